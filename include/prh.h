@@ -1,0 +1,199 @@
+/*
+ * prh.h — C ABI of libprh, the MI355X (gfx950) rasteriser and deferred shader
+ * that sits behind polyred's render.NewRenderer(...).Render() option API.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * an int status (PRH_OK == 0, a negative PRH_E_* code otherwise).  Host memory
+ * passed in is borrowed for the duration of the call only.  Handles are opaque.
+ * No float is passed by value: purego's SyscallN carries integer and pointer
+ * registers only, so every float travels inside a struct passed by pointer.
+ *
+ * Citations are to the polyred sources (paths relative to the polyred repo).
+ * The Go-side binding a maintainer adds is shown in INTEGRATION.md.
+ */
+#ifndef PRH_H
+#define PRH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PRH_OK 0
+#define PRH_E_INVALID (-1)     /* bad argument or handle                        */
+#define PRH_E_UNSUPPORTED (-2) /* scene feature not on the device path; the Go
+                                  caller falls back to the CPU pass, as
+                                  errGPUDeferredUnsupported does
+                                  (render/gpudeferred.go:24, render/raster.go:67-76) */
+#define PRH_E_HIP (-3)         /* a HIP runtime call failed                     */
+#define PRH_E_NOMEM (-4)       /* device allocation failed                      */
+#define PRH_E_NODEVICE (-5)    /* no such GPU                                   */
+
+/* ---- scene contract (host-side, borrowed) ------------------------------ */
+
+/* Blinn-Phong material, one entry of the renderer's flat material table.
+ * Mirrors material.BlinnPhong (material/material.go:44-52).  The texture is a
+ * uniform 1x1 texture (buffer.NewUniformTexture, buffer/texture.go:17-24):
+ * Texture.Query of a 1x1 texture returns its single texel for every
+ * (lod, u, v) (buffer/texture.go:130-160), so the colour is all that matters. */
+typedef struct prh_material {
+    uint8_t diffuse[4];  /* BlinnPhong.Diffuse  RGBA8 */
+    uint8_t specular[4]; /* BlinnPhong.Specular RGBA8 */
+    uint8_t texture[4];  /* uniform texture colour RGBA8 */
+    float shininess;     /* BlinnPhong.Shininess */
+    int32_t flat_shading;      /* Standard.FlatShading     */
+    int32_t ambient_occlusion; /* Standard.AmbientOcclusion */
+    int32_t receive_shadow;    /* Standard.ReceiveShadow   */
+} prh_material;
+
+#define PRH_LIGHT_POINT 0
+#define PRH_LIGHT_DIRECTIONAL 1
+
+/* A light.Source in Scene.Lights() order (scene/scene.go:35-47).
+ * Point: light/point.go:22-46.  Directional: light/directional.go:23-48; the
+ * direction is passed already normalised, as NewDirectional stores it
+ * (light/directional.go:45).  A shadow-casting directional light uses the same
+ * orthographic light camera as a point light, placed at `position` and looking
+ * at the scene centre (render/shadow.go:33-90; the reference builds no camera
+ * for directional lights, see the design notes). */
+typedef struct prh_light {
+    int32_t kind;        /* PRH_LIGHT_POINT or PRH_LIGHT_DIRECTIONAL */
+    float position[3];
+    float direction[3];
+    float intensity;
+    uint8_t color[4];
+    int32_t cast_shadow;
+} prh_light;
+
+/* Scene geometry in the order render.cpuForwardPass visits it
+ * (render/raster.go:226-273): objects in scene.IterObjects order
+ * (scene/core.go:86-111), each object's triangles in Geometry.Triangles()
+ * order.  Meshes hold vertices and triangles; an object instances one mesh
+ * with its composed model matrix (root . groups . geometry, as
+ * `modelMatrix.MulM(g.ModelMatrix())` at render/raster.go:242) and the base of
+ * its materials in the flat table (render/raster.go:252-262). */
+typedef struct prh_scene_desc {
+    int32_t n_meshes;
+    const int32_t* mesh_vtx_off; /* [n_meshes+1] into the vertex arrays        */
+    const int32_t* mesh_tri_off; /* [n_meshes+1] into the triangle arrays      */
+    const float* vpos;           /* [V][4] model-space position (x,y,z,w)     */
+    const float* vnor;           /* [V][4] model-space normal (x,y,z,w)       */
+    const uint8_t* vcol;         /* [V][4] vertex colour RGBA8                */
+    const float* vuv;            /* [V][2] texture coordinates                */
+    const int32_t* tri_v;        /* [T][3] mesh-local vertex indices          */
+    const int32_t* tri_mat;      /* [T] geometry-local material id, <0 = vertex colour */
+
+    int32_t n_objects;
+    const int32_t* obj_mesh;     /* [O] mesh index                            */
+    const float* obj_model;      /* [O][16] row-major Mat4 X00..X33           */
+    const int32_t* obj_mat_base; /* [O] base of the object's materials        */
+
+    int32_t n_materials;
+    const prh_material* materials; /* the flat material table               */
+
+    int32_t n_lights;
+    const prh_light* lights;       /* light sources in Scene.Lights() order */
+    int32_t n_ambient;
+    const float* ambient_intensity; /* light.Ambient intensities in order   */
+
+    float center[3];               /* Scene.Center() (scene/scene.go:49-52) */
+} prh_scene_desc;
+
+#define PRH_CAMERA_PERSPECTIVE 0
+#define PRH_CAMERA_ORTHOGRAPHIC 1
+
+/* camera.Perspective (camera/perspective.go:22-111) or camera.Orthographic
+ * (camera/orthographic.go:22-117). */
+typedef struct prh_camera {
+    int32_t kind;
+    float position[3];
+    float target[3];
+    float up[3];
+    float fov, aspect;                /* perspective: degrees, width/height */
+    float near_, far_;                /* both kinds                         */
+    float left, right, bottom, top;   /* orthographic                       */
+} prh_camera;
+
+/* render.option fields the device path honours (render/options.go:16-33). */
+typedef struct prh_options {
+    int32_t width, height;   /* render.Size                                    */
+    int32_t msaa;            /* render.MSAA; only 1 is on the device path      */
+    uint8_t background[4];   /* render.Background                              */
+    int32_t shadow_map;      /* render.ShadowMap                               */
+    int32_t gamma_correct;   /* render.GammaCorrection                         */
+    int32_t shadow_size;     /* shadow map side; 0 = framebuffer-sized as the
+                                reference (render/shadow.go:87-88)             */
+    int32_t keep_linear;     /* also keep the pre-Round linear shading sums    */
+} prh_options;
+
+/* ---- handles ----------------------------------------------------------- */
+typedef struct prh_ctx prh_ctx;
+typedef struct prh_scene prh_scene;
+typedef struct prh_renderer prh_renderer;
+
+/* Thread-local text of the last error on the calling thread. */
+const char* prh_last_error(void);
+
+/* Open GPU `ordinal` and create the context's stream.  Replaces gpu.Open
+ * (gpu/device.go:77-89) for this option; the caller owns the context. */
+int prh_open(int32_t ordinal, prh_ctx** out);
+int prh_close(prh_ctx* ctx);
+
+/* Copy a scene into device memory (HBM).  The descriptor is borrowed.
+ * Replaces the per-frame CPU marshalling of render/gpuforward.go:271-310 and
+ * render/gpudeferred.go:130-187 with one upload. */
+int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* desc, prh_scene** out);
+int prh_scene_free(prh_scene* scene);
+
+/* Create a renderer: render.NewRenderer(Size, Camera, Scene, Background,
+ * ShadowMap, GammaCorrection, ...) (render/raster.go:84-143), including the
+ * one-time shadow-camera fit of initShadowMaps (render/shadow.go:33-90) and
+ * its zero-initialised shadow maps. */
+int prh_renderer_new(prh_ctx* ctx, prh_scene* scene, const prh_camera* camera,
+                     const prh_options* opts, prh_renderer** out);
+int prh_renderer_free(prh_renderer* r);
+
+/* One frame of (*Renderer).Render() (render/raster.go:155-199): shadow
+ * passes, forward raster, deferred shading, gamma.  Returns when the frame is
+ * complete on the device; the image stays in HBM until read. */
+int prh_render(prh_renderer* r);
+
+/* The rendered RGBA8 image, top row first, the layout of
+ * FragmentBuffer.Image() (buffer/buffer.go:160-166).  bytes >= 4*W*H. */
+int prh_read_image(prh_renderer* r, uint8_t* rgba, size_t bytes);
+
+#define PRH_BUF_VIS 0       /* uint64 [H][W] raster rows: depth|primitive key  */
+#define PRH_BUF_DEPTH 1     /* float  [H][W] fragment depth, -1 if uncovered   */
+#define PRH_BUF_LINEAR 2    /* float  [H][W][3] pre-Round shading sums         */
+#define PRH_BUF_SHADOW 3    /* float  [n_lights][S][S] shadow maps             */
+#define PRH_BUF_PREGAMMA 4  /* uint8  [H][W][4] deferred colour before gamma   */
+
+/* Copy one internal buffer to host memory (parity tests, debugging). */
+int prh_read_buffer(prh_renderer* r, int32_t which, void* dst, size_t bytes);
+
+/* Per-kernel device time of the last frame, from HIP events on the
+ * renderer's stream. */
+typedef struct prh_frame_stats {
+    float ms_total;
+    float ms_shadow;     /* all shadow passes                               */
+    float ms_vertex;     /* vertex transform of the main view               */
+    float ms_setup;      /* cull, clip, bin                                 */
+    float ms_raster;     /* visibility raster                               */
+    float ms_shade;      /* deferred shade + shadows + AO + gamma           */
+    int32_t n_prims;     /* primitives after cull/clip                      */
+    int32_t n_clipped;   /* triangles that went through the clipper         */
+    int32_t n_bin_pairs; /* (primitive, tile) pairs                         */
+    int32_t nan_rerun;   /* 1 if the NaN-depth exact-order pass ran         */
+} prh_frame_stats;
+int prh_last_stats(prh_renderer* r, prh_frame_stats* out);
+
+/* Library build identification ("libprh gfx950 ..."). */
+const char* prh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRH_H */

@@ -1,0 +1,131 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes loader of the C restatement of polyred's
+CPU renderer (oracle/oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the parity checker or the timed CPU baseline.  The product
+(polyred_amd, libprh) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from polyred_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+
+class oracle_out(C.Structure):
+    _fields_ = [
+        ("vis_order", C.POINTER(C.c_uint32)),
+        ("depth", C.POINTER(C.c_float)),
+        ("linear", C.POINTER(C.c_float)),
+        ("pregamma", C.POINTER(C.c_uint8)),
+        ("image", C.POINTER(C.c_uint8)),
+        ("shadow_maps", C.POINTER(C.c_float)),
+        ("shade_mask", C.POINTER(C.c_uint8)),
+        ("shade_err64", C.POINTER(C.c_double)),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        f32p, u8p = C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+        L.oracle_render.argtypes = [C.POINTER(abi.prh_scene_desc), C.POINTER(abi.prh_camera),
+                                    C.POINTER(abi.prh_options), C.POINTER(oracle_out)]
+        L.oracle_barycoord.argtypes = [f32p] * 5
+        L.oracle_viewport_matrix.argtypes = [C.c_float, C.c_float, f32p]
+        L.oracle_view_matrix.argtypes = [f32p] * 4
+        L.oracle_proj_matrix.argtypes = [C.POINTER(abi.prh_camera), f32p]
+        L.oracle_mat4_inv.argtypes = [f32p, f32p]
+        L.oracle_mat4_mulm.argtypes = [f32p, f32p, f32p]
+        L.oracle_aabb_intersect.argtypes = [f32p] * 4
+        L.oracle_aabb_contains.argtypes = [f32p, f32p, C.c_int, f32p]
+        L.oracle_aabb_new.argtypes = [C.c_int, f32p, f32p, f32p]
+        L.oracle_tri_is_valid.argtypes = [f32p] * 3
+        L.oracle_interp_worldpos.argtypes = [f32p] * 5
+        L.oracle_mat_at.argtypes = [C.c_int32, C.c_int64]
+        L.oracle_srgb_from_linear.argtypes = [C.c_float]
+        L.oracle_srgb_from_linear.restype = C.c_float
+        L.oracle_linear_from_srgb.argtypes = [C.c_float]
+        L.oracle_linear_from_srgb.restype = C.c_float
+        L.oracle_gamma_u8.argtypes = [C.c_uint8]
+        L.oracle_gamma_u8.restype = C.c_uint8
+        L.oracle_shadow_quant.argtypes = [C.c_uint8, C.c_int]
+        L.oracle_shadow_quant.restype = C.c_uint8
+        L.oracle_fragment_shader.argtypes = [C.POINTER(abi.prh_material), f32p, f32p, f32p, f32p, C.c_int,
+                                             C.POINTER(abi.prh_light), C.c_int, f32p, u8p, f32p]
+        L.oracle_ao_dirs.argtypes = [f32p]
+        _lib = L
+    return _lib
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def f32(*v) -> np.ndarray:
+    return np.ascontiguousarray(np.array(v, dtype=np.float32).reshape(-1))
+
+
+class Frame:
+    """Outputs of one oracle frame (raster-row layout unless noted)."""
+
+    def __init__(self, scene, mask=None, shadows=True):
+        W, H = scene.width, scene.height
+        n = W * H
+        self.vis_order = np.zeros(n, np.uint32)
+        self.depth = np.zeros(n, np.float32)
+        self.linear = np.zeros(3 * n, np.float32)
+        self.pregamma = np.zeros(4 * n, np.uint8)
+        self.image = np.zeros(4 * n, np.uint8)
+        S = scene.shadow_size if scene.shadow_size > 0 else None
+        nl = max(1, len(scene.lights))
+        self.shadow_maps = np.zeros(nl * (S * S if S else n), np.float32) if scene.shadow_map else None
+        self.err64 = C.c_double(0.0)
+        self.mask = None if mask is None else np.ascontiguousarray(mask.astype(np.uint8).reshape(-1))
+
+
+def render(scene, mask=None) -> Frame:
+    """Render one frame of `scene` with the oracle."""
+    L = lib()
+    fr = Frame(scene, mask)
+    o = oracle_out()
+    o.vis_order = fr.vis_order.ctypes.data_as(C.POINTER(C.c_uint32))
+    o.depth = fptr(fr.depth)
+    o.linear = fptr(fr.linear)
+    o.pregamma = fr.pregamma.ctypes.data_as(C.POINTER(C.c_uint8))
+    o.image = fr.image.ctypes.data_as(C.POINTER(C.c_uint8))
+    if fr.shadow_maps is not None:
+        o.shadow_maps = fptr(fr.shadow_maps)
+    if fr.mask is not None:
+        o.shade_mask = fr.mask.ctypes.data_as(C.POINTER(C.c_uint8))
+    o.shade_err64 = C.pointer(fr.err64)
+    desc = scene.desc()
+    cam = scene.camera_struct()
+    opts = scene.options_struct()
+    rc = L.oracle_render(C.byref(desc), C.byref(cam), C.byref(opts), C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"oracle_render failed: {rc}")
+    W, H = scene.width, scene.height
+    fr.vis_order = fr.vis_order.reshape(H, W)
+    fr.depth = fr.depth.reshape(H, W)
+    fr.linear = fr.linear.reshape(H, W, 3)
+    fr.pregamma = fr.pregamma.reshape(H, W, 4)
+    fr.image = fr.image.reshape(H, W, 4)
+    return fr

@@ -1,0 +1,982 @@
+/*
+ * oracle.c — TEST INFRASTRUCTURE ONLY (see oracle.h for the pinning list).
+ *
+ * Plain-C restatement of polyred's CPU rasteriser and deferred shader,
+ * sequential (Workers(1)) order.  Every function names the reference lines it
+ * follows.  Build flags matter: -O2 -ffp-contract=off -fno-fast-math on
+ * x86-64 (SSE), so every float operation rounds once, as Go's amd64 compiler
+ * (GOAMD64=v1) evaluates the reference.
+ *
+ * Go semantics reproduced here:
+ *   - math.FMA[float32](x,y,z) = float32(math.FMA(float64 x,y,z))
+ *     (math/math.go:265-267): a double FMA then a rounding to float;
+ *   - Sqrt/Pow/Tan/Cos/Sin/Atan/Round go through float64 and back
+ *     (math/math.go:58-233);
+ *   - generic Min/Max fold through math.Min/math.Max on float64 starting at
+ *     +-MaxFloat64 (math/math.go:245-261), with math.Max's special cases
+ *     (+Inf first, then NaN, +0 preferred over -0);
+ *   - float->int is CVTTSS2SQ (NaN and out-of-range give MinInt64) and
+ *     float->uint8 goes through int32 then keeps the low byte.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { float x, y, z, w; } v4;
+typedef struct { float x, y, z; } v3;
+typedef struct { float a[16]; } m4; /* row-major, a[r*4+c] = Xrc */
+
+/* ------------------------------------------------------------------ Go math */
+static inline float gfma(float a, float b, float c) { return (float)fma((double)a, (double)b, (double)c); }
+static inline float gsqrt(float x) { return (float)sqrt((double)x); }
+static inline float ground(float x) { return (float)round((double)x); }
+static inline float gpow(float x, float y) { return (float)pow((double)x, (double)y); }
+
+static double go_max64(double x, double y) {
+    if ((isinf(x) && x > 0) || (isinf(y) && y > 0)) return INFINITY;
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == 0 && x == y) return signbit(x) ? y : x;
+    return x > y ? x : y;
+}
+static double go_min64(double x, double y) {
+    if ((isinf(x) && x < 0) || (isinf(y) && y < 0)) return -INFINITY;
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == 0 && x == y) return signbit(x) ? x : y;
+    return x < y ? x : y;
+}
+/* generic Max[float32](a, b) / Min[float32](a, b) */
+static inline float gmax2(float a, float b) { return (float)go_max64(go_max64(-DBL_MAX, a), b); }
+static inline float gmin2(float a, float b) { return (float)go_min64(go_min64(DBL_MAX, a), b); }
+
+static inline int64_t gf2i(float x) {
+    if (x >= -9223372036854775808.0f && x < 9223372036854775808.0f) return (int64_t)x;
+    return INT64_MIN;
+}
+static inline uint8_t gf2u8(float x) {
+    if (x >= -2147483648.0f && x < 2147483648.0f) return (uint8_t)(uint32_t)(int32_t)x;
+    return 0;
+}
+static inline float gclamp(float n, float lo, float hi) { /* math/clamp.go:8-16 */
+    if (n < lo) return lo;
+    if (n > hi) return hi;
+    return n;
+}
+static inline int approx_eq(float a, float b, float eps) { return (float)fabs((double)(a - b)) <= eps; }
+
+/* ---------------------------------------------------------------- vectors */
+static inline v4 V4(float x, float y, float z, float w) { v4 r = {x, y, z, w}; return r; }
+static inline v3 V3(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v4 sub4(v4 a, v4 b) { return V4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+static inline v4 add4(v4 a, v4 b) { return V4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline v4 scale4(v4 a, float x, float y, float z, float w) { return V4(a.x * x, a.y * y, a.z * z, a.w * w); }
+/* math/vec4.go:89-91 */
+static inline float dot4(v4 v, v4 u) { return gfma(v.x, u.x, gfma(v.y, u.y, gfma(v.z, u.z, v.w * u.w))); }
+static inline float len4(v4 v) { return gsqrt(dot4(v, v)); }
+/* math/vec4.go:96-104 */
+static inline v4 unit4(v4 v) { float n = 1.0f / len4(v); return V4(v.x * n, v.y * n, v.z * n, v.w * n); }
+/* math/vec4.go:107-116 */
+static inline v4 apply4(v4 v, const m4* m) {
+    const float* a = m->a;
+    return V4(gfma(a[0], v.x, gfma(a[1], v.y, gfma(a[2], v.z, a[3] * v.w))),
+              gfma(a[4], v.x, gfma(a[5], v.y, gfma(a[6], v.z, a[7] * v.w))),
+              gfma(a[8], v.x, gfma(a[9], v.y, gfma(a[10], v.z, a[11] * v.w))),
+              gfma(a[12], v.x, gfma(a[13], v.y, gfma(a[14], v.z, a[15] * v.w))));
+}
+/* math/vec4.go:127-137 */
+static inline v4 cross4(v4 v, v4 u) {
+    return V4(gfma(v.y, u.z, -v.z * u.y), gfma(v.z, u.x, -v.x * u.z), gfma(v.x, u.y, -v.y * u.x), 0);
+}
+/* math/vec4.go:140-146 */
+static inline v4 pos4(v4 v) {
+    if (v.w == 1 || v.w == 0) return V4(v.x, v.y, v.z, 1);
+    float inv = 1.0f / v.w;
+    return V4(v.x * inv, v.y * inv, v.z * inv, 1);
+}
+static inline v3 sub3(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+/* math/vec3.go:78-120 */
+static inline float dot3(v3 v, v3 u) { return gfma(v.x, u.x, gfma(v.y, u.y, v.z * u.z)); }
+static inline v3 unit3(v3 v) { float n = 1.0f / gsqrt(dot3(v, v)); return V3(v.x * n, v.y * n, v.z * n); }
+static inline v3 cross3(v3 v, v3 u) {
+    return V3(gfma(v.y, u.z, -v.z * u.y), gfma(v.z, u.x, -v.x * u.z), gfma(v.x, u.y, -v.y * u.x));
+}
+
+/* ---------------------------------------------------------------- matrices */
+/* math/mat4.go:201-219: each entry a left-to-right sum of four products. */
+static m4 mulm(const m4* m, const m4* n) {
+    m4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            r.a[i * 4 + j] = m->a[i * 4 + 0] * n->a[0 * 4 + j] + m->a[i * 4 + 1] * n->a[1 * 4 + j] +
+                             m->a[i * 4 + 2] * n->a[2 * 4 + j] + m->a[i * 4 + 3] * n->a[3 * 4 + j];
+    return r;
+}
+/* math/mat4.go:224-231 */
+static v4 mulv(const m4* m, v4 v) {
+    const float* a = m->a;
+    return V4(a[0] * v.x + a[1] * v.y + a[2] * v.z + a[3] * v.w, a[4] * v.x + a[5] * v.y + a[6] * v.z + a[7] * v.w,
+              a[8] * v.x + a[9] * v.y + a[10] * v.z + a[11] * v.w,
+              a[12] * v.x + a[13] * v.y + a[14] * v.z + a[15] * v.w);
+}
+static m4 transpose(const m4* m) {
+    m4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) r.a[i * 4 + j] = m->a[j * 4 + i];
+    return r;
+}
+/* Determinant terms in the order of math/mat4.go:233-247: {sign, i, j, k, l},
+ * each term ((a_i*a_j)*a_k)*a_l, accumulated left to right. */
+static const signed char DET_T[24][5] = {
+    {+1, 0, 5, 10, 15}, {-1, 0, 5, 11, 14}, {+1, 0, 6, 11, 13}, {-1, 0, 6, 9, 15},  {+1, 0, 7, 9, 14},
+    {-1, 0, 7, 10, 13}, {-1, 1, 6, 11, 12}, {+1, 1, 6, 8, 15},  {-1, 1, 7, 8, 14},  {+1, 1, 7, 10, 12},
+    {-1, 1, 4, 10, 15}, {+1, 1, 4, 11, 14}, {+1, 2, 7, 8, 13},  {-1, 2, 7, 9, 12},  {+1, 2, 4, 9, 15},
+    {-1, 2, 4, 11, 13}, {+1, 2, 5, 11, 12}, {-1, 2, 5, 8, 15},  {-1, 3, 4, 9, 14},  {+1, 3, 4, 10, 13},
+    {-1, 3, 5, 10, 12}, {+1, 3, 5, 8, 14},  {-1, 3, 6, 8, 13},  {+1, 3, 6, 9, 12}};
+/* Cofactor terms of math/mat4.go:259-282 per output entry: {sign, i, j, k}. */
+static const signed char INV_T[16][6][4] = {
+    {{+1, 6, 11, 13}, {-1, 7, 10, 13}, {+1, 7, 9, 14}, {-1, 5, 11, 14}, {-1, 6, 9, 15}, {+1, 5, 10, 15}},
+    {{+1, 3, 10, 13}, {-1, 2, 11, 13}, {-1, 3, 9, 14}, {+1, 1, 11, 14}, {+1, 2, 9, 15}, {-1, 1, 10, 15}},
+    {{+1, 2, 7, 13}, {-1, 3, 6, 13}, {+1, 3, 5, 14}, {-1, 1, 7, 14}, {-1, 2, 5, 15}, {+1, 1, 6, 15}},
+    {{+1, 3, 6, 9}, {-1, 2, 7, 9}, {-1, 3, 5, 10}, {+1, 1, 7, 10}, {+1, 2, 5, 11}, {-1, 1, 6, 11}},
+    {{+1, 7, 10, 12}, {-1, 6, 11, 12}, {-1, 7, 8, 14}, {+1, 4, 11, 14}, {+1, 6, 8, 15}, {-1, 4, 10, 15}},
+    {{+1, 2, 11, 12}, {-1, 3, 10, 12}, {+1, 3, 8, 14}, {-1, 0, 11, 14}, {-1, 2, 8, 15}, {+1, 0, 10, 15}},
+    {{+1, 3, 6, 12}, {-1, 2, 7, 12}, {-1, 3, 4, 14}, {+1, 0, 7, 14}, {+1, 2, 4, 15}, {-1, 0, 6, 15}},
+    {{+1, 2, 7, 8}, {-1, 3, 6, 8}, {+1, 3, 4, 10}, {-1, 0, 7, 10}, {-1, 2, 4, 11}, {+1, 0, 6, 11}},
+    {{+1, 5, 11, 12}, {-1, 7, 9, 12}, {+1, 7, 8, 13}, {-1, 4, 11, 13}, {-1, 5, 8, 15}, {+1, 4, 9, 15}},
+    {{+1, 3, 9, 12}, {-1, 1, 11, 12}, {-1, 3, 8, 13}, {+1, 0, 11, 13}, {+1, 1, 8, 15}, {-1, 0, 9, 15}},
+    {{+1, 1, 7, 12}, {-1, 3, 5, 12}, {+1, 3, 4, 13}, {-1, 0, 7, 13}, {-1, 1, 4, 15}, {+1, 0, 5, 15}},
+    {{+1, 3, 5, 8}, {-1, 1, 7, 8}, {-1, 3, 4, 9}, {+1, 0, 7, 9}, {+1, 1, 4, 11}, {-1, 0, 5, 11}},
+    {{+1, 6, 9, 12}, {-1, 5, 10, 12}, {-1, 6, 8, 13}, {+1, 4, 10, 13}, {+1, 5, 8, 14}, {-1, 4, 9, 14}},
+    {{+1, 1, 10, 12}, {-1, 2, 9, 12}, {+1, 2, 8, 13}, {-1, 0, 10, 13}, {-1, 1, 8, 14}, {+1, 0, 9, 14}},
+    {{+1, 2, 5, 12}, {-1, 1, 6, 12}, {-1, 2, 4, 13}, {+1, 0, 6, 13}, {+1, 1, 4, 14}, {-1, 0, 5, 14}},
+    {{+1, 1, 6, 8}, {-1, 2, 5, 8}, {+1, 2, 4, 9}, {-1, 0, 6, 9}, {-1, 1, 4, 10}, {+1, 0, 5, 10}}};
+
+static float det(const m4* m) {
+    const float* a = m->a;
+    float acc = 0;
+    for (int t = 0; t < 24; t++) {
+        float p = a[DET_T[t][1]] * a[DET_T[t][2]] * a[DET_T[t][3]] * a[DET_T[t][4]];
+        acc = t == 0 ? p : (DET_T[t][0] > 0 ? acc + p : acc - p);
+    }
+    return acc;
+}
+/* math/mat4.go:255-283 (the zero-determinant panic becomes an identity-free
+ * NaN result; the reference scenes never hit it). */
+static m4 inv(const m4* m) {
+    const float* a = m->a;
+    float dinv = 1 / det(m);
+    m4 r;
+    for (int e = 0; e < 16; e++) {
+        float acc = 0;
+        for (int t = 0; t < 6; t++) {
+            float p = a[INV_T[e][t][1]] * a[INV_T[e][t][2]] * a[INV_T[e][t][3]];
+            acc = t == 0 ? p : (INV_T[e][t][0] > 0 ? acc + p : acc - p);
+        }
+        r.a[e] = dinv * acc;
+    }
+    return r;
+}
+/* math/math.go:270-277 */
+static m4 viewport_matrix(float w, float h) {
+    m4 r = {{w / 2, 0, 0, w / 2, 0, h / 2, 0, h / 2, 0, 0, 1, 0, 0, 0, 0, 1}};
+    return r;
+}
+/* camera/camera.go:42-53 */
+static m4 view_matrix(v3 pos, v3 target, v3 up) {
+    v3 l = unit3(sub3(target, pos));
+    v3 lxu = unit3(cross3(l, up));
+    v3 u = unit3(cross3(lxu, l));
+    m4 r = {{lxu.x, lxu.y, lxu.z, -dot3(lxu, pos), u.x, u.y, u.z, -dot3(u, pos), -l.x, -l.y, -l.z, dot3(l, pos), 0, 0,
+             0, 1}};
+    return r;
+}
+/* camera/perspective.go:100-111 */
+static m4 persp_proj(float fov_deg, float aspect, float n, float f) {
+    float fov = (fov_deg * (float)M_PI) / 180;
+    float t = (float)tan((double)(fov / 2));
+    m4 r = {{-1 / (aspect * t), 0, 0, 0, 0, -1 / t, 0, 0, 0, 0, (n + f) / (n - f), (2 * n * f) / (n - f), 0, 0, 1, 0}};
+    return r;
+}
+/* camera/orthographic.go:106-117 */
+static m4 ortho_proj(float l, float r_, float b, float t, float n, float f) {
+    m4 r = {{2 / (r_ - l), 0, 0, (l + r_) / (l - r_), 0, 2 / (t - b), 0, (b + t) / (b - t), 0, 0, 2 / (n - f),
+             (f + n) / (f - n), 0, 0, 0, 1}};
+    return r;
+}
+static m4 cam_proj(const prh_camera* c) {
+    if (c->kind == PRH_CAMERA_PERSPECTIVE) return persp_proj(c->fov, c->aspect, c->near_, c->far_);
+    return ortho_proj(c->left, c->right, c->bottom, c->top, c->near_, c->far_);
+}
+static inline v3 V3a(const float* p) { return V3(p[0], p[1], p[2]); }
+
+/* -------------------------------------------------------------- AABB etc. */
+typedef struct { v3 mn, mx; } aabb_t;
+/* geometry/primitive/box.go:15-25 */
+static aabb_t aabb_new(int n, const v3* vs) {
+    aabb_t b = {V3(FLT_MAX, FLT_MAX, FLT_MAX), V3(-FLT_MAX, -FLT_MAX, -FLT_MAX)};
+    for (int i = 0; i < n; i++) {
+        b.mn.x = gmin2(b.mn.x, vs[i].x);
+        b.mn.y = gmin2(b.mn.y, vs[i].y);
+        b.mn.z = gmin2(b.mn.z, vs[i].z);
+        b.mx.x = gmax2(b.mx.x, vs[i].x);
+        b.mx.y = gmax2(b.mx.y, vs[i].y);
+        b.mx.z = gmax2(b.mx.z, vs[i].z);
+    }
+    return b;
+}
+/* geometry/primitive/box.go:32-41, including maxZ = min(a.Max.Y, b.Max.Z) */
+static int aabb_intersect(const aabb_t* a, const aabb_t* b) {
+    float minX = gmax2(a->mn.x, b->mn.x), minY = gmax2(a->mn.y, b->mn.y), minZ = gmax2(a->mn.z, b->mn.z);
+    float maxX = gmin2(a->mx.x, b->mx.x), maxY = gmin2(a->mx.y, b->mx.y), maxZ = gmin2(a->mx.y, b->mx.z);
+    return minX <= maxX && minY <= maxY && minZ <= maxZ;
+}
+/* geometry/primitive/box.go:61-75 with math.ApproxEq/ApproxLess (math/math.go:41-49) */
+static inline int less_eq(float v1, float v2) {
+    const float eps = 1e-7f;
+    return approx_eq(v1, v2, eps) || (v1 < v2 && (float)fabs((double)(v1 - v2)) > eps);
+}
+static int aabb_contains(const aabb_t* b, int n, const v3* vs) {
+    for (int i = 0; i < n; i++) {
+        int ok = less_eq(b->mn.x, vs[i].x) && less_eq(b->mn.y, vs[i].y) && less_eq(b->mn.z, vs[i].z) &&
+                 less_eq(vs[i].x, b->mx.x) && less_eq(vs[i].y, b->mx.y) && less_eq(vs[i].z, b->mx.z);
+        if (!ok) return 0;
+    }
+    return 1;
+}
+/* geometry/primitive/triangle.go:63-80 */
+static int tri_is_valid(v4 p1, v4 p2, v4 p3) {
+    const float eps = 1e-7f;
+    v4 a = sub4(p2, p1), b = sub4(p3, p1);
+    if (approx_eq(a.x, 0, eps) && approx_eq(a.y, 0, eps) && approx_eq(a.z, 0, eps)) return 0;
+    if (approx_eq(b.x, 0, eps) && approx_eq(b.y, 0, eps) && approx_eq(b.z, 0, eps)) return 0;
+    float d = dot4(a, b) / (len4(a) * len4(b));
+    return !approx_eq(d, 1, eps) && !approx_eq(d, -1, eps);
+}
+/* math/interpolate.go:66-78 */
+static inline void barycoord(float px, float py, float ax, float ay, float bx, float by, float cx, float cy, float out[3]) {
+    v3 ap = V3(px - ax, py - ay, 0), ab = V3(bx - ax, by - ay, 0), ac = V3(cx - ax, cy - ay, 0);
+    v3 bc = V3(cx - bx, cy - by, 0), bp = V3(px - bx, py - by, 0);
+    float Sabc = cross3(ab, ac).z, Sabp = cross3(ab, ap).z, Sapc = cross3(ap, ac).z, Sbcp = cross3(bc, bp).z;
+    out[0] = Sbcp / Sabc;
+    out[1] = Sapc / Sabc;
+    out[2] = Sabp / Sabc;
+}
+
+/* ------------------------------------------------------------ sRGB (color) */
+static double lin2srgb_lut[1025];
+static int lut_ready = 0;
+/* color/srgb.go:57-75,86-93 */
+static void lut_init(void) {
+    if (lut_ready) return;
+    for (int i = 0; i < 1024; i++) {
+        double v = (double)i / 1024;
+        lin2srgb_lut[i] = v <= 0.0031308 ? v * 12.92 : 1.055 * pow(v, 1.0 / 2.4) - 0.055;
+    }
+    lin2srgb_lut[1024] = lin2srgb_lut[1023];
+    lut_ready = 1;
+}
+/* color/srgb.go:15-31 */
+static float from_linear_srgb(float v) {
+    lut_init();
+    if (v <= 0) return 0;
+    if (v == 1) return 1;
+    float i = v * 1024;
+    int64_t ifl = gf2i(i) & 1023;
+    float v0 = (float)lin2srgb_lut[ifl], v1 = (float)lin2srgb_lut[ifl + 1];
+    i -= (float)ifl;
+    return v0 * (1.0f - i) + v1 * i;
+}
+static float from_srgb_linear(float v) { /* color/srgb.go:34-50 with its LUT */
+    if (v <= 0) return 0;
+    if (v >= 1) return 1;
+    float i = v * 1024;
+    int64_t ifl = gf2i(i) & 1023;
+    double l0 = (double)ifl / 1024, l1 = (double)(ifl + 1) / 1024;
+    double t0 = l0 <= 0.04045 ? l0 / 12.92 : pow((l0 + 0.055) / 1.055, 2.4);
+    double t1 = ifl + 1 < 1024 ? (l1 <= 0.04045 ? l1 / 12.92 : pow((l1 + 0.055) / 1.055, 2.4)) : t0;
+    if (ifl + 1 == 1024) t1 = t0;
+    float v0 = (float)t0, v1 = (float)t1;
+    i -= (float)ifl;
+    return v0 * (1.0f - i) + v1 * i;
+}
+/* shader/gamma.go:13-18 */
+static uint8_t gamma_u8(uint8_t c) { return gf2u8(from_linear_srgb((float)c / 0xff) * 0xff + 0.5f); }
+
+/* ------------------------------------------------------------ frame state */
+typedef struct {
+    int ok;
+    int x, y;
+    float depth;
+    v4 nor, facenor, wpos;
+    uint8_t col[4];
+    int64_t matid;
+    uint32_t order; /* oracle bookkeeping: triangle*8+sub of the stored fragment */
+} frag_t;
+
+typedef struct {
+    v4 pos, nor;
+    uint8_t col[4];
+    float uv[2];
+} vert_t;
+
+typedef struct {
+    const prh_scene_desc* s;
+    const prh_camera* cam;
+    const prh_options* o;
+    int W, H;         /* framebuffer (W*MSAA, H*MSAA) */
+    int perspect;
+    frag_t* f;        /* storage rows (H-1-y) as FragmentBuffer */
+    /* shadow */
+    int SW, SH;
+    m4* lview;
+    m4* lproj;
+    int* lactive;
+    float* smaps;
+    /* deferred uniforms */
+    m4 view, proj, vp, viewinv, projinv, vpinv, s2w;
+    float ao_dir[16];
+    float* linear;
+    double shade_err64;
+} ctx_t;
+
+static inline frag_t* fat(ctx_t* c, int x, int y) { return &c->f[(size_t)(c->H - 1 - y) * c->W + x]; }
+static inline int in_buf(const ctx_t* c, int64_t x, int64_t y) { return x >= 0 && x < c->W && y >= 0 && y < c->H; }
+
+/* buffer/buffer.go:269-280 */
+static inline int depth_test(ctx_t* c, int x, int y, float z) {
+    frag_t* f = fat(c, x, y);
+    return !f->ok || z > f->depth;
+}
+/* buffer/buffer.go:221-266 (colour/depth images are produced later) */
+static inline void fset(ctx_t* c, int x, int y, const frag_t* nf) {
+    frag_t* f = fat(c, x, y);
+    if (f->ok && nf->depth <= f->depth) return;
+    *f = *nf;
+}
+
+/* render/raster.go:453-460 */
+static inline v4 interp_wpos(const float bc[3], v4 m1, v4 m2, v4 m3) {
+    return V4(bc[0] * m1.x + bc[1] * m2.x + bc[2] * m3.x, bc[0] * m1.y + bc[1] * m2.y + bc[2] * m3.y,
+              bc[0] * m1.z + bc[1] * m2.z + bc[2] * m3.z, 1);
+}
+
+/* render/raster.go:462-572 (UV/Du/Dv only feed Texture.Query, which is
+ * constant for uniform textures, so they are not materialised) */
+static void draw_clipped(ctx_t* c, const m4* vpinv, const m4* pinv, const m4* vinv, const vert_t* t1,
+                         const vert_t* t2, const vert_t* t3, const float recipw[3], int64_t matid, uint32_t order) {
+    v4 m1 = apply4(apply4(apply4(t1->pos, vpinv), pinv), vinv);
+    v4 m2 = apply4(apply4(apply4(t2->pos, vpinv), pinv), vinv);
+    v4 m3 = apply4(apply4(apply4(t3->pos, vpinv), pinv), vinv);
+    v3 ps[3] = {V3(t1->pos.x, t1->pos.y, t1->pos.z), V3(t2->pos.x, t2->pos.y, t2->pos.z),
+                V3(t3->pos.x, t3->pos.y, t3->pos.z)};
+    aabb_t bb = aabb_new(3, ps);
+    int64_t xmin = gf2i(ground(bb.mn.x) - 1), xmax = gf2i(ground(bb.mx.x) + 1);
+    int64_t ymin = gf2i(ground(bb.mn.y) - 1), ymax = gf2i(ground(bb.mx.y) + 1);
+    v4 fN = unit4(cross4(sub4(m2, m1), sub4(m3, m1)));
+    if (xmin < 0) xmin = 0;
+    if (ymin < 0) ymin = 0;
+    if (xmax > c->W - 1) xmax = c->W - 1;
+    if (ymax > c->H - 1) ymax = c->H - 1;
+    for (int64_t x = xmin; x <= xmax; x++) {
+        for (int64_t y = ymin; y <= ymax; y++) {
+            float bc[3];
+            barycoord((float)x + 0.5f, (float)y + 0.5f, t1->pos.x, t1->pos.y, t2->pos.x, t2->pos.y, t3->pos.x,
+                      t3->pos.y, bc);
+            if (bc[0] < -1e-7f || bc[1] < -1e-7f || bc[2] < -1e-7f) continue;
+            float z = bc[0] * t1->pos.z + bc[1] * t2->pos.z + bc[2] * t3->pos.z;
+            if (!depth_test(c, (int)x, (int)y, z)) continue;
+            float wc1 = recipw[0] * bc[0], wc2 = recipw[1] * bc[1], wc3 = recipw[2] * bc[2];
+            float norm = 1.0f;
+            if (c->perspect) norm = 1 / (wc1 + wc2 + wc3);
+            frag_t nf;
+            nf.ok = 1;
+            nf.x = (int)x;
+            nf.y = (int)y;
+            nf.depth = z;
+            nf.nor = unit4(V4(bc[0] * t1->nor.x + bc[1] * t2->nor.x + bc[2] * t3->nor.x,
+                              bc[0] * t1->nor.y + bc[1] * t2->nor.y + bc[2] * t3->nor.y,
+                              bc[0] * t1->nor.z + bc[1] * t2->nor.z + bc[2] * t3->nor.z, 0));
+            nf.facenor = fN;
+            nf.wpos = interp_wpos(bc, m1, m2, m3);
+            for (int k = 0; k < 4; k++)
+                nf.col[k] = gf2u8(gclamp(
+                    (wc1 * (float)t1->col[k] + wc2 * (float)t2->col[k] + wc3 * (float)t3->col[k]) * norm, 0, 0xff));
+            nf.matid = matid;
+            nf.order = order;
+            fset(c, (int)x, (int)y, &nf);
+        }
+    }
+}
+
+/* render/clipping.go:14-65 */
+typedef struct { v4 pos, nor; } plane_t;
+static inline int in_front(const plane_t* p, v4 v) { return dot4(sub4(v, p->pos), p->nor) > 0; }
+static inline v4 isect(const plane_t* p, v4 v0, v4 v1) {
+    v4 u = sub4(v1, v0), w = sub4(v0, p->pos);
+    float d = dot4(p->nor, u);
+    float n = -dot4(p->nor, w);
+    float s = n / d;
+    return add4(v0, scale4(u, s, s, s, s));
+}
+static int sutherland_hodgman(const v4* pts, int n, float w, float h, v4* out) {
+    plane_t planes[6] = {{V4(w, 0, 0, 1), V4(-1, 0, 0, 1)}, {V4(0, 0, 0, 1), V4(1, 0, 0, 1)},
+                         {V4(0, h, 0, 1), V4(0, -1, 0, 1)}, {V4(0, 0, 0, 1), V4(0, 1, 0, 1)},
+                         {V4(0, 0, 1, 1), V4(0, 0, -1, 1)}, {V4(0, 0, -1, 1), V4(0, 0, 1, 1)}};
+    v4 a[16], b[16];
+    int na = n;
+    for (int i = 0; i < n; i++) a[i] = pts[i];
+    for (int pi = 0; pi < 6; pi++) {
+        if (na == 0) return 0;
+        int nb = 0;
+        v4 s = a[na - 1];
+        for (int i = 0; i < na; i++) {
+            v4 e = a[i];
+            if (in_front(&planes[pi], e)) {
+                if (!in_front(&planes[pi], s)) b[nb++] = isect(&planes[pi], s, e);
+                b[nb++] = e;
+            } else if (in_front(&planes[pi], s)) {
+                b[nb++] = isect(&planes[pi], s, e);
+            }
+            s = e;
+        }
+        for (int i = 0; i < nb; i++) a[i] = b[i];
+        na = nb;
+    }
+    for (int i = 0; i < na; i++) out[i] = a[i];
+    return na;
+}
+/* render/clipping.go:67-157 */
+static int clip_triangle(const vert_t* v1, const vert_t* v2, const vert_t* v3, float w, float h, vert_t (*res)[3]) {
+    v4 pts[3] = {v1->pos, v2->pos, v3->pos}, clips[16];
+    int n = sutherland_hodgman(pts, 3, w, h, clips);
+    int nt = 0;
+    for (int i = 2; i < n; i++) {
+        const v4* cp[3] = {&clips[0], &clips[i - 1], &clips[i]};
+        for (int k = 0; k < 3; k++) {
+            float b[3];
+            barycoord(cp[k]->x, cp[k]->y, v1->pos.x, v1->pos.y, v2->pos.x, v2->pos.y, v3->pos.x, v3->pos.y, b);
+            vert_t* t = &res[nt][k];
+            t->pos = V4(b[0] * v1->pos.x + b[1] * v2->pos.x + b[2] * v3->pos.x,
+                        b[0] * v1->pos.y + b[1] * v2->pos.y + b[2] * v3->pos.y,
+                        b[0] * v1->pos.z + b[1] * v2->pos.z + b[2] * v3->pos.z, 1);
+            t->uv[0] = b[0] * v1->uv[0] + b[1] * v2->uv[0] + b[2] * v3->uv[0];
+            t->uv[1] = b[0] * v1->uv[1] + b[1] * v2->uv[1] + b[2] * v3->uv[1];
+            t->nor = V4(b[0] * v1->nor.x + b[1] * v2->nor.x + b[2] * v3->nor.x,
+                        b[0] * v1->nor.y + b[1] * v2->nor.y + b[2] * v3->nor.y,
+                        b[0] * v1->nor.z + b[1] * v2->nor.z + b[2] * v3->nor.z, 0);
+            for (int q = 0; q < 4; q++)
+                t->col[q] = gf2u8(gclamp(
+                    b[0] * (float)v1->col[q] + b[1] * (float)v2->col[q] + b[2] * (float)v3->col[q], 0, 0xff));
+        }
+        nt++;
+    }
+    return nt;
+}
+
+/* render/cull.go:26-28 */
+static inline int cull_back_face(v4 v1, v4 v2, v4 v3) { return cross4(sub4(v2, v1), sub4(v3, v1)).z < 0; }
+
+static void load_vert(const prh_scene_desc* s, int vi, vert_t* v) {
+    v->pos = V4(s->vpos[4 * vi], s->vpos[4 * vi + 1], s->vpos[4 * vi + 2], s->vpos[4 * vi + 3]);
+    v->nor = V4(s->vnor[4 * vi], s->vnor[4 * vi + 1], s->vnor[4 * vi + 2], s->vnor[4 * vi + 3]);
+    memcpy(v->col, s->vcol + 4 * vi, 4);
+    v->uv[0] = s->vuv[2 * vi];
+    v->uv[1] = s->vuv[2 * vi + 1];
+}
+static m4 obj_model(const prh_scene_desc* s, int o) {
+    m4 m;
+    memcpy(m.a, s->obj_model + 16 * o, sizeof m.a);
+    return m;
+}
+
+/* render/raster.go:226-273 and draw, render/raster.go:380-444 */
+static void forward_pass(ctx_t* c) {
+    const prh_scene_desc* s = c->s;
+    m4 view = c->view, proj = c->proj, vp = c->vp;
+    m4 vinv = inv(&view), pinv = inv(&proj), vpinv = inv(&vp);
+    int64_t gtri = 0;
+    for (int o = 0; o < s->n_objects; o++) {
+        int mesh = s->obj_mesh[o];
+        m4 model = obj_model(s, o);
+        m4 mi = inv(&model);
+        m4 normal = transpose(&mi);
+        m4 pv = mulm(&proj, &view);
+        m4 trans = mulm(&pv, &model);
+        int v0 = s->mesh_vtx_off[mesh];
+        for (int t = s->mesh_tri_off[mesh]; t < s->mesh_tri_off[mesh + 1]; t++, gtri++) {
+            int64_t mat = s->tri_mat[t];
+            if (mat >= 0) mat += s->obj_mat_base[o];
+            vert_t in[3], tv[3];
+            for (int k = 0; k < 3; k++) load_vert(s, v0 + s->tri_v[3 * t + k], &in[k]);
+            if (!tri_is_valid(in[0].pos, in[1].pos, in[2].pos)) continue;
+            for (int k = 0; k < 3; k++) {
+                tv[k] = in[k];
+                tv[k].pos = mulv(&trans, in[k].pos);
+                tv[k].nor = apply4(in[k].nor, &normal);
+            }
+            float recipw[3] = {1, 1, 1};
+            if (c->perspect)
+                for (int k = 0; k < 3; k++) recipw[k] = -1 / tv[k].pos.w;
+            for (int k = 0; k < 3; k++) tv[k].pos = pos4(apply4(tv[k].pos, &vp));
+            if (cull_back_face(tv[0].pos, tv[1].pos, tv[2].pos)) continue;
+            v3 p[3];
+            for (int k = 0; k < 3; k++) p[k] = V3(tv[k].pos.x, tv[k].pos.y, tv[k].pos.z);
+            int msaa = c->o->msaa > 0 ? c->o->msaa : 1;
+            aabb_t vpbox = {V3(0, 0, -1), V3((float)(msaa * c->W), (float)(msaa * c->H), 1)};
+            aabb_t tbox = aabb_new(3, p);
+            if (!aabb_intersect(&vpbox, &tbox)) continue;
+            uint32_t key = (uint32_t)(gtri * 8);
+            if (aabb_contains(&vpbox, 3, p)) {
+                draw_clipped(c, &vpinv, &pinv, &vinv, &tv[0], &tv[1], &tv[2], recipw, mat, key + 7);
+                continue;
+            }
+            vert_t sub[8][3];
+            int ns = clip_triangle(&tv[0], &tv[1], &tv[2], (float)(msaa * c->W), (float)(msaa * c->H), sub);
+            for (int i = 0; i < ns; i++)
+                draw_clipped(c, &vpinv, &pinv, &vinv, &sub[i][0], &sub[i][1], &sub[i][2], recipw, mat, key + i);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ shadows */
+/* render/shadow.go:33-90.  A shadow-casting directional light gets the same
+ * orthographic light camera as a point light (the reference leaves it nil). */
+static void init_shadow_maps(ctx_t* c) {
+    const prh_scene_desc* s = c->s;
+    int L = s->n_lights;
+    c->lview = calloc((size_t)L + 1, sizeof(m4));
+    c->lproj = calloc((size_t)L + 1, sizeof(m4));
+    c->lactive = calloc((size_t)L + 1, sizeof(int));
+    c->smaps = calloc((size_t)L * c->SW * c->SH + 1, sizeof(float));
+    v3 up = V3(0, 1, 0);
+    v3 center = V3a(s->center);
+    m4 cview = view_matrix(V3a(c->cam->position), V3a(c->cam->target), V3a(c->cam->up));
+    m4 cproj = cam_proj(c->cam);
+    m4 cvinv = inv(&cview), cpinv = inv(&cproj);
+    for (int i = 0; i < L; i++) {
+        if (!s->lights[i].cast_shadow) continue;
+        v3 lp = V3a(s->lights[i].position);
+        m4 lv = view_matrix(lp, center, up);
+        m4 t0 = mulm(&lv, &cvinv);
+        m4 tm = mulm(&t0, &cpinv);
+        static const float corners[8][3] = {{1, 1, 1},  {1, 1, -1},  {1, -1, 1},  {-1, 1, 1},
+                                            {-1, -1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}};
+        v3 cv[8];
+        for (int k = 0; k < 8; k++) {
+            v4 q = pos4(apply4(V4(corners[k][0], corners[k][1], corners[k][2], 1), &tm));
+            cv[k] = V3(q.x, q.y, q.z);
+        }
+        aabb_t bb = aabb_new(8, cv);
+        float le = bb.mn.x, ri = bb.mx.x, bo = bb.mn.y, to = bb.mx.y, ne = bb.mx.z, fa = bb.mn.z - 2;
+        c->lview[i] = view_matrix(lp, center, up);
+        c->lproj[i] = ortho_proj(le, ri, bo, to, ne, fa);
+        c->lactive[i] = 1;
+    }
+}
+
+/* render/shadow.go:152-228 */
+static void draw_depth(ctx_t* c, int li, const m4* trans, const m4* vp, const vert_t* in) {
+    v4 p[3];
+    for (int k = 0; k < 3; k++) p[k] = pos4(apply4(mulv(trans, in[k].pos), vp));
+    if (cull_back_face(p[0], p[1], p[2])) return;
+    /* cullViewFrustum, render/cull.go:15-23 */
+    int msaa = c->o->msaa > 0 ? c->o->msaa : 1;
+    v3 vpp[3] = {V3((float)(c->SW * msaa), (float)(c->SH * msaa), 1), V3(0, 0, 0), V3(0, 0, -1)};
+    aabb_t vpbox = aabb_new(3, vpp);
+    v3 q[3];
+    for (int k = 0; k < 3; k++) q[k] = V3(p[k].x, p[k].y, p[k].z);
+    aabb_t tb = aabb_new(3, q);
+    if (!aabb_intersect(&vpbox, &tb)) return;
+    int64_t xmin = gf2i(ground(tb.mn.x) - 1), xmax = gf2i(ground(tb.mx.x) + 1);
+    int64_t ymin = gf2i(ground(tb.mn.y) - 1), ymax = gf2i(ground(tb.mx.y) + 1);
+    if (xmin < 0) xmin = 0;
+    if (ymin < 0) ymin = 0;
+    if (xmax > c->SW - 1) xmax = c->SW - 1;
+    if (ymax > c->SH - 1) ymax = c->SH - 1;
+    float* d = c->smaps + (size_t)li * c->SW * c->SH;
+    for (int64_t x = xmin; x <= xmax; x++)
+        for (int64_t y = ymin; y <= ymax; y++) {
+            float bc[3];
+            barycoord((float)x + 0.5f, (float)y + 0.5f, p[0].x, p[0].y, p[1].x, p[1].y, p[2].x, p[2].y, bc);
+            if (bc[0] < -1e-7f || bc[1] < -1e-7f || bc[2] < -1e-7f) continue;
+            float z = bc[0] * p[0].z + bc[1] * p[1].z + bc[2] * p[2].z;
+            size_t idx = (size_t)(x + y * c->SW);
+            if (!(z <= d[idx])) d[idx] = z;
+        }
+}
+/* render/shadow.go:92-150 */
+static void pass_shadows(ctx_t* c, int li) {
+    const prh_scene_desc* s = c->s;
+    if (!s->lights[li].cast_shadow) return;
+    m4 vp = viewport_matrix((float)c->SW, (float)c->SH);
+    m4 pv = mulm(&c->lproj[li], &c->lview[li]);
+    for (int o = 0; o < s->n_objects; o++) {
+        int mesh = s->obj_mesh[o];
+        m4 model = obj_model(s, o);
+        m4 trans = mulm(&pv, &model);
+        int v0 = s->mesh_vtx_off[mesh];
+        for (int t = s->mesh_tri_off[mesh]; t < s->mesh_tri_off[mesh + 1]; t++) {
+            vert_t in[3];
+            for (int k = 0; k < 3; k++) load_vert(s, v0 + s->tri_v[3 * t + k], &in[k]);
+            if (!tri_is_valid(in[0].pos, in[1].pos, in[2].pos)) continue;
+            draw_depth(c, li, &trans, &vp, in);
+        }
+    }
+}
+/* render/shadow.go:230-283: true when the fragment is occluded (the name is
+ * inverted in the reference; non-casting lights report true). */
+static int shading_visibility(ctx_t* c, int li, const frag_t* info) {
+    if (!c->s->lights[li].cast_shadow) return 1;
+    m4 vps = viewport_matrix((float)c->SW, (float)c->SH);
+    v4 sc = V4((float)info->x, (float)info->y, info->depth, 1);
+    sc = apply4(sc, &c->s2w);
+    sc = apply4(sc, &c->lview[li]);
+    sc = apply4(sc, &c->lproj[li]);
+    sc = pos4(apply4(sc, &vps));
+    int64_t lx = gf2i(sc.x), ly = gf2i(sc.y);
+    int64_t idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)c->SW);
+    int64_t len = (int64_t)c->SW * c->SH;
+    int shadow = 0;
+    if (idx > 0 && idx < len) {
+        float sz = c->smaps[(size_t)li * c->SW * c->SH + (size_t)idx];
+        if (sc.z < sz - 0.03f) shadow++;
+    }
+    return shadow > 0;
+}
+
+/* -------------------------------------------------------------- shading */
+static const prh_material* mat_at(const prh_scene_desc* s, int64_t id) { /* render/material_table.go */
+    if (id < 0 || id >= s->n_materials) return NULL;
+    return &s->materials[id];
+}
+
+/* shader/blinn_cpu.go:24-106; lin receives the pre-Round sums.  Returns 0 on
+ * the no-lights early return (texture colour). */
+static int fragment_shader(const prh_material* m, v4 nor, v4 facenor, v4 x, v3 cam, int nl, const prh_light* ls,
+                           int na, const float* amb, uint8_t out[4], float lin[3], double lin64[3]) {
+    const uint8_t* col = m->texture;
+    if (nl == 0) {
+        memcpy(out, col, 4);
+        return 0;
+    }
+    float LaR = 0, LaG = 0, LaB = 0;
+    for (int e = 0; e < na; e++) {
+        LaR += amb[e] * (float)col[0];
+        LaG += amb[e] * (float)col[1];
+        LaB += amb[e] * (float)col[2];
+    }
+    float LdR = 0, LdG = 0, LdB = 0, LsR = 0, LsG = 0, LsB = 0;
+    v4 n = m->flat_shading ? facenor : nor;
+    double dLa[3] = {0, 0, 0}, dLd[3] = {0, 0, 0}, dLs[3] = {0, 0, 0};
+    for (int e = 0; e < na && lin64; e++)
+        for (int k = 0; k < 3; k++) dLa[k] += (double)amb[e] * col[k];
+    for (int li = 0; li < nl; li++) {
+        const prh_light* l = &ls[li];
+        v4 L;
+        float I;
+        if (l->kind == PRH_LIGHT_POINT) {
+            v4 ldir = sub4(V4(l->position[0], l->position[1], l->position[2], 1), x);
+            L = unit4(ldir);
+            I = l->intensity / len4(ldir);
+        } else {
+            L = scale4(V4(l->direction[0], l->direction[1], l->direction[2], 0), -1, -1, -1, 1);
+            I = l->intensity;
+        }
+        v4 V = unit4(sub4(V4(cam.x, cam.y, cam.z, 1), x));
+        v4 H = unit4(add4(L, V));
+        float Ld = gclamp(dot4(n, L), 0, 1);
+        float Ls = gpow(gclamp(dot4(n, H), 0, 1), m->shininess);
+        LdR += Ld * (float)col[0] * I;
+        LdG += Ld * (float)col[1] * I;
+        LdB += Ld * (float)col[2] * I;
+        LsR += Ls * (float)l->color[0] * I;
+        LsG += Ls * (float)l->color[1] * I;
+        LsB += Ls * (float)l->color[2] * I;
+        if (lin64) { /* the same shading, every operation in double */
+            double nx = n.x, ny = n.y, nz = n.z, Lx, Ly, Lz, dI;
+            if (l->kind == PRH_LIGHT_POINT) {
+                double dx = (double)l->position[0] - x.x, dy = (double)l->position[1] - x.y,
+                       dz = (double)l->position[2] - x.z;
+                double dl = sqrt(dx * dx + dy * dy + dz * dz);
+                Lx = dx / dl, Ly = dy / dl, Lz = dz / dl;
+                dI = l->intensity / dl;
+            } else {
+                Lx = -(double)l->direction[0], Ly = -(double)l->direction[1], Lz = -(double)l->direction[2];
+                dI = l->intensity;
+            }
+            double vx = (double)cam.x - x.x, vy = (double)cam.y - x.y, vz = (double)cam.z - x.z;
+            double vl = sqrt(vx * vx + vy * vy + vz * vz);
+            vx /= vl, vy /= vl, vz /= vl;
+            double hx = Lx + vx, hy = Ly + vy, hz = Lz + vz, hl = sqrt(hx * hx + hy * hy + hz * hz);
+            hx /= hl, hy /= hl, hz /= hl;
+            double dld = nx * Lx + ny * Ly + nz * Lz, dls = nx * hx + ny * hy + nz * hz;
+            dld = dld < 0 ? 0 : (dld > 1 ? 1 : dld);
+            dls = dls < 0 ? 0 : (dls > 1 ? 1 : dls);
+            dls = pow(dls, (double)m->shininess);
+            for (int k = 0; k < 3; k++) {
+                dLd[k] += dld * col[k] * dI;
+                dLs[k] += dls * l->color[k] * dI;
+            }
+        }
+    }
+    float r = LaR + ((float)m->diffuse[0] * LdR / 255.0f) + ((float)m->specular[0] * LsR / 255.0f);
+    float g = LaG + ((float)m->diffuse[1] * LdG / 255.0f) + ((float)m->specular[1] * LsG / 255.0f);
+    float b = LaB + ((float)m->diffuse[2] * LdB / 255.0f) + ((float)m->specular[2] * LsB / 255.0f);
+    if (lin) lin[0] = r, lin[1] = g, lin[2] = b;
+    if (lin64)
+        for (int k = 0; k < 3; k++)
+            lin64[k] = dLa[k] + (double)m->diffuse[k] * dLd[k] / 255.0 + (double)m->specular[k] * dLs[k] / 255.0;
+    out[0] = gf2u8(gclamp(ground(r), 0, 0xff));
+    out[1] = gf2u8(gclamp(ground(g), 0, 0xff));
+    out[2] = gf2u8(gclamp(ground(b), 0, 0xff));
+    out[3] = gf2u8(gclamp((float)col[3], 0, 0xff));
+    return 1;
+}
+
+/* material/ao.go:26-28: the eight march directions. */
+static void ao_dirs(float out[16]) {
+    int k = 0;
+    for (float a = 0.0f; a < (float)(2 * M_PI - 1e-4); a += (float)(M_PI / 4)) {
+        out[2 * k] = (float)cos((double)a);
+        out[2 * k + 1] = (float)sin((double)a);
+        k++;
+    }
+}
+/* material/ao.go:40-73 */
+static float max_elevation_angle(ctx_t* c, int px, int py, float dx, float dy) {
+    v4 p = V4((float)px, (float)py, 0, 1);
+    v4 dir = V4(dx, dy, 0, 0);
+    float maxangle = 0.0f;
+    for (float t = 0.0f; t < 100; t += 1) {
+        v4 cur = add4(p, scale4(dir, t, t, 1, 1));
+        int64_t cx = gf2i(cur.x), cy = gf2i(cur.y);
+        if (!in_buf(c, cx, cy)) return maxangle;
+        float distance = len4(sub4(p, cur));
+        if (distance < 1) continue;
+        const frag_t* sh = fat(c, (int)cx, (int)cy);
+        const frag_t* tr = fat(c, (int)gf2i(p.x), (int)gf2i(p.y));
+        float sd = sh->ok ? sh->depth : -1, td = tr->ok ? tr->depth : -1;
+        float elevation = sd - td;
+        maxangle = gmax2(maxangle, (float)atan((double)(elevation / distance)));
+    }
+    return maxangle;
+}
+/* material/ao.go:20-38 */
+static void ao_shade(ctx_t* c, int fx, int fy, const uint8_t col[4], const prh_material* mat, uint8_t out[4]) {
+    if (!mat || !mat->ambient_occlusion) {
+        memcpy(out, col, 4);
+        return;
+    }
+    float total = 0.0f;
+    for (int k = 0; k < 8; k++)
+        total += (float)(M_PI / 2) - max_elevation_angle(c, fx, fy, c->ao_dir[2 * k], c->ao_dir[2 * k + 1]);
+    total /= (float)((M_PI / 2) * 8);
+    total = gpow(total, 10000);
+    out[0] = gf2u8(total * (float)col[0]);
+    out[1] = gf2u8(total * (float)col[1]);
+    out[2] = gf2u8(total * (float)col[2]);
+    out[3] = col[3];
+}
+
+/* render/raster.go:324-359, called from DrawFragment (render/raster_screen.go:75-87)
+ * with frag = the pixel's stored fragment. */
+static void shade(ctx_t* c, int x, int y) {
+    frag_t* fr = fat(c, x, y); /* DrawFragment's info (copy written back below) */
+    frag_t frag = *fr;
+    const frag_t* info = fat(c, frag.x, frag.y);
+    uint8_t col[4];
+    size_t lid = (size_t)y * c->W + x;
+    if (!info->ok) {
+        memcpy(col, c->o->background, 4);
+    } else {
+        memcpy(col, info->col, 4);
+        const prh_material* mat = mat_at(c->s, frag.matid);
+        if (mat) {
+            float lin[3];
+            double lin64[3];
+            int lit = fragment_shader(mat, info->nor, info->facenor, info->wpos, V3a(c->cam->position),
+                                      c->s->n_lights, c->s->lights, c->s->n_ambient, c->s->ambient_intensity, col,
+                                      lin, lin64);
+            if (lit && c->linear) {
+                c->linear[3 * lid] = lin[0];
+                c->linear[3 * lid + 1] = lin[1];
+                c->linear[3 * lid + 2] = lin[2];
+                for (int k = 0; k < 3; k++) {
+                    double e = fabs((double)lin[k] - lin64[k]);
+                    if (e > c->shade_err64) c->shade_err64 = e;
+                }
+            }
+            if (c->o->shadow_map && mat->receive_shadow) {
+                float vis = 0.0f;
+                for (int i = 0; i < c->s->n_lights; i++)
+                    if (shading_visibility(c, i, info)) vis++;
+                float w = gpow(0.5f, vis);
+                for (int k = 0; k < 3; k++) col[k] = gf2u8((float)col[k] * w);
+            }
+        }
+        uint8_t outc[4];
+        ao_shade(c, frag.x, frag.y, col, mat_at(c->s, frag.matid), outc);
+        memcpy(col, outc, 4);
+    }
+    memcpy(fr->col, col, 4);
+}
+
+/* ----------------------------------------------------------------- render */
+int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_options* o, oracle_out* out) {
+    if (!s || !cam || !o || !out) return PRH_E_INVALID;
+    int msaa = o->msaa > 0 ? o->msaa : 1;
+    if (msaa != 1) return PRH_E_UNSUPPORTED;
+    ctx_t c;
+    memset(&c, 0, sizeof c);
+    c.s = s;
+    c.cam = cam;
+    c.o = o;
+    c.W = o->width * msaa;
+    c.H = o->height * msaa;
+    c.perspect = cam->kind == PRH_CAMERA_PERSPECTIVE;
+    c.SW = o->shadow_size > 0 ? o->shadow_size : c.W;
+    c.SH = o->shadow_size > 0 ? o->shadow_size : c.H;
+    size_t np = (size_t)c.W * c.H;
+    c.f = calloc(np, sizeof(frag_t));
+    if (!c.f) return PRH_E_NOMEM;
+    c.linear = out->linear;
+    if (c.linear)
+        for (size_t i = 0; i < 3 * np; i++) c.linear[i] = NAN;
+    ao_dirs(c.ao_dir);
+    c.view = view_matrix(V3a(cam->position), V3a(cam->target), V3a(cam->up));
+    c.proj = cam_proj(cam);
+    c.vp = viewport_matrix((float)c.W, (float)c.H);
+    c.viewinv = inv(&c.view);
+    c.projinv = inv(&c.proj);
+    c.vpinv = inv(&c.vp);
+    m4 t0 = mulm(&c.viewinv, &c.projinv);
+    c.s2w = mulm(&t0, &c.vpinv);
+
+    if (o->shadow_map) {
+        init_shadow_maps(&c);
+        for (int i = 0; i < s->n_lights; i++) pass_shadows(&c, i);
+    }
+    forward_pass(&c);
+    for (int y = 0; y < c.H; y++)
+        for (int x = 0; x < c.W; x++) {
+            const frag_t* f = fat(&c, x, y);
+            size_t i = (size_t)y * c.W + x;
+            if (out->vis_order) out->vis_order[i] = f->ok ? f->order : 0xFFFFFFFFu;
+            if (out->depth) out->depth[i] = f->ok ? f->depth : -1.0f;
+        }
+    /* deferred: DrawFragments order j = 0..n-1, x = j%w, y = j/w; pixel (0,0)
+     * is first, so every later pixel may read its shaded colour. */
+    shade(&c, 0, 0);
+    const uint8_t* mask = out->shade_mask;
+#ifdef _OPENMP
+#pragma omp parallel
+    {
+        ctx_t cl = c;
+        cl.shade_err64 = 0;
+#pragma omp for schedule(dynamic, 256)
+        for (long j = 1; j < (long)np; j++) {
+            int x = (int)(j % c.W), y = (int)(j / c.W);
+            if (mask && !mask[j]) continue;
+            shade(&cl, x, y);
+        }
+#pragma omp critical
+        if (cl.shade_err64 > c.shade_err64) c.shade_err64 = cl.shade_err64;
+    }
+#else
+    for (size_t j = 1; j < np; j++) {
+        int x = (int)(j % c.W), y = (int)(j / c.W);
+        if (mask && !mask[j]) continue;
+        shade(&c, x, y);
+    }
+#endif
+    for (int y = 0; y < c.H; y++)
+        for (int x = 0; x < c.W; x++) {
+            size_t i = (size_t)y * c.W + x;
+            const frag_t* f = fat(&c, x, y);
+            uint8_t col[4];
+            memcpy(col, f->col, 4);
+            if (mask && !mask[i] && i != 0) memset(col, 0, 4);
+            if (out->pregamma) memcpy(out->pregamma + 4 * i, col, 4);
+            if (o->gamma_correct) /* shader/gamma.go:13-18 */
+                for (int k = 0; k < 3; k++) col[k] = gamma_u8(col[k]);
+            if (out->image) memcpy(out->image + 4 * ((size_t)(c.H - 1 - y) * c.W + x), col, 4);
+        }
+    if (out->shadow_maps && c.smaps)
+        memcpy(out->shadow_maps, c.smaps, sizeof(float) * (size_t)s->n_lights * c.SW * c.SH);
+    if (out->shade_err64) *out->shade_err64 = c.shade_err64;
+    free(c.f);
+    free(c.lview);
+    free(c.lproj);
+    free(c.lactive);
+    free(c.smaps);
+    return PRH_OK;
+}
+
+/* ----------------------------------------------------- known-answer hooks */
+void oracle_barycoord(const float p[2], const float a[2], const float b[2], const float c[2], float out[3]) {
+    barycoord(p[0], p[1], a[0], a[1], b[0], b[1], c[0], c[1], out);
+}
+void oracle_viewport_matrix(float w, float h, float out[16]) { memcpy(out, viewport_matrix(w, h).a, 64); }
+void oracle_view_matrix(const float pos[3], const float target[3], const float up[3], float out[16]) {
+    memcpy(out, view_matrix(V3a(pos), V3a(target), V3a(up)).a, 64);
+}
+void oracle_proj_matrix(const prh_camera* cam, float out[16]) { memcpy(out, cam_proj(cam).a, 64); }
+void oracle_mat4_inv(const float m[16], float out[16]) {
+    m4 a;
+    memcpy(a.a, m, 64);
+    memcpy(out, inv(&a).a, 64);
+}
+void oracle_mat4_mulm(const float a[16], const float b[16], float out[16]) {
+    m4 x, y;
+    memcpy(x.a, a, 64);
+    memcpy(y.a, b, 64);
+    memcpy(out, mulm(&x, &y).a, 64);
+}
+int oracle_aabb_intersect(const float min1[3], const float max1[3], const float min2[3], const float max2[3]) {
+    aabb_t a = {V3a(min1), V3a(max1)}, b = {V3a(min2), V3a(max2)};
+    return aabb_intersect(&a, &b);
+}
+int oracle_aabb_contains(const float mn[3], const float mx[3], int n, const float* pts) {
+    aabb_t a = {V3a(mn), V3a(mx)};
+    v3 vs[64];
+    if (n > 64) return -1;
+    for (int i = 0; i < n; i++) vs[i] = V3a(pts + 3 * i);
+    return aabb_contains(&a, n, vs);
+}
+void oracle_aabb_new(int n, const float* pts, float mn[3], float mx[3]) {
+    v3 vs[64];
+    if (n > 64) n = 64;
+    for (int i = 0; i < n; i++) vs[i] = V3a(pts + 3 * i);
+    aabb_t b = aabb_new(n, vs);
+    mn[0] = b.mn.x, mn[1] = b.mn.y, mn[2] = b.mn.z;
+    mx[0] = b.mx.x, mx[1] = b.mx.y, mx[2] = b.mx.z;
+}
+int oracle_tri_is_valid(const float p1[4], const float p2[4], const float p3[4]) {
+    return tri_is_valid(V4(p1[0], p1[1], p1[2], p1[3]), V4(p2[0], p2[1], p2[2], p2[3]),
+                        V4(p3[0], p3[1], p3[2], p3[3]));
+}
+void oracle_interp_worldpos(const float bc[3], const float m1[4], const float m2[4], const float m3[4],
+                            float out[4]) {
+    v4 r = interp_wpos(bc, V4(m1[0], m1[1], m1[2], m1[3]), V4(m2[0], m2[1], m2[2], m2[3]),
+                       V4(m3[0], m3[1], m3[2], m3[3]));
+    out[0] = r.x, out[1] = r.y, out[2] = r.z, out[3] = r.w;
+}
+int oracle_mat_at(int32_t n, int64_t id) { return id >= 0 && id < n ? (int)id : -1; }
+float oracle_srgb_from_linear(float v) { return from_linear_srgb(v); }
+float oracle_linear_from_srgb(float v) { return from_srgb_linear(v); }
+uint8_t oracle_gamma_u8(uint8_t c) { return gamma_u8(c); }
+uint8_t oracle_shadow_quant(uint8_t c, int occluded) { return gf2u8((float)c * gpow(0.5f, (float)occluded)); }
+void oracle_fragment_shader(const prh_material* m, const float nor[4], const float facenor[4], const float wpos[4],
+                            const float cam[3], int n_lights, const prh_light* lights, int n_amb, const float* amb,
+                            uint8_t out[4], float lin[3]) {
+    float l[3] = {NAN, NAN, NAN};
+    fragment_shader(m, V4(nor[0], nor[1], nor[2], nor[3]), V4(facenor[0], facenor[1], facenor[2], facenor[3]),
+                    V4(wpos[0], wpos[1], wpos[2], wpos[3]), V3a(cam), n_lights, lights, n_amb, amb, out, l, NULL);
+    if (lin) memcpy(lin, l, sizeof l);
+}
+void oracle_ao_dirs(float out[16]) { ao_dirs(out); }

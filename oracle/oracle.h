@@ -1,0 +1,91 @@
+/*
+ * oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of polyred's CPU renderer (the parity oracle for
+ * libprh).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker or the timed CPU baseline; the
+ * product path never calls it.
+ *
+ * It follows the reference in sequential Workers(1) order (one worker makes
+ * internal/sched a single FIFO channel, internal/sched/sched.go:40-78), which
+ * is the order the reference's own parity tests pin
+ * (render/gpudeferred_compare_test.go:13-22).
+ *
+ * Pinning (each check lives in tests/test_oracle_*.py):
+ *   - Barycoord, ViewportMatrix, ViewMatrix, Perspective/Orthographic
+ *     ProjMatrix, AABB Intersect/Contains/Add, Triangle.IsValid,
+ *     interpWorldPos, matAt, sRGB endpoints and LUT round trip, the shadow
+ *     quantisation: against the reference's known-answer tests
+ *     (math/interpolate_test.go:104-115, math/math_test.go:40-51,
+ *     camera/camera_test.go:106-196, geometry/primitive/box_test.go:30-110,
+ *     geometry/primitive/triangle_test.go:29-52, render/worldpos_test.go:17-31,
+ *     render/material_table_test.go:17-34, color/color_test.go:36-67,
+ *     gpu/shader/gpumath/kernels/shadow_test.go:12-31);
+ *   - FragmentShader: the six fragments of
+ *     render/shading_equiv_test.go:33-107 against an independent FP64
+ *     Blinn-Phong, and a brute-force FP64 re-evaluation on whole frames;
+ *   - the raster: brute force on tiny scenes (every pixel's coverage from an
+ *     exact rational edge test), ties and clipping fans by construction;
+ *   - AO: range and closed-form cases (flat depth gives a factor of exactly
+ *     one) only.  Go's math.Atan/Cos/Sin last bits are not reproducible
+ *     without Go: AO is "parity unpinned" beyond those checks.
+ */
+#ifndef POLYRED_ORACLE_H
+#define POLYRED_ORACLE_H
+
+#include <stdint.h>
+#include "../include/prh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_out {
+    uint32_t* vis_order;   /* [H][W] raster rows: triangle*8+sub, 0xFFFFFFFF if not Ok */
+    float* depth;          /* [H][W] fragment depth, -1 if not Ok                      */
+    float* linear;         /* [H][W][3] pre-Round shading sums, NaN if not shaded      */
+    uint8_t* pregamma;     /* [H][W][4] raster rows, colour after the deferred pass    */
+    uint8_t* image;        /* [H][W][4] final image, top row first                     */
+    float* shadow_maps;    /* [n_lights][S][S]                                          */
+    const uint8_t* shade_mask; /* optional [H][W] raster rows: deferred-shade only these */
+    double* shade_err64;   /* optional: max |f32 linear - f64 re-evaluation|            */
+} oracle_out;
+
+int oracle_render(const prh_scene_desc* scene, const prh_camera* cam,
+                  const prh_options* opts, oracle_out* out);
+
+/* Known-answer entry points (row-major Mat4 X00..X33). */
+void oracle_barycoord(const float p[2], const float a[2], const float b[2],
+                      const float c[2], float out[3]);
+void oracle_viewport_matrix(float w, float h, float out[16]);
+void oracle_view_matrix(const float pos[3], const float target[3],
+                        const float up[3], float out[16]);
+void oracle_proj_matrix(const prh_camera* cam, float out[16]);
+void oracle_mat4_inv(const float m[16], float out[16]);
+void oracle_mat4_mulm(const float a[16], const float b[16], float out[16]);
+int oracle_aabb_intersect(const float min1[3], const float max1[3],
+                          const float min2[3], const float max2[3]);
+int oracle_aabb_contains(const float mn[3], const float mx[3], int n,
+                         const float* pts);
+void oracle_aabb_new(int n, const float* pts, float mn[3], float mx[3]);
+int oracle_tri_is_valid(const float p1[4], const float p2[4], const float p3[4]);
+void oracle_interp_worldpos(const float bc[3], const float m1[4],
+                            const float m2[4], const float m3[4], float out[4]);
+int oracle_mat_at(int32_t n, int64_t id);
+float oracle_srgb_from_linear(float v);
+float oracle_linear_from_srgb(float v);
+uint8_t oracle_gamma_u8(uint8_t c);
+uint8_t oracle_shadow_quant(uint8_t c, int occluded);
+/* shader.FragmentShader on one fragment; lin may be NULL. */
+void oracle_fragment_shader(const prh_material* m, const float nor[4],
+                            const float facenor[4], const float wpos[4],
+                            const float cam[3], int n_lights,
+                            const prh_light* lights, int n_amb,
+                            const float* amb, uint8_t out[4], float lin[3]);
+/* AO direction table (8 x {cos, sin}) as material/ao.go:26 builds it. */
+void oracle_ao_dirs(float out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

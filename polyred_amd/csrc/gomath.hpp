@@ -1,0 +1,224 @@
+// gomath.hpp — polyred's float32 arithmetic, bit for bit, on host and device.
+//
+// The reference renderer is Go; its results depend on the exact sequence of
+// float32 roundings.  These helpers reproduce that sequence:
+//   * math.FMA[float32](x,y,z) = float32(math.FMA(float64 x,y,z))
+//     (math/math.go:265-267): a double FMA (v_fma_f64 on gfx950) then one
+//     rounding to float;
+//   * Sqrt/Pow/Atan go through float64 (math/math.go:91-233): float32 sqrt of
+//     a float32 is the correctly rounded f32 sqrt, which HIP emits by default;
+//   * plain float32 expressions round once per operation (hipcc is always
+//     invoked with -ffp-contract=off);
+//   * generic Min/Max fold through float64 math.Min/Max from +-MaxFloat64
+//     (math/math.go:245-261) with +Inf-before-NaN and +0-over--0 rules;
+//   * float->int is Go's amd64 conversion (NaN/out of range -> MinInt64),
+//     float->uint8 goes through int32 and keeps the low byte.
+// Vectors/matrices follow math/vec3.go, math/vec4.go and math/mat4.go.
+#pragma once
+
+#include <stdint.h>
+#include <float.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define GM_HD __host__ __device__ __forceinline__
+#else
+#define GM_HD inline
+#endif
+
+#ifndef __HIP_DEVICE_COMPILE__
+#include <cmath>
+#endif
+
+namespace gm {
+
+GM_HD float fma64(float a, float b, float c) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return (float)__builtin_fma((double)a, (double)b, (double)c);
+#else
+    return (float)std::fma((double)a, (double)b, (double)c);
+#endif
+}
+GM_HD float sqrt32(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_sqrtf(x);
+#else
+    return (float)std::sqrt((double)x);
+#endif
+}
+GM_HD float round32(float x) {  // math.Round: half away from zero
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_roundf(x);
+#else
+    return (float)std::round((double)x);
+#endif
+}
+GM_HD double pow64(double x, double y) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return pow(x, y);
+#else
+    return std::pow(x, y);
+#endif
+}
+GM_HD double atan64(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return atan(x);
+#else
+    return std::atan(x);
+#endif
+}
+GM_HD bool isnan32(float x) { return x != x; }
+GM_HD bool isinf64(double x) { return x - x != 0 && x == x; }
+GM_HD bool signbit64(double x) {
+    union { double d; uint64_t u; } c;
+    c.d = x;
+    return (c.u >> 63) != 0;
+}
+GM_HD double go_max64(double x, double y) {
+    if ((isinf64(x) && x > 0) || (isinf64(y) && y > 0)) return __builtin_inf();
+    if (x != x || y != y) return __builtin_nan("");
+    if (x == 0 && x == y) return signbit64(x) ? y : x;
+    return x > y ? x : y;
+}
+GM_HD double go_min64(double x, double y) {
+    if ((isinf64(x) && x < 0) || (isinf64(y) && y < 0)) return -__builtin_inf();
+    if (x != x || y != y) return __builtin_nan("");
+    if (x == 0 && x == y) return signbit64(x) ? x : y;
+    return x < y ? x : y;
+}
+// generic Max[float32](a,b) / Min[float32](a,b)
+GM_HD float max2(float a, float b) { return (float)go_max64(go_max64(-DBL_MAX, a), b); }
+GM_HD float min2(float a, float b) { return (float)go_min64(go_min64(DBL_MAX, a), b); }
+
+GM_HD int64_t f2i(float x) {
+    if (x >= -9223372036854775808.0f && x < 9223372036854775808.0f) return (int64_t)x;
+    return INT64_MIN;
+}
+GM_HD uint8_t f2u8(float x) {
+    if (x >= -2147483648.0f && x < 2147483648.0f) return (uint8_t)(uint32_t)(int32_t)x;
+    return 0;
+}
+GM_HD float clamp(float n, float lo, float hi) {  // math/clamp.go:8-16
+    if (n < lo) return lo;
+    if (n > hi) return hi;
+    return n;
+}
+GM_HD float abs32(float x) { return x < 0 ? -x : (x == 0 ? 0.0f : x); }
+GM_HD bool approx_eq(float a, float b, float eps) { return abs32(a - b) <= eps; }
+
+struct V4 {
+    float x, y, z, w;
+};
+struct V3 {
+    float x, y, z;
+};
+GM_HD V4 v4(float x, float y, float z, float w) { return V4{x, y, z, w}; }
+GM_HD V4 sub(V4 a, V4 b) { return v4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+GM_HD V4 add(V4 a, V4 b) { return v4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+GM_HD V4 scale(V4 a, float x, float y, float z, float w) { return v4(a.x * x, a.y * y, a.z * z, a.w * w); }
+GM_HD float dot(V4 v, V4 u) { return fma64(v.x, u.x, fma64(v.y, u.y, fma64(v.z, u.z, v.w * u.w))); }
+GM_HD float len(V4 v) { return sqrt32(dot(v, v)); }
+GM_HD V4 unit(V4 v) {
+    float n = 1.0f / len(v);
+    return v4(v.x * n, v.y * n, v.z * n, v.w * n);
+}
+GM_HD V4 cross(V4 v, V4 u) {
+    return v4(fma64(v.y, u.z, -v.z * u.y), fma64(v.z, u.x, -v.x * u.z), fma64(v.x, u.y, -v.y * u.x), 0);
+}
+GM_HD V4 pos(V4 v) {
+    if (v.w == 1 || v.w == 0) return v4(v.x, v.y, v.z, 1);
+    float inv = 1.0f / v.w;
+    return v4(v.x * inv, v.y * inv, v.z * inv, 1);
+}
+GM_HD V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+GM_HD V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+GM_HD float dot(V3 v, V3 u) { return fma64(v.x, u.x, fma64(v.y, u.y, v.z * u.z)); }
+GM_HD V3 unit(V3 v) {
+    float n = 1.0f / sqrt32(dot(v, v));
+    return v3(v.x * n, v.y * n, v.z * n);
+}
+GM_HD V3 cross(V3 v, V3 u) {
+    return v3(fma64(v.y, u.z, -v.z * u.y), fma64(v.z, u.x, -v.x * u.z), fma64(v.x, u.y, -v.y * u.x));
+}
+
+// Row-major 4x4 (X00..X33).
+struct M4 {
+    float a[16];
+};
+GM_HD V4 apply(V4 v, const M4& m) {  // math/vec4.go:107-116
+    const float* a = m.a;
+    return v4(fma64(a[0], v.x, fma64(a[1], v.y, fma64(a[2], v.z, a[3] * v.w))),
+              fma64(a[4], v.x, fma64(a[5], v.y, fma64(a[6], v.z, a[7] * v.w))),
+              fma64(a[8], v.x, fma64(a[9], v.y, fma64(a[10], v.z, a[11] * v.w))),
+              fma64(a[12], v.x, fma64(a[13], v.y, fma64(a[14], v.z, a[15] * v.w))));
+}
+GM_HD V4 mulv(const M4& m, V4 v) {  // math/mat4.go:224-231
+    const float* a = m.a;
+    return v4(a[0] * v.x + a[1] * v.y + a[2] * v.z + a[3] * v.w, a[4] * v.x + a[5] * v.y + a[6] * v.z + a[7] * v.w,
+              a[8] * v.x + a[9] * v.y + a[10] * v.z + a[11] * v.w,
+              a[12] * v.x + a[13] * v.y + a[14] * v.z + a[15] * v.w);
+}
+
+// Screen-space barycentric coordinates, math/interpolate.go:66-78.  The
+// per-triangle part (edges ab, ac, bc and the doubled area Sabc) is split
+// out so a rasteriser can hoist it; the result is bit-identical.
+struct TriSetup {
+    float ax, ay, bx, by;
+    float abx, aby, acx, acy, bcx, bcy, sabc;
+};
+GM_HD TriSetup tri_setup(float ax, float ay, float bx, float by, float cx, float cy) {
+    TriSetup t;
+    t.ax = ax, t.ay = ay, t.bx = bx, t.by = by;
+    t.abx = bx - ax, t.aby = by - ay;
+    t.acx = cx - ax, t.acy = cy - ay;
+    t.bcx = cx - bx, t.bcy = cy - by;
+    t.sabc = fma64(t.abx, t.acy, -t.aby * t.acx);
+    return t;
+}
+// The three edge areas (Sbcp, Sapc, Sabp) at p.
+GM_HD void tri_areas(const TriSetup& t, float px, float py, float s[3]) {
+    float apx = px - t.ax, apy = py - t.ay;
+    float bpx = px - t.bx, bpy = py - t.by;
+    s[2] = fma64(t.abx, apy, -t.aby * apx);  // Sabp
+    s[1] = fma64(apx, t.acy, -apy * t.acx);  // Sapc
+    s[0] = fma64(t.bcx, bpy, -t.bcy * bpx);  // Sbcp
+}
+GM_HD void barycoord(const TriSetup& t, float px, float py, float bc[3]) {
+    float s[3];
+    tri_areas(t, px, py, s);
+    bc[0] = s[0] / t.sabc;
+    bc[1] = s[1] / t.sabc;
+    bc[2] = s[2] / t.sabc;
+}
+GM_HD bool inside(const float bc[3]) {  // render/raster.go:491
+    return !(bc[0] < -1e-7f || bc[1] < -1e-7f || bc[2] < -1e-7f);
+}
+
+// Depth ordering for a 64-bit atomic-min visibility key: larger depth gives a
+// smaller key; -0 and +0 tie (they compare equal under '>').  NaN is handled
+// out of band by the caller.
+GM_HD uint32_t depth_key_hi(float z) {
+    union { float f; uint32_t u; } c;
+    c.f = (z == 0.0f) ? 0.0f : z;
+    uint32_t ord = (c.u & 0x80000000u) ? ~c.u : (c.u | 0x80000000u);
+    return ~ord;
+}
+GM_HD float depth_from_key_hi(uint32_t hi) {
+    uint32_t ord = ~hi;
+    union { float f; uint32_t u; } c;
+    c.u = (ord & 0x80000000u) ? (ord & 0x7fffffffu) : ~ord;
+    return c.f;
+}
+// Ordered bits for atomic max of a shadow-map depth (value only).
+GM_HD uint32_t depth_ord(float z) {
+    union { float f; uint32_t u; } c;
+    c.f = (z == 0.0f) ? 0.0f : z;
+    return (c.u & 0x80000000u) ? ~c.u : (c.u | 0x80000000u);
+}
+GM_HD float depth_from_ord(uint32_t ord) {
+    union { float f; uint32_t u; } c;
+    c.u = (ord & 0x80000000u) ? (ord & 0x7fffffffu) : ~ord;
+    return c.f;
+}
+
+}  // namespace gm

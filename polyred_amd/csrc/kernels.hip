@@ -1,0 +1,891 @@
+// kernels.hip — libprh's gfx950 kernels: vertex transform, triangle setup /
+// clip / bin, LDS-binned visibility and shadow rasters, deferred shading with
+// shadow maps, SSAO and sRGB fused.  All arithmetic that decides coverage,
+// depth, primitive order or colour follows polyred's CPU renderer bit for bit
+// (render/raster.go, render/clipping.go, render/shadow.go,
+// shader/blinn_cpu.go, material/ao.go, shader/gamma.go) through gomath.hpp.
+#include "kernels.hpp"
+
+using namespace gm;
+
+namespace prh {
+
+namespace {
+
+__device__ __forceinline__ V4 ld(float4 a) { return v4(a.x, a.y, a.z, a.w); }
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+
+// One atomic per wave for an append: returns this lane's slot (valid when pred).
+__device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Pixel box of a triangle, render/raster.go:473-477 (and shadow.go:187-191):
+// int(Round(min)-1) .. int(Round(max)+1) from the AABB of the three vertices,
+// then restricted to the buffer (FragmentBuffer.In).  Returns false if empty.
+__device__ bool pixel_box(float ax, float ay, float az, float bx, float by, float bz, float cx, float cy, float cz,
+                          int W, int H, int& x0, int& y0, int& x1, int& y1) {
+    float mnx = min2(min2(min2(FLT_MAX, ax), bx), cx);
+    float mny = min2(min2(min2(FLT_MAX, ay), by), cy);
+    float mxx = max2(max2(max2(-FLT_MAX, ax), bx), cx);
+    float mxy = max2(max2(max2(-FLT_MAX, ay), by), cy);
+    (void)az, (void)bz, (void)cz;
+    int64_t xmin = f2i(round32(mnx) - 1), xmax = f2i(round32(mxx) + 1);
+    int64_t ymin = f2i(round32(mny) - 1), ymax = f2i(round32(mxy) + 1);
+    if (xmin < 0) xmin = 0;
+    if (ymin < 0) ymin = 0;
+    if (xmax > W - 1) xmax = W - 1;
+    if (ymax > H - 1) ymax = H - 1;
+    if (xmin > xmax || ymin > ymax) return false;
+    x0 = (int)xmin, y0 = (int)ymin, x1 = (int)xmax, y1 = (int)ymax;
+    return true;
+}
+
+struct Box3 {
+    float mnx, mny, mnz, mxx, mxy, mxz;
+};
+__device__ Box3 box_of(V4 a, V4 b, V4 c) {  // primitive.NewAABB, box.go:15-25
+    Box3 r;
+    r.mnx = min2(min2(min2(FLT_MAX, a.x), b.x), c.x);
+    r.mny = min2(min2(min2(FLT_MAX, a.y), b.y), c.y);
+    r.mnz = min2(min2(min2(FLT_MAX, a.z), b.z), c.z);
+    r.mxx = max2(max2(max2(-FLT_MAX, a.x), b.x), c.x);
+    r.mxy = max2(max2(max2(-FLT_MAX, a.y), b.y), c.y);
+    r.mxz = max2(max2(max2(-FLT_MAX, a.z), b.z), c.z);
+    return r;
+}
+// AABB.Intersect with the viewport box Min (0,0,-1), Max (w,h,1); the z test
+// compares against the viewport's Max.Y (geometry/primitive/box.go:38).
+__device__ bool vp_intersect(const Box3& t, float w, float h) {
+    float minX = max2(0.0f, t.mnx), minY = max2(0.0f, t.mny), minZ = max2(-1.0f, t.mnz);
+    float maxX = min2(w, t.mxx), maxY = min2(h, t.mxy), maxZ = min2(h, t.mxz);
+    return minX <= maxX && minY <= maxY && minZ <= maxZ;
+}
+__device__ __forceinline__ bool less_eq(float v1, float v2) {  // box.go:62-64
+    const float eps = 1e-7f;
+    return approx_eq(v1, v2, eps) || (v1 < v2 && abs32(v1 - v2) > eps);
+}
+__device__ bool vp_contains(V4 p, float w, float h) {  // box.go:61-75 for one vertex
+    return less_eq(0.0f, p.x) && less_eq(0.0f, p.y) && less_eq(-1.0f, p.z) && less_eq(p.x, w) && less_eq(p.y, h) &&
+           less_eq(p.z, 1.0f);
+}
+
+__device__ void count_tiles(int32_t* tile_count, int base, int tiles_x, int x0, int y0, int x1, int y1) {
+    for (int ty = y0 / TILE; ty <= y1 / TILE; ty++)
+        for (int tx = x0 / TILE; tx <= x1 / TILE; tx++) atomicAdd(&tile_count[base + ty * tiles_x + tx], 1);
+}
+
+__device__ __forceinline__ void store_prim(Prim* prims, int slot, V4 a, V4 b, V4 c, uint32_t key, int x0, int y0,
+                                           int x1, int y1) {
+    Prim p;
+    p.p0 = make_float4(a.x, a.y, a.z, b.x);
+    p.p1 = make_float4(b.y, b.z, c.x, c.y);
+    p.p2 = make_float4(c.z, bitsf(key), bitsf(pack_bbox(x0, y0)), bitsf(pack_bbox(x1, y1)));
+    prims[slot] = p;
+}
+
+// Sutherland-Hodgman against the six screen-space planes, clipping.go:31-65.
+struct Plane {
+    V4 pos, nor;
+};
+__device__ __forceinline__ bool in_front(const Plane& p, V4 v) { return dot(sub(v, p.pos), p.nor) > 0; }
+__device__ __forceinline__ V4 isect(const Plane& p, V4 v0, V4 v1) {
+    V4 u = sub(v1, v0), w = sub(v0, p.pos);
+    float d = dot(p.nor, u);
+    float n = -dot(p.nor, w);
+    float s = n / d;
+    return add(v0, scale(u, s, s, s, s));
+}
+__device__ int sutherland_hodgman(V4* a, float w, float h) {
+    Plane planes[6] = {{v4(w, 0, 0, 1), v4(-1, 0, 0, 1)}, {v4(0, 0, 0, 1), v4(1, 0, 0, 1)},
+                       {v4(0, h, 0, 1), v4(0, -1, 0, 1)}, {v4(0, 0, 0, 1), v4(0, 1, 0, 1)},
+                       {v4(0, 0, 1, 1), v4(0, 0, -1, 1)}, {v4(0, 0, -1, 1), v4(0, 0, 1, 1)}};
+    V4 b[12];
+    int na = 3;
+    for (int pi = 0; pi < 6; pi++) {
+        if (na == 0) return 0;
+        int nb = 0;
+        V4 s = a[na - 1];
+        for (int i = 0; i < na; i++) {
+            V4 e = a[i];
+            if (in_front(planes[pi], e)) {
+                if (!in_front(planes[pi], s)) b[nb++] = isect(planes[pi], s, e);
+                b[nb++] = e;
+            } else if (in_front(planes[pi], s)) {
+                b[nb++] = isect(planes[pi], s, e);
+            }
+            s = e;
+        }
+        for (int i = 0; i < nb; i++) a[i] = b[i];
+        na = nb;
+    }
+    return na;
+}
+
+__device__ __forceinline__ uint32_t u8x4(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
+    return (uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | ((uint32_t)a << 24);
+}
+__device__ __forceinline__ float chan(uint32_t c, int k) { return (float)((c >> (8 * k)) & 0xff); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- fills
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------- vertex transform
+// render/raster.go:381-410 per vertex: Pos = trans.MulV(p) with
+// trans = Proj.View.Model, recipw = -1/W (perspective), Pos.Apply(Viewport).Pos();
+// Nor = n.Apply(Normal); and drawClipped's back transform
+// m = Pos.Apply(ViewportInv).Apply(ProjInv).Apply(ViewInv) (raster.go:467-469).
+__global__ __launch_bounds__(256) void k_vertex_main(DevScene s, const M4* __restrict__ trans,
+                                                     const M4* __restrict__ normal, ViewConsts vc,
+                                                     float4* __restrict__ tv_scr, float4* __restrict__ tv_nor,
+                                                     float4* __restrict__ tv_wp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n_tverts) return;
+    int o = s.vtx_obj[i];
+    int mv = s.mesh_vtx_off[s.obj_mesh[o]] + (i - s.obj_vtx_start[o]);
+    V4 p = ld(s.vpos[mv]), n = ld(s.vnor[mv]);
+    V4 clip = mulv(trans[o], p);
+    float rw = vc.perspect ? -1.0f / clip.w : 1.0f;
+    V4 sp = pos(apply(clip, vc.vp));
+    V4 nn = apply(n, normal[o]);
+    V4 m = apply(apply(apply(sp, vc.vpinv), vc.pinv), vc.vinv);
+    tv_scr[i] = make_float4(sp.x, sp.y, sp.z, rw);
+    tv_nor[i] = make_float4(nn.x, nn.y, nn.z, 0.0f);
+    tv_wp[i] = make_float4(m.x, m.y, m.z, 0.0f);
+}
+
+// render/shadow.go:155-173: light-space screen position of every vertex for
+// every shadow slot (grid.y = slot).
+__global__ __launch_bounds__(256) void k_vertex_shadow(DevScene s, const M4* __restrict__ strans, ShadowConsts sc,
+                                                       float4* __restrict__ ts) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int slot = blockIdx.y;
+    if (i >= s.n_tverts) return;
+    int o = s.vtx_obj[i];
+    int mv = s.mesh_vtx_off[s.obj_mesh[o]] + (i - s.obj_vtx_start[o]);
+    V4 q = pos(apply(mulv(strans[(size_t)slot * s.n_objects + o], ld(s.vpos[mv])), sc.vp));
+    ts[(size_t)slot * s.n_tverts + i] = make_float4(q.x, q.y, q.z, 0.0f);
+}
+
+// ------------------------------------------------------- triangle setup
+// render/raster.go:257-269 (IsValid) and draw, raster.go:380-444: back-face
+// cull, viewport AABB test, then either one primitive (all vertices inside)
+// or the Sutherland-Hodgman fan (clipTriangle, clipping.go:67-157), whose
+// pieces keep the parent's recipw (raster.go:440-443).  Each primitive is
+// counted into every 16x16 tile its pixel box touches.
+__global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
+                                                    const float4* __restrict__ tv_nor, Prim* __restrict__ prims,
+                                                    ClipRec* __restrict__ clips, int32_t* __restrict__ tri_clip_slot,
+                                                    int32_t* __restrict__ tile_count, Counters* cnt, Capacities cap) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    bool direct = false, doclip = false;
+    V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
+    float4 A = make_float4(0, 0, 0, 0), B = A, Cc = A;
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0, o = 0, t = 0, i0 = 0, i1 = 0, i2 = 0;
+    const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
+    if (g < s.n_tris) {
+        o = s.tri_obj[g];
+        int mesh = s.obj_mesh[o];
+        t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
+        if (s.tri_valid[t]) {
+            int vb = s.obj_vtx_start[o];
+            i0 = vb + s.tri_v[3 * t], i1 = vb + s.tri_v[3 * t + 1], i2 = vb + s.tri_v[3 * t + 2];
+            A = tv_scr[i0], B = tv_scr[i1], Cc = tv_scr[i2];
+            p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(Cc.x, Cc.y, Cc.z, 1);
+            if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {  // cull.go:26-28
+                Box3 tb = box_of(p1, p2, p3);
+                if (vp_intersect(tb, w, h)) {
+                    if (vp_contains(p1, w, h) && vp_contains(p2, w, h) && vp_contains(p3, w, h))
+                        direct = pixel_box(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, vc.W, vc.H, x0, y0,
+                                           x1, y1);
+                    else
+                        doclip = true;
+                }
+            }
+        }
+    }
+    int slot = wave_append(&cnt->n_prims, direct);
+    if (direct) {
+        if (slot < cap.prims) {
+            store_prim(prims, slot, p1, p2, p3, (uint32_t)g * 8u + SUB_DIRECT, x0, y0, x1, y1);
+            count_tiles(tile_count, 0, vc.tiles_x, x0, y0, x1, y1);
+        } else {
+            atomicOr(&cnt->overflow, 1);
+        }
+    }
+    if (!doclip) return;
+
+    V4 cl[12];
+    cl[0] = p1, cl[1] = p2, cl[2] = p3;
+    int nc = sutherland_hodgman(cl, w, h);
+    int nsub = nc - 2;
+    if (nsub <= 0) return;
+    int base = atomicAdd(&cnt->n_clip_subs, nsub);
+    atomicAdd(&cnt->n_clipped, 1);
+    if (base + nsub > cap.clip_subs) {
+        atomicOr(&cnt->overflow, 1);
+        return;
+    }
+    tri_clip_slot[g] = base;
+    int mesh = s.obj_mesh[o];
+    int mvb = s.mesh_vtx_off[mesh];
+    uint32_t c0 = s.vcol[mvb + s.tri_v[3 * t]], c1 = s.vcol[mvb + s.tri_v[3 * t + 1]],
+             c2 = s.vcol[mvb + s.tri_v[3 * t + 2]];
+    V4 n0 = ld(tv_nor[i0]), n1 = ld(tv_nor[i1]), n2 = ld(tv_nor[i2]);
+    float rw0 = A.w, rw1 = B.w, rw2 = Cc.w;
+    TriSetup par = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+    for (int i = 2; i < nc; i++) {
+        ClipRec rec;
+        V4 sv[3];
+        int idx[3] = {0, i - 1, i};
+        for (int k = 0; k < 3; k++) {
+            float b[3];
+            barycoord(par, cl[idx[k]].x, cl[idx[k]].y, b);
+            V4 q = v4(b[0] * p1.x + b[1] * p2.x + b[2] * p3.x, b[0] * p1.y + b[1] * p2.y + b[2] * p3.y,
+                      b[0] * p1.z + b[1] * p2.z + b[2] * p3.z, 1);
+            V4 nq = v4(b[0] * n0.x + b[1] * n1.x + b[2] * n2.x, b[0] * n0.y + b[1] * n1.y + b[2] * n2.y,
+                       b[0] * n0.z + b[1] * n1.z + b[2] * n2.z, 0);
+            uint8_t cc[4];
+            for (int q4 = 0; q4 < 4; q4++)
+                cc[q4] = f2u8(clamp(b[0] * chan(c0, q4) + b[1] * chan(c1, q4) + b[2] * chan(c2, q4), 0, 0xff));
+            V4 m = apply(apply(apply(q, vc.vpinv), vc.pinv), vc.vinv);
+            float rws[3] = {rw0, rw1, rw2};
+            rec.pos[k] = make_float4(q.x, q.y, q.z, rws[k]);
+            rec.nor[k] = make_float4(nq.x, nq.y, nq.z, bitsf(u8x4(cc[0], cc[1], cc[2], cc[3])));
+            rec.wp[k] = make_float4(m.x, m.y, m.z, 0.0f);
+            sv[k] = q;
+        }
+        clips[base + i - 2] = rec;
+        int bx0, by0, bx1, by1;
+        if (!pixel_box(sv[0].x, sv[0].y, sv[0].z, sv[1].x, sv[1].y, sv[1].z, sv[2].x, sv[2].y, sv[2].z, vc.W, vc.H,
+                       bx0, by0, bx1, by1))
+            continue;
+        int ps = atomicAdd(&cnt->n_prims, 1);
+        if (ps >= cap.prims) {
+            atomicOr(&cnt->overflow, 1);
+            continue;
+        }
+        store_prim(prims, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)(i - 2), bx0, by0, bx1, by1);
+        count_tiles(tile_count, 0, vc.tiles_x, bx0, by0, bx1, by1);
+    }
+}
+
+// render/shadow.go:152-176 (drawDepth up to the pixel loop) for every
+// (triangle, shadow slot): back-face cull, cullViewFrustum, pixel box.
+__global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts sc, const float4* __restrict__ ts,
+                                                      Prim* __restrict__ prims, int32_t* __restrict__ tile_count,
+                                                      Counters* cnt, Capacities cap) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int slot = blockIdx.y;
+    bool emit = false;
+    V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    if (g < s.n_tris) {
+        int o = s.tri_obj[g];
+        int mesh = s.obj_mesh[o];
+        int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
+        if (s.tri_valid[t]) {
+            const float4* T = ts + (size_t)slot * s.n_tverts;
+            int vb = s.obj_vtx_start[o];
+            float4 A = T[vb + s.tri_v[3 * t]], B = T[vb + s.tri_v[3 * t + 1]], C = T[vb + s.tri_v[3 * t + 2]];
+            p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(C.x, C.y, C.z, 1);
+            if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {
+                Box3 tb = box_of(p1, p2, p3);
+                if (vp_intersect(tb, (float)sc.SW, (float)sc.SH))
+                    emit = pixel_box(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, sc.SW, sc.SH, x0, y0, x1, y1);
+            }
+        }
+    }
+    int ps = wave_append(&cnt->n_prims, emit);
+    if (!emit) return;
+    if (ps >= cap.prims) {
+        atomicOr(&cnt->overflow, 1);
+        return;
+    }
+    store_prim(prims, ps, p1, p2, p3, (uint32_t)slot, x0, y0, x1, y1);
+    count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, x0, y0, x1, y1);
+}
+
+// ------------------------------------------------------------ tile scan
+// One workgroup: exclusive prefix sum of the per-tile counts.
+__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ counts, int32_t n,
+                                               int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
+                                               Counters* cnt) {
+    __shared__ int32_t part[1024];
+    int tid = threadIdx.x;
+    int per = (n + 1023) / 1024;
+    int b = tid * per, e = min(n, b + per);
+    int sum = 0;
+    for (int i = b; i < e; i++) sum += counts[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int i = b; i < e; i++) {
+        offsets[i] = run;
+        cursor[i] = run;
+        run += counts[i];
+    }
+    if (tid == 1023) {
+        offsets[n] = part[1023];
+        cnt->n_pairs = part[1023];
+    }
+}
+
+// Scatter each primitive into the lists of the tiles its pixel box touches.
+__global__ __launch_bounds__(256) void k_bin(const Prim* __restrict__ prims, Counters* cnt, int32_t tiles_x,
+                                             int32_t tiles_per_view, int32_t shadow, int32_t* __restrict__ cursor,
+                                             int32_t* __restrict__ list, Capacities cap, int32_t max_prims) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cnt->n_pairs > cap.pairs) {
+        if (i == 0) atomicOr(&cnt->overflow, 1);
+        return;
+    }
+    int np = min(cnt->n_prims, max_prims);
+    if (i >= np || cnt->overflow) return;
+    float4 p2 = prims[i].p2;
+    uint32_t lo = fbits(p2.z), hi = fbits(p2.w);
+    int x0 = lo & 0xffff, y0 = lo >> 16, x1 = hi & 0xffff, y1 = hi >> 16;
+    int base = shadow ? (int)fbits(p2.y) * tiles_per_view : 0;
+    for (int ty = y0 / TILE; ty <= y1 / TILE; ty++)
+        for (int tx = x0 / TILE; tx <= x1 / TILE; tx++) {
+            int pos = atomicAdd(&cursor[base + ty * tiles_x + tx], 1);
+            list[pos] = i;
+        }
+}
+
+// ------------------------------------------------------------- rasters
+namespace {
+// A primitive staged in LDS: the hoisted Barycoord terms, the three depths,
+// the order key and its pixel rectangle inside the tile.
+struct alignas(16) LPrim {
+    float ax, ay, bx, by;
+    float abx, aby, acx, acy;
+    float bcx, bcy, sabc, az;
+    float bz, cz;
+    uint32_t key;
+    uint32_t rect;  // rx0 | ry0<<4 | (w-1)<<8 | (h-1)<<12 (tile-local)
+};
+
+// Loads one primitive and intersects its pixel box with the tile; returns
+// the candidate count.
+__device__ __forceinline__ int stage_prim(const Prim& p, int px0, int py0, LPrim& lp) {
+    uint32_t lo = fbits(p.p2.z), hi = fbits(p.p2.w);
+    int x0 = max((int)(lo & 0xffff), px0), y0 = max((int)(lo >> 16), py0);
+    int x1 = min((int)(hi & 0xffff), px0 + TILE - 1), y1 = min((int)(hi >> 16), py0 + TILE - 1);
+    if (x0 > x1 || y0 > y1) return 0;
+    TriSetup t = tri_setup(p.p0.x, p.p0.y, p.p0.w, p.p1.x, p.p1.z, p.p1.w);
+    lp.ax = t.ax, lp.ay = t.ay, lp.bx = t.bx, lp.by = t.by;
+    lp.abx = t.abx, lp.aby = t.aby, lp.acx = t.acx, lp.acy = t.acy;
+    lp.bcx = t.bcx, lp.bcy = t.bcy, lp.sabc = t.sabc;
+    lp.az = p.p0.z, lp.bz = p.p1.y, lp.cz = p.p2.x;
+    lp.key = fbits(p.p2.y);
+    int w = x1 - x0 + 1, h = y1 - y0 + 1;
+    lp.rect = (uint32_t)(x0 - px0) | ((uint32_t)(y0 - py0) << 4) | ((uint32_t)(w - 1) << 8) | ((uint32_t)(h - 1) << 12);
+    return w * h;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ void block_scan_256(int v, int32_t* start /*[257]*/, int32_t* wsum /*[4]*/) {
+    int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < wv; k++) before += wsum[k];
+    start[tid] = before + incl - v;
+    if (tid == 255) start[256] = before + incl;
+    __syncthreads();
+}
+
+// Candidate idx -> (primitive j, tile-local pixel).  l / w by multiply-shift
+// is exact for l < 256, w <= 16.
+__device__ __forceinline__ void candidate(const int32_t* start, const LPrim* sp, int idx, int& j, int& lx, int& ly) {
+    int lo = 0, hi = 255;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    j = lo;
+    int l = idx - start[j];
+    uint32_t r = sp[j].rect;
+    int w = ((r >> 8) & 15) + 1;
+    int q = (int)(((uint32_t)l * ((65536u + (uint32_t)w - 1u) / (uint32_t)w)) >> 16);
+    lx = (int)(r & 15) + (l - q * w);
+    ly = (int)((r >> 4) & 15) + q;
+}
+
+__device__ __forceinline__ bool eval_candidate(const LPrim& p, float px, float py, float& z) {
+    TriSetup t;
+    t.ax = p.ax, t.ay = p.ay, t.bx = p.bx, t.by = p.by;
+    t.abx = p.abx, t.aby = p.aby, t.acx = p.acx, t.acy = p.acy;
+    t.bcx = p.bcx, t.bcy = p.bcy, t.sabc = p.sabc;
+    float bc[3];
+    barycoord(t, px, py, bc);
+    if (!inside(bc)) return false;
+    z = bc[0] * p.az + bc[1] * p.bz + bc[2] * p.cz;  // raster.go:496
+    return true;
+}
+}  // namespace
+
+// Visibility raster, one workgroup per 16x16 tile.  For each pixel it keeps
+// the 64-bit minimum of (~ordered(z) << 32 | triangle*8+sub): the largest
+// depth wins and, among equal depths, the first primitive in sequential
+// order -- exactly FragmentBuffer.DepthTest/Set under Workers(1)
+// (buffer/buffer.go:221-280).  A NaN depth can only win as the very first
+// fragment of a pixel; the fast pass flags it and the exact-order pass
+// (NAN_MODE) tracks the first fragment of every pixel to decide.
+template <bool NAN_MODE>
+__global__ __launch_bounds__(256) void k_raster_vis(const Prim* __restrict__ prims, const int32_t* __restrict__ tile_off,
+                                                    const int32_t* __restrict__ tile_list, Counters* cnt, int32_t W,
+                                                    int32_t H, int32_t tiles_x, uint64_t* __restrict__ vis,
+                                                    float* __restrict__ depth) {
+    __shared__ LPrim sp[256];
+    __shared__ int32_t start[257];
+    __shared__ int32_t wsum[4];
+    __shared__ unsigned long long svis[TILE_PX];
+    __shared__ uint32_t sfirst[NAN_MODE ? TILE_PX : 1];
+    __shared__ uint32_t snan[NAN_MODE ? TILE_PX : 1];
+    if (cnt->overflow) return;
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int px0 = (tile % tiles_x) * TILE, py0 = (tile / tiles_x) * TILE;
+    svis[tid] = VIS_EMPTY;
+    if (NAN_MODE) sfirst[tid] = 0xffffffffu, snan[tid] = 0xffffffffu;
+    const int beg = tile_off[tile], n = tile_off[tile + 1] - beg;
+    bool saw_nan = false;
+    for (int base = 0; base < n; base += 256) {
+        int c = 0;
+        if (base + tid < n) c = stage_prim(prims[tile_list[beg + base + tid]], px0, py0, sp[tid]);
+        block_scan_256(c, start, wsum);
+        const int total = start[256];
+        for (int idx = tid; idx < total; idx += 256) {
+            int j, lx, ly;
+            candidate(start, sp, idx, j, lx, ly);
+            float z;
+            if (!eval_candidate(sp[j], (float)(px0 + lx) + 0.5f, (float)(py0 + ly) + 0.5f, z)) continue;
+            const int pix = ly * TILE + lx;
+            const uint32_t key = sp[j].key;
+            if (NAN_MODE) atomicMin(&sfirst[pix], key);
+            if (z != z) {
+                saw_nan = true;
+                if (NAN_MODE) atomicMin(&snan[pix], key);
+                continue;
+            }
+            atomicMin(&svis[pix], ((unsigned long long)depth_key_hi(z) << 32) | key);
+        }
+        __syncthreads();
+    }
+    if (!NAN_MODE && saw_nan) atomicOr(&cnt->nan_flag, 1);
+    const int x = px0 + (tid & (TILE - 1)), y = py0 + tid / TILE;
+    if (x >= W || y >= H) return;
+    unsigned long long v = svis[tid];
+    if (NAN_MODE && snan[tid] != 0xffffffffu && snan[tid] == sfirst[tid])
+        v = ((unsigned long long)KEY_HI_NAN << 32) | snan[tid];
+    const size_t o = (size_t)y * W + x;
+    vis[o] = v;
+    uint32_t hi = (uint32_t)(v >> 32);
+    depth[o] = v == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
+}
+
+// Shadow-map raster, one workgroup per (slot, 16x16 tile): the running
+// maximum of render/shadow.go:214-228 (`!(z <= d)` then `d = z`, maps zeroed
+// once at renderer creation and never cleared) as an LDS atomic max over
+// ordered depth bits, merged into the persistent map.  A NaN depth would make
+// the result order-dependent; it is flagged and the frame reports it.
+__global__ __launch_bounds__(256) void k_raster_shadow(const Prim* __restrict__ prims,
+                                                       const int32_t* __restrict__ tile_off,
+                                                       const int32_t* __restrict__ tile_list, Counters* cnt,
+                                                       ShadowConsts sc, float* __restrict__ smaps) {
+    __shared__ LPrim sp[256];
+    __shared__ int32_t start[257];
+    __shared__ int32_t wsum[4];
+    __shared__ uint32_t smax[TILE_PX];
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int beg = tile_off[tile], n = tile_off[tile + 1] - beg;
+    if (n == 0 || cnt->overflow) return;
+    const int slot = tile / sc.tiles_per_map, t = tile % sc.tiles_per_map;
+    const int px0 = (t % sc.tiles_x) * TILE, py0 = (t / sc.tiles_x) * TILE;
+    smax[tid] = depth_ord(0.0f);
+    bool saw_nan = false;
+    for (int base = 0; base < n; base += 256) {
+        int c = 0;
+        if (base + tid < n) c = stage_prim(prims[tile_list[beg + base + tid]], px0, py0, sp[tid]);
+        block_scan_256(c, start, wsum);
+        const int total = start[256];
+        for (int idx = tid; idx < total; idx += 256) {
+            int j, lx, ly;
+            candidate(start, sp, idx, j, lx, ly);
+            float z;
+            if (!eval_candidate(sp[j], (float)(px0 + lx) + 0.5f, (float)(py0 + ly) + 0.5f, z)) continue;
+            if (z != z) {
+                saw_nan = true;
+                continue;
+            }
+            atomicMax(&smax[ly * TILE + lx], depth_ord(z));
+        }
+        __syncthreads();
+    }
+    if (saw_nan) atomicOr(&cnt->shadow_nan, 1);
+    const int x = px0 + (tid & (TILE - 1)), y = py0 + tid / TILE;
+    if (x >= sc.SW || y >= sc.SH) return;
+    float* m = smaps + (size_t)slot * sc.SW * sc.SH + (size_t)y * sc.SW + x;
+    uint32_t nv = smax[tid];
+    if (nv > depth_ord(*m)) *m = depth_from_ord(nv);
+}
+
+// ---------------------------------------------------------------- shade
+namespace {
+
+struct GFrag {
+    float depth;
+    V4 nor, facenor, wpos;
+    uint32_t col;
+    int mat;  // flat material id (-1 when matAt gives nil)
+};
+
+// Rebuilds the G-buffer fragment of pixel (x, y) from its visibility key,
+// as drawClipped wrote it (render/raster.go:487-568): Barycoord at the pixel
+// centre, linear depth, perspective-correct vertex colour, unit
+// interpolated normal, face normal and interpWorldPos of the back-transformed
+// vertices.  UV/Du/Dv only feed Texture.Query, constant for 1x1 textures.
+__device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int x, int y,
+                                uint64_t key) {
+    const uint32_t ord = (uint32_t)key;
+    const int g = (int)(ord >> 3), piece = (int)(ord & 7);
+    const int o = s.tri_obj[g];
+    const int mesh = s.obj_mesh[o];
+    const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
+    V4 P[3], N[3], M[3];
+    float rw[3];
+    uint32_t C3[3];
+    if (piece == (int)SUB_DIRECT) {
+        const int vb = s.obj_vtx_start[o], mvb = s.mesh_vtx_off[mesh];
+        for (int k = 0; k < 3; k++) {
+            int lv = s.tri_v[3 * t + k];
+            float4 a = b.tv_scr[vb + lv];
+            P[k] = v4(a.x, a.y, a.z, 1);
+            rw[k] = a.w;
+            N[k] = ld(b.tv_nor[vb + lv]);
+            M[k] = ld(b.tv_wp[vb + lv]);
+            C3[k] = s.vcol[mvb + lv];
+        }
+    } else {
+        const ClipRec& r = b.clips[b.tri_clip_slot[g] + piece];
+        for (int k = 0; k < 3; k++) {
+            P[k] = v4(r.pos[k].x, r.pos[k].y, r.pos[k].z, 1);
+            rw[k] = r.pos[k].w;
+            N[k] = v4(r.nor[k].x, r.nor[k].y, r.nor[k].z, 0);
+            C3[k] = fbits(r.nor[k].w);
+            M[k] = ld(r.wp[k]);
+        }
+    }
+    TriSetup ts = tri_setup(P[0].x, P[0].y, P[1].x, P[1].y, P[2].x, P[2].y);
+    float bc[3];
+    barycoord(ts, (float)x + 0.5f, (float)y + 0.5f, bc);
+    GFrag f;
+    f.depth = bc[0] * P[0].z + bc[1] * P[1].z + bc[2] * P[2].z;
+    float wc1 = rw[0] * bc[0], wc2 = rw[1] * bc[1], wc3 = rw[2] * bc[2];
+    float norm = 1.0f;
+    if (c.v.perspect) norm = 1 / (wc1 + wc2 + wc3);
+    uint8_t cc[4];
+    for (int k = 0; k < 4; k++)
+        cc[k] = f2u8(clamp((wc1 * chan(C3[0], k) + wc2 * chan(C3[1], k) + wc3 * chan(C3[2], k)) * norm, 0, 0xff));
+    f.col = u8x4(cc[0], cc[1], cc[2], cc[3]);
+    f.nor = unit(v4(bc[0] * N[0].x + bc[1] * N[1].x + bc[2] * N[2].x, bc[0] * N[0].y + bc[1] * N[1].y + bc[2] * N[2].y,
+                    bc[0] * N[0].z + bc[1] * N[1].z + bc[2] * N[2].z, 0));
+    V4 m1 = v4(M[0].x, M[0].y, M[0].z, 0), m2 = v4(M[1].x, M[1].y, M[1].z, 0), m3 = v4(M[2].x, M[2].y, M[2].z, 0);
+    f.facenor = unit(cross(sub(m2, m1), sub(m3, m1)));
+    f.wpos = v4(bc[0] * m1.x + bc[1] * m2.x + bc[2] * m3.x, bc[0] * m1.y + bc[1] * m2.y + bc[2] * m3.y,
+                bc[0] * m1.z + bc[1] * m2.z + bc[2] * m3.z, 1);
+    int mat = s.tri_mat[t];
+    if (mat >= 0) mat += s.obj_mat_base[o];
+    f.mat = (mat >= 0 && mat < s.n_materials) ? mat : -1;
+    return f;
+}
+
+// shader.FragmentShader (shader/blinn_cpu.go:24-106).  Returns false on the
+// no-lights early return (texture colour, no linear sums).
+__device__ bool fragment_shader(const DevScene& s, const ShadeConsts& c, const DevMaterial& m, const GFrag& f,
+                                uint32_t& out, float lin[3]) {
+    if (s.n_lights == 0) {
+        out = m.tex_rgba;
+        return false;
+    }
+    float LdR = 0, LdG = 0, LdB = 0, LsR = 0, LsG = 0, LsB = 0;
+    const V4 n = m.flat_shading ? f.facenor : f.nor;
+    const V4 x = f.wpos;
+    const V4 V = unit(sub(v4(c.v.cam[0], c.v.cam[1], c.v.cam[2], 1), x));
+    for (int i = 0; i < s.n_lights; i++) {
+        const DevLight& l = s.lights[i];
+        V4 L;
+        float I;
+        if (l.kind == 0) {
+            V4 ldir = sub(v4(l.pos[0], l.pos[1], l.pos[2], 1), x);
+            L = unit(ldir);
+            I = l.intensity / len(ldir);
+        } else {
+            L = scale(v4(l.dir[0], l.dir[1], l.dir[2], 0), -1, -1, -1, 1);
+            I = l.intensity;
+        }
+        V4 H = unit(add(L, V));
+        float Ld = clamp(dot(n, L), 0, 1);
+        float Ls = (float)pow64((double)clamp(dot(n, H), 0, 1), (double)m.shininess);
+        LdR += Ld * m.tex[0] * I;
+        LdG += Ld * m.tex[1] * I;
+        LdB += Ld * m.tex[2] * I;
+        LsR += Ls * l.color[0] * I;
+        LsG += Ls * l.color[1] * I;
+        LsB += Ls * l.color[2] * I;
+    }
+    float r = m.la[0] + (m.diffuse[0] * LdR / 255.0f) + (m.specular[0] * LsR / 255.0f);
+    float g = m.la[1] + (m.diffuse[1] * LdG / 255.0f) + (m.specular[1] * LsG / 255.0f);
+    float bb = m.la[2] + (m.diffuse[2] * LdB / 255.0f) + (m.specular[2] * LsB / 255.0f);
+    lin[0] = r, lin[1] = g, lin[2] = bb;
+    out = u8x4(f2u8(clamp(round32(r), 0, 0xff)), f2u8(clamp(round32(g), 0, 0xff)), f2u8(clamp(round32(bb), 0, 0xff)),
+               (uint8_t)(m.tex_rgba >> 24));
+    return true;
+}
+
+// render/shadow.go:230-283 summed over all lights (raster.go:339-347): the
+// number of lights for which shadingVisibility reports true.
+__device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int fx, int fy, float depth) {
+    int occ = 0;
+    const int64_t len = (int64_t)c.SW * c.SH;
+    for (int i = 0; i < s.n_lights; i++) {
+        const DevLight& l = s.lights[i];
+        if (!l.cast_shadow) {
+            occ++;
+            continue;
+        }
+        V4 q = v4((float)fx, (float)fy, depth, 1);
+        q = apply(q, c.v.s2w);
+        q = apply(q, b.light_view[i]);
+        q = apply(q, b.light_proj[i]);
+        q = pos(apply(q, c.svp));
+        int64_t lx = f2i(q.x), ly = f2i(q.y);
+        int64_t idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)c.SW);
+        if (idx > 0 && idx < len) {
+            float sz = b.smaps[(size_t)l.slot * len + idx];
+            if (q.z < sz - 0.03f) occ++;
+        }
+    }
+    return occ;
+}
+
+// maxElevationAngle (material/ao.go:40-73).  atan is monotone and so is its
+// rounding to float32, so the running Max of atan over the taps equals atan
+// of the largest elevation/distance ratio; NaN ratios poison the result as
+// math.Max does.
+__device__ float max_elevation_angle(const float* __restrict__ depth, int W, int H, int px, int py, float dx, float dy,
+                                     float tdepth) {
+    const float X = (float)px, Y = (float)py;
+    float maxr = -__builtin_inff();
+    bool nan = false;
+    for (float t = 0.0f; t < 100; t += 1) {
+        float cx = X + dx * t, cy = Y + dy * t;
+        int64_t ix = f2i(cx), iy = f2i(cy);
+        if (!(ix >= 0 && ix < W && iy >= 0 && iy < H)) break;
+        float ex = X - cx, ey = Y - cy;
+        float dist = sqrt32(fma64(ex, ex, fma64(ey, ey, fma64(0.0f, 0.0f, 0.0f * 0.0f))));
+        if (dist < 1) continue;
+        float r = (depth[(size_t)iy * W + ix] - tdepth) / dist;
+        if (r != r) nan = true;
+        else if (r > maxr) maxr = r;
+    }
+    if (nan) return __builtin_nanf("");
+    return maxr > 0 ? (float)atan64((double)maxr) : 0.0f;
+}
+
+// AmbientOcclusionShade (material/ao.go:20-38) of a fragment at (px, py).
+__device__ uint32_t ao_shade(const ShadeConsts& c, const ShadeBuffers& b, int px, int py, uint32_t col) {
+    const int W = c.v.W, H = c.v.H;
+    const float tdepth = b.depth[(size_t)py * W + px];
+    float total = 0.0f;
+    for (int k = 0; k < 8; k++)
+        total += 1.57079632679489661923f - max_elevation_angle(b.depth, W, H, px, py, c.ao_dir[2 * k], c.ao_dir[2 * k + 1], tdepth);
+    total /= 12.5663706143591729539f;
+    total = (float)pow64((double)total, 10000.0);
+    return u8x4(f2u8(total * chan(col, 0)), f2u8(total * chan(col, 1)), f2u8(total * chan(col, 2)), (uint8_t)(col >> 24));
+}
+
+// render/raster.go:324-359 for a fragment f (the stored fragment of pixel
+// (fx, fy)) shaded with material `mat`; `col` is the fragment's stored colour.
+__device__ uint32_t shade_fragment(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, const GFrag& f,
+                                   int fx, int fy, int mat, uint32_t col, float lin[3], bool& has_lin) {
+    has_lin = false;
+    if (mat < 0) return col;  // vertex colour; AO with a nil material returns it
+    const DevMaterial& m = s.mats[mat];
+    has_lin = fragment_shader(s, c, m, f, col, lin);
+    if (c.shadow_map && m.receive_shadow) {
+        int occ = occluded(s, c, b, fx, fy, f.depth);
+        float w = __builtin_ldexpf(1.0f, -occ);  // math.Pow(0.5, visibles), exact
+        col = u8x4(f2u8(chan(col, 0) * w), f2u8(chan(col, 1) * w), f2u8(chan(col, 2) * w), (uint8_t)(col >> 24));
+    }
+    if (m.ambient_occlusion) col = ao_shade(c, b, fx, fy, col);
+    return col;
+}
+
+__device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
+    if (!c.gamma_correct) return col;
+    return (uint32_t)c.gamma_lut[col & 0xff] | ((uint32_t)c.gamma_lut[(col >> 8) & 0xff] << 8) |
+           ((uint32_t)c.gamma_lut[(col >> 16) & 0xff] << 16) | (col & 0xff000000u);
+}
+
+}  // namespace
+
+// The colour of every uncovered pixel (render/raster.go:324-328 with
+// raster_screen.go:75-87): DrawFragment hands `shade` the zero fragment, whose
+// X = Y = 0 and MaterialID = 0, so it is shaded from pixel (0,0)'s stored
+// fragment with material table[0]; if pixel (0,0) is uncovered it is the
+// Background.  With no material 0 the colour is pixel (0,0)'s own final one.
+__global__ void k_shade_bg(DevScene s, ShadeConsts c, ShadeBuffers b) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || b.cnt->overflow) return;
+    const uint64_t k00 = b.vis[0];
+    float lin[3] = {__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf("")};
+    uint32_t col = c.background;
+    if (k00 != VIS_EMPTY) {
+        GFrag f = fetch_fragment(s, c, b, 0, 0, k00);
+        bool hl;
+        float own_lin[3];
+        uint32_t own = shade_fragment(s, c, b, f, 0, 0, f.mat, f.col, own_lin, hl);
+        if (s.n_materials > 0) {
+            col = shade_fragment(s, c, b, f, 0, 0, 0, f.col, lin, hl);
+            if (!hl) lin[0] = lin[1] = lin[2] = __builtin_nanf("");
+        } else {
+            col = own;
+        }
+    }
+    b.bg[0] = col;
+    b.bg_lin[0] = lin[0], b.bg_lin[1] = lin[1], b.bg_lin[2] = lin[2];
+}
+
+// Deferred pass (DrawFragments(shade) then DrawFragments(GammaCorrection),
+// render/raster.go:300-314,369-376): one thread per pixel, each wave an 8x8
+// block; workgroups are spread so that consecutive workgroups (which the
+// dispatcher deals round-robin to the 8 XCDs) land in one XCD's screen band.
+__global__ __launch_bounds__(256) void k_shade(DevScene s, ShadeConsts c, ShadeBuffers b) {
+    if (b.cnt->overflow) return;
+    const int tiles_x = c.v.tiles_x, n_tiles = tiles_x * c.v.tiles_y;
+    const int per_xcd = (n_tiles + 7) / 8;
+    const int bid = blockIdx.x;
+    const int tile = (bid % 8) * per_xcd + bid / 8;
+    if (tile >= n_tiles) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int x = (tile % tiles_x) * TILE + (wv & 1) * 8 + (lane & 7);
+    const int y = (tile / tiles_x) * TILE + (wv >> 1) * 8 + (lane >> 3);
+    if (x >= c.v.W || y >= c.v.H) return;
+    const size_t o = (size_t)y * c.v.W + x;
+    const uint64_t key = b.vis[o];
+    uint32_t col;
+    float lin[3] = {__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf("")};
+    bool hl = false;
+    if (key == VIS_EMPTY) {
+        col = b.bg[0];
+        lin[0] = b.bg_lin[0], lin[1] = b.bg_lin[1], lin[2] = b.bg_lin[2];
+        hl = true;
+    } else {
+        GFrag f = fetch_fragment(s, c, b, x, y, key);
+        col = shade_fragment(s, c, b, f, x, y, f.mat, f.col, lin, hl);
+    }
+    if (b.pregamma) b.pregamma[o] = col;
+    if (b.linear) {
+        b.linear[3 * o] = hl ? lin[0] : __builtin_nanf("");
+        b.linear[3 * o + 1] = hl ? lin[1] : __builtin_nanf("");
+        b.linear[3 * o + 2] = hl ? lin[2] : __builtin_nanf("");
+    }
+    b.image[(size_t)(c.v.H - 1 - y) * c.v.W + x] = gamma(c, col);
+}
+
+// ------------------------------------------------------------- launchers
+static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min<size_t>(blocks_for(n, 256), 4096)), dim3(256), 0, st, p, n, v);
+}
+void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3(std::min<size_t>(blocks_for(n, 256), 4096)), dim3(256), 0, st, p, n, v);
+}
+void launch_vertex_main(hipStream_t st, const DevScene& s, const M4* trans, const M4* normal, const ViewConsts& vc,
+                        float4* tv_scr, float4* tv_nor, float4* tv_wp) {
+    if (!s.n_tverts) return;
+    hipLaunchKernelGGL(k_vertex_main, dim3(blocks_for(s.n_tverts, 256)), dim3(256), 0, st, s, trans, normal, vc, tv_scr,
+                       tv_nor, tv_wp);
+}
+void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float4* ts) {
+    if (!s.n_tverts || !sc.n_slots) return;
+    hipLaunchKernelGGL(k_vertex_shadow, dim3(blocks_for(s.n_tverts, 256), sc.n_slots), dim3(256), 0, st, s, strans, sc,
+                       ts);
+}
+void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
+                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot, int32_t* tile_count,
+                       Counters* cnt, Capacities cap) {
+    if (!s.n_tris) return;
+    hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, tv_nor, prims,
+                       clips, tri_clip_slot, tile_count, cnt, cap);
+}
+void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
+                         int32_t* tile_count, Counters* cnt, Capacities cap) {
+    if (!s.n_tris || !sc.n_slots) return;
+    hipLaunchKernelGGL(k_setup_shadow, dim3(blocks_for(s.n_tris, 256), sc.n_slots), dim3(256), 0, st, s, sc, ts, prims,
+                       tile_count, cnt, cap);
+}
+void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* cursor, Counters* cnt) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, offsets, cursor, cnt);
+}
+void launch_bin(hipStream_t st, const Prim* prims, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
+                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t max_prims) {
+    if (max_prims <= 0) return;
+    hipLaunchKernelGGL(k_bin, dim3(blocks_for(max_prims, 256)), dim3(256), 0, st, prims, cnt, tiles_x, tiles_per_view,
+                       shadow, cursor, list, cap, max_prims);
+}
+void launch_raster_vis(hipStream_t st, bool nan_mode, const Prim* prims, const int32_t* tile_off,
+                       const int32_t* tile_list, Counters* cnt, int32_t W, int32_t H, int32_t tiles_x, int32_t n_tiles,
+                       uint64_t* vis, float* depth) {
+    if (nan_mode)
+        hipLaunchKernelGGL(k_raster_vis<true>, dim3(n_tiles), dim3(256), 0, st, prims, tile_off, tile_list, cnt, W, H,
+                           tiles_x, vis, depth);
+    else
+        hipLaunchKernelGGL(k_raster_vis<false>, dim3(n_tiles), dim3(256), 0, st, prims, tile_off, tile_list, cnt, W, H,
+                           tiles_x, vis, depth);
+}
+void launch_raster_shadow(hipStream_t st, const Prim* prims, const int32_t* tile_off, const int32_t* tile_list,
+                          Counters* cnt, const ShadowConsts& sc, int32_t n_tiles_total, float* smaps) {
+    if (!n_tiles_total) return;
+    hipLaunchKernelGGL(k_raster_shadow, dim3(n_tiles_total), dim3(256), 0, st, prims, tile_off, tile_list, cnt, sc,
+                       smaps);
+}
+void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
+    hipLaunchKernelGGL(k_shade_bg, dim3(1), dim3(64), 0, st, s, c, b);
+    int n_tiles = c.v.tiles_x * c.v.tiles_y;
+    int per_xcd = (n_tiles + 7) / 8;
+    hipLaunchKernelGGL(k_shade, dim3(per_xcd * 8), dim3(256), 0, st, s, c, b);
+}
+
+}  // namespace prh

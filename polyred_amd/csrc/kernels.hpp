@@ -1,0 +1,61 @@
+// kernels.hpp — host-side launchers of libprh's gfx950 kernels.
+#pragma once
+
+#include "prh_internal.hpp"
+
+namespace prh {
+
+struct ShadeConsts {
+    ViewConsts v;
+    uint8_t gamma_lut[256];   // shader.GammaCorrection per 8-bit value
+    uint32_t background;      // render.Background RGBA8
+    int32_t shadow_map;       // render.ShadowMap
+    int32_t gamma_correct;    // render.GammaCorrection
+    int32_t n_lights;
+    float ao_dir[16];         // material/ao.go:26 directions {cos, sin} x 8
+    int32_t SW, SH;           // shadow map size
+    gm::M4 svp;               // ViewportMatrix(SW, SH)
+};
+
+struct ShadeBuffers {
+    const uint64_t* vis;
+    const float* depth;
+    const float4* tv_scr;
+    const float4* tv_nor;
+    const float4* tv_wp;
+    const ClipRec* clips;
+    const int32_t* tri_clip_slot;
+    const float* smaps;        // [slots][SH][SW]
+    const gm::M4* light_view;  // [n_lights]
+    const gm::M4* light_proj;  // [n_lights]
+    uint32_t* image;           // [H][W] RGBA8, top row first
+    uint32_t* pregamma;        // optional [H][W] raster rows
+    float* linear;             // optional [H][W][3]
+    uint32_t* bg;              // [2] uncovered colour, pixel (0,0)'s colour
+    float* bg_lin;             // [3]
+    const Counters* cnt;       // main-pass counters: nothing is shaded after an overflow
+};
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
+void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
+void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, const gm::M4* normal,
+                        const ViewConsts& vc, float4* tv_scr, float4* tv_nor, float4* tv_wp);
+void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
+                          float4* ts);
+void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
+                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot,
+                       int32_t* tile_count, Counters* cnt, Capacities cap);
+void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
+                         Prim* prims, int32_t* tile_count, Counters* cnt, Capacities cap);
+void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* cursor,
+                 Counters* cnt);
+void launch_bin(hipStream_t st, const Prim* prims, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
+                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t max_prims);
+void launch_raster_vis(hipStream_t st, bool nan_mode, const Prim* prims, const int32_t* tile_off,
+                       const int32_t* tile_list, Counters* cnt, int32_t W, int32_t H, int32_t tiles_x,
+                       int32_t n_tiles, uint64_t* vis, float* depth);
+void launch_raster_shadow(hipStream_t st, const Prim* prims, const int32_t* tile_off, const int32_t* tile_list,
+                          Counters* cnt, const ShadowConsts& sc, int32_t n_tiles_total, float* smaps);
+void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
+
+}  // namespace prh

@@ -1,0 +1,115 @@
+// prh_internal.hpp — device data layout shared by libprh's host runtime and
+// its gfx950 kernels.  Everything lives in HBM, structure-of-arrays where a
+// kernel streams it, fixed-size records where a kernel gathers it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gomath.hpp"
+
+namespace prh {
+
+constexpr int TILE = 16;                 // raster/shade tile side (pixels)
+constexpr int TILE_PX = TILE * TILE;     // 256 pixels = 4 wave64s
+constexpr uint32_t SUB_DIRECT = 7;       // key sub-index of an unclipped triangle
+constexpr uint64_t VIS_EMPTY = ~0ull;    // no fragment
+constexpr uint32_t KEY_HI_NAN = 0u;      // NaN-depth winner (exact-order pass only)
+
+// Material as the shading kernel reads it (from prh_material).
+struct DevMaterial {
+    float diffuse[3];   // float32(Diffuse.R/G/B)
+    float specular[3];  // float32(Specular.R/G/B)
+    float tex[3];       // float32(texture R/G/B)
+    float la[3];        // sum_e Intensity_e * float32(tex)  (blinn_cpu.go:44-52)
+    float shininess;
+    uint32_t tex_rgba;  // texture RGBA8 (alpha is the shaded alpha)
+    int32_t flat_shading, ambient_occlusion, receive_shadow;
+};
+
+struct DevLight {
+    int32_t kind;        // 0 point, 1 directional
+    float pos[3];
+    float dir[3];
+    float intensity;
+    float color[3];      // float32(Color.R/G/B)
+    int32_t cast_shadow;
+    int32_t slot;        // shadow-map slot, -1 if none
+};
+
+// Scene geometry in HBM.
+struct DevScene {
+    const float4* vpos;       // [Vm] model-space positions (mesh vertices)
+    const float4* vnor;       // [Vm] model-space normals
+    const uint32_t* vcol;     // [Vm] RGBA8
+    const int32_t* tri_v;     // [Tm][3] mesh-local vertex ids
+    const int32_t* tri_mat;   // [Tm] geometry-local material id
+    const uint8_t* tri_valid; // [Tm] Triangle.IsValid (model space, per mesh triangle)
+    const int32_t* mesh_vtx_off;
+    const int32_t* mesh_tri_off;
+    const int32_t* obj_mesh;      // [O]
+    const int32_t* obj_mat_base;  // [O]
+    const int32_t* obj_tri_start; // [O+1] first global triangle of each object
+    const int32_t* obj_vtx_start; // [O+1] first transformed vertex of each object
+    const int32_t* tri_obj;       // [Tg] object of each global triangle
+    const int32_t* vtx_obj;       // [Vt] object of each transformed vertex
+    int32_t n_objects;
+    int32_t n_tris;               // Tg = sum over objects of mesh triangles
+    int32_t n_tverts;             // Vt = sum over objects of mesh vertices
+    int32_t n_materials;
+    const DevMaterial* mats;
+    int32_t n_lights;
+    const DevLight* lights;
+};
+
+// Per-view constants of the main camera (passed by value).
+struct ViewConsts {
+    gm::M4 vp, vpinv, pinv, vinv, s2w;
+    float cam[3];
+    int32_t W, H, tiles_x, tiles_y;
+    int32_t perspect;
+    float vpbox_max_x, vpbox_max_y;  // viewport AABB max (MSAA*W, MSAA*H)
+};
+
+// Per-frame shadow constants (all shadow-casting lights at once).
+struct ShadowConsts {
+    gm::M4 vp;                 // ViewportMatrix(SW, SH)
+    int32_t SW, SH, tiles_x, tiles_y;
+    int32_t n_slots, tiles_per_map;
+};
+
+// A rasterisable primitive: a whole triangle (sub = 7) or one clipped fan
+// piece.  p2.y holds the order key (main view) or the shadow slot.
+struct Prim {
+    float4 p0;  // ax ay az bx
+    float4 p1;  // by bz cx cy
+    float4 p2;  // cz key bbox_lo bbox_hi (bit patterns)
+};
+
+// Vertex data of one clipped fan piece (render/clipping.go:67-157).
+struct ClipRec {
+    float4 pos[3];  // screen x y z, parent recipw
+    float4 nor[3];  // normal x y z, colour RGBA8 bits
+    float4 wp[3];   // back-transformed m_i x y z (render/raster.go:467-469)
+};
+
+struct Counters {
+    int32_t n_prims;
+    int32_t n_clip_subs;
+    int32_t n_clipped;
+    int32_t n_pairs;
+    int32_t nan_flag;
+    int32_t overflow;
+    int32_t shadow_nan;
+    int32_t pad;
+};
+
+struct Capacities {
+    int32_t prims;
+    int32_t clip_subs;
+    int32_t pairs;
+};
+
+__host__ __device__ inline uint32_t pack_bbox(int x, int y) { return (uint32_t)(x & 0xffff) | ((uint32_t)y << 16); }
+
+}  // namespace prh

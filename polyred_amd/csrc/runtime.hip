@@ -1,0 +1,748 @@
+// runtime.hip — libprh's host runtime and C ABI (include/prh.h).
+//
+// Host-side work mirrors what render.NewRenderer / (*Renderer).Render do on
+// the CPU before touching pixels (render/raster.go:84-143,155-199,226-315,
+// render/shadow.go:33-90): camera, projection and viewport matrices and their
+// inverses, per-object Model^-1^T, the orthographic light cameras fitted to
+// the camera frustum, the sRGB table.  Everything per pixel or per triangle
+// runs in kernels.hip on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/prh.h"
+#include "kernels.hpp"
+
+using namespace gm;
+using namespace prh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? PRH_E_NOMEM : PRH_E_HIP,                       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+    } while (0)
+
+// ---------------------------------------------------------- host matrices
+M4 mat(const float* a) {
+    M4 m;
+    memcpy(m.a, a, sizeof m.a);
+    return m;
+}
+M4 mulm(const M4& m, const M4& n) {  // math/mat4.go:201-219
+    M4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            r.a[i * 4 + j] = m.a[i * 4] * n.a[j] + m.a[i * 4 + 1] * n.a[4 + j] + m.a[i * 4 + 2] * n.a[8 + j] +
+                             m.a[i * 4 + 3] * n.a[12 + j];
+    return r;
+}
+M4 transpose(const M4& m) {
+    M4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) r.a[i * 4 + j] = m.a[j * 4 + i];
+    return r;
+}
+// Mat4.Det / Mat4.Inv (math/mat4.go:233-283).  The float32 result depends on
+// the order of the products and sums, so the terms are listed in the
+// reference's order: a sign and the element indices of each product.
+const char* const kDetTerms =
+    "+0,5,10,15 -0,5,11,14 +0,6,11,13 -0,6,9,15 +0,7,9,14 -0,7,10,13 -1,6,11,12 +1,6,8,15 "
+    "-1,7,8,14 +1,7,10,12 -1,4,10,15 +1,4,11,14 +2,7,8,13 -2,7,9,12 +2,4,9,15 -2,4,11,13 "
+    "+2,5,11,12 -2,5,8,15 -3,4,9,14 +3,4,10,13 -3,5,10,12 +3,5,8,14 -3,6,8,13 +3,6,9,12";
+const char* const kInvTerms[16] = {
+    "+6,11,13 -7,10,13 +7,9,14 -5,11,14 -6,9,15 +5,10,15", "+3,10,13 -2,11,13 -3,9,14 +1,11,14 +2,9,15 -1,10,15",
+    "+2,7,13 -3,6,13 +3,5,14 -1,7,14 -2,5,15 +1,6,15",     "+3,6,9 -2,7,9 -3,5,10 +1,7,10 +2,5,11 -1,6,11",
+    "+7,10,12 -6,11,12 -7,8,14 +4,11,14 +6,8,15 -4,10,15", "+2,11,12 -3,10,12 +3,8,14 -0,11,14 -2,8,15 +0,10,15",
+    "+3,6,12 -2,7,12 -3,4,14 +0,7,14 +2,4,15 -0,6,15",     "+2,7,8 -3,6,8 +3,4,10 -0,7,10 -2,4,11 +0,6,11",
+    "+5,11,12 -7,9,12 +7,8,13 -4,11,13 -5,8,15 +4,9,15",   "+3,9,12 -1,11,12 -3,8,13 +0,11,13 +1,8,15 -0,9,15",
+    "+1,7,12 -3,5,12 +3,4,13 -0,7,13 -1,4,15 +0,5,15",     "+3,5,8 -1,7,8 -3,4,9 +0,7,9 +1,4,11 -0,5,11",
+    "+6,9,12 -5,10,12 -6,8,13 +4,10,13 +5,8,14 -4,9,14",   "+1,10,12 -2,9,12 +2,8,13 -0,10,13 -1,8,14 +0,9,14",
+    "+2,5,12 -1,6,12 -2,4,13 +0,6,13 +1,4,14 -0,5,14",     "+1,6,8 -2,5,8 +2,4,9 -0,6,9 -1,4,10 +0,5,10"};
+
+// Evaluates "+i,j,k ..." left to right: acc = p0; acc = acc +/- p_t.
+float eval_terms(const float* a, const char* terms) {
+    float acc = 0;
+    bool first = true;
+    const char* p = terms;
+    while (*p) {
+        while (*p == ' ') p++;
+        if (!*p) break;
+        char sign = *p++;
+        float prod = 0;
+        bool firstf = true;
+        for (;;) {
+            int idx = (int)strtol(p, (char**)&p, 10);
+            prod = firstf ? a[idx] : prod * a[idx];
+            firstf = false;
+            if (*p == ',') {
+                p++;
+                continue;
+            }
+            break;
+        }
+        acc = first ? prod : (sign == '+' ? acc + prod : acc - prod);
+        first = false;
+    }
+    return acc;
+}
+float det(const M4& m) { return eval_terms(m.a, kDetTerms); }
+M4 inv(const M4& m) {
+    float dinv = 1 / det(m);
+    M4 r;
+    for (int e = 0; e < 16; e++) r.a[e] = dinv * eval_terms(m.a, kInvTerms[e]);
+    return r;
+}
+M4 viewport_matrix(float w, float h) {  // math/math.go:270-277
+    M4 r = {{w / 2, 0, 0, w / 2, 0, h / 2, 0, h / 2, 0, 0, 1, 0, 0, 0, 0, 1}};
+    return r;
+}
+V3 v3a(const float* p) { return v3(p[0], p[1], p[2]); }
+M4 view_matrix(V3 pos, V3 target, V3 up) {  // camera/camera.go:42-53
+    V3 l = unit(sub(target, pos));
+    V3 lxu = unit(cross(l, up));
+    V3 u = unit(cross(lxu, l));
+    M4 r = {{lxu.x, lxu.y, lxu.z, -dot(lxu, pos), u.x, u.y, u.z, -dot(u, pos), -l.x, -l.y, -l.z, dot(l, pos), 0, 0, 0,
+             1}};
+    return r;
+}
+M4 persp_proj(float fov_deg, float aspect, float n, float f) {  // camera/perspective.go:100-111
+    float fov = (fov_deg * (float)M_PI) / 180;
+    float t = (float)std::tan((double)(fov / 2));
+    M4 r = {{-1 / (aspect * t), 0, 0, 0, 0, -1 / t, 0, 0, 0, 0, (n + f) / (n - f), (2 * n * f) / (n - f), 0, 0, 1, 0}};
+    return r;
+}
+M4 ortho_proj(float l, float r_, float b, float t, float n, float f) {  // camera/orthographic.go:106-117
+    M4 r = {{2 / (r_ - l), 0, 0, (l + r_) / (l - r_), 0, 2 / (t - b), 0, (b + t) / (b - t), 0, 0, 2 / (n - f),
+             (f + n) / (f - n), 0, 0, 0, 1}};
+    return r;
+}
+M4 cam_proj(const prh_camera& c) {
+    if (c.kind == PRH_CAMERA_PERSPECTIVE) return persp_proj(c.fov, c.aspect, c.near_, c.far_);
+    return ortho_proj(c.left, c.right, c.bottom, c.top, c.near_, c.far_);
+}
+// Triangle.IsValid (geometry/primitive/triangle.go:63-80), model space.
+bool tri_is_valid(V4 p1, V4 p2, V4 p3) {
+    const float eps = 1e-7f;
+    V4 a = sub(p2, p1), b = sub(p3, p1);
+    if (approx_eq(a.x, 0, eps) && approx_eq(a.y, 0, eps) && approx_eq(a.z, 0, eps)) return false;
+    if (approx_eq(b.x, 0, eps) && approx_eq(b.y, 0, eps) && approx_eq(b.z, 0, eps)) return false;
+    float d = dot(a, b) / (len(a) * len(b));
+    return !approx_eq(d, 1, eps) && !approx_eq(d, -1, eps);
+}
+// color/srgb.go:57-93 and shader/gamma.go:13-18 for each 8-bit value.
+void gamma_table(uint8_t out[256]) {
+    double lut[1025];
+    for (int i = 0; i < 1024; i++) {
+        double v = (double)i / 1024;
+        lut[i] = v <= 0.0031308 ? v * 12.92 : 1.055 * std::pow(v, 1.0 / 2.4) - 0.055;
+    }
+    lut[1024] = lut[1023];
+    for (int c = 0; c < 256; c++) {
+        float v = (float)c / 0xff, s;
+        if (v <= 0) s = 0;
+        else if (v == 1) s = 1;
+        else {
+            float i = v * 1024;
+            int64_t fl = f2i(i) & 1023;
+            float v0 = (float)lut[fl], v1 = (float)lut[fl + 1];
+            i -= (float)fl;
+            s = v0 * (1.0f - i) + v1 * i;
+        }
+        out[c] = f2u8(s * 0xff + 0.5f);
+    }
+}
+// The eight AO directions (material/ao.go:26-28), float32 angle accumulated.
+void ao_dirs(float out[16]) {
+    int k = 0;
+    for (float a = 0.0f; a < (float)(2 * M_PI - 1e-4) && k < 8; a += (float)(M_PI / 4)) {
+        out[2 * k] = (float)std::cos((double)a);
+        out[2 * k + 1] = (float)std::sin((double)a);
+        k++;
+    }
+}
+
+template <class T>
+int dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(PRH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return PRH_OK;
+}
+template <class T>
+int upload(T** p, const T* src, size_t n) {
+    int rc = dalloc(p, n);
+    if (rc) return rc;
+    if (n) HIP_TRY(hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return PRH_OK;
+}
+
+}  // namespace
+
+struct prh_ctx {
+    int device;
+    hipStream_t stream;
+};
+
+struct prh_scene {
+    prh_ctx* ctx;
+    DevScene d;
+    std::vector<void*> allocs;
+    std::vector<M4> model;
+    std::vector<prh_light> lights;
+    std::vector<int> light_slot;
+    int n_slots;
+    float center[3];
+};
+
+struct prh_renderer {
+    prh_ctx* ctx;
+    prh_scene* sc;
+    prh_camera cam;
+    prh_options opt;
+    int W, H, tiles_x, tiles_y, n_tiles;
+    int SW, SH, stiles_x, stiles_y, stiles_per_map, n_slots;
+    ViewConsts vc;
+    ShadowConsts shc;
+    ShadeConsts sh;
+    std::vector<void*> allocs;
+    M4 *d_trans, *d_normal, *d_strans, *d_lview, *d_lproj;
+    float4 *tv_scr, *tv_nor, *tv_wp, *ts;
+    Prim* prims;
+    ClipRec* clips;
+    int32_t* tri_clip_slot;
+    int32_t *tile_count, *tile_off, *tile_cursor, *tile_list;
+    int32_t tile_slots;
+    uint64_t* vis;
+    float* depth;
+    uint32_t *image, *pregamma, *bg;
+    float *linear, *bg_lin, *smaps;
+    Counters* cnt;      // [2]: main, shadow
+    Counters* h_cnt;    // pinned
+    Capacities cap_main, cap_shadow;
+    hipEvent_t ev[7];
+    prh_frame_stats stats;
+};
+
+namespace {
+
+template <class T>
+int ralloc(prh_renderer* r, T** p, size_t n) {
+    int rc = dalloc(p, n);
+    if (!rc) r->allocs.push_back((void*)*p);
+    return rc;
+}
+
+void free_list(std::vector<void*>& v) {
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
+}
+
+int alloc_work(prh_renderer* r) {
+    int rc;
+    if ((rc = ralloc(r, &r->prims, (size_t)std::max(r->cap_main.prims, r->cap_shadow.prims)))) return rc;
+    if ((rc = ralloc(r, &r->clips, (size_t)r->cap_main.clip_subs))) return rc;
+    if ((rc = ralloc(r, &r->tile_list, (size_t)std::max(r->cap_main.pairs, r->cap_shadow.pairs)))) return rc;
+    return PRH_OK;
+}
+void free_work(prh_renderer* r) {
+    for (void* p : {(void*)r->prims, (void*)r->clips, (void*)r->tile_list}) {
+        auto it = std::find(r->allocs.begin(), r->allocs.end(), p);
+        if (it != r->allocs.end()) {
+            (void)hipFree(p);
+            r->allocs.erase(it);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* prh_last_error(void) { return g_err.c_str(); }
+
+const char* prh_version(void) {
+    static char v[96];
+    snprintf(v, sizeof v, "libprh gfx950 (HIP %d.%d.%d)", HIP_VERSION_MAJOR, HIP_VERSION_MINOR, HIP_VERSION_PATCH);
+    return v;
+}
+
+int prh_open(int32_t ordinal, prh_ctx** out) {
+    if (!out) return fail(PRH_E_INVALID, "prh_open: null out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(PRH_E_NODEVICE, "no HIP device");
+    if (ordinal < 0 || ordinal >= n) return fail(PRH_E_NODEVICE, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(ordinal));
+    hipStream_t st;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    prh_ctx* c = new prh_ctx{ordinal, st};
+    *out = c;
+    return PRH_OK;
+}
+
+int prh_close(prh_ctx* ctx) {
+    if (!ctx) return fail(PRH_E_INVALID, "prh_close: null context");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PRH_OK;
+}
+
+int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
+    if (!ctx || !d || !out) return fail(PRH_E_INVALID, "prh_scene_upload: null argument");
+    *out = nullptr;
+    if (d->n_meshes <= 0 || d->n_objects <= 0) return fail(PRH_E_INVALID, "scene has no meshes or objects");
+    if (d->n_lights < 0 || d->n_ambient < 0 || d->n_materials < 0) return fail(PRH_E_INVALID, "negative count");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int nm = d->n_meshes, O = d->n_objects;
+    const int Vm = d->mesh_vtx_off[nm], Tm = d->mesh_tri_off[nm];
+    for (int m = 0; m < nm; m++)
+        if (d->mesh_vtx_off[m] > d->mesh_vtx_off[m + 1] || d->mesh_tri_off[m] > d->mesh_tri_off[m + 1])
+            return fail(PRH_E_INVALID, "mesh offsets are not increasing");
+    std::vector<float4> vpos(Vm), vnor(Vm);
+    std::vector<uint32_t> vcol(Vm);
+    for (int i = 0; i < Vm; i++) {
+        vpos[i] = make_float4(d->vpos[4 * i], d->vpos[4 * i + 1], d->vpos[4 * i + 2], d->vpos[4 * i + 3]);
+        vnor[i] = make_float4(d->vnor[4 * i], d->vnor[4 * i + 1], d->vnor[4 * i + 2], d->vnor[4 * i + 3]);
+        memcpy(&vcol[i], d->vcol + 4 * i, 4);
+    }
+    std::vector<uint8_t> valid(Tm);
+    for (int m = 0; m < nm; m++) {
+        int nv = d->mesh_vtx_off[m + 1] - d->mesh_vtx_off[m];
+        for (int t = d->mesh_tri_off[m]; t < d->mesh_tri_off[m + 1]; t++) {
+            V4 p[3];
+            for (int k = 0; k < 3; k++) {
+                int lv = d->tri_v[3 * t + k];
+                if (lv < 0 || lv >= nv) return fail(PRH_E_INVALID, "triangle vertex index out of range");
+                const float* q = d->vpos + 4 * (d->mesh_vtx_off[m] + lv);
+                p[k] = v4(q[0], q[1], q[2], q[3]);
+            }
+            valid[t] = tri_is_valid(p[0], p[1], p[2]) ? 1 : 0;
+        }
+    }
+    std::vector<int32_t> tri_start(O + 1), vtx_start(O + 1);
+    int64_t tg = 0, vt = 0;
+    for (int o = 0; o < O; o++) {
+        int m = d->obj_mesh[o];
+        if (m < 0 || m >= nm) return fail(PRH_E_INVALID, "object mesh index out of range");
+        tri_start[o] = (int32_t)tg, vtx_start[o] = (int32_t)vt;
+        tg += d->mesh_tri_off[m + 1] - d->mesh_tri_off[m];
+        vt += d->mesh_vtx_off[m + 1] - d->mesh_vtx_off[m];
+    }
+    if (tg >= (1ll << 29) || vt >= (1ll << 31)) return fail(PRH_E_UNSUPPORTED, "scene exceeds 2^29 triangles");
+    tri_start[O] = (int32_t)tg, vtx_start[O] = (int32_t)vt;
+    std::vector<int32_t> tri_obj(tg), vtx_obj(vt);
+    for (int o = 0; o < O; o++) {
+        for (int32_t g = tri_start[o]; g < tri_start[o + 1]; g++) tri_obj[g] = o;
+        for (int32_t v = vtx_start[o]; v < vtx_start[o + 1]; v++) vtx_obj[v] = o;
+    }
+    std::vector<DevMaterial> mats(d->n_materials);
+    for (int i = 0; i < d->n_materials; i++) {
+        const prh_material& m = d->materials[i];
+        DevMaterial& dm = mats[i];
+        for (int k = 0; k < 3; k++) {
+            dm.diffuse[k] = (float)m.diffuse[k];
+            dm.specular[k] = (float)m.specular[k];
+            dm.tex[k] = (float)m.texture[k];
+            float la = 0.0f;  // shader/blinn_cpu.go:48-52
+            for (int e = 0; e < d->n_ambient; e++) la += d->ambient_intensity[e] * (float)m.texture[k];
+            dm.la[k] = la;
+        }
+        dm.shininess = m.shininess;
+        memcpy(&dm.tex_rgba, m.texture, 4);
+        dm.flat_shading = m.flat_shading != 0;
+        dm.ambient_occlusion = m.ambient_occlusion != 0;
+        dm.receive_shadow = m.receive_shadow != 0;
+    }
+    prh_scene* s = new prh_scene();
+    s->ctx = ctx;
+    s->n_slots = 0;
+    std::vector<DevLight> lights(d->n_lights);
+    for (int i = 0; i < d->n_lights; i++) {
+        const prh_light& l = d->lights[i];
+        if (l.kind != PRH_LIGHT_POINT && l.kind != PRH_LIGHT_DIRECTIONAL) {
+            delete s;
+            return fail(PRH_E_UNSUPPORTED, "light kind not supported");
+        }
+        DevLight& dl = lights[i];
+        dl.kind = l.kind;
+        for (int k = 0; k < 3; k++) {
+            dl.pos[k] = l.position[k];
+            dl.dir[k] = l.direction[k];
+            dl.color[k] = (float)l.color[k];
+        }
+        dl.intensity = l.intensity;
+        dl.cast_shadow = l.cast_shadow != 0;
+        dl.slot = l.cast_shadow ? s->n_slots++ : -1;
+        s->lights.push_back(l);
+        s->light_slot.push_back(dl.slot);
+    }
+    for (int o = 0; o < O; o++) s->model.push_back(mat(d->obj_model + 16 * o));
+    memcpy(s->center, d->center, sizeof s->center);
+
+    DevScene& ds = s->d;
+    memset(&ds, 0, sizeof ds);
+    int rc = 0;
+    auto up = [&](auto** p, const auto* src, size_t n) {
+        if (rc) return;
+        rc = upload(p, src, n);
+        if (!rc) s->allocs.push_back((void*)*p);
+    };
+    float4 *dvpos, *dvnor;
+    uint32_t* dvcol;
+    int32_t *dtri_v, *dtri_mat, *dmvo, *dmto, *dom, *domb, *dts, *dvs, *dto, *dvo;
+    uint8_t* dvalid;
+    DevMaterial* dmats;
+    DevLight* dlights;
+    up(&dvpos, vpos.data(), Vm);
+    up(&dvnor, vnor.data(), Vm);
+    up(&dvcol, vcol.data(), Vm);
+    up(&dtri_v, d->tri_v, (size_t)3 * Tm);
+    up(&dtri_mat, d->tri_mat, Tm);
+    up(&dvalid, valid.data(), Tm);
+    up(&dmvo, d->mesh_vtx_off, nm + 1);
+    up(&dmto, d->mesh_tri_off, nm + 1);
+    up(&dom, d->obj_mesh, O);
+    up(&domb, d->obj_mat_base, O);
+    up(&dts, tri_start.data(), O + 1);
+    up(&dvs, vtx_start.data(), O + 1);
+    up(&dto, tri_obj.data(), tri_obj.size());
+    up(&dvo, vtx_obj.data(), vtx_obj.size());
+    up(&dmats, mats.data(), mats.size());
+    up(&dlights, lights.data(), lights.size());
+    if (rc) {
+        free_list(s->allocs);
+        delete s;
+        return rc;
+    }
+    ds.vpos = dvpos, ds.vnor = dvnor, ds.vcol = dvcol, ds.tri_v = dtri_v, ds.tri_mat = dtri_mat, ds.tri_valid = dvalid;
+    ds.mesh_vtx_off = dmvo, ds.mesh_tri_off = dmto, ds.obj_mesh = dom, ds.obj_mat_base = domb;
+    ds.obj_tri_start = dts, ds.obj_vtx_start = dvs, ds.tri_obj = dto, ds.vtx_obj = dvo;
+    ds.n_objects = O, ds.n_tris = (int32_t)tg, ds.n_tverts = (int32_t)vt;
+    ds.n_materials = d->n_materials, ds.mats = dmats;
+    ds.n_lights = d->n_lights, ds.lights = dlights;
+    *out = s;
+    return PRH_OK;
+}
+
+int prh_scene_free(prh_scene* s) {
+    if (!s) return fail(PRH_E_INVALID, "prh_scene_free: null scene");
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    free_list(s->allocs);
+    delete s;
+    return PRH_OK;
+}
+
+int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const prh_options* o, prh_renderer** out) {
+    if (!ctx || !s || !cam || !o || !out) return fail(PRH_E_INVALID, "prh_renderer_new: null argument");
+    *out = nullptr;
+    if (o->width <= 0 || o->height <= 0 || o->width > 32768 || o->height > 32768)
+        return fail(PRH_E_INVALID, "framebuffer size out of range");
+    if (o->msaa > 1) return fail(PRH_E_UNSUPPORTED, "MSAA > 1 is not on the device path");
+    if (o->shadow_size < 0 || o->shadow_size > 32768) return fail(PRH_E_INVALID, "shadow size out of range");
+    if (cam->kind != PRH_CAMERA_PERSPECTIVE && cam->kind != PRH_CAMERA_ORTHOGRAPHIC)
+        return fail(PRH_E_INVALID, "unknown camera kind");
+    HIP_TRY(hipSetDevice(ctx->device));
+    prh_renderer* r = new prh_renderer();
+    r->ctx = ctx, r->sc = s, r->cam = *cam, r->opt = *o;
+    r->W = o->width, r->H = o->height;
+    r->tiles_x = (r->W + TILE - 1) / TILE, r->tiles_y = (r->H + TILE - 1) / TILE;
+    r->n_tiles = r->tiles_x * r->tiles_y;
+    // shadow_size 0: framebuffer-sized maps as the reference (shadow.go:87-88)
+    r->SW = o->shadow_size > 0 ? o->shadow_size : r->W;
+    r->SH = o->shadow_size > 0 ? o->shadow_size : r->H;
+    const bool shadows = o->shadow_map && s->n_slots > 0;
+    r->n_slots = shadows ? s->n_slots : 0;
+    r->stiles_x = (r->SW + TILE - 1) / TILE;
+    r->stiles_y = (r->SH + TILE - 1) / TILE;
+    r->stiles_per_map = r->stiles_x * r->stiles_y;
+
+    // Main camera (render/raster.go:232-246,281-295).
+    M4 view = view_matrix(v3a(cam->position), v3a(cam->target), v3a(cam->up));
+    M4 proj = cam_proj(*cam);
+    M4 vp = viewport_matrix((float)r->W, (float)r->H);
+    M4 vinv = inv(view), pinv = inv(proj), vpinv = inv(vp);
+    ViewConsts& vc = r->vc;
+    vc.vp = vp, vc.vpinv = vpinv, vc.pinv = pinv, vc.vinv = vinv;
+    vc.s2w = mulm(mulm(vinv, pinv), vpinv);
+    memcpy(vc.cam, cam->position, sizeof vc.cam);
+    vc.W = r->W, vc.H = r->H, vc.tiles_x = r->tiles_x, vc.tiles_y = r->tiles_y;
+    vc.perspect = cam->kind == PRH_CAMERA_PERSPECTIVE;
+    vc.vpbox_max_x = (float)r->W, vc.vpbox_max_y = (float)r->H;
+
+    const int O = s->d.n_objects, L = s->d.n_lights;
+    std::vector<M4> trans(O), normal(O);
+    M4 pv = mulm(proj, view);
+    for (int i = 0; i < O; i++) {
+        trans[i] = mulm(pv, s->model[i]);          // raster.go:382
+        normal[i] = transpose(inv(s->model[i]));   // raster.go:243
+    }
+    // Light cameras, render/shadow.go:33-90 (directional lights included).
+    std::vector<M4> lview(std::max(1, L)), lproj(std::max(1, L)), strans((size_t)std::max(1, r->n_slots) * O);
+    if (shadows) {
+        V3 up = v3(0, 1, 0), center = v3a(s->center);
+        for (int i = 0; i < L; i++) {
+            if (!s->lights[i].cast_shadow) continue;
+            V3 lp = v3a(s->lights[i].position);
+            M4 lv = view_matrix(lp, center, up);
+            M4 tm = mulm(mulm(lv, vinv), pinv);
+            static const float corners[8][3] = {{1, 1, 1},  {1, 1, -1},  {1, -1, 1},  {-1, 1, 1},
+                                                {-1, -1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}};
+            float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            for (int k = 0; k < 8; k++) {
+                V4 q = pos(apply(v4(corners[k][0], corners[k][1], corners[k][2], 1), tm));
+                float qv[3] = {q.x, q.y, q.z};
+                for (int a = 0; a < 3; a++) mn[a] = min2(mn[a], qv[a]), mx[a] = max2(mx[a], qv[a]);
+            }
+            lview[i] = lv;
+            lproj[i] = ortho_proj(mn[0], mx[0], mn[1], mx[1], mx[2], mn[2] - 2);
+            M4 lpv = mulm(lproj[i], lview[i]);
+            int slot = s->light_slot[i];
+            for (int o2 = 0; o2 < O; o2++) strans[(size_t)slot * O + o2] = mulm(lpv, s->model[o2]);
+        }
+    }
+    ShadowConsts& shc = r->shc;
+    shc.vp = viewport_matrix((float)r->SW, (float)r->SH);
+    shc.SW = r->SW, shc.SH = r->SH, shc.tiles_x = r->stiles_x, shc.tiles_y = r->stiles_y;
+    shc.n_slots = r->n_slots, shc.tiles_per_map = r->stiles_per_map;
+
+    ShadeConsts& sh = r->sh;
+    memset(&sh, 0, sizeof sh);
+    sh.v = vc;
+    gamma_table(sh.gamma_lut);
+    memcpy(&sh.background, o->background, 4);
+    sh.shadow_map = o->shadow_map != 0;
+    sh.gamma_correct = o->gamma_correct != 0;
+    sh.n_lights = L;
+    ao_dirs(sh.ao_dir);
+    sh.SW = r->SW, sh.SH = r->SH;
+    sh.svp = shc.vp;
+
+    const size_t P = (size_t)r->W * r->H;
+    const int Tg = s->d.n_tris, Vt = s->d.n_tverts;
+    r->cap_main = Capacities{Tg + 7 * std::min(Tg, 1 << 16) + 16, 7 * std::min(Tg, 1 << 16) + 16,
+                             4 * Tg + 4 * r->n_tiles + 1024};
+    r->cap_shadow = Capacities{std::max(1, r->n_slots) * Tg + 16, 0,
+                               std::max(1, r->n_slots) * (2 * Tg + 4 * r->stiles_per_map) + 1024};
+    r->tile_slots = std::max(r->n_tiles, r->n_slots * r->stiles_per_map) + 1;
+    int rc = 0;
+    auto chk = [&](int v) {
+        if (!rc) rc = v;
+    };
+    chk(upload(&r->d_trans, trans.data(), O));
+    if (!rc) r->allocs.push_back(r->d_trans);
+    chk(upload(&r->d_normal, normal.data(), O));
+    if (!rc) r->allocs.push_back(r->d_normal);
+    chk(upload(&r->d_strans, strans.data(), strans.size()));
+    if (!rc) r->allocs.push_back(r->d_strans);
+    chk(upload(&r->d_lview, lview.data(), lview.size()));
+    if (!rc) r->allocs.push_back(r->d_lview);
+    chk(upload(&r->d_lproj, lproj.data(), lproj.size()));
+    if (!rc) r->allocs.push_back(r->d_lproj);
+    if (!rc) chk(ralloc(r, &r->tv_scr, Vt));
+    if (!rc) chk(ralloc(r, &r->tv_nor, Vt));
+    if (!rc) chk(ralloc(r, &r->tv_wp, Vt));
+    if (!rc) chk(ralloc(r, &r->ts, (size_t)std::max(1, r->n_slots) * Vt));
+    if (!rc) chk(ralloc(r, &r->tri_clip_slot, Tg));
+    if (!rc) chk(ralloc(r, &r->tile_count, r->tile_slots));
+    if (!rc) chk(ralloc(r, &r->tile_off, r->tile_slots + 1));
+    if (!rc) chk(ralloc(r, &r->tile_cursor, r->tile_slots));
+    if (!rc) chk(ralloc(r, &r->vis, P));
+    if (!rc) chk(ralloc(r, &r->depth, P));
+    if (!rc) chk(ralloc(r, &r->image, P));
+    if (!rc) chk(ralloc(r, &r->pregamma, P));
+    if (!rc && o->keep_linear) chk(ralloc(r, &r->linear, 3 * P));
+    if (!rc) chk(ralloc(r, &r->bg, 4));
+    if (!rc) chk(ralloc(r, &r->bg_lin, 4));
+    if (!rc) chk(ralloc(r, &r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH));
+    if (!rc) chk(ralloc(r, &r->cnt, 2));
+    if (!rc) chk(alloc_work(r));
+    if (!rc) {
+        hipError_t e = hipHostMalloc((void**)&r->h_cnt, 2 * sizeof(Counters), hipHostMallocDefault);
+        if (e != hipSuccess) rc = fail(PRH_E_NOMEM, "hipHostMalloc failed");
+    }
+    for (int i = 0; i < 7 && !rc; i++)
+        if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
+    if (rc) {
+        free_list(r->allocs);
+        delete r;
+        return rc;
+    }
+    // Shadow maps start at zero once, here, as make([]float32) in initShadowMaps.
+    launch_fill_u32(ctx->stream, (uint32_t*)r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH, 0u);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    memset(&r->stats, 0, sizeof r->stats);
+    *out = r;
+    return PRH_OK;
+}
+
+int prh_renderer_free(prh_renderer* r) {
+    if (!r) return fail(PRH_E_INVALID, "prh_renderer_free: null renderer");
+    (void)hipSetDevice(r->ctx->device);
+    (void)hipStreamSynchronize(r->ctx->stream);
+    free_list(r->allocs);
+    if (r->h_cnt) (void)hipHostFree(r->h_cnt);
+    for (int i = 0; i < 7; i++)
+        if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
+    delete r;
+    return PRH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Enqueues one frame; nan_mode re-runs the visibility raster in exact order.
+void enqueue_frame(prh_renderer* r, bool nan_mode) {
+    hipStream_t st = r->ctx->stream;
+    const DevScene& d = r->sc->d;
+    Counters* cm = r->cnt;
+    Counters* cs = r->cnt + 1;
+    (void)hipEventRecord(r->ev[0], st);
+    launch_fill_u32(st, (uint32_t*)r->cnt, 2 * sizeof(Counters) / 4, 0u);
+    if (r->n_slots > 0 && !nan_mode) {
+        const int nt = r->n_slots * r->stiles_per_map;
+        launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
+        launch_fill_u32(st, (uint32_t*)r->tile_count, nt, 0u);
+        launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->tile_count, cs, r->cap_shadow);
+        launch_scan(st, r->tile_count, nt, r->tile_off, r->tile_cursor, cs);
+        launch_bin(st, r->prims, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
+                   r->cap_shadow.prims);
+        launch_raster_shadow(st, r->prims, r->tile_off, r->tile_list, cs, r->shc, nt, r->smaps);
+    }
+    (void)hipEventRecord(r->ev[1], st);
+    launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
+    (void)hipEventRecord(r->ev[2], st);
+    launch_fill_u32(st, (uint32_t*)r->tile_count, r->n_tiles, 0u);
+    launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->clips, r->tri_clip_slot, r->tile_count, cm,
+                      r->cap_main);
+    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->tile_cursor, cm);
+    launch_bin(st, r->prims, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main,
+               r->cap_main.prims);
+    (void)hipEventRecord(r->ev[3], st);
+    launch_raster_vis(st, nan_mode, r->prims, r->tile_off, r->tile_list, cm, r->W, r->H, r->tiles_x, r->n_tiles,
+                      r->vis, r->depth);
+    (void)hipEventRecord(r->ev[4], st);
+    ShadeBuffers b;
+    b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
+    b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
+    b.light_view = r->d_lview, b.light_proj = r->d_lproj;
+    b.image = r->image, b.pregamma = r->pregamma, b.linear = r->linear, b.bg = r->bg, b.bg_lin = r->bg_lin;
+    b.cnt = cm;
+    launch_shade(st, d, r->sh, b);
+    (void)hipEventRecord(r->ev[5], st);
+    (void)hipMemcpyAsync(r->h_cnt, r->cnt, 2 * sizeof(Counters), hipMemcpyDeviceToHost, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int prh_render(prh_renderer* r) {
+    if (!r) return fail(PRH_E_INVALID, "prh_render: null renderer");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    hipStream_t st = r->ctx->stream;
+    bool nan_rerun = false;
+    for (int attempt = 0;; attempt++) {
+        enqueue_frame(r, false);
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipGetLastError());
+        const Counters cm = r->h_cnt[0], cs = r->h_cnt[1];
+        bool grow = false;
+        if (cs.overflow || cs.n_prims > r->cap_shadow.prims || cs.n_pairs > r->cap_shadow.pairs) {
+            r->cap_shadow.prims = std::max(r->cap_shadow.prims, cs.n_prims + cs.n_prims / 4 + 16);
+            r->cap_shadow.pairs = std::max(r->cap_shadow.pairs, cs.n_pairs + cs.n_pairs / 4 + 16);
+            grow = true;
+        }
+        if (cm.overflow || cm.n_prims > r->cap_main.prims || cm.n_pairs > r->cap_main.pairs ||
+            cm.n_clip_subs > r->cap_main.clip_subs) {
+            r->cap_main.prims = std::max(r->cap_main.prims, cm.n_prims + cm.n_prims / 4 + 16);
+            r->cap_main.pairs = std::max(r->cap_main.pairs, cm.n_pairs + cm.n_pairs / 4 + 16);
+            r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, cm.n_clip_subs + cm.n_clip_subs / 4 + 16);
+            r->cap_main.prims = std::max(r->cap_main.prims, cm.n_prims + 7 * r->cap_main.clip_subs);
+            grow = true;
+        }
+        if (grow) {
+            if (attempt >= 3) return fail(PRH_E_NOMEM, "work buffers keep overflowing");
+            free_work(r);
+            int rc = alloc_work(r);
+            if (rc) return rc;
+            continue;
+        }
+        if (cs.shadow_nan)
+            return fail(PRH_E_UNSUPPORTED, "NaN depth in a shadow pass: its result depends on sequential order");
+        if (cm.nan_flag) {
+            enqueue_frame(r, true);
+            HIP_TRY(hipStreamSynchronize(st));
+            nan_rerun = true;
+        }
+        prh_frame_stats& S = r->stats;
+        float ms[6] = {0};
+        for (int i = 0; i < 5; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
+        (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[5]);
+        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_shade = ms[4];
+        S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
+        S.nan_rerun = nan_rerun ? 1 : 0;
+        return PRH_OK;
+    }
+}
+
+int prh_read_image(prh_renderer* r, uint8_t* rgba, size_t bytes) {
+    if (!r || !rgba) return fail(PRH_E_INVALID, "prh_read_image: null argument");
+    size_t need = (size_t)4 * r->W * r->H;
+    if (bytes < need) return fail(PRH_E_INVALID, "prh_read_image: buffer too small");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(hipMemcpyAsync(rgba, r->image, need, hipMemcpyDeviceToHost, r->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+    return PRH_OK;
+}
+
+int prh_read_buffer(prh_renderer* r, int32_t which, void* dst, size_t bytes) {
+    if (!r || !dst) return fail(PRH_E_INVALID, "prh_read_buffer: null argument");
+    const size_t P = (size_t)r->W * r->H;
+    const void* src = nullptr;
+    size_t need = 0;
+    switch (which) {
+        case PRH_BUF_VIS: src = r->vis, need = 8 * P; break;
+        case PRH_BUF_DEPTH: src = r->depth, need = 4 * P; break;
+        case PRH_BUF_LINEAR:
+            if (!r->linear) return fail(PRH_E_INVALID, "renderer was created without keep_linear");
+            src = r->linear, need = 12 * P;
+            break;
+        case PRH_BUF_SHADOW: src = r->smaps, need = (size_t)4 * r->n_slots * r->SW * r->SH; break;
+        case PRH_BUF_PREGAMMA: src = r->pregamma, need = 4 * P; break;
+        default: return fail(PRH_E_INVALID, "unknown buffer id");
+    }
+    if (bytes < need) return fail(PRH_E_INVALID, "prh_read_buffer: buffer too small");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    if (need) HIP_TRY(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, r->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+    return PRH_OK;
+}
+
+int prh_last_stats(prh_renderer* r, prh_frame_stats* out) {
+    if (!r || !out) return fail(PRH_E_INVALID, "prh_last_stats: null argument");
+    *out = r->stats;
+    return PRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+"""Host-side mirror of polyred's render option API over libprh.
+
+    ctx = render.Device(0)
+    r = render.NewRenderer(render.Size(w, h), render.Camera(cam), render.Scene(scene),
+                           render.ShadowMap(True), render.GammaCorrection(True),
+                           render.HIP(ctx))
+    img = r.Render()            # (H, W, 4) uint8, top row first
+
+Option names, argument meaning and the pass-record follow render/options.go
+(16-141) and render/raster.go (63-199).  ``HIP(dev)`` is the new option that
+sits beside ``render.GPU(dev)``: every pass of the frame runs in libprh on the
+given device.  There is no CPU fallback in this package -- a frame that libprh
+cannot render raises ``abi.PrhError`` (the Go integration falls back to the
+reference CPU passes instead, see the binding notes).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+
+
+class Device:
+    """An opened GPU (prh_open / prh_close): the context owns one HIP stream."""
+
+    def __init__(self, ordinal: int = 0):
+        self._lib = abi.load()
+        h = C.c_void_p()
+        abi.check(self._lib.prh_open(ordinal, C.byref(h)), "prh_open")
+        self.handle = h
+        self.ordinal = ordinal
+
+    def close(self) -> None:
+        if self.handle:
+            abi.check(self._lib.prh_close(self.handle), "prh_close")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GPUScene:
+    """A scene uploaded to HBM (prh_scene_upload)."""
+
+    def __init__(self, dev: Device, scene):
+        self._lib = abi.load()
+        self.dev = dev
+        self.scene = scene
+        desc = scene.desc()
+        h = C.c_void_p()
+        abi.check(self._lib.prh_scene_upload(dev.handle, C.byref(desc), C.byref(h)), "prh_scene_upload")
+        self.handle = h
+
+    def free(self) -> None:
+        if self.handle:
+            abi.check(self._lib.prh_scene_free(self.handle), "prh_scene_free")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class _Option:  # render/options.go:16-33
+    BatchSize: int = 32
+    Workers: int = 0
+    Width: int = 800
+    Height: int = 600
+    Background: tuple = (0, 0, 0, 0)
+    MSAA: int = 1
+    ShadowMap: bool = False
+    ShadowSize: int = 0
+    GammaCorrect: bool = False
+    Camera: Optional[dict] = None
+    Scene: object = None
+    HIPDevice: Optional[Device] = None
+    KeepLinear: bool = False
+    extra: dict = field(default_factory=dict)
+
+
+def Size(width: int, height: int):
+    def f(o: _Option):
+        o.Width, o.Height = width, height
+    return f
+
+
+def Camera(cam: dict):
+    def f(o: _Option):
+        o.Camera = cam
+    return f
+
+
+def Scene(s):
+    def f(o: _Option):
+        o.Scene = s
+    return f
+
+
+def Background(rgba):
+    def f(o: _Option):
+        o.Background = tuple(rgba)
+    return f
+
+
+def MSAA(n: int):
+    def f(o: _Option):
+        o.MSAA = n
+    return f
+
+
+def ShadowMap(enable: bool):
+    def f(o: _Option):
+        o.ShadowMap = enable
+    return f
+
+
+def ShadowSize(n: int):
+    """Extension: shadow-map side in texels (0 = framebuffer-sized)."""
+    def f(o: _Option):
+        o.ShadowSize = n
+    return f
+
+
+def GammaCorrection(enable: bool):
+    def f(o: _Option):
+        o.GammaCorrect = enable
+    return f
+
+
+def BatchSize(n: int):  # accepted for API parity; the device path has no CPU batches
+    def f(o: _Option):
+        o.BatchSize = n
+    return f
+
+
+def Workers(n: int):  # accepted for API parity
+    def f(o: _Option):
+        o.Workers = n
+    return f
+
+
+def HIP(dev: Device):
+    """Run every pass on `dev` through libprh (the option beside render.GPU)."""
+    def f(o: _Option):
+        o.HIPDevice = dev
+    return f
+
+
+def KeepLinear(enable: bool):
+    """Also keep the pre-Round linear shading sums (parity/debug)."""
+    def f(o: _Option):
+        o.KeepLinear = enable
+    return f
+
+
+class Renderer:
+    def __init__(self, cfg: _Option):
+        self.cfg = cfg
+        self._lib = abi.load()
+        self.passGPU = {}
+        if cfg.HIPDevice is None:
+            raise ValueError("polyred_amd renders on the device only: pass render.HIP(dev)")
+        if cfg.Scene is None or cfg.Camera is None:
+            raise ValueError("NewRenderer needs Scene and Camera options")
+        s = cfg.Scene
+        s.width, s.height = cfg.Width, cfg.Height
+        s.shadow_map, s.gamma, s.shadow_size = cfg.ShadowMap, cfg.GammaCorrect, cfg.ShadowSize
+        s.background = cfg.Background
+        s.camera = cfg.Camera
+        self.gscene = GPUScene(cfg.HIPDevice, s)
+        cam = s.camera_struct()
+        opts = s.options_struct(keep_linear=cfg.KeepLinear)
+        opts.msaa = cfg.MSAA
+        h = C.c_void_p()
+        abi.check(self._lib.prh_renderer_new(cfg.HIPDevice.handle, self.gscene.handle, C.byref(cam), C.byref(opts),
+                                             C.byref(h)), "prh_renderer_new")
+        self.handle = h
+        self.W, self.H = cfg.Width, cfg.Height
+        self.n_shadow = sum(1 for l in s.lights if l.cast_shadow) if cfg.ShadowMap else 0
+        self.S = (cfg.ShadowSize if cfg.ShadowSize > 0 else None)
+
+    # (*Renderer).Render, render/raster.go:155-199
+    def Render(self) -> np.ndarray:
+        self.render_device()
+        return self.image()
+
+    def render_device(self) -> None:
+        """One frame; the image stays in HBM."""
+        abi.check(self._lib.prh_render(self.handle), "prh_render")
+        for name in ("shadow", "forward", "deferred", "gamma"):
+            self.passGPU[name] = True
+
+    def passOnGPU(self, name: str) -> bool:  # render/raster.go:79
+        return self.passGPU.get(name, False)
+
+    def image(self) -> np.ndarray:
+        out = np.empty((self.H, self.W, 4), np.uint8)
+        abi.check(self._lib.prh_read_image(self.handle, out.ctypes.data, out.nbytes), "prh_read_image")
+        return out
+
+    def read(self, which: int) -> np.ndarray:
+        W, H = self.W, self.H
+        if which == abi.PRH_BUF_VIS:
+            out = np.empty((H, W), np.uint64)
+        elif which == abi.PRH_BUF_DEPTH:
+            out = np.empty((H, W), np.float32)
+        elif which == abi.PRH_BUF_LINEAR:
+            out = np.empty((H, W, 3), np.float32)
+        elif which == abi.PRH_BUF_PREGAMMA:
+            out = np.empty((H, W, 4), np.uint8)
+        elif which == abi.PRH_BUF_SHADOW:
+            sw, sh = (self.S, self.S) if self.S else (W, H)
+            out = np.empty((max(1, self.n_shadow), sh, sw), np.float32)
+            if self.n_shadow == 0:
+                return out[:0]
+        else:
+            raise ValueError(which)
+        abi.check(self._lib.prh_read_buffer(self.handle, which, out.ctypes.data, out.nbytes), "prh_read_buffer")
+        return out
+
+    def stats(self) -> abi.prh_frame_stats:
+        st = abi.prh_frame_stats()
+        abi.check(self._lib.prh_last_stats(self.handle, C.byref(st)), "prh_last_stats")
+        return st
+
+    def free(self) -> None:
+        if getattr(self, "handle", None):
+            abi.check(self._lib.prh_renderer_free(self.handle), "prh_renderer_free")
+            self.handle = None
+        if getattr(self, "gscene", None) is not None:
+            self.gscene.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def NewRenderer(*opts) -> Renderer:  # render/raster.go:84-143
+    cfg = _Option()
+    for opt in opts:
+        opt(cfg)
+    return Renderer(cfg)
+
+
+def renderer_for(scene, dev: Device, keep_linear: bool = False) -> Renderer:
+    """A renderer configured from a generated scene's own settings."""
+    return NewRenderer(Size(scene.width, scene.height), Camera(scene.camera), Scene(scene),
+                       Background(scene.background), ShadowMap(scene.shadow_map), ShadowSize(scene.shadow_size),
+                       GammaCorrection(scene.gamma), KeepLinear(keep_linear), HIP(dev))
