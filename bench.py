@@ -1,0 +1,165 @@
+"""Benchmark: one step = one full frame of the 1080p workload.
+
+Workload (the configuration the metric is quoted on): 1920x1080, 1.04M
+triangles (1000 instanced UV spheres + a 200x200-quad ground grid), 8
+Blinn-Phong materials, 4 point + 1 directional light all casting 2048^2 shadow
+maps, AO on, sRGB.  A frame is (*Renderer).Render(): shadow passes, forward
+raster, deferred shading with shadows and AO, gamma.  The scene is resident
+in HBM before the timed region; each step is one synchronous prh_render.
+
+Prints one JSON line.  `value` is frames/s on the one GPU; Mtris/s and Mpix/s
+ride along.  `roofline` is for the dominant kernel, timed with HIP events on
+libprh's stream over the timed region; `cpu_baseline` is the C restatement of
+the reference CPU renderer (the parity oracle) timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "1080p frames/s + Mtris/s + HBM GB/s; batched & fwd+adjoint vs plain 1-view"
+
+
+def shade_bytes(scene, vis: np.ndarray, n_shadow: int) -> dict:
+    """Algorithmic HBM bytes of one k_shade launch: per pixel the 8-byte
+    visibility key, its 4-byte depth and the 4-byte RGBA8 output; per covered
+    pixel one 4-byte shadow texel per shadow map; per distinct visible
+    triangle its 12-byte index triple and 3 vertices x 48 bytes
+    (screen xyz+1/w, normal, back-transformed position)."""
+    P = vis.size
+    cov = vis != np.uint64(0xFFFFFFFFFFFFFFFF)
+    n_cov = int(cov.sum())
+    tris = np.unique((vis[cov] & np.uint64(0xFFFFFFFF)) >> np.uint64(3))
+    t_vis = int(tris.size)
+    b = P * (8 + 4 + 4) + n_cov * 4 * n_shadow + t_vis * (12 + 3 * 48)
+    return dict(bytes=b, pixels=P, covered=n_cov, visible_tris=t_vis)
+
+
+def pmc_traffic(path: str):
+    """HBM bytes per k_shade dispatch from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_dispatch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(scene, frames: int) -> dict:
+    import oracle  # the parity oracle, timed as the CPU baseline only
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    oracle.render(scene)  # warm the page cache / lazy init
+    t = time.perf_counter()
+    fr = None
+    for _ in range(frames):
+        fr = oracle.render(scene)
+    dt = time.perf_counter() - t
+    return dict(value=frames / dt, unit="frames/s", cores=threads, kind="port",
+                sample=f"{frames} full frames of the same 1080p workload through the C restatement of the "
+                       f"reference CPU renderer (raster passes sequential, deferred pass on {threads} OpenMP "
+                       f"threads), {dt:.1f} s", _frame=fr)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--small", action="store_true", help="reduced scene for a quick check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build renders on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    from polyred_amd import abi, render, scenes
+
+    if args.small:
+        scene = scenes.c2_scale(width=480, height=270, n_spheres=200, grid=50, shadow_size=512)
+    else:
+        scene = scenes.c2_scale()
+    dev = render.Device(0)
+    r = render.renderer_for(scene, dev)
+    for _ in range(args.warmup):
+        r.render_device()
+    acc = dict(total=0.0, shadow=0.0, vertex=0.0, setup=0.0, raster=0.0, shade=0.0, shade_bg=0.0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render_device()  # synchronous: returns when the frame is complete
+        st = r.stats()
+        acc["total"] += st.ms_total
+        acc["shadow"] += st.ms_shadow
+        acc["vertex"] += st.ms_vertex
+        acc["setup"] += st.ms_setup
+        acc["raster"] += st.ms_raster
+        acc["shade"] += st.ms_shade
+        acc["shade_bg"] += st.ms_shade_bg
+    dt = time.perf_counter() - t0
+    ms = {k: v / args.steps for k, v in acc.items()}
+    vis = r.read(abi.PRH_BUF_VIS)
+    img = r.image()
+    st = r.stats()
+    n_shadow = sum(1 for l in scene.lights if l.cast_shadow)
+    sb = shade_bytes(scene, vis, n_shadow)
+    ms_step = 1000.0 * dt / args.steps
+    fps = args.steps / dt
+    achieved = sb["bytes"] / (ms["shade"] * 1e-3) / 1e9
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "k_shade_c2_1080p_pmc.md"))
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, no assets)",
+        "config": {"workload": scene.name, "width": scene.width, "height": scene.height,
+                   "triangles": scene.n_triangles, "lights": len(scene.lights), "shadow_maps": n_shadow,
+                   "shadow_size": scene.shadow_size, "ao": True, "srgb": True},
+        "mtris_per_s": round(scene.n_triangles * fps / 1e6, 2),
+        "mpix_per_s": round(scene.n_pixels * fps / 1e6, 2),
+        "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "roofline": {"kernel": "k_shade", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes": sb["bytes"],
+                     "covered_px": sb["covered"], "visible_tris": sb["visible_tris"]},
+        "frame": {"prims": st.n_prims, "clipped": st.n_clipped, "bin_pairs": st.n_bin_pairs},
+    }
+    r.free()
+    dev.close()
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(scene, args.cpu_frames)
+        fr = cb.pop("_frame")
+        order = (vis & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        order[vis == np.uint64(0xFFFFFFFFFFFFFFFF)] = 0xFFFFFFFF
+        out["parity_vs_cpu"] = bool(np.array_equal(order, fr.vis_order) and np.array_equal(img, fr.image))
+        cb["value"] = round(cb["value"], 4)
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

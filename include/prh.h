@@ -126,7 +126,8 @@ typedef struct prh_options {
     int32_t gamma_correct;   /* render.GammaCorrection                         */
     int32_t shadow_size;     /* shadow map side; 0 = framebuffer-sized as the
                                 reference (render/shadow.go:87-88)             */
-    int32_t keep_linear;     /* also keep the pre-Round linear shading sums    */
+    int32_t keep_linear;     /* also keep the debug buffers: pre-Round linear
+                                shading sums and the pre-gamma colour          */
 } prh_options;
 
 /* ---- handles ----------------------------------------------------------- */
@@ -183,6 +184,7 @@ typedef struct prh_frame_stats {
     float ms_setup;      /* cull, clip, bin                                 */
     float ms_raster;     /* visibility raster                               */
     float ms_shade;      /* deferred shade + shadows + AO + gamma           */
+    float ms_shade_bg;   /* the uncovered-pixel colour (one fragment)       */
     int32_t n_prims;     /* primitives after cull/clip                      */
     int32_t n_clipped;   /* triangles that went through the clipper         */
     int32_t n_bin_pairs; /* (primitive, tile) pairs                         */

@@ -881,8 +881,10 @@ void launch_raster_shadow(hipStream_t st, const Prim* prims, const int32_t* tile
     hipLaunchKernelGGL(k_raster_shadow, dim3(n_tiles_total), dim3(256), 0, st, prims, tile_off, tile_list, cnt, sc,
                        smaps);
 }
-void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
+void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     hipLaunchKernelGGL(k_shade_bg, dim3(1), dim3(64), 0, st, s, c, b);
+}
+void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     int n_tiles = c.v.tiles_x * c.v.tiles_y;
     int per_xcd = (n_tiles + 7) / 8;
     hipLaunchKernelGGL(k_shade, dim3(per_xcd * 8), dim3(256), 0, st, s, c, b);

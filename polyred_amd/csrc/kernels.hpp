@@ -56,6 +56,7 @@ void launch_raster_vis(hipStream_t st, bool nan_mode, const Prim* prims, const i
                        int32_t n_tiles, uint64_t* vis, float* depth);
 void launch_raster_shadow(hipStream_t st, const Prim* prims, const int32_t* tile_off, const int32_t* tile_list,
                           Counters* cnt, const ShadowConsts& sc, int32_t n_tiles_total, float* smaps);
+void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 
 }  // namespace prh

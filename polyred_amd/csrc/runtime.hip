@@ -236,7 +236,7 @@ struct prh_renderer {
     Counters* cnt;      // [2]: main, shadow
     Counters* h_cnt;    // pinned
     Capacities cap_main, cap_shadow;
-    hipEvent_t ev[7];
+    hipEvent_t ev[8];
     prh_frame_stats stats;
 };
 
@@ -569,7 +569,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->vis, P));
     if (!rc) chk(ralloc(r, &r->depth, P));
     if (!rc) chk(ralloc(r, &r->image, P));
-    if (!rc) chk(ralloc(r, &r->pregamma, P));
+    if (!rc && o->keep_linear) chk(ralloc(r, &r->pregamma, P));
     if (!rc && o->keep_linear) chk(ralloc(r, &r->linear, 3 * P));
     if (!rc) chk(ralloc(r, &r->bg, 4));
     if (!rc) chk(ralloc(r, &r->bg_lin, 4));
@@ -580,7 +580,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
         hipError_t e = hipHostMalloc((void**)&r->h_cnt, 2 * sizeof(Counters), hipHostMallocDefault);
         if (e != hipSuccess) rc = fail(PRH_E_NOMEM, "hipHostMalloc failed");
     }
-    for (int i = 0; i < 7 && !rc; i++)
+    for (int i = 0; i < 8 && !rc; i++)
         if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
     if (rc) {
         free_list(r->allocs);
@@ -601,7 +601,7 @@ int prh_renderer_free(prh_renderer* r) {
     (void)hipStreamSynchronize(r->ctx->stream);
     free_list(r->allocs);
     if (r->h_cnt) (void)hipHostFree(r->h_cnt);
-    for (int i = 0; i < 7; i++)
+    for (int i = 0; i < 8; i++)
         if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
     delete r;
     return PRH_OK;
@@ -648,8 +648,10 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     b.light_view = r->d_lview, b.light_proj = r->d_lproj;
     b.image = r->image, b.pregamma = r->pregamma, b.linear = r->linear, b.bg = r->bg, b.bg_lin = r->bg_lin;
     b.cnt = cm;
-    launch_shade(st, d, r->sh, b);
+    launch_shade_bg(st, d, r->sh, b);
     (void)hipEventRecord(r->ev[5], st);
+    launch_shade(st, d, r->sh, b);
+    (void)hipEventRecord(r->ev[6], st);
     (void)hipMemcpyAsync(r->h_cnt, r->cnt, 2 * sizeof(Counters), hipMemcpyDeviceToHost, st);
 }
 
@@ -697,9 +699,10 @@ int prh_render(prh_renderer* r) {
         }
         prh_frame_stats& S = r->stats;
         float ms[6] = {0};
-        for (int i = 0; i < 5; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
-        (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[5]);
-        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_shade = ms[4];
+        for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
+        (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
+        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_shade_bg = ms[4],
+        S.ms_shade = ms[5];
         S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
         S.nan_rerun = nan_rerun ? 1 : 0;
         return PRH_OK;
@@ -729,7 +732,10 @@ int prh_read_buffer(prh_renderer* r, int32_t which, void* dst, size_t bytes) {
             src = r->linear, need = 12 * P;
             break;
         case PRH_BUF_SHADOW: src = r->smaps, need = (size_t)4 * r->n_slots * r->SW * r->SH; break;
-        case PRH_BUF_PREGAMMA: src = r->pregamma, need = 4 * P; break;
+        case PRH_BUF_PREGAMMA:
+            if (!r->pregamma) return fail(PRH_E_INVALID, "renderer was created without keep_linear");
+            src = r->pregamma, need = 4 * P;
+            break;
         default: return fail(PRH_E_INVALID, "unknown buffer id");
     }
     if (bytes < need) return fail(PRH_E_INVALID, "prh_read_buffer: buffer too small");

@@ -1,0 +1,71 @@
+"""Hand-built edge-case scenes shared by the oracle tests and the GPU parity
+tests."""
+import numpy as np
+
+from polyred_amd import abi, scenes
+
+
+def _scene(P, tri, mats_local, width, height, camera, materials=None, lights=None, col=None):
+    P = np.asarray(P, np.float32)
+    n = len(P)
+    N = np.tile(np.array([0, 0, 1, 0], np.float32), (n, 1))
+    if col is None:
+        col = np.tile(np.array([200, 100, 50, 255], np.uint8), (n, 1))
+    mesh = scenes.Mesh(P, N, col, np.zeros((n, 2), np.float32), np.asarray(tri, np.int32),
+                       np.asarray(mats_local, np.int32))
+    return scenes.Scene(meshes=[mesh], obj_mesh=np.array([0]), obj_model=np.eye(4, dtype=np.float32).reshape(1, 16),
+                        obj_mat_base=np.array([0]),
+                        materials=materials if materials is not None else [scenes._mat_from((0.5, 0.6, 0.7), 0.5, 8)],
+                        lights=lights if lights is not None else [scenes.Light(position=(1.0, 2.0, 3.0))],
+                        ambient=[0.1], camera=camera, width=width, height=height, name="case")
+
+
+def big_triangle(width=48, height=36):
+    """A large front-facing triangle crossing three viewport borders (so
+    clipTriangle fans it) plus a small vertex-coloured one in front."""
+    P = [[-9, -1.0, 0.2, 1], [2.5, -9, 0.2, 1], [1.0, 9, 0.2, 1],
+         [-0.3, -0.2, 0.6, 1], [0.4, -0.2, 0.6, 1], [0.0, 0.5, 0.6, 1]]
+    tri = scenes._orient(np.asarray(P, np.float32), np.array([[0, 1, 2], [3, 4, 5]], np.int32),
+                         lambda a, b, c: np.tile([0.0, 0.0, 1.0], (len(a), 1)))
+    return _scene(P, tri, [0, -1], width, height, dict(position=(0.0, 0.0, 3.0), fov=60.0, near=0.1, far=100.0))
+
+
+def depth_tie(width=48, height=36):
+    """The same triangle twice: equal depths, the first one must win."""
+    s = big_triangle(width, height)
+    m = s.meshes[0]
+    m.tri = np.concatenate([m.tri[1:2], m.tri[1:2]])
+    m.mat = np.array([0, -1], np.int32)
+    return s
+
+
+ORTHO = dict(kind=abi.PRH_CAMERA_ORTHOGRAPHIC, position=(0.0, 0.0, 3.0), near=1.0, far=-10.0,
+             frustum=(-1.0, 1.0, -1.0, 1.0))
+
+
+def nan_depth(nan_first: bool, width=32, height=32):
+    """A triangle that is valid in model space but projects onto the pixel-
+    centre row y = 10.5 under an axis-aligned orthographic camera: every
+    pixel centre of its box on that row gets Barycoord 0/0 = NaN, which passes
+    the inside test (render/raster.go:491) with a NaN depth.  A NaN fragment
+    only survives as the first fragment of a pixel (buffer/buffer.go:230,279):
+    with nan_first it precedes a covering triangle, otherwise it follows it."""
+    Y = -0.34375  # 16*Y + 16 = 10.5
+    flat = [[-0.5, Y, 0.5, 1], [0.5, Y, 0.5, 1], [0.0, Y, -0.5, 1]]
+    cover = [[-0.9, -0.9, 0.0, 1], [0.9, -0.9, 0.0, 1], [0.0, 0.6, 0.0, 1]]
+    P = flat + cover if nan_first else cover + flat
+    tri = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    mats = [0, -1] if nan_first else [-1, 0]
+    return _scene(P, tri, mats, width, height, ORTHO)
+
+
+def no_lights(width=40, height=30):
+    s = scenes.tiny(7, width=width, height=height, n_tris=10)
+    s.lights = []
+    return s
+
+
+def no_materials(width=40, height=30):
+    s = scenes.tiny(8, width=width, height=height, n_tris=10)
+    s.materials = []
+    return s

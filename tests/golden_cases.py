@@ -1,0 +1,32 @@
+"""The seeded scenes whose oracle outputs are hashed in
+tests/golden/oracle_hashes.json."""
+from polyred_amd import scenes
+
+from . import cases as edge
+
+
+def c1_shadow_ao():
+    s = scenes.c1_plain()
+    s.name = "c1_shadow_ao"
+    s.lights[0].cast_shadow = True
+    s.shadow_map = True
+    for m in s.materials:
+        m.receive_shadow = True
+        m.ambient_occlusion = True
+    return s
+
+
+def cases_dict():
+    d = {"c1_plain": scenes.c1_plain(), "c1_shadow_ao": c1_shadow_ao(),
+         "big_triangle": edge.big_triangle(), "depth_tie": edge.depth_tie(),
+         "nan_first": edge.nan_depth(True), "nan_second": edge.nan_depth(False),
+         "no_lights": edge.no_lights(), "no_materials": edge.no_materials()}
+    for seed in range(4):
+        d[f"tiny_{seed}"] = scenes.tiny(seed, width=40, height=30, n_tris=12)
+    for seed in range(2):
+        d[f"tiny_ortho_{seed}"] = scenes.tiny(100 + seed, width=40, height=30, n_tris=12, perspective=False)
+    return d
+
+
+def all_cases():
+    return cases_dict()
