@@ -139,7 +139,9 @@ typedef struct prh_renderer prh_renderer;
 const char* prh_last_error(void);
 
 /* Open GPU `ordinal` and create the context's stream.  Replaces gpu.Open
- * (gpu/device.go:77-89) for this option; the caller owns the context. */
+ * (gpu/device.go:77-89) for this option; the caller owns the context.
+ * Lifetimes nest: a context outlives its scenes and a scene its renderers;
+ * prh_close / prh_scene_free return PRH_E_INVALID while children live. */
 int prh_open(int32_t ordinal, prh_ctx** out);
 int prh_close(prh_ctx* ctx);
 

@@ -198,10 +198,12 @@ int upload(T** p, const T* src, size_t n) {
 struct prh_ctx {
     int device;
     hipStream_t stream;
+    int live_scenes;  // a context outlives every scene uploaded to it
 };
 
 struct prh_scene {
     prh_ctx* ctx;
+    int live_renderers;  // a scene outlives every renderer drawing it
     DevScene d;
     std::vector<void*> allocs;
     std::vector<M4> model;
@@ -292,13 +294,15 @@ int prh_open(int32_t ordinal, prh_ctx** out) {
     HIP_TRY(hipSetDevice(ordinal));
     hipStream_t st;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    prh_ctx* c = new prh_ctx{ordinal, st};
+    prh_ctx* c = new prh_ctx{ordinal, st, 0};
     *out = c;
     return PRH_OK;
 }
 
 int prh_close(prh_ctx* ctx) {
     if (!ctx) return fail(PRH_E_INVALID, "prh_close: null context");
+    if (ctx->live_scenes > 0)
+        return fail(PRH_E_INVALID, "prh_close: " + std::to_string(ctx->live_scenes) + " scene(s) still alive");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
@@ -439,12 +443,16 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
     ds.n_objects = O, ds.n_tris = (int32_t)tg, ds.n_tverts = (int32_t)vt;
     ds.n_materials = d->n_materials, ds.mats = dmats;
     ds.n_lights = d->n_lights, ds.lights = dlights;
+    ctx->live_scenes++;
     *out = s;
     return PRH_OK;
 }
 
 int prh_scene_free(prh_scene* s) {
     if (!s) return fail(PRH_E_INVALID, "prh_scene_free: null scene");
+    if (s->live_renderers > 0)
+        return fail(PRH_E_INVALID, "prh_scene_free: " + std::to_string(s->live_renderers) + " renderer(s) still alive");
+    s->ctx->live_scenes--;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     free_list(s->allocs);
@@ -455,6 +463,7 @@ int prh_scene_free(prh_scene* s) {
 int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const prh_options* o, prh_renderer** out) {
     if (!ctx || !s || !cam || !o || !out) return fail(PRH_E_INVALID, "prh_renderer_new: null argument");
     *out = nullptr;
+    if (s->ctx != ctx) return fail(PRH_E_INVALID, "prh_renderer_new: scene belongs to another context");
     if (o->width <= 0 || o->height <= 0 || o->width > 32768 || o->height > 32768)
         return fail(PRH_E_INVALID, "framebuffer size out of range");
     if (o->msaa > 1) return fail(PRH_E_UNSUPPORTED, "MSAA > 1 is not on the device path");
@@ -591,6 +600,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     launch_fill_u32(ctx->stream, (uint32_t*)r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH, 0u);
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     memset(&r->stats, 0, sizeof r->stats);
+    s->live_renderers++;
     *out = r;
     return PRH_OK;
 }
@@ -603,6 +613,7 @@ int prh_renderer_free(prh_renderer* r) {
     if (r->h_cnt) (void)hipHostFree(r->h_cnt);
     for (int i = 0; i < 8; i++)
         if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
+    r->sc->live_renderers--;
     delete r;
     return PRH_OK;
 }

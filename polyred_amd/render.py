@@ -15,7 +15,9 @@ reference CPU passes instead, see the binding notes).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -24,8 +26,13 @@ import numpy as np
 from . import abi
 
 
+_live_devices: "weakref.WeakSet" = weakref.WeakSet()
+
+
 class Device:
-    """An opened GPU (prh_open / prh_close): the context owns one HIP stream."""
+    """An opened GPU (prh_open / prh_close): the context owns one HIP stream.
+    Lifetimes nest as in the C ABI: closing a device first frees the scenes
+    (and their renderers) still alive on it."""
 
     def __init__(self, ordinal: int = 0):
         self._lib = abi.load()
@@ -33,9 +40,13 @@ class Device:
         abi.check(self._lib.prh_open(ordinal, C.byref(h)), "prh_open")
         self.handle = h
         self.ordinal = ordinal
+        self._scenes: "weakref.WeakSet" = weakref.WeakSet()
+        _live_devices.add(self)
 
     def close(self) -> None:
         if self.handle:
+            for sc in list(self._scenes):
+                sc.free()
             abi.check(self._lib.prh_close(self.handle), "prh_close")
             self.handle = None
 
@@ -53,13 +64,18 @@ class GPUScene:
         self._lib = abi.load()
         self.dev = dev
         self.scene = scene
+        self.handle = None
+        self._renderers: "weakref.WeakSet" = weakref.WeakSet()
         desc = scene.desc()
         h = C.c_void_p()
         abi.check(self._lib.prh_scene_upload(dev.handle, C.byref(desc), C.byref(h)), "prh_scene_upload")
         self.handle = h
+        dev._scenes.add(self)
 
     def free(self) -> None:
         if self.handle:
+            for r in list(self._renderers):
+                r.free()
             abi.check(self._lib.prh_scene_free(self.handle), "prh_scene_free")
             self.handle = None
 
@@ -177,14 +193,20 @@ class Renderer:
         s.shadow_map, s.gamma, s.shadow_size = cfg.ShadowMap, cfg.GammaCorrect, cfg.ShadowSize
         s.background = cfg.Background
         s.camera = cfg.Camera
+        self.handle = None
         self.gscene = GPUScene(cfg.HIPDevice, s)
         cam = s.camera_struct()
         opts = s.options_struct(keep_linear=cfg.KeepLinear)
         opts.msaa = cfg.MSAA
         h = C.c_void_p()
-        abi.check(self._lib.prh_renderer_new(cfg.HIPDevice.handle, self.gscene.handle, C.byref(cam), C.byref(opts),
-                                             C.byref(h)), "prh_renderer_new")
+        try:
+            abi.check(self._lib.prh_renderer_new(cfg.HIPDevice.handle, self.gscene.handle, C.byref(cam),
+                                                 C.byref(opts), C.byref(h)), "prh_renderer_new")
+        except abi.PrhError:
+            self.gscene.free()
+            raise
         self.handle = h
+        self.gscene._renderers.add(self)
         self.W, self.H = cfg.Width, cfg.Height
         self.n_shadow = sum(1 for l in s.lights if l.cast_shadow) if cfg.ShadowMap else 0
         self.S = (cfg.ShadowSize if cfg.ShadowSize > 0 else None)
@@ -237,12 +259,23 @@ class Renderer:
         if getattr(self, "handle", None):
             abi.check(self._lib.prh_renderer_free(self.handle), "prh_renderer_free")
             self.handle = None
-        if getattr(self, "gscene", None) is not None:
-            self.gscene.free()
+        gs = getattr(self, "gscene", None)
+        if gs is not None:
+            self.gscene = None
+            gs.free()
 
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+@atexit.register
+def _close_all() -> None:
+    for d in list(_live_devices):
+        try:
+            d.close()
         except Exception:
             pass
 

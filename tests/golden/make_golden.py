@@ -18,6 +18,14 @@ import oracle  # noqa: E402
 from tests import golden_cases  # noqa: E402
 
 
+def canon(a: np.ndarray) -> np.ndarray:
+    """float32 bits with every NaN as 0x7FC00000 (NaN sign/payload differ
+    between x86 SSE and the GPU and carry no meaning)."""
+    b = np.ascontiguousarray(a, np.float32).view(np.uint32).copy()
+    b[np.isnan(a)] = 0x7FC00000
+    return b
+
+
 def digest(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
@@ -28,12 +36,12 @@ def main():
         fr = oracle.render(scene)
         out[name] = {
             "vis_order": digest(fr.vis_order),
-            "depth": digest(fr.depth.view(np.uint32)),
+            "depth": digest(canon(fr.depth)),
             "pregamma": digest(fr.pregamma),
             "image": digest(fr.image),
         }
         if fr.shadow_maps is not None:
-            out[name]["shadow_maps"] = digest(fr.shadow_maps.view(np.uint32))
+            out[name]["shadow_maps"] = digest(canon(fr.shadow_maps))
     path = os.path.join(ROOT, "tests", "golden", "oracle_hashes.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
