@@ -97,7 +97,7 @@ def main() -> int:
     r = render.renderer_for(scene, dev)
     for _ in range(args.warmup):
         r.render_device()
-    acc = dict(total=0.0, shadow=0.0, vertex=0.0, setup=0.0, raster=0.0, shade=0.0, shade_bg=0.0)
+    acc = dict(total=0.0, shadow=0.0, vertex=0.0, setup=0.0, raster=0.0, ao=0.0, shade=0.0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r.render_device()  # synchronous: returns when the frame is complete
@@ -108,7 +108,7 @@ def main() -> int:
         acc["setup"] += st.ms_setup
         acc["raster"] += st.ms_raster
         acc["shade"] += st.ms_shade
-        acc["shade_bg"] += st.ms_shade_bg
+        acc["ao"] += st.ms_ao
     dt = time.perf_counter() - t0
     ms = {k: v / args.steps for k, v in acc.items()}
     vis = r.read(abi.PRH_BUF_VIS)

@@ -185,8 +185,8 @@ typedef struct prh_frame_stats {
     float ms_vertex;     /* vertex transform of the main view               */
     float ms_setup;      /* cull, clip, bin                                 */
     float ms_raster;     /* visibility raster                               */
-    float ms_shade;      /* deferred shade + shadows + AO + gamma           */
-    float ms_shade_bg;   /* the uncovered-pixel colour (one fragment)       */
+    float ms_shade;      /* deferred shade + shadow lookups + gamma          */
+    float ms_ao;         /* screen-space ambient occlusion                  */
     int32_t n_prims;     /* primitives after cull/clip                      */
     int32_t n_clipped;   /* triangles that went through the clipper         */
     int32_t n_bin_pairs; /* (primitive, tile) pairs                         */

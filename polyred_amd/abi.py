@@ -123,7 +123,7 @@ class prh_frame_stats(C.Structure):
         ("ms_setup", C.c_float),
         ("ms_raster", C.c_float),
         ("ms_shade", C.c_float),
-        ("ms_shade_bg", C.c_float),
+        ("ms_ao", C.c_float),
         ("n_prims", C.c_int32),
         ("n_clipped", C.c_int32),
         ("n_bin_pairs", C.c_int32),

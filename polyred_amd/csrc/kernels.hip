@@ -16,16 +16,18 @@ __device__ __forceinline__ V4 ld(float4 a) { return v4(a.x, a.y, a.z, a.w); }
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
 
-// One atomic per wave for an append: returns this lane's slot (valid when pred).
-__device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0;
-    int lane = __lane_id();
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    return base + __popcll(mask & ((1ull << lane) - 1ull));
+// Go's math.Min/Max folds (NaN- and Inf-aware, -0 below +0) reduce to plain
+// min/max when every input is finite; the sign of a zero extreme never
+// changes a result here (they feed comparisons and Round), so finite inputs
+// take the fast path and anything else the exact fold.
+__device__ __forceinline__ bool fin(float a) { return __builtin_isfinite(a); }
+__device__ __forceinline__ float gmin3(float a, float b, float c) {
+    if (fin(a) && fin(b) && fin(c)) return fminf(fminf(a, b), c);
+    return min2(min2(min2(FLT_MAX, a), b), c);
+}
+__device__ __forceinline__ float gmax3(float a, float b, float c) {
+    if (fin(a) && fin(b) && fin(c)) return fmaxf(fmaxf(a, b), c);
+    return max2(max2(max2(-FLT_MAX, a), b), c);
 }
 
 // Pixel box of a triangle, render/raster.go:473-477 (and shadow.go:187-191):
@@ -33,10 +35,8 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
 // then restricted to the buffer (FragmentBuffer.In).  Returns false if empty.
 __device__ bool pixel_box(float ax, float ay, float az, float bx, float by, float bz, float cx, float cy, float cz,
                           int W, int H, int& x0, int& y0, int& x1, int& y1) {
-    float mnx = min2(min2(min2(FLT_MAX, ax), bx), cx);
-    float mny = min2(min2(min2(FLT_MAX, ay), by), cy);
-    float mxx = max2(max2(max2(-FLT_MAX, ax), bx), cx);
-    float mxy = max2(max2(max2(-FLT_MAX, ay), by), cy);
+    float mnx = gmin3(ax, bx, cx), mny = gmin3(ay, by, cy);
+    float mxx = gmax3(ax, bx, cx), mxy = gmax3(ay, by, cy);
     (void)az, (void)bz, (void)cz;
     int64_t xmin = f2i(round32(mnx) - 1), xmax = f2i(round32(mxx) + 1);
     int64_t ymin = f2i(round32(mny) - 1), ymax = f2i(round32(mxy) + 1);
@@ -54,17 +54,16 @@ struct Box3 {
 };
 __device__ Box3 box_of(V4 a, V4 b, V4 c) {  // primitive.NewAABB, box.go:15-25
     Box3 r;
-    r.mnx = min2(min2(min2(FLT_MAX, a.x), b.x), c.x);
-    r.mny = min2(min2(min2(FLT_MAX, a.y), b.y), c.y);
-    r.mnz = min2(min2(min2(FLT_MAX, a.z), b.z), c.z);
-    r.mxx = max2(max2(max2(-FLT_MAX, a.x), b.x), c.x);
-    r.mxy = max2(max2(max2(-FLT_MAX, a.y), b.y), c.y);
-    r.mxz = max2(max2(max2(-FLT_MAX, a.z), b.z), c.z);
+    r.mnx = gmin3(a.x, b.x, c.x), r.mny = gmin3(a.y, b.y, c.y), r.mnz = gmin3(a.z, b.z, c.z);
+    r.mxx = gmax3(a.x, b.x, c.x), r.mxy = gmax3(a.y, b.y, c.y), r.mxz = gmax3(a.z, b.z, c.z);
     return r;
 }
 // AABB.Intersect with the viewport box Min (0,0,-1), Max (w,h,1); the z test
 // compares against the viewport's Max.Y (geometry/primitive/box.go:38).
 __device__ bool vp_intersect(const Box3& t, float w, float h) {
+    if (fin(t.mnx) && fin(t.mny) && fin(t.mnz) && fin(t.mxx) && fin(t.mxy) && fin(t.mxz))
+        return fmaxf(0.0f, t.mnx) <= fminf(w, t.mxx) && fmaxf(0.0f, t.mny) <= fminf(h, t.mxy) &&
+               fmaxf(-1.0f, t.mnz) <= fminf(h, t.mxz);
     float minX = max2(0.0f, t.mnx), minY = max2(0.0f, t.mny), minZ = max2(-1.0f, t.mnz);
     float maxX = min2(w, t.mxx), maxY = min2(h, t.mxy), maxZ = min2(h, t.mxz);
     return minX <= maxX && minY <= maxY && minZ <= maxZ;
@@ -78,12 +77,55 @@ __device__ bool vp_contains(V4 p, float w, float h) {  // box.go:61-75 for one v
            less_eq(p.z, 1.0f);
 }
 
-__device__ void count_tiles(int32_t* tile_count, int base, int tiles_x, int x0, int y0, int x1, int y1) {
-    for (int ty = y0 / TILE; ty <= y1 / TILE; ty++)
-        for (int tx = x0 / TILE; tx <= x1 / TILE; tx++) atomicAdd(&tile_count[base + ty * tiles_x + tx], 1);
+// Adds one to counter[key] for every active lane with a single atomic per
+// distinct key in the wave (neighbouring triangles share tiles, and a shadow
+// map concentrates a scene in few tiles, so per-lane atomics collide).
+__device__ __forceinline__ void wave_count(int32_t* counter, int key, bool active) {
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lkey = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lkey) & pending;
+        if (__lane_id() == leader) atomicAdd(&counter[lkey], __popcll(same));
+        pending &= ~same;
+    }
+}
+// counter[key]++ for every active lane, one atomic per distinct key; returns
+// the lane's pre-increment value.
+__device__ __forceinline__ int wave_alloc(int32_t* counter, int key, bool active) {
+    uint64_t pending = __ballot(active);
+    int pos = 0;
+    const int lane = __lane_id();
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lkey = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lkey) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counter[lkey], __popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) pos = base + __popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return pos;
+}
+// Counts a primitive into every tile its pixel box touches (wave-uniform call).
+__device__ __forceinline__ void count_tiles(int32_t* tile_count, int base, int tiles_x, bool active, int x0, int y0,
+                                            int x1, int y1) {
+    const int tx0 = x0 / TILE, ty0 = y0 / TILE, ntx = active ? x1 / TILE - tx0 + 1 : 0;
+    const int n = active ? ntx * (y1 / TILE - ty0 + 1) : 0;
+    for (int j = 0; __ballot(j < n) != 0; j++) {
+        const bool act = j < n;
+        const int key = act ? base + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx : 0;
+        wave_count(tile_count, key, act);
+    }
 }
 
-__device__ __forceinline__ void store_prim(Prim* prims, int slot, V4 a, V4 b, V4 c, uint32_t key, int x0, int y0,
+constexpr uint32_t PRIM_EMPTY = 0xffffffffu;  // bbox word of an unused primitive slot
+
+__device__ __forceinline__ void store_empty(Prim* prims, size_t slot) {
+    prims[slot].p2 = make_float4(0.0f, 0.0f, __uint_as_float(PRIM_EMPTY), __uint_as_float(PRIM_EMPTY));
+}
+__device__ __forceinline__ void store_prim(Prim* prims, size_t slot, V4 a, V4 b, V4 c, uint32_t key, int x0, int y0,
                                            int x1, int y1) {
     Prim p;
     p.p0 = make_float4(a.x, a.y, a.z, b.x);
@@ -184,28 +226,25 @@ __global__ __launch_bounds__(256) void k_vertex_shadow(DevScene s, const M4* __r
 
 // ------------------------------------------------------- triangle setup
 // render/raster.go:257-269 (IsValid) and draw, raster.go:380-444: back-face
-// cull, viewport AABB test, then either one primitive (all vertices inside)
-// or the Sutherland-Hodgman fan (clipTriangle, clipping.go:67-157), whose
-// pieces keep the parent's recipw (raster.go:440-443).  Each primitive is
-// counted into every 16x16 tile its pixel box touches.
+// cull, viewport AABB test, then either one primitive in the triangle's own
+// slot (all vertices inside) or a place in the clipper's queue.  Each
+// primitive is counted into every 16x16 tile its pixel box touches.
 __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
-                                                    const float4* __restrict__ tv_nor, Prim* __restrict__ prims,
-                                                    ClipRec* __restrict__ clips, int32_t* __restrict__ tri_clip_slot,
+                                                    Prim* __restrict__ prims, int32_t* __restrict__ clip_list,
                                                     int32_t* __restrict__ tile_count, Counters* cnt, Capacities cap) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
     bool direct = false, doclip = false;
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
-    float4 A = make_float4(0, 0, 0, 0), B = A, Cc = A;
-    int x0 = 0, y0 = 0, x1 = 0, y1 = 0, o = 0, t = 0, i0 = 0, i1 = 0, i2 = 0;
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
     if (g < s.n_tris) {
-        o = s.tri_obj[g];
-        int mesh = s.obj_mesh[o];
-        t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
+        const int o = s.tri_obj[g];
+        const int mesh = s.obj_mesh[o];
+        const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
         if (s.tri_valid[t]) {
-            int vb = s.obj_vtx_start[o];
-            i0 = vb + s.tri_v[3 * t], i1 = vb + s.tri_v[3 * t + 1], i2 = vb + s.tri_v[3 * t + 2];
-            A = tv_scr[i0], B = tv_scr[i1], Cc = tv_scr[i2];
+            const int vb = s.obj_vtx_start[o];
+            const float4 A = tv_scr[vb + s.tri_v[3 * t]], B = tv_scr[vb + s.tri_v[3 * t + 1]],
+                         Cc = tv_scr[vb + s.tri_v[3 * t + 2]];
             p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(Cc.x, Cc.y, Cc.z, 1);
             if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {  // cull.go:26-28
                 Box3 tb = box_of(p1, p2, p3);
@@ -218,31 +257,50 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
                 }
             }
         }
+        if (direct) store_prim(prims, g, p1, p2, p3, (uint32_t)g * 8u + SUB_DIRECT, x0, y0, x1, y1);
+        else store_empty(prims, g);
     }
-    int slot = wave_append(&cnt->n_prims, direct);
-    if (direct) {
-        if (slot < cap.prims) {
-            store_prim(prims, slot, p1, p2, p3, (uint32_t)g * 8u + SUB_DIRECT, x0, y0, x1, y1);
-            count_tiles(tile_count, 0, vc.tiles_x, x0, y0, x1, y1);
-        } else {
-            atomicOr(&cnt->overflow, 1);
-        }
+    count_tiles(tile_count, 0, vc.tiles_x, direct, x0, y0, x1, y1);
+    const int pos = wave_alloc(&cnt->n_clipped, 0, doclip);
+    if (doclip) {
+        if (pos < cap.clip_tris) clip_list[pos] = g;
+        else atomicOr(&cnt->overflow, 1);
     }
-    if (!doclip) return;
+    const int nd = __syncthreads_count(direct);
+    if (threadIdx.x == 0 && nd) atomicAdd(&cnt->n_prims, nd);
+}
 
+// The clipper (clipTriangle, clipping.go:67-157) for the queued triangles:
+// Sutherland-Hodgman against the six screen-space planes, a fan of pieces
+// with screen-barycentric attributes that keep the parent's recipw
+// (raster.go:440-443), appended after the per-triangle primitive slots.
+__global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
+                                             const float4* __restrict__ tv_nor, const int32_t* __restrict__ clip_list,
+                                             Prim* __restrict__ prims, ClipRec* __restrict__ clips,
+                                             int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
+                                             Counters* cnt, Capacities cap) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= min(cnt->n_clipped, cap.clip_tris) || cnt->overflow) return;
+    const int g = clip_list[q];
+    const int o = s.tri_obj[g];
+    const int mesh = s.obj_mesh[o];
+    const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
+    const int vb = s.obj_vtx_start[o];
+    const int i0 = vb + s.tri_v[3 * t], i1 = vb + s.tri_v[3 * t + 1], i2 = vb + s.tri_v[3 * t + 2];
+    const float4 A = tv_scr[i0], B = tv_scr[i1], Cc = tv_scr[i2];
+    const V4 p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(Cc.x, Cc.y, Cc.z, 1);
+    const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
     V4 cl[12];
     cl[0] = p1, cl[1] = p2, cl[2] = p3;
     int nc = sutherland_hodgman(cl, w, h);
     int nsub = nc - 2;
     if (nsub <= 0) return;
     int base = atomicAdd(&cnt->n_clip_subs, nsub);
-    atomicAdd(&cnt->n_clipped, 1);
     if (base + nsub > cap.clip_subs) {
         atomicOr(&cnt->overflow, 1);
         return;
     }
     tri_clip_slot[g] = base;
-    int mesh = s.obj_mesh[o];
     int mvb = s.mesh_vtx_off[mesh];
     uint32_t c0 = s.vcol[mvb + s.tri_v[3 * t]], c1 = s.vcol[mvb + s.tri_v[3 * t + 1]],
              c2 = s.vcol[mvb + s.tri_v[3 * t + 2]];
@@ -273,15 +331,15 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
         clips[base + i - 2] = rec;
         int bx0, by0, bx1, by1;
         if (!pixel_box(sv[0].x, sv[0].y, sv[0].z, sv[1].x, sv[1].y, sv[1].z, sv[2].x, sv[2].y, sv[2].z, vc.W, vc.H,
-                       bx0, by0, bx1, by1))
-            continue;
-        int ps = atomicAdd(&cnt->n_prims, 1);
-        if (ps >= cap.prims) {
-            atomicOr(&cnt->overflow, 1);
+                       bx0, by0, bx1, by1)) {
+            store_empty(prims, (size_t)s.n_tris + base + i - 2);
             continue;
         }
+        atomicAdd(&cnt->n_prims, 1);
+        const size_t ps = (size_t)s.n_tris + base + i - 2;
         store_prim(prims, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)(i - 2), bx0, by0, bx1, by1);
-        count_tiles(tile_count, 0, vc.tiles_x, bx0, by0, bx1, by1);
+        for (int ty = by0 / TILE; ty <= by1 / TILE; ty++)
+            for (int tx = bx0 / TILE; tx <= bx1 / TILE; tx++) atomicAdd(&tile_count[ty * vc.tiles_x + tx], 1);
     }
 }
 
@@ -290,8 +348,8 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
 __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts sc, const float4* __restrict__ ts,
                                                       Prim* __restrict__ prims, int32_t* __restrict__ tile_count,
                                                       Counters* cnt, Capacities cap) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    int slot = blockIdx.y;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = blockIdx.y;
     bool emit = false;
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
@@ -311,67 +369,102 @@ __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts s
             }
         }
     }
-    int ps = wave_append(&cnt->n_prims, emit);
-    if (!emit) return;
-    if (ps >= cap.prims) {
-        atomicOr(&cnt->overflow, 1);
-        return;
+    if (g < s.n_tris) {
+        const size_t ps = (size_t)slot * s.n_tris + g;
+        if (emit) store_prim(prims, ps, p1, p2, p3, (uint32_t)slot, x0, y0, x1, y1);
+        else store_empty(prims, ps);
     }
-    store_prim(prims, ps, p1, p2, p3, (uint32_t)slot, x0, y0, x1, y1);
-    count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, x0, y0, x1, y1);
+    count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, emit, x0, y0, x1, y1);
+    const int nd = __syncthreads_count(emit);
+    if (threadIdx.x == 0 && nd) atomicAdd(&cnt->n_prims, nd);
 }
 
 // ------------------------------------------------------------ tile scan
-// One workgroup: exclusive prefix sum of the per-tile counts.
+// One workgroup: exclusive prefix sums of the per-tile primitive counts and
+// of the per-tile work items (ceil(count / ITEM_PRIMS)), 4096 tiles per step (four
+// consecutive tiles per thread, wave scans, then the 16 wave totals),
+// carrying the running totals across steps.
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+    for (int off = 1; off < 64; off <<= 1) {
+        int u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
 __global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ counts, int32_t n,
-                                               int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
-                                               Counters* cnt) {
-    __shared__ int32_t part[1024];
-    int tid = threadIdx.x;
-    int per = (n + 1023) / 1024;
-    int b = tid * per, e = min(n, b + per);
-    int sum = 0;
-    for (int i = b; i < e; i++) sum += counts[i];
-    part[tid] = sum;
+                                               int32_t* __restrict__ offsets, int32_t* __restrict__ chunk_off,
+                                               int32_t* __restrict__ cursor, Counters* cnt) {
+    __shared__ int32_t wtot[16], wch[16];
+    __shared__ int32_t carry_s, carry_c;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) carry_s = 0, carry_c = 0;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = tid >= off ? part[tid - off] : 0;
+    for (int base = 0; base < n; base += 4096) {
+        const int i0 = base + 4 * tid;
+        int v[4], ch[4];
+        for (int k = 0; k < 4; k++) {
+            v[k] = i0 + k < n ? counts[i0 + k] : 0;
+            ch[k] = (v[k] + ITEM_PRIMS - 1) / ITEM_PRIMS;
+        }
+        const int local = v[0] + v[1] + v[2] + v[3], localc = ch[0] + ch[1] + ch[2] + ch[3];
+        const int incl = wave_incl_scan(local, lane), inclc = wave_incl_scan(localc, lane);
+        if (lane == 63) wtot[wv] = incl, wch[wv] = inclc;
         __syncthreads();
-        part[tid] += v;
+        int before = carry_s, beforec = carry_c;
+        for (int k = 0; k < wv; k++) before += wtot[k], beforec += wch[k];
+        int run = before + incl - local, runc = beforec + inclc - localc;
+        for (int k = 0; k < 4; k++)
+            if (i0 + k < n) {
+                offsets[i0 + k] = run;
+                cursor[i0 + k] = run;
+                chunk_off[i0 + k] = runc;
+                run += v[k];
+                runc += ch[k];
+            }
+        __syncthreads();
+        if (tid == 1023) carry_s = before + incl, carry_c = beforec + inclc;
         __syncthreads();
     }
-    int run = part[tid] - sum;
-    for (int i = b; i < e; i++) {
-        offsets[i] = run;
-        cursor[i] = run;
-        run += counts[i];
-    }
-    if (tid == 1023) {
-        offsets[n] = part[1023];
-        cnt->n_pairs = part[1023];
+    if (tid == 0) {
+        offsets[n] = carry_s;
+        chunk_off[n] = carry_c;
+        cnt->n_pairs = carry_s;
+        cnt->n_chunks = carry_c;
     }
 }
 
 // Scatter each primitive into the lists of the tiles its pixel box touches.
+// Slots [0, n_fixed) are one per triangle (per shadow slot); the main view's
+// clipped pieces follow them.
 __global__ __launch_bounds__(256) void k_bin(const Prim* __restrict__ prims, Counters* cnt, int32_t tiles_x,
                                              int32_t tiles_per_view, int32_t shadow, int32_t* __restrict__ cursor,
-                                             int32_t* __restrict__ list, Capacities cap, int32_t max_prims) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                             int32_t* __restrict__ list, Capacities cap, int32_t n_fixed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (cnt->n_pairs > cap.pairs) {
         if (i == 0) atomicOr(&cnt->overflow, 1);
         return;
     }
-    int np = min(cnt->n_prims, max_prims);
-    if (i >= np || cnt->overflow) return;
-    float4 p2 = prims[i].p2;
-    uint32_t lo = fbits(p2.z), hi = fbits(p2.w);
-    int x0 = lo & 0xffff, y0 = lo >> 16, x1 = hi & 0xffff, y1 = hi >> 16;
-    int base = shadow ? (int)fbits(p2.y) * tiles_per_view : 0;
-    for (int ty = y0 / TILE; ty <= y1 / TILE; ty++)
-        for (int tx = x0 / TILE; tx <= x1 / TILE; tx++) {
-            int pos = atomicAdd(&cursor[base + ty * tiles_x + tx], 1);
-            list[pos] = i;
+    if (cnt->overflow) return;
+    const int np = n_fixed + (shadow ? 0 : min(cnt->n_clip_subs, cap.clip_subs));
+    bool active = false;
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0, base = 0;
+    if (i < np) {
+        const float4 p2 = prims[i].p2;
+        const uint32_t lo = fbits(p2.z), hi = fbits(p2.w);
+        if (hi != PRIM_EMPTY) {
+            active = true;
+            x0 = lo & 0xffff, y0 = lo >> 16, x1 = hi & 0xffff, y1 = hi >> 16;
+            base = shadow ? (int)fbits(p2.y) * tiles_per_view : 0;
         }
+    }
+    const int tx0 = x0 / TILE, ty0 = y0 / TILE, ntx = active ? x1 / TILE - tx0 + 1 : 0;
+    const int n = active ? ntx * (y1 / TILE - ty0 + 1) : 0;
+    for (int j = 0; __ballot(j < n) != 0; j++) {
+        const bool act = j < n;
+        const int key = act ? base + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx : 0;
+        const int pos = wave_alloc(cursor, key, act);
+        if (act) list[pos] = i;
+    }
 }
 
 // ------------------------------------------------------------- rasters
@@ -453,109 +546,125 @@ __device__ __forceinline__ bool eval_candidate(const LPrim& p, float px, float p
 }
 }  // namespace
 
-// Visibility raster, one workgroup per 16x16 tile.  For each pixel it keeps
-// the 64-bit minimum of (~ordered(z) << 32 | triangle*8+sub): the largest
-// depth wins and, among equal depths, the first primitive in sequential
-// order -- exactly FragmentBuffer.DepthTest/Set under Workers(1)
-// (buffer/buffer.go:221-280).  A NaN depth can only win as the very first
-// fragment of a pixel; the fast pass flags it and the exact-order pass
-// (NAN_MODE) tracks the first fragment of every pixel to decide.
-template <bool NAN_MODE>
-__global__ __launch_bounds__(256) void k_raster_vis(const Prim* __restrict__ prims, const int32_t* __restrict__ tile_off,
-                                                    const int32_t* __restrict__ tile_list, Counters* cnt, int32_t W,
-                                                    int32_t H, int32_t tiles_x, uint64_t* __restrict__ vis,
-                                                    float* __restrict__ depth) {
-    __shared__ LPrim sp[256];
-    __shared__ int32_t start[257];
-    __shared__ int32_t wsum[4];
-    __shared__ unsigned long long svis[TILE_PX];
-    __shared__ uint32_t sfirst[NAN_MODE ? TILE_PX : 1];
-    __shared__ uint32_t snan[NAN_MODE ? TILE_PX : 1];
-    if (cnt->overflow) return;
-    const int tile = blockIdx.x, tid = threadIdx.x;
-    const int px0 = (tile % tiles_x) * TILE, py0 = (tile / tiles_x) * TILE;
-    svis[tid] = VIS_EMPTY;
-    if (NAN_MODE) sfirst[tid] = 0xffffffffu, snan[tid] = 0xffffffffu;
-    const int beg = tile_off[tile], n = tile_off[tile + 1] - beg;
-    bool saw_nan = false;
-    for (int base = 0; base < n; base += 256) {
-        int c = 0;
-        if (base + tid < n) c = stage_prim(prims[tile_list[beg + base + tid]], px0, py0, sp[tid]);
-        block_scan_256(c, start, wsum);
-        const int total = start[256];
-        for (int idx = tid; idx < total; idx += 256) {
-            int j, lx, ly;
-            candidate(start, sp, idx, j, lx, ly);
-            float z;
-            if (!eval_candidate(sp[j], (float)(px0 + lx) + 0.5f, (float)(py0 + ly) + 0.5f, z)) continue;
-            const int pix = ly * TILE + lx;
-            const uint32_t key = sp[j].key;
-            if (NAN_MODE) atomicMin(&sfirst[pix], key);
-            if (z != z) {
-                saw_nan = true;
-                if (NAN_MODE) atomicMin(&snan[pix], key);
-                continue;
-            }
-            atomicMin(&svis[pix], ((unsigned long long)depth_key_hi(z) << 32) | key);
-        }
-        __syncthreads();
-    }
-    if (!NAN_MODE && saw_nan) atomicOr(&cnt->nan_flag, 1);
-    const int x = px0 + (tid & (TILE - 1)), y = py0 + tid / TILE;
-    if (x >= W || y >= H) return;
-    unsigned long long v = svis[tid];
-    if (NAN_MODE && snan[tid] != 0xffffffffu && snan[tid] == sfirst[tid])
-        v = ((unsigned long long)KEY_HI_NAN << 32) | snan[tid];
-    const size_t o = (size_t)y * W + x;
-    vis[o] = v;
-    uint32_t hi = (uint32_t)(v >> 32);
-    depth[o] = v == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
+// Work items of the rasters: (tile, up to ITEM_PRIMS primitives of its
+// list), so that a tile a scene crowds -- a shadow map fitted to the whole
+// camera frustum concentrates the scene in a few hundred of its tiles -- is
+// spread over many workgroups.  k_items lists each item's tile.
+__global__ __launch_bounds__(256) void k_items(const int32_t* __restrict__ chunk_off, int32_t n_tiles,
+                                               int32_t* __restrict__ item_tile, const Counters* cnt,
+                                               int32_t cap_items) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles || cnt->n_chunks > cap_items) return;
+    for (int i = chunk_off[t]; i < chunk_off[t + 1]; i++) item_tile[i] = t;
 }
 
-// Shadow-map raster, one workgroup per (slot, 16x16 tile): the running
-// maximum of render/shadow.go:214-228 (`!(z <= d)` then `d = z`, maps zeroed
-// once at renderer creation and never cleared) as an LDS atomic max over
-// ordered depth bits, merged into the persistent map.  A NaN depth would make
-// the result order-dependent; it is flagged and the frame reports it.
-__global__ __launch_bounds__(256) void k_raster_shadow(const Prim* __restrict__ prims,
-                                                       const int32_t* __restrict__ tile_off,
-                                                       const int32_t* __restrict__ tile_list, Counters* cnt,
-                                                       ShadowConsts sc, float* __restrict__ smaps) {
-    __shared__ LPrim sp[256];
-    __shared__ int32_t start[257];
+// Persistent workgroups take items i = blockIdx.x, blockIdx.x + gridDim.x, ...
+// Each item evaluates its candidates (pixel box x tile) exactly as
+// drawClipped / drawDepth do, reduces them in LDS and writes the tile: with
+// plain stores when the tile is a single item, with global atomics (an
+// order-independent merge) when several items share it:
+//   MODE_VIS    64-bit min of (~ordered(z) << 32 | triangle*8+sub): largest
+//               depth wins, ties go to the first primitive in sequential
+//               order -- FragmentBuffer.DepthTest/Set under Workers(1)
+//               (buffer/buffer.go:221-280); a NaN depth is flagged;
+//   MODE_EXACT  also the first fragment and the first NaN fragment of each
+//               pixel, for the NaN rule (a NaN depth only survives as the
+//               very first fragment of a pixel);
+//   MODE_SHADOW the running maximum of render/shadow.go:214-228 (maps zeroed
+//               once, never cleared) as a max over ordered depth bits.
+enum { MODE_VIS = 0, MODE_EXACT = 1, MODE_SHADOW = 2 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_raster(const Prim* __restrict__ prims, const int32_t* __restrict__ tile_off,
+                                                const int32_t* __restrict__ chunk_off,
+                                                const int32_t* __restrict__ item_tile,
+                                                const int32_t* __restrict__ tile_list, Counters* cnt, RasterTarget rt,
+                                                int32_t cap_items) {
+    constexpr bool SHADOW = MODE == MODE_SHADOW, EXACT = MODE == MODE_EXACT;
+    __shared__ LPrim sp[CHUNK];
+    __shared__ int32_t start[CHUNK + 1];
     __shared__ int32_t wsum[4];
-    __shared__ uint32_t smax[TILE_PX];
-    const int tile = blockIdx.x, tid = threadIdx.x;
-    const int beg = tile_off[tile], n = tile_off[tile + 1] - beg;
-    if (n == 0 || cnt->overflow) return;
-    const int slot = tile / sc.tiles_per_map, t = tile % sc.tiles_per_map;
-    const int px0 = (t % sc.tiles_x) * TILE, py0 = (t / sc.tiles_x) * TILE;
-    smax[tid] = depth_ord(0.0f);
+    __shared__ unsigned long long svis[SHADOW ? 1 : TILE_PX];
+    __shared__ uint32_t smax[SHADOW ? TILE_PX : 1];
+    __shared__ uint32_t sfirst[EXACT ? TILE_PX : 1];
+    __shared__ uint32_t snan[EXACT ? TILE_PX : 1];
+    const int tid = threadIdx.x;
+    const int n_items = cnt->n_chunks;
+    if (cnt->overflow || n_items > cap_items) return;
     bool saw_nan = false;
-    for (int base = 0; base < n; base += 256) {
-        int c = 0;
-        if (base + tid < n) c = stage_prim(prims[tile_list[beg + base + tid]], px0, py0, sp[tid]);
-        block_scan_256(c, start, wsum);
-        const int total = start[256];
-        for (int idx = tid; idx < total; idx += 256) {
-            int j, lx, ly;
-            candidate(start, sp, idx, j, lx, ly);
-            float z;
-            if (!eval_candidate(sp[j], (float)(px0 + lx) + 0.5f, (float)(py0 + ly) + 0.5f, z)) continue;
-            if (z != z) {
-                saw_nan = true;
-                continue;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int tile = item_tile[item];
+        const int slot = SHADOW ? tile / rt.tiles_per_map : 0;
+        const int t = SHADOW ? tile % rt.tiles_per_map : tile;
+        const int px0 = (t % rt.tiles_x) * TILE, py0 = (t / rt.tiles_x) * TILE;
+        const int first_item = chunk_off[tile];
+        const bool shared_tile = chunk_off[tile + 1] - first_item > 1;
+        const int beg = tile_off[tile] + (item - first_item) * ITEM_PRIMS;
+        const int n = min(ITEM_PRIMS, tile_off[tile + 1] - beg);
+        if (SHADOW) smax[tid] = depth_ord(0.0f);
+        else svis[tid] = VIS_EMPTY;
+        if (EXACT) sfirst[tid] = 0xffffffffu, snan[tid] = 0xffffffffu;
+        for (int b0 = 0; b0 < n; b0 += CHUNK) {
+            int c = 0;
+            if (b0 + tid < n) c = stage_prim(prims[tile_list[beg + b0 + tid]], px0, py0, sp[tid]);
+            block_scan_256(c, start, wsum);
+            const int total = start[CHUNK];
+            for (int idx = tid; idx < total; idx += 256) {
+                int j, lx, ly;
+                candidate(start, sp, idx, j, lx, ly);
+                float z;
+                if (!eval_candidate(sp[j], (float)(px0 + lx) + 0.5f, (float)(py0 + ly) + 0.5f, z)) continue;
+                const int pix = ly * TILE + lx;
+                const uint32_t key = sp[j].key;
+                if (EXACT) atomicMin(&sfirst[pix], key);
+                if (z != z) {
+                    saw_nan = true;
+                    if (EXACT) atomicMin(&snan[pix], key);
+                    continue;
+                }
+                if (SHADOW) atomicMax(&smax[pix], depth_ord(z));
+                else atomicMin(&svis[pix], ((unsigned long long)depth_key_hi(z) << 32) | key);
             }
-            atomicMax(&smax[ly * TILE + lx], depth_ord(z));
+            __syncthreads();
+        }
+        const int x = px0 + (tid & (TILE - 1)), y = py0 + tid / TILE;
+        if (SHADOW) {
+            if (x < rt.W && y < rt.H && smax[tid] > depth_ord(0.0f)) {
+                uint32_t* m = &rt.smap_ord[(size_t)slot * rt.W * rt.H + (size_t)y * rt.W + x];
+                if (shared_tile) atomicMax(m, smax[tid]);
+                else if (smax[tid] > *m) *m = smax[tid];
+            }
+        } else if (x < rt.W && y < rt.H) {
+            const size_t o = (size_t)y * rt.W + x;
+            if (!shared_tile) rt.vis[o] = svis[tid];
+            else if (svis[tid] != VIS_EMPTY) atomicMin(&rt.vis[o], svis[tid]);
+            if (EXACT) {
+                if (sfirst[tid] != 0xffffffffu) atomicMin(&rt.first_all[o], sfirst[tid]);
+                if (snan[tid] != 0xffffffffu) atomicMin(&rt.first_nan[o], snan[tid]);
+            }
         }
         __syncthreads();
     }
-    if (saw_nan) atomicOr(&cnt->shadow_nan, 1);
-    const int x = px0 + (tid & (TILE - 1)), y = py0 + tid / TILE;
-    if (x >= sc.SW || y >= sc.SH) return;
-    float* m = smaps + (size_t)slot * sc.SW * sc.SH + (size_t)y * sc.SW + x;
-    uint32_t nv = smax[tid];
-    if (nv > depth_ord(*m)) *m = depth_from_ord(nv);
+    if (saw_nan) atomicOr(SHADOW ? &cnt->shadow_nan : &cnt->nan_flag, 1);
+}
+
+// Fragment depth per pixel from the visibility key (-1 where uncovered, as
+// ao.go reads an empty fragment); after the exact-order pass, a NaN-depth
+// fragment that came first at its pixel takes it.
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, const Counters* cnt, float* __restrict__ depth) {
+    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (o >= (size_t)rt.W * rt.H || cnt->overflow) return;
+    unsigned long long v = rt.vis[o];
+    if (EXACT) {
+        const uint32_t fn = rt.first_nan[o];
+        if (fn != 0xffffffffu && fn == rt.first_all[o]) {
+            v = ((unsigned long long)KEY_HI_NAN << 32) | fn;
+            rt.vis[o] = v;
+        }
+    }
+    const uint32_t hi = (uint32_t)(v >> 32);
+    depth[o] = v == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
 }
 
 // ---------------------------------------------------------------- shade
@@ -690,46 +799,16 @@ __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuff
         int64_t lx = f2i(q.x), ly = f2i(q.y);
         int64_t idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)c.SW);
         if (idx > 0 && idx < len) {
-            float sz = b.smaps[(size_t)l.slot * len + idx];
+            float sz = depth_from_ord(b.smaps[(size_t)l.slot * len + idx]);
             if (q.z < sz - 0.03f) occ++;
         }
     }
     return occ;
 }
 
-// maxElevationAngle (material/ao.go:40-73).  atan is monotone and so is its
-// rounding to float32, so the running Max of atan over the taps equals atan
-// of the largest elevation/distance ratio; NaN ratios poison the result as
-// math.Max does.
-__device__ float max_elevation_angle(const float* __restrict__ depth, int W, int H, int px, int py, float dx, float dy,
-                                     float tdepth) {
-    const float X = (float)px, Y = (float)py;
-    float maxr = -__builtin_inff();
-    bool nan = false;
-    for (float t = 0.0f; t < 100; t += 1) {
-        float cx = X + dx * t, cy = Y + dy * t;
-        int64_t ix = f2i(cx), iy = f2i(cy);
-        if (!(ix >= 0 && ix < W && iy >= 0 && iy < H)) break;
-        float ex = X - cx, ey = Y - cy;
-        float dist = sqrt32(fma64(ex, ex, fma64(ey, ey, fma64(0.0f, 0.0f, 0.0f * 0.0f))));
-        if (dist < 1) continue;
-        float r = (depth[(size_t)iy * W + ix] - tdepth) / dist;
-        if (r != r) nan = true;
-        else if (r > maxr) maxr = r;
-    }
-    if (nan) return __builtin_nanf("");
-    return maxr > 0 ? (float)atan64((double)maxr) : 0.0f;
-}
-
-// AmbientOcclusionShade (material/ao.go:20-38) of a fragment at (px, py).
-__device__ uint32_t ao_shade(const ShadeConsts& c, const ShadeBuffers& b, int px, int py, uint32_t col) {
-    const int W = c.v.W, H = c.v.H;
-    const float tdepth = b.depth[(size_t)py * W + px];
-    float total = 0.0f;
-    for (int k = 0; k < 8; k++)
-        total += 1.57079632679489661923f - max_elevation_angle(b.depth, W, H, px, py, c.ao_dir[2 * k], c.ao_dir[2 * k + 1], tdepth);
-    total /= 12.5663706143591729539f;
-    total = (float)pow64((double)total, 10000.0);
+// AmbientOcclusionShade's last step (material/ao.go:34-37): the fragment
+// colour scaled by the AO factor k_ao computed for its pixel.
+__device__ __forceinline__ uint32_t ao_apply(uint32_t col, float total) {
     return u8x4(f2u8(total * chan(col, 0)), f2u8(total * chan(col, 1)), f2u8(total * chan(col, 2)), (uint8_t)(col >> 24));
 }
 
@@ -746,7 +825,7 @@ __device__ uint32_t shade_fragment(const DevScene& s, const ShadeConsts& c, cons
         float w = __builtin_ldexpf(1.0f, -occ);  // math.Pow(0.5, visibles), exact
         col = u8x4(f2u8(chan(col, 0) * w), f2u8(chan(col, 1) * w), f2u8(chan(col, 2) * w), (uint8_t)(col >> 24));
     }
-    if (m.ambient_occlusion) col = ao_shade(c, b, fx, fy, col);
+    if (m.ambient_occlusion) col = ao_apply(col, b.ao[(size_t)fy * c.v.W + fx]);
     return col;
 }
 
@@ -757,6 +836,88 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 }
 
 }  // namespace
+
+// Screen-space AO (material/ao.go:20-73) for every pixel whose material has
+// it, plus pixel (0,0) (the uncovered pixels are shaded from it).  One
+// workgroup = one 8x8 pixel block x the 8 march directions, one wave per
+// direction.  maxElevationAngle returns at the first tap outside the buffer;
+// the tap coordinates move monotonically along the ray, so a tap never
+// re-enters once out and masking every later tap gives the same result --
+// which turns the march into a fixed 99-step loop with no divergence.  The
+// running Max of atan over the taps equals atan of the largest
+// elevation/distance ratio (atan and its rounding to float32 are monotone);
+// a NaN ratio poisons it as math.Max does.  The per-pixel factor
+// pow(total, 10000) (ao.go:31-33) is written for k_shade.
+__global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uint64_t* __restrict__ vis,
+                                            const float* __restrict__ depth, float* __restrict__ ao,
+                                            const Counters* cnt) {
+    __shared__ float s_m[8][64];
+    __shared__ float s_dp[64];
+    __shared__ int32_t s_need[64];
+    if (cnt->overflow) return;
+    const int W = c.v.W, H = c.v.H;
+    const int bx = (W + 7) / 8, nb = bx * ((H + 7) / 8);
+    const int per_xcd = (nb + 7) / 8;
+    const int blk = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (blk >= nb) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x = (blk % bx) * 8 + (lane & 7), y = (blk / bx) * 8 + (lane >> 3);
+    if (wv == 0) {
+        bool need = false;
+        float dp = -1.0f;
+        if (x < W && y < H) {
+            const uint64_t key = vis[(size_t)y * W + x];
+            if (key != VIS_EMPTY) {
+                dp = depth[(size_t)y * W + x];
+                if (x == 0 && y == 0) {
+                    need = true;
+                } else {
+                    const int g = (int)((uint32_t)key >> 3);
+                    const int o = s.tri_obj[g];
+                    const int mesh = s.obj_mesh[o];
+                    int mat = s.tri_mat[s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o])];
+                    if (mat >= 0) mat += s.obj_mat_base[o];
+                    need = mat >= 0 && mat < s.n_materials && s.mats[mat].ambient_occlusion;
+                }
+            }
+        }
+        s_need[lane] = need;
+        s_dp[lane] = dp;
+    }
+    __syncthreads();
+    const bool need = s_need[lane] != 0;
+    float m = 0.0f;
+    if (__ballot(need) != 0) {
+        const float dx = c.ao_dir[2 * wv], dy = c.ao_dir[2 * wv + 1];
+        const float X = (float)x, Y = (float)y, dp = s_dp[lane];
+        float maxr = -__builtin_inff();
+        bool nan = false;
+        // t = 0 is the pixel itself: distance 0 < 1, always skipped
+        for (int ti = 1; ti < 100; ti++) {
+            const float t = (float)ti;
+            const float cx = X + dx * t, cy = Y + dy * t;
+            const int ix = (int)cx, iy = (int)cy;  // |cx|, |cy| < 2^31: Go's int() truncation
+            const float ex = X - cx, ey = Y - cy;
+            const float dist = sqrt32(fma64(ex, ex, fma64(ey, ey, fma64(0.0f, 0.0f, 0.0f * 0.0f))));
+            const bool v = need && (unsigned)ix < (unsigned)W && (unsigned)iy < (unsigned)H && !(dist < 1);
+            const float d = depth[v ? (size_t)iy * W + ix : 0];
+            const float r = (d - dp) / dist;
+            if (v) {
+                if (r != r) nan = true;
+                else if (r > maxr) maxr = r;
+            }
+        }
+        m = nan ? __builtin_nanf("") : (maxr > 0 ? (float)atan64((double)maxr) : 0.0f);
+    }
+    s_m[wv][lane] = m;
+    __syncthreads();
+    if (wv == 0 && need) {
+        float total = 0.0f;
+        for (int k = 0; k < 8; k++) total += 1.57079632679489661923f - s_m[k][lane];
+        total /= 12.5663706143591729539f;
+        ao[(size_t)y * W + x] = (float)pow64((double)total, 10000.0);
+    }
+}
 
 // The colour of every uncovered pixel (render/raster.go:324-328 with
 // raster_screen.go:75-87): DrawFragment hands `shade` the zero fragment, whose
@@ -844,11 +1005,13 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, c
                        ts);
 }
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
-                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot, int32_t* tile_count,
-                       Counters* cnt, Capacities cap) {
+                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot, int32_t* clip_list,
+                       int32_t* tile_count, Counters* cnt, Capacities cap) {
     if (!s.n_tris) return;
-    hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, tv_nor, prims,
-                       clips, tri_clip_slot, tile_count, cnt, cap);
+    hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, prims,
+                       clip_list, tile_count, cnt, cap);
+    hipLaunchKernelGGL(k_clip, dim3(blocks_for(cap.clip_tris, 64)), dim3(64), 0, st, s, vc, tv_scr, tv_nor, clip_list,
+                       prims, clips, tri_clip_slot, tile_count, cnt, cap);
 }
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
                          int32_t* tile_count, Counters* cnt, Capacities cap) {
@@ -856,30 +1019,40 @@ void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& 
     hipLaunchKernelGGL(k_setup_shadow, dim3(blocks_for(s.n_tris, 256), sc.n_slots), dim3(256), 0, st, s, sc, ts, prims,
                        tile_count, cnt, cap);
 }
-void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* cursor, Counters* cnt) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, offsets, cursor, cnt);
+void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off,
+                 int32_t* cursor, Counters* cnt) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, offsets, chunk_off, cursor, cnt);
 }
 void launch_bin(hipStream_t st, const Prim* prims, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
-                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t max_prims) {
+                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed) {
+    const int max_prims = n_fixed + (shadow ? 0 : cap.clip_subs);
     if (max_prims <= 0) return;
     hipLaunchKernelGGL(k_bin, dim3(blocks_for(max_prims, 256)), dim3(256), 0, st, prims, cnt, tiles_x, tiles_per_view,
-                       shadow, cursor, list, cap, max_prims);
+                       shadow, cursor, list, cap, n_fixed);
 }
-void launch_raster_vis(hipStream_t st, bool nan_mode, const Prim* prims, const int32_t* tile_off,
-                       const int32_t* tile_list, Counters* cnt, int32_t W, int32_t H, int32_t tiles_x, int32_t n_tiles,
-                       uint64_t* vis, float* depth) {
-    if (nan_mode)
-        hipLaunchKernelGGL(k_raster_vis<true>, dim3(n_tiles), dim3(256), 0, st, prims, tile_off, tile_list, cnt, W, H,
-                           tiles_x, vis, depth);
+void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
+                   const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
+                   Counters* cnt, const RasterTarget& rt) {
+    hipLaunchKernelGGL(k_items, dim3(blocks_for(rt.n_tiles, 256)), dim3(256), 0, st, chunk_off, rt.n_tiles, item_tile,
+                       cnt, cap_items);
+    if (mode == MODE_SHADOW)
+        hipLaunchKernelGGL(k_raster<MODE_SHADOW>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
+                           item_tile, tile_list, cnt, rt, cap_items);
+    else if (mode == MODE_EXACT)
+        hipLaunchKernelGGL(k_raster<MODE_EXACT>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
+                           item_tile, tile_list, cnt, rt, cap_items);
     else
-        hipLaunchKernelGGL(k_raster_vis<false>, dim3(n_tiles), dim3(256), 0, st, prims, tile_off, tile_list, cnt, W, H,
-                           tiles_x, vis, depth);
+        hipLaunchKernelGGL(k_raster<MODE_VIS>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
+                           item_tile, tile_list, cnt, rt, cap_items);
 }
-void launch_raster_shadow(hipStream_t st, const Prim* prims, const int32_t* tile_off, const int32_t* tile_list,
-                          Counters* cnt, const ShadowConsts& sc, int32_t n_tiles_total, float* smaps) {
-    if (!n_tiles_total) return;
-    hipLaunchKernelGGL(k_raster_shadow, dim3(n_tiles_total), dim3(256), 0, st, prims, tile_off, tile_list, cnt, sc,
-                       smaps);
+void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, const Counters* cnt, float* depth) {
+    const size_t P = (size_t)rt.W * rt.H;
+    if (exact) hipLaunchKernelGGL(k_resolve<true>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth);
+    else hipLaunchKernelGGL(k_resolve<false>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth);
+}
+void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
+    const int nb = ((c.v.W + 7) / 8) * ((c.v.H + 7) / 8);
+    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(512), 0, st, s, c, b.vis, b.depth, b.ao, b.cnt);
 }
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     hipLaunchKernelGGL(k_shade_bg, dim3(1), dim3(64), 0, st, s, c, b);

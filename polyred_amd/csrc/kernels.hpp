@@ -25,7 +25,7 @@ struct ShadeBuffers {
     const float4* tv_wp;
     const ClipRec* clips;
     const int32_t* tri_clip_slot;
-    const float* smaps;        // [slots][SH][SW]
+    const uint32_t* smaps;     // [slots][SH][SW] ordered depth bits
     const gm::M4* light_view;  // [n_lights]
     const gm::M4* light_proj;  // [n_lights]
     uint32_t* image;           // [H][W] RGBA8, top row first
@@ -34,6 +34,7 @@ struct ShadeBuffers {
     uint32_t* bg;              // [2] uncovered colour, pixel (0,0)'s colour
     float* bg_lin;             // [3]
     const Counters* cnt;       // main-pass counters: nothing is shaded after an overflow
+    float* ao;                 // [H][W] AO factor of the pixels that need it
 };
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
@@ -43,19 +44,20 @@ void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, 
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
                           float4* ts);
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
-                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot,
+                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot, int32_t* clip_list,
                        int32_t* tile_count, Counters* cnt, Capacities cap);
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
                          Prim* prims, int32_t* tile_count, Counters* cnt, Capacities cap);
-void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* cursor,
-                 Counters* cnt);
+void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off,
+                 int32_t* cursor, Counters* cnt);
 void launch_bin(hipStream_t st, const Prim* prims, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
-                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t max_prims);
-void launch_raster_vis(hipStream_t st, bool nan_mode, const Prim* prims, const int32_t* tile_off,
-                       const int32_t* tile_list, Counters* cnt, int32_t W, int32_t H, int32_t tiles_x,
-                       int32_t n_tiles, uint64_t* vis, float* depth);
-void launch_raster_shadow(hipStream_t st, const Prim* prims, const int32_t* tile_off, const int32_t* tile_list,
-                          Counters* cnt, const ShadowConsts& sc, int32_t n_tiles_total, float* smaps);
+                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed);
+// mode: 0 visibility, 1 visibility in exact order (NaN rule), 2 shadow maps
+void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
+                   const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
+                   Counters* cnt, const RasterTarget& rt);
+void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, const Counters* cnt, float* depth);
+void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 

@@ -96,18 +96,33 @@ struct ClipRec {
 struct Counters {
     int32_t n_prims;
     int32_t n_clip_subs;
-    int32_t n_clipped;
-    int32_t n_pairs;
+    int32_t n_clipped;      // triangles queued for the clipper
+    int32_t n_pairs;        // (primitive, tile) pairs
     int32_t nan_flag;
     int32_t overflow;
     int32_t shadow_nan;
-    int32_t pad;
+    int32_t n_chunks;       // raster work items: (tile, up to ITEM_PRIMS primitives)
+    int32_t next_item;      // persistent raster work queue head
+    int32_t pad[7];
 };
 
 struct Capacities {
     int32_t prims;
     int32_t clip_subs;
     int32_t pairs;
+    int32_t clip_tris;
+};
+
+constexpr int CHUNK = 256;          // primitives staged in LDS at once
+constexpr int ITEM_PRIMS = 1024;    // primitives per raster work item
+
+// Where a raster pass writes.
+struct RasterTarget {
+    int32_t W, H, tiles_x, tiles_per_map, n_tiles;
+    unsigned long long* vis;  // main view: 64-bit depth|order keys
+    uint32_t* first_all;      // exact-order pass: first fragment per pixel
+    uint32_t* first_nan;      // exact-order pass: first NaN-depth fragment
+    uint32_t* smap_ord;       // shadow maps as ordered depth bits
 };
 
 __host__ __device__ inline uint32_t pack_bbox(int x, int y) { return (uint32_t)(x & 0xffff) | ((uint32_t)y << 16); }

@@ -229,12 +229,15 @@ struct prh_renderer {
     Prim* prims;
     ClipRec* clips;
     int32_t* tri_clip_slot;
-    int32_t *tile_count, *tile_off, *tile_cursor, *tile_list;
+    int32_t *tile_count, *tile_off, *chunk_off, *tile_cursor, *tile_list, *clip_list, *item_tile;
+    int32_t cap_items;
     int32_t tile_slots;
+    int32_t n_cu;
+    uint32_t *first_all, *first_nan;
     uint64_t* vis;
     float* depth;
-    uint32_t *image, *pregamma, *bg;
-    float *linear, *bg_lin, *smaps;
+    uint32_t *image, *pregamma, *bg, *smaps;
+    float *linear, *bg_lin, *ao;
     Counters* cnt;      // [2]: main, shadow
     Counters* h_cnt;    // pinned
     Capacities cap_main, cap_shadow;
@@ -261,10 +264,14 @@ int alloc_work(prh_renderer* r) {
     if ((rc = ralloc(r, &r->prims, (size_t)std::max(r->cap_main.prims, r->cap_shadow.prims)))) return rc;
     if ((rc = ralloc(r, &r->clips, (size_t)r->cap_main.clip_subs))) return rc;
     if ((rc = ralloc(r, &r->tile_list, (size_t)std::max(r->cap_main.pairs, r->cap_shadow.pairs)))) return rc;
+    if ((rc = ralloc(r, &r->clip_list, (size_t)r->cap_main.clip_tris))) return rc;
+    // every item holds >= 1 primitive of one tile: items <= tiles + pairs / ITEM_PRIMS
+    r->cap_items = r->tile_slots + std::max(r->cap_main.pairs, r->cap_shadow.pairs) / ITEM_PRIMS + 1;
+    if ((rc = ralloc(r, &r->item_tile, (size_t)r->cap_items))) return rc;
     return PRH_OK;
 }
 void free_work(prh_renderer* r) {
-    for (void* p : {(void*)r->prims, (void*)r->clips, (void*)r->tile_list}) {
+    for (void* p : {(void*)r->prims, (void*)r->clips, (void*)r->tile_list, (void*)r->clip_list, (void*)r->item_tile}) {
         auto it = std::find(r->allocs.begin(), r->allocs.end(), p);
         if (it != r->allocs.end()) {
             (void)hipFree(p);
@@ -548,10 +555,12 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
 
     const size_t P = (size_t)r->W * r->H;
     const int Tg = s->d.n_tris, Vt = s->d.n_tverts;
-    r->cap_main = Capacities{Tg + 7 * std::min(Tg, 1 << 16) + 16, 7 * std::min(Tg, 1 << 16) + 16,
-                             4 * Tg + 4 * r->n_tiles + 1024};
-    r->cap_shadow = Capacities{std::max(1, r->n_slots) * Tg + 16, 0,
-                               std::max(1, r->n_slots) * (2 * Tg + 4 * r->stiles_per_map) + 1024};
+    // primitive slots: one per triangle (per shadow slot), then the main
+    // view's clipped fan pieces
+    const int clip_tris = std::min(Tg, 1 << 16) + 16, clip0 = 7 * clip_tris;
+    r->cap_main = Capacities{Tg + clip0, clip0, 4 * Tg + 4 * r->n_tiles + 1024, clip_tris};
+    r->cap_shadow = Capacities{std::max(1, r->n_slots) * Tg, 0,
+                               std::max(1, r->n_slots) * (2 * Tg + 4 * r->stiles_per_map) + 1024, 0};
     r->tile_slots = std::max(r->n_tiles, r->n_slots * r->stiles_per_map) + 1;
     int rc = 0;
     auto chk = [&](int v) {
@@ -574,9 +583,14 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->tri_clip_slot, Tg));
     if (!rc) chk(ralloc(r, &r->tile_count, r->tile_slots));
     if (!rc) chk(ralloc(r, &r->tile_off, r->tile_slots + 1));
+    if (!rc) chk(ralloc(r, &r->chunk_off, r->tile_slots + 1));
+
+    if (!rc) chk(ralloc(r, &r->first_all, P));
+    if (!rc) chk(ralloc(r, &r->first_nan, P));
     if (!rc) chk(ralloc(r, &r->tile_cursor, r->tile_slots));
     if (!rc) chk(ralloc(r, &r->vis, P));
     if (!rc) chk(ralloc(r, &r->depth, P));
+    if (!rc) chk(ralloc(r, &r->ao, P));
     if (!rc) chk(ralloc(r, &r->image, P));
     if (!rc && o->keep_linear) chk(ralloc(r, &r->pregamma, P));
     if (!rc && o->keep_linear) chk(ralloc(r, &r->linear, 3 * P));
@@ -596,8 +610,15 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
         delete r;
         return rc;
     }
-    // Shadow maps start at zero once, here, as make([]float32) in initShadowMaps.
-    launch_fill_u32(ctx->stream, (uint32_t*)r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH, 0u);
+    // Shadow maps start at zero once, here, as make([]float32) in initShadowMaps
+    // (stored as ordered depth bits).
+    launch_fill_u32(ctx->stream, r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH, depth_ord(0.0f));
+    {
+        int n_cu = 0;
+        if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || n_cu <= 0)
+            n_cu = 256;
+        r->n_cu = n_cu;
+    }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     memset(&r->stats, 0, sizeof r->stats);
     s->live_renderers++;
@@ -622,12 +643,15 @@ int prh_renderer_free(prh_renderer* r) {
 
 namespace {
 
+int d_n_tris(const prh_renderer* r) { return r->sc->d.n_tris; }
+
 // Enqueues one frame; nan_mode re-runs the visibility raster in exact order.
 void enqueue_frame(prh_renderer* r, bool nan_mode) {
     hipStream_t st = r->ctx->stream;
     const DevScene& d = r->sc->d;
     Counters* cm = r->cnt;
     Counters* cs = r->cnt + 1;
+    const int persistent = r->n_cu * 4;
     (void)hipEventRecord(r->ev[0], st);
     launch_fill_u32(st, (uint32_t*)r->cnt, 2 * sizeof(Counters) / 4, 0u);
     if (r->n_slots > 0 && !nan_mode) {
@@ -635,23 +659,33 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
         launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
         launch_fill_u32(st, (uint32_t*)r->tile_count, nt, 0u);
         launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->tile_count, cs, r->cap_shadow);
-        launch_scan(st, r->tile_count, nt, r->tile_off, r->tile_cursor, cs);
+        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, cs);
         launch_bin(st, r->prims, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
-                   r->cap_shadow.prims);
-        launch_raster_shadow(st, r->prims, r->tile_off, r->tile_list, cs, r->shc, nt, r->smaps);
+                   r->n_slots * d.n_tris);
+        RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
+        launch_raster(st, 2, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
+                      r->tile_list, cs, rt);
     }
     (void)hipEventRecord(r->ev[1], st);
     launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
     (void)hipEventRecord(r->ev[2], st);
     launch_fill_u32(st, (uint32_t*)r->tile_count, r->n_tiles, 0u);
-    launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->clips, r->tri_clip_slot, r->tile_count, cm,
-                      r->cap_main);
-    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->tile_cursor, cm);
-    launch_bin(st, r->prims, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main,
-               r->cap_main.prims);
+    launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->clips, r->tri_clip_slot, r->clip_list,
+                      r->tile_count, cm, r->cap_main);
+    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, cm);
+    launch_bin(st, r->prims, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris);
     (void)hipEventRecord(r->ev[3], st);
-    launch_raster_vis(st, nan_mode, r->prims, r->tile_off, r->tile_list, cm, r->W, r->H, r->tiles_x, r->n_tiles,
-                      r->vis, r->depth);
+    const size_t P = (size_t)r->W * r->H;
+    launch_fill_u64(st, r->vis, P, VIS_EMPTY);
+    if (nan_mode) {
+        launch_fill_u32(st, r->first_all, P, 0xffffffffu);
+        launch_fill_u32(st, r->first_nan, P, 0xffffffffu);
+    }
+    RasterTarget rt{r->W, r->H, r->tiles_x, r->n_tiles, r->n_tiles, (unsigned long long*)r->vis, r->first_all,
+                    r->first_nan, nullptr};
+    launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
+                  r->tile_list, cm, rt);
+    launch_resolve(st, nan_mode, rt, cm, r->depth);
     (void)hipEventRecord(r->ev[4], st);
     ShadeBuffers b;
     b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
@@ -659,8 +693,10 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     b.light_view = r->d_lview, b.light_proj = r->d_lproj;
     b.image = r->image, b.pregamma = r->pregamma, b.linear = r->linear, b.bg = r->bg, b.bg_lin = r->bg_lin;
     b.cnt = cm;
-    launch_shade_bg(st, d, r->sh, b);
+    b.ao = r->ao;
+    launch_ao(st, d, r->sh, b);
     (void)hipEventRecord(r->ev[5], st);
+    launch_shade_bg(st, d, r->sh, b);
     launch_shade(st, d, r->sh, b);
     (void)hipEventRecord(r->ev[6], st);
     (void)hipMemcpyAsync(r->h_cnt, r->cnt, 2 * sizeof(Counters), hipMemcpyDeviceToHost, st);
@@ -681,17 +717,17 @@ int prh_render(prh_renderer* r) {
         HIP_TRY(hipGetLastError());
         const Counters cm = r->h_cnt[0], cs = r->h_cnt[1];
         bool grow = false;
-        if (cs.overflow || cs.n_prims > r->cap_shadow.prims || cs.n_pairs > r->cap_shadow.pairs) {
-            r->cap_shadow.prims = std::max(r->cap_shadow.prims, cs.n_prims + cs.n_prims / 4 + 16);
+        if (cs.overflow || cs.n_pairs > r->cap_shadow.pairs) {
             r->cap_shadow.pairs = std::max(r->cap_shadow.pairs, cs.n_pairs + cs.n_pairs / 4 + 16);
             grow = true;
         }
-        if (cm.overflow || cm.n_prims > r->cap_main.prims || cm.n_pairs > r->cap_main.pairs ||
-            cm.n_clip_subs > r->cap_main.clip_subs) {
-            r->cap_main.prims = std::max(r->cap_main.prims, cm.n_prims + cm.n_prims / 4 + 16);
+        if (cm.overflow || cm.n_pairs > r->cap_main.pairs || cm.n_clip_subs > r->cap_main.clip_subs ||
+            cm.n_clipped > r->cap_main.clip_tris) {
             r->cap_main.pairs = std::max(r->cap_main.pairs, cm.n_pairs + cm.n_pairs / 4 + 16);
             r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, cm.n_clip_subs + cm.n_clip_subs / 4 + 16);
-            r->cap_main.prims = std::max(r->cap_main.prims, cm.n_prims + 7 * r->cap_main.clip_subs);
+            r->cap_main.clip_tris = std::max(r->cap_main.clip_tris, cm.n_clipped + cm.n_clipped / 4 + 16);
+            r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, 7 * r->cap_main.clip_tris);
+            r->cap_main.prims = d_n_tris(r) + r->cap_main.clip_subs;
             grow = true;
         }
         if (grow) {
@@ -712,7 +748,7 @@ int prh_render(prh_renderer* r) {
         float ms[6] = {0};
         for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
         (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
-        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_shade_bg = ms[4],
+        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4],
         S.ms_shade = ms[5];
         S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
         S.nan_rerun = nan_rerun ? 1 : 0;
@@ -742,7 +778,19 @@ int prh_read_buffer(prh_renderer* r, int32_t which, void* dst, size_t bytes) {
             if (!r->linear) return fail(PRH_E_INVALID, "renderer was created without keep_linear");
             src = r->linear, need = 12 * P;
             break;
-        case PRH_BUF_SHADOW: src = r->smaps, need = (size_t)4 * r->n_slots * r->SW * r->SH; break;
+        case PRH_BUF_SHADOW: {
+            need = (size_t)4 * r->n_slots * r->SW * r->SH;
+            if (bytes < need) return fail(PRH_E_INVALID, "prh_read_buffer: buffer too small");
+            HIP_TRY(hipSetDevice(r->ctx->device));
+            if (need) HIP_TRY(hipMemcpyAsync(dst, r->smaps, need, hipMemcpyDeviceToHost, r->ctx->stream));
+            HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+            uint32_t* u = (uint32_t*)dst;
+            for (size_t i = 0; i < need / 4; i++) {  // ordered depth bits -> float32
+                float f = depth_from_ord(u[i]);
+                memcpy(&u[i], &f, 4);
+            }
+            return PRH_OK;
+        }
         case PRH_BUF_PREGAMMA:
             if (!r->pregamma) return fail(PRH_E_INVALID, "renderer was created without keep_linear");
             src = r->pregamma, need = 4 * P;
