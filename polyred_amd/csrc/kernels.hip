@@ -806,6 +806,19 @@ __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuff
     return occ;
 }
 
+// Correctly rounded float32 sqrt for finite x >= 2^-96: the hardware root
+// (within 1 ulp) corrected by the sign of the residuals of its neighbours,
+// the expansion the compiler emits for sqrtf minus its small-input scaling
+// and special-class handling, which these inputs never need.
+__device__ __forceinline__ float sqrt32_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u), up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rdn = __builtin_fmaf(-dn, s, x), rup = __builtin_fmaf(-up, s, x);
+    float r = rdn <= 0.0f ? dn : s;
+    r = rup > 0.0f ? up : r;
+    return r;
+}
+
 // AmbientOcclusionShade's last step (material/ao.go:34-37): the fragment
 // colour scaled by the AO factor k_ao computed for its pixel.
 __device__ __forceinline__ uint32_t ao_apply(uint32_t col, float total) {
@@ -890,23 +903,55 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
     if (__ballot(need) != 0) {
         const float dx = c.ao_dir[2 * wv], dy = c.ao_dir[2 * wv + 1];
         const float X = (float)x, Y = (float)y, dp = s_dp[lane];
-        float maxr = -__builtin_inff();
+        // the tap with the largest exact elevation/distance quotient, tracked
+        // with exact float64 cross products (e * d_best > e_best * d: both
+        // products of two float32 are exact); its rounded quotient is the
+        // largest rounded quotient
+        double e_best = -__builtin_inf(), d_best = 1.0;
         bool nan = false;
-        // t = 0 is the pixel itself: distance 0 < 1, always skipped
-        for (int ti = 1; ti < 100; ti++) {
-            const float t = (float)ti;
-            const float cx = X + dx * t, cy = Y + dy * t;
-            const int ix = (int)cx, iy = (int)cy;  // |cx|, |cy| < 2^31: Go's int() truncation
-            const float ex = X - cx, ey = Y - cy;
-            const float dist = sqrt32(fma64(ex, ex, fma64(ey, ey, fma64(0.0f, 0.0f, 0.0f * 0.0f))));
-            const bool v = need && (unsigned)ix < (unsigned)W && (unsigned)iy < (unsigned)H && !(dist < 1);
-            const float d = depth[v ? (size_t)iy * W + ix : 0];
-            const float r = (d - dp) / dist;
-            if (v) {
-                if (r != r) nan = true;
-                else if (r > maxr) maxr = r;
+        float t = 1.0f;  // t = 0 is the pixel itself: distance 0 < 1, skipped
+        if ((wv & 1) == 0 && W <= 4096 && H <= 4096) {
+            // Axis directions (a = 0, pi/2, pi, 3pi/2 in float32): one offset
+            // is exactly +-t, the other the tiny cos/sin residue (<= 8.8e-8 t)
+            // rounded onto the coordinate grid, under 2e-5 for coordinates
+            // below 4096.  Its square (< 4e-10) is under half an ulp of t^2
+            // (>= 6e-8), so Len = sqrt(t^2) = t exactly (also checked over
+            // every coordinate pair below 4096 and every t).
+#pragma unroll 3
+            for (int ti = 1; ti < 100; ti++, t += 1.0f) {
+                const float cx = X + dx * t, cy = Y + dy * t;
+                const int ix = (int)cx, iy = (int)cy;
+                const bool v = need && (unsigned)ix < (unsigned)W && (unsigned)iy < (unsigned)H;
+                const uint32_t off = v ? ((unsigned)iy * (unsigned)W + (unsigned)ix) * 4u : 0u;
+                const float e = *(const float*)((const char*)depth + off) - dp;
+                const double e64 = (double)e, d64 = (double)t;
+                const bool better = v && e64 * d_best > e_best * d64;
+                e_best = better ? e64 : e_best;
+                d_best = better ? d64 : d_best;
+                nan |= v && e != e;
+            }
+        } else {
+#pragma unroll 3
+            for (int ti = 1; ti < 100; ti++, t += 1.0f) {
+                const float cx = X + dx * t, cy = Y + dy * t;
+                const int ix = (int)cx, iy = (int)cy;  // |cx|, |cy| < 2^31: Go's int() truncation
+                const float ex = X - cx, ey = Y - cy;
+                // Len of (ex, ey, 0, 0): FMA(ex,ex, FMA(ey,ey, FMA(0,0,0*0))) where
+                // the inner FMA(ey,ey,+0) is exactly the float32 product ey*ey
+                const float dist = sqrt32_normal(fma64(ex, ex, ey * ey));
+                const bool v = need && (unsigned)ix < (unsigned)W && (unsigned)iy < (unsigned)H && dist >= 1.0f;
+                // 32-bit byte offset (the frame is < 2^30 pixels) so the load can
+                // take a scalar base plus a 32-bit vector offset
+                const uint32_t off = v ? ((unsigned)iy * (unsigned)W + (unsigned)ix) * 4u : 0u;
+                const float e = *(const float*)((const char*)depth + off) - dp;
+                const double e64 = (double)e, d64 = (double)dist;
+                const bool better = v && e64 * d_best > e_best * d64;
+                e_best = better ? e64 : e_best;
+                d_best = better ? d64 : d_best;
+                nan |= v && e != e;
             }
         }
+        const float maxr = (float)e_best / (float)d_best;
         m = nan ? __builtin_nanf("") : (maxr > 0 ? (float)atan64((double)maxr) : 0.0f);
     }
     s_m[wv][lane] = m;
