@@ -81,6 +81,9 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced scene for a quick check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run 3 elementwise copies of MIB MiB (torch.neg), the known-size kernel that "
+                         "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build renders on one GPU; --gpus {args.gpus} is not supported "
@@ -88,6 +91,17 @@ def main() -> int:
         return 2
 
     from polyred_amd import abi, render, scenes
+
+    if args.copy_probe > 0:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)  # an elementwise kernel: reads and writes MIB MiB
+        torch.cuda.synchronize()
+        del src, dst
 
     if args.small:
         scene = scenes.c2_scale(width=480, height=270, n_spheres=200, grid=50, shadow_size=512)

@@ -187,7 +187,8 @@ typedef struct prh_frame_stats {
     float ms_raster;     /* visibility raster                               */
     float ms_shade;      /* deferred shade + shadow lookups + gamma          */
     float ms_ao;         /* screen-space ambient occlusion                  */
-    int32_t n_prims;     /* primitives after cull/clip                      */
+    int32_t n_prims;     /* primitives after cull/clip: unclipped triangles
+                            plus the clipper's pieces                       */
     int32_t n_clipped;   /* triangles that went through the clipper         */
     int32_t n_bin_pairs; /* (primitive, tile) pairs                         */
     int32_t nan_rerun;   /* 1 if the NaN-depth exact-order pass ran         */

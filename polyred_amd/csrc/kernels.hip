@@ -231,7 +231,8 @@ __global__ __launch_bounds__(256) void k_vertex_shadow(DevScene s, const M4* __r
 // primitive is counted into every 16x16 tile its pixel box touches.
 __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
                                                     Prim* __restrict__ prims, int32_t* __restrict__ clip_list,
-                                                    int32_t* __restrict__ tile_count, Counters* cnt, Capacities cap) {
+                                                    int32_t* __restrict__ tile_count, int32_t* __restrict__ blk_prims,
+                                                    Counters* cnt, Capacities cap) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     bool direct = false, doclip = false;
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
@@ -266,8 +267,10 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
         if (pos < cap.clip_tris) clip_list[pos] = g;
         else atomicOr(&cnt->overflow, 1);
     }
+    // the primitive count of the frame statistics, summed by k_scan (one
+    // global counter hit by every workgroup would serialise them)
     const int nd = __syncthreads_count(direct);
-    if (threadIdx.x == 0 && nd) atomicAdd(&cnt->n_prims, nd);
+    if (threadIdx.x == 0) blk_prims[blockIdx.x] = nd;
 }
 
 // The clipper (clipTriangle, clipping.go:67-157) for the queued triangles:
@@ -335,7 +338,6 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
             store_empty(prims, (size_t)s.n_tris + base + i - 2);
             continue;
         }
-        atomicAdd(&cnt->n_prims, 1);
         const size_t ps = (size_t)s.n_tris + base + i - 2;
         store_prim(prims, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)(i - 2), bx0, by0, bx1, by1);
         for (int ty = by0 / TILE; ty <= by1 / TILE; ty++)
@@ -375,8 +377,6 @@ __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts s
         else store_empty(prims, ps);
     }
     count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, emit, x0, y0, x1, y1);
-    const int nd = __syncthreads_count(emit);
-    if (threadIdx.x == 0 && nd) atomicAdd(&cnt->n_prims, nd);
 }
 
 // ------------------------------------------------------------ tile scan
@@ -393,7 +393,8 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 }
 __global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ counts, int32_t n,
                                                int32_t* __restrict__ offsets, int32_t* __restrict__ chunk_off,
-                                               int32_t* __restrict__ cursor, Counters* cnt) {
+                                               int32_t* __restrict__ cursor, const int32_t* __restrict__ blk_prims,
+                                               int32_t n_blk, int32_t max_clip_subs, Counters* cnt) {
     __shared__ int32_t wtot[16], wch[16];
     __shared__ int32_t carry_s, carry_c;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -425,11 +426,21 @@ __global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ count
         if (tid == 1023) carry_s = before + incl, carry_c = beforec + inclc;
         __syncthreads();
     }
+    // statistics: unclipped primitives (per-workgroup counts of the setup)
+    // plus the clipper's pieces
+    int np = 0;
+    for (int i = tid; i < n_blk; i += 1024) np += blk_prims[i];
+    for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
+    if (lane == 0) wtot[wv] = np;
+    __syncthreads();
     if (tid == 0) {
         offsets[n] = carry_s;
         chunk_off[n] = carry_c;
         cnt->n_pairs = carry_s;
         cnt->n_chunks = carry_c;
+        int tot = min(cnt->n_clip_subs, max_clip_subs);
+        for (int k = 0; k < 16; k++) tot += wtot[k];
+        cnt->n_prims = tot;
     }
 }
 
@@ -1051,10 +1062,10 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, c
 }
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot, int32_t* clip_list,
-                       int32_t* tile_count, Counters* cnt, Capacities cap) {
+                       int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap) {
     if (!s.n_tris) return;
     hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, prims,
-                       clip_list, tile_count, cnt, cap);
+                       clip_list, tile_count, blk_prims, cnt, cap);
     hipLaunchKernelGGL(k_clip, dim3(blocks_for(cap.clip_tris, 64)), dim3(64), 0, st, s, vc, tv_scr, tv_nor, clip_list,
                        prims, clips, tri_clip_slot, tile_count, cnt, cap);
 }
@@ -1065,8 +1076,9 @@ void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& 
                        tile_count, cnt, cap);
 }
 void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off,
-                 int32_t* cursor, Counters* cnt) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, offsets, chunk_off, cursor, cnt);
+                 int32_t* cursor, const int32_t* blk_prims, int32_t n_blk, int32_t max_clip_subs, Counters* cnt) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, offsets, chunk_off, cursor, blk_prims, n_blk,
+                       max_clip_subs, cnt);
 }
 void launch_bin(hipStream_t st, const Prim* prims, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed) {

@@ -229,6 +229,7 @@ struct prh_renderer {
     Prim* prims;
     ClipRec* clips;
     int32_t* tri_clip_slot;
+    int32_t* blk_prims;  // per setup workgroup: unclipped primitives (statistics)
     int32_t *tile_count, *tile_off, *chunk_off, *tile_cursor, *tile_list, *clip_list, *item_tile;
     int32_t cap_items;
     int32_t tile_slots;
@@ -581,6 +582,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->tv_wp, Vt));
     if (!rc) chk(ralloc(r, &r->ts, (size_t)std::max(1, r->n_slots) * Vt));
     if (!rc) chk(ralloc(r, &r->tri_clip_slot, Tg));
+    if (!rc) chk(ralloc(r, &r->blk_prims, (size_t)(Tg + 255) / 256));
     if (!rc) chk(ralloc(r, &r->tile_count, r->tile_slots));
     if (!rc) chk(ralloc(r, &r->tile_off, r->tile_slots + 1));
     if (!rc) chk(ralloc(r, &r->chunk_off, r->tile_slots + 1));
@@ -659,7 +661,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
         launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
         launch_fill_u32(st, (uint32_t*)r->tile_count, nt, 0u);
         launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->tile_count, cs, r->cap_shadow);
-        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, cs);
+        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, nullptr, 0, 0, cs);
         launch_bin(st, r->prims, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
                    r->n_slots * d.n_tris);
         RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
@@ -671,8 +673,9 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     (void)hipEventRecord(r->ev[2], st);
     launch_fill_u32(st, (uint32_t*)r->tile_count, r->n_tiles, 0u);
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->clips, r->tri_clip_slot, r->clip_list,
-                      r->tile_count, cm, r->cap_main);
-    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, cm);
+                      r->tile_count, r->blk_prims, cm, r->cap_main);
+    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->blk_prims,
+                (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
     launch_bin(st, r->prims, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris);
     (void)hipEventRecord(r->ev[3], st);
     const size_t P = (size_t)r->W * r->H;
