@@ -122,16 +122,19 @@ __device__ __forceinline__ void count_tiles(int32_t* tile_count, int base, int t
 
 constexpr uint32_t PRIM_EMPTY = 0xffffffffu;  // bbox word of an unused primitive slot
 
-__device__ __forceinline__ void store_empty(Prim* prims, size_t slot) {
-    prims[slot].p2 = make_float4(0.0f, 0.0f, __uint_as_float(PRIM_EMPTY), __uint_as_float(PRIM_EMPTY));
+// A primitive slot is its 48-byte record plus its pixel box in a separate
+// 8-byte array, which is all k_bin reads; an empty slot only marks its box.
+__device__ __forceinline__ void store_empty(uint2* pbox, size_t slot) {
+    pbox[slot] = make_uint2(PRIM_EMPTY, PRIM_EMPTY);
 }
-__device__ __forceinline__ void store_prim(Prim* prims, size_t slot, V4 a, V4 b, V4 c, uint32_t key, int x0, int y0,
-                                           int x1, int y1) {
+__device__ __forceinline__ void store_prim(Prim* prims, uint2* pbox, size_t slot, V4 a, V4 b, V4 c, uint32_t key,
+                                           int x0, int y0, int x1, int y1) {
     Prim p;
     p.p0 = make_float4(a.x, a.y, a.z, b.x);
     p.p1 = make_float4(b.y, b.z, c.x, c.y);
     p.p2 = make_float4(c.z, bitsf(key), bitsf(pack_bbox(x0, y0)), bitsf(pack_bbox(x1, y1)));
     prims[slot] = p;
+    pbox[slot] = make_uint2(pack_bbox(x0, y0), pack_bbox(x1, y1));
 }
 
 // Sutherland-Hodgman against the six screen-space planes, clipping.go:31-65.
@@ -230,7 +233,8 @@ __global__ __launch_bounds__(256) void k_vertex_shadow(DevScene s, const M4* __r
 // slot (all vertices inside) or a place in the clipper's queue.  Each
 // primitive is counted into every 16x16 tile its pixel box touches.
 __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
-                                                    Prim* __restrict__ prims, int32_t* __restrict__ clip_list,
+                                                    Prim* __restrict__ prims, uint2* __restrict__ pbox,
+                                                    int32_t* __restrict__ clip_list,
                                                     int32_t* __restrict__ tile_count, int32_t* __restrict__ blk_prims,
                                                     Counters* cnt, Capacities cap) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,8 +262,8 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
                 }
             }
         }
-        if (direct) store_prim(prims, g, p1, p2, p3, (uint32_t)g * 8u + SUB_DIRECT, x0, y0, x1, y1);
-        else store_empty(prims, g);
+        if (direct) store_prim(prims, pbox, g, p1, p2, p3, (uint32_t)g * 8u + SUB_DIRECT, x0, y0, x1, y1);
+        else store_empty(pbox, g);
     }
     count_tiles(tile_count, 0, vc.tiles_x, direct, x0, y0, x1, y1);
     const int pos = wave_alloc(&cnt->n_clipped, 0, doclip);
@@ -279,7 +283,8 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
 // (raster.go:440-443), appended after the per-triangle primitive slots.
 __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
                                              const float4* __restrict__ tv_nor, const int32_t* __restrict__ clip_list,
-                                             Prim* __restrict__ prims, ClipRec* __restrict__ clips,
+                                             Prim* __restrict__ prims, uint2* __restrict__ pbox,
+                                             ClipRec* __restrict__ clips,
                                              int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
                                              Counters* cnt, Capacities cap) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -335,11 +340,11 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
         int bx0, by0, bx1, by1;
         if (!pixel_box(sv[0].x, sv[0].y, sv[0].z, sv[1].x, sv[1].y, sv[1].z, sv[2].x, sv[2].y, sv[2].z, vc.W, vc.H,
                        bx0, by0, bx1, by1)) {
-            store_empty(prims, (size_t)s.n_tris + base + i - 2);
+            store_empty(pbox, (size_t)s.n_tris + base + i - 2);
             continue;
         }
         const size_t ps = (size_t)s.n_tris + base + i - 2;
-        store_prim(prims, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)(i - 2), bx0, by0, bx1, by1);
+        store_prim(prims, pbox, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)(i - 2), bx0, by0, bx1, by1);
         for (int ty = by0 / TILE; ty <= by1 / TILE; ty++)
             for (int tx = bx0 / TILE; tx <= bx1 / TILE; tx++) atomicAdd(&tile_count[ty * vc.tiles_x + tx], 1);
     }
@@ -348,7 +353,8 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
 // render/shadow.go:152-176 (drawDepth up to the pixel loop) for every
 // (triangle, shadow slot): back-face cull, cullViewFrustum, pixel box.
 __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts sc, const float4* __restrict__ ts,
-                                                      Prim* __restrict__ prims, int32_t* __restrict__ tile_count,
+                                                      Prim* __restrict__ prims, uint2* __restrict__ pbox,
+                                                      int32_t* __restrict__ tile_count,
                                                       Counters* cnt, Capacities cap) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
@@ -373,17 +379,20 @@ __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts s
     }
     if (g < s.n_tris) {
         const size_t ps = (size_t)slot * s.n_tris + g;
-        if (emit) store_prim(prims, ps, p1, p2, p3, (uint32_t)slot, x0, y0, x1, y1);
-        else store_empty(prims, ps);
+        if (emit) store_prim(prims, pbox, ps, p1, p2, p3, (uint32_t)slot, x0, y0, x1, y1);
+        else store_empty(pbox, ps);
     }
     count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, emit, x0, y0, x1, y1);
 }
 
 // ------------------------------------------------------------ tile scan
-// One workgroup: exclusive prefix sums of the per-tile primitive counts and
-// of the per-tile work items (ceil(count / ITEM_PRIMS)), 4096 tiles per step (four
-// consecutive tiles per thread, wave scans, then the 16 wave totals),
-// carrying the running totals across steps.
+// Exclusive prefix sums of the per-tile primitive counts and of the per-tile
+// work items (ceil(count / ITEM_PRIMS)) in two launches over blocks of
+// SCAN_BLOCK tiles: k_scan_sums reduces each block, k_scan_apply adds the
+// sums of the blocks before it and scans its own block (four consecutive
+// tiles per thread, wave scans, then the 16 wave totals).
+constexpr int SCAN_BLOCK = 4096;
+
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     for (int off = 1; off < 64; off <<= 1) {
         int u = __shfl_up(v, off);
@@ -391,65 +400,105 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     }
     return v;
 }
-__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ counts, int32_t n,
-                                               int32_t* __restrict__ offsets, int32_t* __restrict__ chunk_off,
-                                               int32_t* __restrict__ cursor, const int32_t* __restrict__ blk_prims,
-                                               int32_t n_blk, int32_t max_clip_subs, Counters* cnt) {
-    __shared__ int32_t wtot[16], wch[16];
-    __shared__ int32_t carry_s, carry_c;
+__device__ __forceinline__ int wave_sum(int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ int item_count(int v) { return (v + ITEM_PRIMS - 1) / ITEM_PRIMS; }
+
+__global__ __launch_bounds__(1024) void k_scan_sums(const int32_t* __restrict__ counts, int32_t n,
+                                                    int32_t* __restrict__ bsum) {
+    __shared__ int32_t ws[16], wc[16];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid == 0) carry_s = 0, carry_c = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 4096) {
-        const int i0 = base + 4 * tid;
-        int v[4], ch[4];
-        for (int k = 0; k < 4; k++) {
-            v[k] = i0 + k < n ? counts[i0 + k] : 0;
-            ch[k] = (v[k] + ITEM_PRIMS - 1) / ITEM_PRIMS;
-        }
-        const int local = v[0] + v[1] + v[2] + v[3], localc = ch[0] + ch[1] + ch[2] + ch[3];
-        const int incl = wave_incl_scan(local, lane), inclc = wave_incl_scan(localc, lane);
-        if (lane == 63) wtot[wv] = incl, wch[wv] = inclc;
-        __syncthreads();
-        int before = carry_s, beforec = carry_c;
-        for (int k = 0; k < wv; k++) before += wtot[k], beforec += wch[k];
-        int run = before + incl - local, runc = beforec + inclc - localc;
-        for (int k = 0; k < 4; k++)
-            if (i0 + k < n) {
-                offsets[i0 + k] = run;
-                cursor[i0 + k] = run;
-                chunk_off[i0 + k] = runc;
-                run += v[k];
-                runc += ch[k];
-            }
-        __syncthreads();
-        if (tid == 1023) carry_s = before + incl, carry_c = beforec + inclc;
-        __syncthreads();
+    const int i0 = blockIdx.x * SCAN_BLOCK + 4 * tid;
+    int sv = 0, sc = 0;
+    for (int k = 0; k < 4; k++) {
+        const int v = i0 + k < n ? counts[i0 + k] : 0;
+        sv += v;
+        sc += item_count(v);
     }
-    // statistics: unclipped primitives (per-workgroup counts of the setup)
-    // plus the clipper's pieces
-    int np = 0;
-    for (int i = tid; i < n_blk; i += 1024) np += blk_prims[i];
-    for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
-    if (lane == 0) wtot[wv] = np;
+    sv = wave_sum(sv), sc = wave_sum(sc);
+    if (lane == 0) ws[wv] = sv, wc[wv] = sc;
     __syncthreads();
     if (tid == 0) {
-        offsets[n] = carry_s;
-        chunk_off[n] = carry_c;
-        cnt->n_pairs = carry_s;
-        cnt->n_chunks = carry_c;
-        int tot = min(cnt->n_clip_subs, max_clip_subs);
-        for (int k = 0; k < 16; k++) tot += wtot[k];
-        cnt->n_prims = tot;
+        int a = 0, c = 0;
+        for (int k = 0; k < 16; k++) a += ws[k], c += wc[k];
+        bsum[2 * blockIdx.x] = a;
+        bsum[2 * blockIdx.x + 1] = c;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(const int32_t* __restrict__ counts, int32_t n,
+                                                     const int32_t* __restrict__ bsum, int32_t* __restrict__ offsets,
+                                                     int32_t* __restrict__ chunk_off, int32_t* __restrict__ cursor,
+                                                     const int32_t* __restrict__ blk_prims, int32_t n_blk,
+                                                     int32_t max_clip_subs, Counters* cnt) {
+    __shared__ int32_t ws[16], wc[16];
+    __shared__ int32_t carry_s, carry_c;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int b = blockIdx.x;
+    // sums of the blocks before this one
+    int ps = 0, pc = 0;
+    for (int k = tid; k < b; k += 1024) ps += bsum[2 * k], pc += bsum[2 * k + 1];
+    ps = wave_sum(ps), pc = wave_sum(pc);
+    if (lane == 0) ws[wv] = ps, wc[wv] = pc;
+    __syncthreads();
+    if (tid == 0) {
+        int a = 0, c = 0;
+        for (int k = 0; k < 16; k++) a += ws[k], c += wc[k];
+        carry_s = a, carry_c = c;
+    }
+    __syncthreads();
+    const int i0 = b * SCAN_BLOCK + 4 * tid;
+    int v[4], ch[4];
+    for (int k = 0; k < 4; k++) {
+        v[k] = i0 + k < n ? counts[i0 + k] : 0;
+        ch[k] = item_count(v[k]);
+    }
+    const int local = v[0] + v[1] + v[2] + v[3], localc = ch[0] + ch[1] + ch[2] + ch[3];
+    const int incl = wave_incl_scan(local, lane), inclc = wave_incl_scan(localc, lane);
+    if (lane == 63) ws[wv] = incl, wc[wv] = inclc;
+    __syncthreads();
+    int before = carry_s, beforec = carry_c;
+    for (int k = 0; k < wv; k++) before += ws[k], beforec += wc[k];
+    int run = before + incl - local, runc = beforec + inclc - localc;
+    for (int k = 0; k < 4; k++)
+        if (i0 + k < n) {
+            offsets[i0 + k] = run;
+            cursor[i0 + k] = run;
+            chunk_off[i0 + k] = runc;
+            run += v[k];
+            runc += ch[k];
+        }
+    if (b != gridDim.x - 1) return;
+    // the last block writes the totals and the frame statistics: unclipped
+    // primitives (per-workgroup counts of the setup) plus the clipper's pieces
+    if (tid == 1023) {
+        offsets[n] = run;
+        chunk_off[n] = runc;
+        cnt->n_pairs = run;
+        cnt->n_chunks = runc;
+    }
+    int np = 0;
+    for (int k = tid; k < n_blk; k += 1024) np += blk_prims[k];
+    np = wave_sum(np);
+    __syncthreads();  // every wave has read ws[] above
+    if (lane == 0) ws[wv] = np;
+    __syncthreads();
+    if (tid == 0) {
+        int a = 0;
+        for (int k = 0; k < 16; k++) a += ws[k];
+        cnt->n_prims = a + min(cnt->n_clip_subs, max_clip_subs);
     }
 }
 
 // Scatter each primitive into the lists of the tiles its pixel box touches.
 // Slots [0, n_fixed) are one per triangle (per shadow slot); the main view's
 // clipped pieces follow them.
-__global__ __launch_bounds__(256) void k_bin(const Prim* __restrict__ prims, Counters* cnt, int32_t tiles_x,
+__global__ __launch_bounds__(256) void k_bin(const uint2* __restrict__ pbox, Counters* cnt, int32_t tiles_x,
                                              int32_t tiles_per_view, int32_t shadow, int32_t* __restrict__ cursor,
-                                             int32_t* __restrict__ list, Capacities cap, int32_t n_fixed) {
+                                             int32_t* __restrict__ list, Capacities cap, int32_t n_fixed,
+                                             int32_t n_tris) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (cnt->n_pairs > cap.pairs) {
         if (i == 0) atomicOr(&cnt->overflow, 1);
@@ -460,12 +509,11 @@ __global__ __launch_bounds__(256) void k_bin(const Prim* __restrict__ prims, Cou
     bool active = false;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0, base = 0;
     if (i < np) {
-        const float4 p2 = prims[i].p2;
-        const uint32_t lo = fbits(p2.z), hi = fbits(p2.w);
-        if (hi != PRIM_EMPTY) {
+        const uint2 bb = pbox[i];
+        if (bb.y != PRIM_EMPTY) {
             active = true;
-            x0 = lo & 0xffff, y0 = lo >> 16, x1 = hi & 0xffff, y1 = hi >> 16;
-            base = shadow ? (int)fbits(p2.y) * tiles_per_view : 0;
+            x0 = bb.x & 0xffff, y0 = bb.x >> 16, x1 = bb.y & 0xffff, y1 = bb.y >> 16;
+            base = shadow ? (i / n_tris) * tiles_per_view : 0;  // shadow slot s holds [s*Tg, (s+1)*Tg)
         }
     }
     const int tx0 = x0 / TILE, ty0 = y0 / TILE, ntx = active ? x1 / TILE - tx0 + 1 : 0;
@@ -1061,31 +1109,35 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, c
                        ts);
 }
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
-                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot, int32_t* clip_list,
-                       int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap) {
+                       const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
+                       int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap) {
     if (!s.n_tris) return;
-    hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, prims,
+    hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, prims, pbox,
                        clip_list, tile_count, blk_prims, cnt, cap);
     hipLaunchKernelGGL(k_clip, dim3(blocks_for(cap.clip_tris, 64)), dim3(64), 0, st, s, vc, tv_scr, tv_nor, clip_list,
-                       prims, clips, tri_clip_slot, tile_count, cnt, cap);
+                       prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
 }
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
-                         int32_t* tile_count, Counters* cnt, Capacities cap) {
+                         uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap) {
     if (!s.n_tris || !sc.n_slots) return;
     hipLaunchKernelGGL(k_setup_shadow, dim3(blocks_for(s.n_tris, 256), sc.n_slots), dim3(256), 0, st, s, sc, ts, prims,
-                       tile_count, cnt, cap);
+                       pbox, tile_count, cnt, cap);
 }
+int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK); }
 void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off,
-                 int32_t* cursor, const int32_t* blk_prims, int32_t n_blk, int32_t max_clip_subs, Counters* cnt) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, counts, n, offsets, chunk_off, cursor, blk_prims, n_blk,
-                       max_clip_subs, cnt);
+                 int32_t* cursor, int32_t* bsum, const int32_t* blk_prims, int32_t n_blk, int32_t max_clip_subs,
+                 Counters* cnt) {
+    const int nb = scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(1024), 0, st, counts, n, bsum);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, st, counts, n, bsum, offsets, chunk_off, cursor,
+                       blk_prims, n_blk, max_clip_subs, cnt);
 }
-void launch_bin(hipStream_t st, const Prim* prims, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
-                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed) {
+void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
+                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris) {
     const int max_prims = n_fixed + (shadow ? 0 : cap.clip_subs);
     if (max_prims <= 0) return;
-    hipLaunchKernelGGL(k_bin, dim3(blocks_for(max_prims, 256)), dim3(256), 0, st, prims, cnt, tiles_x, tiles_per_view,
-                       shadow, cursor, list, cap, n_fixed);
+    hipLaunchKernelGGL(k_bin, dim3(blocks_for(max_prims, 256)), dim3(256), 0, st, pbox, cnt, tiles_x, tiles_per_view,
+                       shadow, cursor, list, cap, n_fixed, n_tris);
 }
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,

@@ -43,17 +43,21 @@ void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, 
                         const ViewConsts& vc, float4* tv_scr, float4* tv_nor, float4* tv_wp);
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
                           float4* ts);
+// pbox: the pixel box of every primitive slot (k_bin's input)
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
-                       const float4* tv_nor, Prim* prims, ClipRec* clips, int32_t* tri_clip_slot, int32_t* clip_list,
-                       int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap);
+                       const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
+                       int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap);
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
-                         Prim* prims, int32_t* tile_count, Counters* cnt, Capacities cap);
+                         Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap);
 // n_blk > 0: also sum the per-workgroup primitive counts of the setup into
 // the frame statistics (the main view)
+// bsum: 2 * scan_blocks(n) ints of scratch
+int scan_blocks(int32_t n);
 void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off,
-                 int32_t* cursor, const int32_t* blk_prims, int32_t n_blk, int32_t max_clip_subs, Counters* cnt);
-void launch_bin(hipStream_t st, const Prim* prims, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
-                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed);
+                 int32_t* cursor, int32_t* bsum, const int32_t* blk_prims, int32_t n_blk, int32_t max_clip_subs,
+                 Counters* cnt);
+void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
+                int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris);
 // mode: 0 visibility, 1 visibility in exact order (NaN rule), 2 shadow maps
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,

@@ -227,9 +227,11 @@ struct prh_renderer {
     M4 *d_trans, *d_normal, *d_strans, *d_lview, *d_lproj;
     float4 *tv_scr, *tv_nor, *tv_wp, *ts;
     Prim* prims;
+    uint2* pbox;  // pixel box per primitive slot
     ClipRec* clips;
     int32_t* tri_clip_slot;
     int32_t* blk_prims;  // per setup workgroup: unclipped primitives (statistics)
+    int32_t* scan_sums;  // per scan block: primitive and work-item sums
     int32_t *tile_count, *tile_off, *chunk_off, *tile_cursor, *tile_list, *clip_list, *item_tile;
     int32_t cap_items;
     int32_t tile_slots;
@@ -263,6 +265,7 @@ void free_list(std::vector<void*>& v) {
 int alloc_work(prh_renderer* r) {
     int rc;
     if ((rc = ralloc(r, &r->prims, (size_t)std::max(r->cap_main.prims, r->cap_shadow.prims)))) return rc;
+    if ((rc = ralloc(r, &r->pbox, (size_t)std::max(r->cap_main.prims, r->cap_shadow.prims)))) return rc;
     if ((rc = ralloc(r, &r->clips, (size_t)r->cap_main.clip_subs))) return rc;
     if ((rc = ralloc(r, &r->tile_list, (size_t)std::max(r->cap_main.pairs, r->cap_shadow.pairs)))) return rc;
     if ((rc = ralloc(r, &r->clip_list, (size_t)r->cap_main.clip_tris))) return rc;
@@ -272,7 +275,7 @@ int alloc_work(prh_renderer* r) {
     return PRH_OK;
 }
 void free_work(prh_renderer* r) {
-    for (void* p : {(void*)r->prims, (void*)r->clips, (void*)r->tile_list, (void*)r->clip_list, (void*)r->item_tile}) {
+    for (void* p : {(void*)r->prims, (void*)r->pbox, (void*)r->clips, (void*)r->tile_list, (void*)r->clip_list, (void*)r->item_tile}) {
         auto it = std::find(r->allocs.begin(), r->allocs.end(), p);
         if (it != r->allocs.end()) {
             (void)hipFree(p);
@@ -583,6 +586,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->ts, (size_t)std::max(1, r->n_slots) * Vt));
     if (!rc) chk(ralloc(r, &r->tri_clip_slot, Tg));
     if (!rc) chk(ralloc(r, &r->blk_prims, (size_t)(Tg + 255) / 256));
+    if (!rc) chk(ralloc(r, &r->scan_sums, 2 * (size_t)scan_blocks(r->tile_slots)));
     if (!rc) chk(ralloc(r, &r->tile_count, r->tile_slots));
     if (!rc) chk(ralloc(r, &r->tile_off, r->tile_slots + 1));
     if (!rc) chk(ralloc(r, &r->chunk_off, r->tile_slots + 1));
@@ -660,10 +664,11 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
         const int nt = r->n_slots * r->stiles_per_map;
         launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
         launch_fill_u32(st, (uint32_t*)r->tile_count, nt, 0u);
-        launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->tile_count, cs, r->cap_shadow);
-        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, nullptr, 0, 0, cs);
-        launch_bin(st, r->prims, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
-                   r->n_slots * d.n_tris);
+        launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->pbox, r->tile_count, cs, r->cap_shadow);
+        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, nullptr, 0, 0,
+                    cs);
+        launch_bin(st, r->pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
+                   r->n_slots * d.n_tris, d.n_tris);
         RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
         launch_raster(st, 2, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                       r->tile_list, cs, rt);
@@ -672,11 +677,12 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
     (void)hipEventRecord(r->ev[2], st);
     launch_fill_u32(st, (uint32_t*)r->tile_count, r->n_tiles, 0u);
-    launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->clips, r->tri_clip_slot, r->clip_list,
+    launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
                       r->tile_count, r->blk_prims, cm, r->cap_main);
-    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->blk_prims,
+    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->blk_prims,
                 (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
-    launch_bin(st, r->prims, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris);
+    launch_bin(st, r->pbox, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
+               d.n_tris);
     (void)hipEventRecord(r->ev[3], st);
     const size_t P = (size_t)r->W * r->H;
     launch_fill_u64(st, r->vis, P, VIS_EMPTY);
