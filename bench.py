@@ -29,32 +29,80 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "1080p frames/s + Mtris/s + HBM GB/s; batched & fwd+adjoint vs plain 1-view"
 
 
-def shade_bytes(scene, vis: np.ndarray, n_shadow: int) -> dict:
-    """Algorithmic HBM bytes of one k_shade launch: per pixel the 8-byte
-    visibility key, its 4-byte depth and the 4-byte RGBA8 output; per covered
-    pixel one 4-byte shadow texel per shadow map; per distinct visible
-    triangle its 12-byte index triple and 3 vertices x 48 bytes
-    (screen xyz+1/w, normal, back-transformed position)."""
+EMPTY = np.uint64(0xFFFFFFFFFFFFFFFF)
+VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: 256 CUs x 4 SIMDs, 4 cycles each at 2.4 GHz
+
+
+def shade_bytes(scene, vis: np.ndarray) -> dict:
+    """Algorithmic (distinct) DRAM bytes of one k_shade launch.  Per pixel:
+    the 8-byte visibility key and the 4-byte RGBA8 output.  Per covered pixel:
+    the 4-byte AO factor if its material has AO, and one 4-byte shadow texel
+    per shadow map if it receives shadows.  Per distinct visible triangle: its
+    4-byte object index (an unclipped one) or its 144-byte clip record (a clip
+    piece).  Per distinct vertex of the visible unclipped triangles: the
+    3 x 16-byte transformed vertex (screen xyz + 1/w, normal, back-transformed
+    position).  Per distinct visible mesh triangle: its 12-byte index triple
+    and 4-byte material; per distinct mesh vertex, its 4-byte colour."""
     P = vis.size
-    cov = vis != np.uint64(0xFFFFFFFFFFFFFFFF)
-    n_cov = int(cov.sum())
-    tris = np.unique((vis[cov] & np.uint64(0xFFFFFFFF)) >> np.uint64(3))
-    t_vis = int(tris.size)
-    b = P * (8 + 4 + 4) + n_cov * 4 * n_shadow + t_vis * (12 + 3 * 48)
-    return dict(bytes=b, pixels=P, covered=n_cov, visible_tris=t_vis)
+    cov = vis != EMPTY
+    order = (vis[cov] & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    g, piece = order >> 3, order & 7
+    tri_counts = np.array([scene.meshes[m].tri.shape[0] for m in scene.obj_mesh])
+    vtx_counts = np.array([scene.meshes[m].pos.shape[0] for m in scene.obj_mesh])
+    obj_tri_start = np.concatenate([[0], np.cumsum(tri_counts)])
+    obj_vtx_start = np.concatenate([[0], np.cumsum(vtx_counts)])
+    mesh_tri_off = np.concatenate([[0], np.cumsum([m.tri.shape[0] for m in scene.meshes])])
+    mesh_vtx_off = np.concatenate([[0], np.cumsum([m.pos.shape[0] for m in scene.meshes])])
+    tri_v_all = np.concatenate([m.tri for m in scene.meshes]).astype(np.int64)
+    tri_mat_all = np.concatenate([m.mat for m in scene.meshes]).astype(np.int64)
+    o = np.searchsorted(obj_tri_start, g, side="right") - 1
+    mesh = np.asarray(scene.obj_mesh)[o]
+    t = mesh_tri_off[mesh] + (g - obj_tri_start[o])
+    direct = piece == 7
+    ug = np.unique(g[direct])
+    n_clip = np.unique(order[~direct]).size
+    od = np.searchsorted(obj_tri_start, ug, side="right") - 1
+    td = mesh_tri_off[np.asarray(scene.obj_mesh)[od]] + (ug - obj_tri_start[od])
+    verts = np.unique((obj_vtx_start[od][:, None] + tri_v_all[td]).ravel())
+    mesh_tris = np.unique(t)
+    mesh_verts = np.unique((mesh_vtx_off[mesh][:, None] + tri_v_all[t]).ravel())
+    mat = tri_mat_all[t] + np.where(tri_mat_all[t] >= 0, np.asarray(scene.obj_mat_base)[o], 0)
+    ok = (mat >= 0) & (mat < len(scene.materials))
+    ao_m = np.array([m.ambient_occlusion for m in scene.materials] + [False])
+    rs_m = np.array([m.receive_shadow for m in scene.materials] + [False])
+    mi = np.where(ok, mat, len(scene.materials))
+    n_shadow = sum(1 for l in scene.lights if l.cast_shadow) if scene.shadow_map else 0
+    n_ao = int(ao_m[mi].sum())
+    n_rs = int(rs_m[mi].sum()) if n_shadow else 0
+    b = (P * (8 + 4) + n_ao * 4 + n_rs * 4 * n_shadow + ug.size * 4 + n_clip * 144 + verts.size * 48
+         + mesh_tris.size * 16 + mesh_verts.size * 4)
+    return dict(bytes=int(b), pixels=P, covered=int(cov.sum()), visible_tris=int(ug.size + n_clip),
+                visible_verts=int(verts.size), ao_px=n_ao)
 
 
-def pmc_traffic(path: str):
-    """HBM bytes per k_shade dispatch from a committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration)."""
+def ao_bytes(scene, vis: np.ndarray, n_ao: int) -> int:
+    """Algorithmic DRAM bytes of one k_ao launch: every pixel's 8-byte key,
+    each covered pixel's 4-byte depth (the taps re-read the depth buffer,
+    which stays in cache) and each AO pixel's 4-byte factor."""
+    return int(vis.size * 8 + int((vis != EMPTY).sum()) * 4 + n_ao * 4)
+
+
+def pmc_summary(path: str) -> dict:
+    """Per-kernel figures from a committed rocprofv3 --pmc summary: lines
+    `kernel.<name>.<field>: <value>` (traffic_bytes, valu_insts per
+    dispatch, with FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
+    calibration)."""
+    out = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_dispatch:"):
-                    return float(line.split(":", 1)[1])
+                if line.startswith("kernel.") and ":" in line:
+                    k, v = line.split(":", 1)
+                    _, name, field_ = k.strip().split(".", 2)
+                    out.setdefault(name, {})[field_] = float(v)
     except OSError:
         pass
-    return None
+    return out
 
 
 def cpu_baseline(scene, frames: int) -> dict:
@@ -129,11 +177,22 @@ def main() -> int:
     img = r.image()
     st = r.stats()
     n_shadow = sum(1 for l in scene.lights if l.cast_shadow)
-    sb = shade_bytes(scene, vis, n_shadow)
+    sb = shade_bytes(scene, vis)
+    ab = ao_bytes(scene, vis, sb["ao_px"])
     ms_step = 1000.0 * dt / args.steps
     fps = args.steps / dt
-    achieved = sb["bytes"] / (ms["shade"] * 1e-3) / 1e9
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "k_shade_c2_1080p_pmc.md"))
+    pmc = pmc_summary(os.path.join(ROOT, "profiles", "c2_1080p_pmc.md"))
+
+    def roof(kernel: str, nbytes: int, ms_kernel: float) -> dict:
+        achieved = nbytes / (ms_kernel * 1e-3) / 1e9
+        p = pmc.get(kernel, {})
+        r = {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": p.get("traffic_bytes"),
+             "algorithmic_bytes": nbytes, "ms": round(ms_kernel, 4)}
+        if "valu_insts" in p:
+            r["valu_issue_frac"] = round(p["valu_insts"] / (ms_kernel * 1e-3) / VALU_PEAK, 3)
+        return r
+
     out = {
         "metric": METRIC,
         "value": round(fps, 3),
@@ -153,10 +212,11 @@ def main() -> int:
         "mtris_per_s": round(scene.n_triangles * fps / 1e6, 2),
         "mpix_per_s": round(scene.n_pixels * fps / 1e6, 2),
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
-        "roofline": {"kernel": "k_shade", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes": sb["bytes"],
-                     "covered_px": sb["covered"], "visible_tris": sb["visible_tris"]},
+        # the dominant kernel (AO, VALU-bound: its HBM fraction is small by
+        # construction) and the deferred-shading kernel the HBM target names
+        "roofline": roof("k_ao", ab, ms["ao"]),
+        "roofline_shade": dict(roof("k_shade", sb["bytes"], ms["shade"]), covered_px=sb["covered"],
+                               visible_tris=sb["visible_tris"], visible_verts=sb["visible_verts"]),
         "frame": {"prims": st.n_prims, "clipped": st.n_clipped, "bin_pairs": st.n_bin_pairs},
     }
     r.free()

@@ -185,7 +185,7 @@ typedef struct prh_frame_stats {
     float ms_vertex;     /* vertex transform of the main view               */
     float ms_setup;      /* cull, clip, bin                                 */
     float ms_raster;     /* visibility raster                               */
-    float ms_shade;      /* deferred shade + shadow lookups + gamma          */
+    float ms_shade;      /* deferred shade + shadow lookups + gamma (k_shade) */
     float ms_ao;         /* screen-space ambient occlusion                  */
     int32_t n_prims;     /* primitives after cull/clip: unclipped triangles
                             plus the clipper's pieces                       */

@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
         L.oracle_fragment_shader.argtypes = [C.POINTER(abi.prh_material), f32p, f32p, f32p, f32p, C.c_int,
                                              C.POINTER(abi.prh_light), C.c_int, f32p, u8p, f32p]
         L.oracle_ao_dirs.argtypes = [f32p]
+        L.oracle_resize.argtypes = [u8p, C.c_int, C.c_int, u8p, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -87,13 +88,14 @@ class Frame:
     """Outputs of one oracle frame (raster-row layout unless noted)."""
 
     def __init__(self, scene, mask=None, shadows=True):
-        W, H = scene.width, scene.height
+        m = getattr(scene, "msaa", 1)
+        W, H = scene.width * m, scene.height * m  # the MSAA-sized framebuffer
         n = W * H
         self.vis_order = np.zeros(n, np.uint32)
         self.depth = np.zeros(n, np.float32)
         self.linear = np.zeros(3 * n, np.float32)
         self.pregamma = np.zeros(4 * n, np.uint8)
-        self.image = np.zeros(4 * n, np.uint8)
+        self.image = np.zeros(4 * scene.width * scene.height, np.uint8)  # after Resize
         S = scene.shadow_size if scene.shadow_size > 0 else None
         nl = max(1, len(scene.lights))
         self.shadow_maps = np.zeros(nl * (S * S if S else n), np.float32) if scene.shadow_map else None
@@ -122,10 +124,23 @@ def render(scene, mask=None) -> Frame:
     rc = L.oracle_render(C.byref(desc), C.byref(cam), C.byref(opts), C.byref(o))
     if rc != 0:
         raise RuntimeError(f"oracle_render failed: {rc}")
-    W, H = scene.width, scene.height
+    m = getattr(scene, "msaa", 1)
+    W, H = scene.width * m, scene.height * m
     fr.vis_order = fr.vis_order.reshape(H, W)
     fr.depth = fr.depth.reshape(H, W)
     fr.linear = fr.linear.reshape(H, W, 3)
     fr.pregamma = fr.pregamma.reshape(H, W, 4)
-    fr.image = fr.image.reshape(H, W, 4)
+    fr.image = fr.image.reshape(scene.height, scene.width, 4)
     return fr
+
+
+def resize(img: np.ndarray, width: int, height: int) -> np.ndarray:
+    """imageutil.Resize of an (ih, iw, 4) uint8 image (the oracle's restatement)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    ih, iw = img.shape[:2]
+    out = np.zeros((height, width, 4), np.uint8)
+    rc = lib().oracle_resize(img.ctypes.data_as(C.POINTER(C.c_uint8)), iw, ih,
+                             out.ctypes.data_as(C.POINTER(C.c_uint8)), width, height)
+    if rc != 0:
+        raise RuntimeError(f"oracle_resize failed: {rc}")
+    return out

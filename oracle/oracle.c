@@ -826,11 +826,95 @@ static void shade(ctx_t* c, int x, int y) {
     memcpy(fr->col, col, 4);
 }
 
+/* ----------------------------------------------------------------- resize */
+/* imageutil.Resize (internal/imageutil/resize.go:16-66), the MSAA
+ * antialiasing pass (render/raster.go:361-378): a separable bilinear filter
+ * with 8-bit fixed-point weights; the first pass filters the rows of the
+ * input into a transposed temporary, the second filters the rows of that. */
+static float resize_linear(float in) { /* resize.go:68-74 */
+    in = (float)fabs((double)in);
+    if (in <= 1) return 1 - in;
+    return 0;
+}
+static int resize_filter_len(float scale) { /* resize.go:149 with filterLength 2 */
+    return 2 * (int)gmax2((float)ceil((double)scale), 1.0f);
+}
+/* createWeights8, resize.go:148-164 */
+static void resize_weights(int dy, int fl, float scale, int16_t* coeffs, int* start) {
+    const float ff = gmin2(1.0f / scale, 1.0f);
+    for (int y = 0; y < dy; y++) {
+        float interp = scale * ((float)y + 0.5f) - 0.5f;
+        start[y] = (int)interp - fl / 2 + 1;
+        interp -= (float)start[y];
+        for (int i = 0; i < fl; i++) {
+            float in = (interp - (float)i) * ff;
+            coeffs[y * fl + i] = (int16_t)(int32_t)(resize_linear(in) * 256);
+        }
+    }
+}
+/* resizeRGBA, resize.go:76-118: `in` has in_rows rows of in_w RGBA pixels;
+ * out[y][x] (out_rows x out_w, row-major) = filter of input row x at y. */
+static void resize_rgba(const uint8_t* in, int in_w, uint8_t* out, int out_w, int out_rows, const int16_t* coeffs,
+                        const int* start, int fl) {
+    const int maxX = in_w - 1;
+    for (int x = 0; x < out_w; x++) {
+        const uint8_t* row = in + (size_t)x * in_w * 4;
+        for (int y = 0; y < out_rows; y++) {
+            int32_t rgba[4] = {0, 0, 0, 0}, sum = 0;
+            for (int i = 0; i < fl; i++) {
+                const int16_t coeff = coeffs[y * fl + i];
+                if (coeff == 0) continue;
+                int xi = start[y] + i;
+                if ((unsigned)xi < (unsigned)maxX) xi *= 4;
+                else if (xi >= maxX) xi = 4 * maxX;
+                else xi = 0;
+                for (int k = 0; k < 4; k++) rgba[k] += (int32_t)coeff * (int32_t)row[xi + k];
+                sum += coeff;
+            }
+            for (int k = 0; k < 4; k++) {
+                int v = rgba[k] / sum; /* Go's integer division truncates toward zero, as C's */
+                out[((size_t)y * out_w + x) * 4 + k] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+            }
+        }
+    }
+}
+/* Resize(width, height, img) with img (iw x ih RGBA8, top row first);
+ * returns 0, or -1 on allocation failure.  The identity when the sizes match. */
+static int resize_image(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height) {
+    if (iw == width && ih == height) {
+        memcpy(out, img, (size_t)iw * ih * 4);
+        return 0;
+    }
+    const float sx = (float)iw / (float)width, sy = (float)ih / (float)height; /* calcFactors, resize.go:121-138 */
+    const int flx = resize_filter_len(sx), fly = resize_filter_len(sy);
+    uint8_t* temp = malloc((size_t)ih * width * 4); /* image.Rect(0, 0, ih, width): width rows of ih pixels */
+    int16_t* cx = malloc(sizeof(int16_t) * (size_t)width * flx);
+    int16_t* cy = malloc(sizeof(int16_t) * (size_t)height * fly);
+    int* ox = malloc(sizeof(int) * (size_t)width);
+    int* oy = malloc(sizeof(int) * (size_t)height);
+    int rc = -1;
+    if (temp && cx && cy && ox && oy) {
+        resize_weights(width, flx, sx, cx, ox);
+        resize_rgba(img, iw, temp, ih, width, cx, ox, flx);
+        resize_weights(height, fly, sy, cy, oy);
+        resize_rgba(temp, ih, out, width, height, cy, oy, fly);
+        rc = 0;
+    }
+    free(temp);
+    free(cx);
+    free(cy);
+    free(ox);
+    free(oy);
+    return rc;
+}
+int oracle_resize(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height) {
+    return resize_image(img, iw, ih, out, width, height) ? PRH_E_NOMEM : PRH_OK;
+}
+
 /* ----------------------------------------------------------------- render */
 int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_options* o, oracle_out* out) {
     if (!s || !cam || !o || !out) return PRH_E_INVALID;
     int msaa = o->msaa > 0 ? o->msaa : 1;
-    if (msaa != 1) return PRH_E_UNSUPPORTED;
     ctx_t c;
     memset(&c, 0, sizeof c);
     c.s = s;
@@ -894,7 +978,12 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
         shade(&c, x, y);
     }
 #endif
-    for (int y = 0; y < c.H; y++)
+    /* passAntialiasing (render/raster.go:361-378): gamma on the
+     * MSAA-sized buffer, then Resize to the requested size */
+    uint8_t* big = msaa > 1 && out->image ? malloc(np * 4) : out->image;
+    int rc = PRH_OK;
+    if (msaa > 1 && out->image && !big) rc = PRH_E_NOMEM;
+    for (int y = 0; y < c.H && rc == PRH_OK; y++)
         for (int x = 0; x < c.W; x++) {
             size_t i = (size_t)y * c.W + x;
             const frag_t* f = fat(&c, x, y);
@@ -904,8 +993,12 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
             if (out->pregamma) memcpy(out->pregamma + 4 * i, col, 4);
             if (o->gamma_correct) /* shader/gamma.go:13-18 */
                 for (int k = 0; k < 3; k++) col[k] = gamma_u8(col[k]);
-            if (out->image) memcpy(out->image + 4 * ((size_t)(c.H - 1 - y) * c.W + x), col, 4);
+            if (big) memcpy(big + 4 * ((size_t)(c.H - 1 - y) * c.W + x), col, 4);
         }
+    if (rc == PRH_OK && msaa > 1 && out->image) {
+        if (resize_image(big, c.W, c.H, out->image, o->width, o->height)) rc = PRH_E_NOMEM;
+        free(big);
+    }
     if (out->shadow_maps && c.smaps)
         memcpy(out->shadow_maps, c.smaps, sizeof(float) * (size_t)s->n_lights * c.SW * c.SH);
     if (out->shade_err64) *out->shade_err64 = c.shade_err64;
@@ -914,7 +1007,7 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
     free(c.lproj);
     free(c.lactive);
     free(c.smaps);
-    return PRH_OK;
+    return rc;
 }
 
 /* ----------------------------------------------------- known-answer hooks */

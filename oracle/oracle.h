@@ -51,8 +51,14 @@ typedef struct oracle_out {
     double* shade_err64;   /* optional: max |f32 linear - f64 re-evaluation|            */
 } oracle_out;
 
+/* With MSAA m > 1 every buffer is (height*m) x (width*m) except the final
+ * image, which is resized to height x width (render/raster.go:361-378). */
 int oracle_render(const prh_scene_desc* scene, const prh_camera* cam,
                   const prh_options* opts, oracle_out* out);
+
+/* imageutil.Resize (internal/imageutil/resize.go:16-66) of an iw x ih RGBA8
+ * image to width x height. */
+int oracle_resize(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height);
 
 /* Known-answer entry points (row-major Mat4 X00..X33). */
 void oracle_barycoord(const float p[2], const float a[2], const float b[2],

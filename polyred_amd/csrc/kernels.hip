@@ -372,7 +372,7 @@ __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts s
             p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(C.x, C.y, C.z, 1);
             if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {
                 Box3 tb = box_of(p1, p2, p3);
-                if (vp_intersect(tb, (float)sc.SW, (float)sc.SH))
+                if (vp_intersect(tb, sc.vpbox_max_x, sc.vpbox_max_y))
                     emit = pixel_box(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, sc.SW, sc.SH, x0, y0, x1, y1);
             }
         }
@@ -922,7 +922,7 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 // pow(total, 10000) (ao.go:31-33) is written for k_shade.
 __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uint64_t* __restrict__ vis,
                                             const float* __restrict__ depth, float* __restrict__ ao,
-                                            const Counters* cnt) {
+                                            const float* __restrict__ ao_tab, int32_t ao_fast, const Counters* cnt) {
     __shared__ float s_m[8][64];
     __shared__ float s_dp[64];
     __shared__ int32_t s_need[64];
@@ -968,8 +968,25 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
         // largest rounded quotient
         double e_best = -__builtin_inf(), d_best = 1.0;
         bool nan = false;
-        float t = 1.0f;  // t = 0 is the pixel itself: distance 0 < 1, skipped
-        if ((wv & 1) == 0 && W <= 4096 && H <= 4096) {
+        // depth taps through a buffer descriptor: an out-of-range offset
+        // reads 0 instead of faulting, so a masked lane needs no clamped
+        // address (its value is never used)
+        const __amdgpu_buffer_rsrc_t drs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)depth, (short)0, W * H * 4, BUF_DWORD3);
+        const int wu = __builtin_amdgcn_readfirstlane(wv);
+        const float* off_x = ao_tab + AO_TAB_OFF + wu * 200;  // fl(dir.x * t), t = 0..99 (uniform)
+        const float* off_y = off_x + 100;                     // fl(dir.y * t)
+        // one tap: the running argmax of elevation / distance, exact
+        // (bitwise & keeps it a select, not a branch that would serialise
+        // the loads of consecutive taps)
+        auto take = [&](bool v, float e, double d64) {
+            const double e64 = (double)e;
+            const bool better = v & (e64 * d_best > e_best * d64);
+            e_best = better ? e64 : e_best;
+            d_best = better ? d64 : d_best;
+            nan |= v & (e != e);
+        };
+        if (ao_fast && (wu & 1) == 0) {
             // Axis directions (a = 0, pi/2, pi, 3pi/2 in float32): one offset
             // is exactly +-t, the other the tiny cos/sin residue (<= 8.8e-8 t)
             // rounded onto the coordinate grid, under 2e-5 for coordinates
@@ -977,19 +994,43 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
             // (>= 6e-8), so Len = sqrt(t^2) = t exactly (also checked over
             // every coordinate pair below 4096 and every t).
 #pragma unroll 3
-            for (int ti = 1; ti < 100; ti++, t += 1.0f) {
-                const float cx = X + dx * t, cy = Y + dy * t;
+            for (int ti = 1; ti < 100; ti++) {
+                const float cx = X + off_x[ti], cy = Y + off_y[ti];
                 const int ix = (int)cx, iy = (int)cy;
-                const bool v = need && (unsigned)ix < (unsigned)W && (unsigned)iy < (unsigned)H;
-                const uint32_t off = v ? ((unsigned)iy * (unsigned)W + (unsigned)ix) * 4u : 0u;
-                const float e = *(const float*)((const char*)depth + off) - dp;
-                const double e64 = (double)e, d64 = (double)t;
-                const bool better = v && e64 * d_best > e_best * d64;
-                e_best = better ? e64 : e_best;
-                d_best = better ? d64 : d_best;
-                nan |= v && e != e;
+                const bool v = need & ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
+                const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
+                take(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, (double)ti);
             }
+        } else if (ao_fast) {
+            // Diagonal directions: X - fl(X + fl(dir.x * t)) is fixed by t and
+            // the binade of the tap coordinate (checked when the table is
+            // built, for every coordinate below 4096), so the distance
+            // Len = sqrt32(fma64(ex, ex, ey * ey)) is a table over
+            // (binade of cx, binade of cy, t), held as float64.
+            const double* dtab =
+                reinterpret_cast<const double*>(ao_tab + AO_TAB_DIST) + ((wu - 1) >> 1) * AO_DIST_PER_DIR;
+            const __amdgpu_buffer_rsrc_t trs =
+                __builtin_amdgcn_make_buffer_rsrc((void*)dtab, (short)0, AO_DIST_PER_DIR * 8, BUF_DWORD3);
+            constexpr int KMAX = AO_EXP_MIN + AO_EXP_N - 1;
+            // byte offset of (kx, ky, t) = kx * ROW_X + ky * ROW_Y + t * 8 + BIAS
+            constexpr int ROW_Y = 100 * 8, ROW_X = AO_EXP_N * ROW_Y, BIAS = -AO_EXP_MIN * (ROW_X + ROW_Y);
+            auto tap = [&](int ti, bool first) {
+                const float cx = X + off_x[ti], cy = Y + off_y[ti];
+                const int ix = (int)cx, iy = (int)cy;
+                const int kx = min(max(__builtin_amdgcn_frexp_expf(cx), AO_EXP_MIN), KMAX);
+                const int ky = min(max(__builtin_amdgcn_frexp_expf(cy), AO_EXP_MIN), KMAX);
+                const int toff = kx * ROW_X + ky * ROW_Y + (ti * 8 + BIAS);
+                const double dist = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(trs, toff, 0, 0));
+                bool v = need & ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
+                if (first) v = v & (dist >= 1.0);  // distance < 1 only occurs at t = 1
+                const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
+                take(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, dist);
+            };
+            tap(1, true);
+#pragma unroll 3
+            for (int ti = 2; ti < 100; ti++) tap(ti, false);
         } else {
+            float t = 1.0f;  // t = 0 is the pixel itself: distance 0 < 1, skipped
 #pragma unroll 3
             for (int ti = 1; ti < 100; ti++, t += 1.0f) {
                 const float cx = X + dx * t, cy = Y + dy * t;
@@ -998,16 +1039,9 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
                 // Len of (ex, ey, 0, 0): FMA(ex,ex, FMA(ey,ey, FMA(0,0,0*0))) where
                 // the inner FMA(ey,ey,+0) is exactly the float32 product ey*ey
                 const float dist = sqrt32_normal(fma64(ex, ex, ey * ey));
-                const bool v = need && (unsigned)ix < (unsigned)W && (unsigned)iy < (unsigned)H && dist >= 1.0f;
-                // 32-bit byte offset (the frame is < 2^30 pixels) so the load can
-                // take a scalar base plus a 32-bit vector offset
-                const uint32_t off = v ? ((unsigned)iy * (unsigned)W + (unsigned)ix) * 4u : 0u;
-                const float e = *(const float*)((const char*)depth + off) - dp;
-                const double e64 = (double)e, d64 = (double)dist;
-                const bool better = v && e64 * d_best > e_best * d64;
-                e_best = better ? e64 : e_best;
-                d_best = better ? d64 : d_best;
-                nan |= v && e != e;
+                const bool v = need & ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H) & (dist >= 1.0f);
+                const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
+                take(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, (double)dist);
             }
         }
         const float maxr = (float)e_best / (float)d_best;
@@ -1089,6 +1123,37 @@ __global__ __launch_bounds__(256) void k_shade(DevScene s, ShadeConsts c, ShadeB
 // ------------------------------------------------------------- launchers
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// ------------------------------------------------------------- resize
+// resizeRGBA (internal/imageutil/resize.go:76-118), one output pixel per
+// thread: out[y][x] filters row x of `in` (rows of in_w RGBA8 pixels) at
+// output position y with createWeights8's 8-bit weights.  Two launches make
+// imageutil.Resize: the first writes its result transposed.
+__global__ __launch_bounds__(256) void k_resize(const uint32_t* __restrict__ in, int32_t in_w,
+                                                uint32_t* __restrict__ out, int32_t out_w, int32_t out_rows,
+                                                const int16_t* __restrict__ coeffs, const int32_t* __restrict__ start,
+                                                int32_t fl) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= out_w || y >= out_rows) return;
+    const uint32_t* row = in + (size_t)x * in_w;
+    const int maxX = in_w - 1;
+    int32_t acc[4] = {0, 0, 0, 0}, sum = 0;
+    for (int i = 0; i < fl; i++) {
+        const int32_t coeff = coeffs[y * fl + i];
+        if (coeff == 0) continue;
+        int xi = start[y] + i;
+        xi = (unsigned)xi < (unsigned)maxX ? xi : (xi >= maxX ? maxX : 0);
+        const uint32_t p = row[xi];
+        for (int k = 0; k < 4; k++) acc[k] += coeff * (int32_t)((p >> (8 * k)) & 0xffu);
+        sum += coeff;
+    }
+    uint32_t o = 0;
+    for (int k = 0; k < 4; k++) {
+        const int v = acc[k] / sum;  // truncating division, as Go's
+        o |= (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v)) << (8 * k);
+    }
+    out[(size_t)y * out_w + x] = o;
+}
+
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min<size_t>(blocks_for(n, 256), 4096)), dim3(256), 0, st, p, n, v);
@@ -1161,7 +1226,8 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, const Co
 }
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const int nb = ((c.v.W + 7) / 8) * ((c.v.H + 7) / 8);
-    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(512), 0, st, s, c, b.vis, b.depth, b.ao, b.cnt);
+    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(512), 0, st, s, c, b.vis, b.depth, b.ao, b.ao_tab,
+                       b.ao_fast, b.cnt);
 }
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     hipLaunchKernelGGL(k_shade_bg, dim3(1), dim3(64), 0, st, s, c, b);
@@ -1170,6 +1236,13 @@ void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const
     int n_tiles = c.v.tiles_x * c.v.tiles_y;
     int per_xcd = (n_tiles + 7) / 8;
     hipLaunchKernelGGL(k_shade, dim3(per_xcd * 8), dim3(256), 0, st, s, c, b);
+}
+
+void launch_resize(hipStream_t st, const uint32_t* in, int32_t in_w, uint32_t* out, int32_t out_w, int32_t out_rows,
+                   const int16_t* coeffs, const int32_t* start, int32_t fl) {
+    if (out_w <= 0 || out_rows <= 0) return;
+    hipLaunchKernelGGL(k_resize, dim3(blocks_for(out_w, 256), out_rows), dim3(256), 0, st, in, in_w, out, out_w,
+                       out_rows, coeffs, start, fl);
 }
 
 }  // namespace prh

@@ -35,7 +35,21 @@ struct ShadeBuffers {
     float* bg_lin;             // [3]
     const Counters* cnt;       // main-pass counters: nothing is shaded after an overflow
     float* ao;                 // [H][W] AO factor of the pixels that need it
+    const float* ao_tab;       // AO march tables (tap offsets, diagonal distances)
+    int32_t ao_fast;           // the tables hold for this frame size
 };
+
+// Layout of the AO march tables, in float (4-byte) units.  Binade buckets are
+// frexp exponents AO_EXP_MIN .. AO_EXP_MIN + AO_EXP_N - 1.
+constexpr int AO_EXP_MIN = -10;
+constexpr int AO_EXP_N = 24;
+constexpr int AO_DIST_PER_DIR = AO_EXP_N * AO_EXP_N * 100;  // doubles
+constexpr int AO_TAB_OFF = 0;           // float [8 directions][2][100 t]: fl(dir * t)
+constexpr int AO_TAB_T64 = 8 * 200;     // double [100]: t
+constexpr int AO_TAB_DIST = AO_TAB_T64 + 200;  // double [4 diagonals][AO_EXP_N][AO_EXP_N][100 t]
+constexpr int AO_TAB_SIZE = AO_TAB_DIST + 2 * 4 * AO_DIST_PER_DIR;
+// Third descriptor word of a raw (unstrided, untyped) buffer on gfx9 parts.
+constexpr uint32_t BUF_DWORD3 = 0x00020000u;
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
@@ -66,5 +80,8 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, const Co
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
+// out[y][x] (out_rows x out_w) = 8-bit bilinear filter of row x of `in` at y
+void launch_resize(hipStream_t st, const uint32_t* in, int32_t in_w, uint32_t* out, int32_t out_w, int32_t out_rows,
+                   const int16_t* coeffs, const int32_t* start, int32_t fl);
 
 }  // namespace prh

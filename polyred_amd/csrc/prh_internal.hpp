@@ -76,6 +76,7 @@ struct ShadowConsts {
     gm::M4 vp;                 // ViewportMatrix(SW, SH)
     int32_t SW, SH, tiles_x, tiles_y;
     int32_t n_slots, tiles_per_map;
+    float vpbox_max_x, vpbox_max_y;  // cullViewFrustum's box: (SW*MSAA, SH*MSAA) (render/cull.go:16-22)
 };
 
 // A rasterisable primitive: a whole triangle (sub = 7) or one clipped fan

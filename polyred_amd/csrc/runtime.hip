@@ -177,6 +177,84 @@ void ao_dirs(float out[16]) {
     }
 }
 
+// Tables of the AO march for frames up to 4096 x 4096 (k_ao):
+//   [AO_TAB_OFF]  per direction, fl(dir.x * t) and fl(dir.y * t), t = 0..99;
+//   [AO_TAB_DIST] per diagonal direction, the tap distance
+//                 Len = sqrt32(fma64(ex, ex, ey * ey)) indexed by the binades
+//                 (frexp exponents) of the tap coordinates and t.
+// The second table rests on ex = X - fl(X + fl(dir.x * t)) being fixed by t
+// and the binade of fl(X + fl(dir.x * t)) for every integer X in [0, 4096);
+// that is checked here over every such X, and a failed check (or a binade
+// outside the table) returns false, which keeps k_ao on its direct path.
+uint32_t bitsof(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+bool ao_tables(const float dirs[16], std::vector<float>& tab) {
+    tab.assign(AO_TAB_SIZE, __builtin_nanf(""));
+    for (int d = 0; d < 8; d++)
+        for (int t = 0; t < 100; t++) {
+            tab[AO_TAB_OFF + d * 200 + t] = dirs[2 * d] * (float)t;
+            tab[AO_TAB_OFF + d * 200 + 100 + t] = dirs[2 * d + 1] * (float)t;
+        }
+    double* t64 = reinterpret_cast<double*>(tab.data() + AO_TAB_T64);
+    for (int t = 0; t < 100; t++) t64[t] = (double)t;
+    double* dist = reinterpret_cast<double*>(tab.data() + AO_TAB_DIST);
+    for (int q = 0; q < 4; q++) {
+        const int d = 2 * q + 1;
+        // e[comp][t][bucket]: the offset for each binade bucket (NaN: unused)
+        std::vector<float> e(2 * 100 * AO_EXP_N, __builtin_nanf(""));
+        for (int comp = 0; comp < 2; comp++)
+            for (int t = 1; t < 100; t++) {
+                const float c = dirs[2 * d + comp] * (float)t;
+                for (int X = 0; X < 4096; X++) {
+                    const float fx = (float)X, cx = fx + c, ex = fx - cx;
+                    int k = 0;
+                    (void)std::frexp(cx, &k);
+                    const int b = k - AO_EXP_MIN;
+                    if (b < 0 || b >= AO_EXP_N) return false;
+                    float& slot = e[(comp * 100 + t) * AO_EXP_N + b];
+                    if (slot != slot) slot = ex;
+                    else if (bitsof(slot) != bitsof(ex)) return false;
+                }
+            }
+        for (int bx = 0; bx < AO_EXP_N; bx++)
+            for (int by = 0; by < AO_EXP_N; by++)
+                for (int t = 1; t < 100; t++) {
+                    const float ex = e[(0 * 100 + t) * AO_EXP_N + bx], ey = e[(1 * 100 + t) * AO_EXP_N + by];
+                    if (ex != ex || ey != ey) continue;
+                    const float s2 = (float)std::fma((double)ex, (double)ex, (double)(ey * ey));
+                    const float len = std::sqrt(s2);
+                    // from t = 2 on every distance is >= 1 (checked): the
+                    // march tests distance < 1 at t = 1 only
+                    if (t >= 2 && len < 1.0f) return false;
+                    dist[q * AO_DIST_PER_DIR + (bx * AO_EXP_N + by) * 100 + t] = (double)len;
+                }
+    }
+    return true;
+}
+
+// createWeights8 (internal/imageutil/resize.go:148-164) with the linear
+// kernel (:68-74) and filterLength 2, in float32 as the reference.
+int resize_weights(int dy, float scale, std::vector<int16_t>& coeffs, std::vector<int32_t>& start) {
+    const int fl = 2 * (int)max2((float)std::ceil((double)scale), 1.0f);
+    const float ff = min2(1.0f / scale, 1.0f);
+    coeffs.assign((size_t)dy * fl, 0);
+    start.assign(dy, 0);
+    for (int y = 0; y < dy; y++) {
+        float interp = scale * ((float)y + 0.5f) - 0.5f;
+        start[y] = (int)interp - fl / 2 + 1;
+        interp -= (float)start[y];
+        for (int i = 0; i < fl; i++) {
+            float in = std::fabs((interp - (float)i) * ff);
+            const float k = in <= 1 ? 1 - in : 0;
+            coeffs[(size_t)y * fl + i] = (int16_t)(int32_t)(k * 256);
+        }
+    }
+    return fl;
+}
+
 template <class T>
 int dalloc(T** p, size_t n) {
     *p = nullptr;
@@ -218,7 +296,8 @@ struct prh_renderer {
     prh_scene* sc;
     prh_camera cam;
     prh_options opt;
-    int W, H, tiles_x, tiles_y, n_tiles;
+    int W, H, tiles_x, tiles_y, n_tiles;  // the MSAA-sized framebuffer (width*MSAA x height*MSAA)
+    int msaa, OW, OH;                       // MSAA factor and the output image size
     int SW, SH, stiles_x, stiles_y, stiles_per_map, n_slots;
     ViewConsts vc;
     ShadowConsts shc;
@@ -240,7 +319,14 @@ struct prh_renderer {
     uint64_t* vis;
     float* depth;
     uint32_t *image, *pregamma, *bg, *smaps;
+    // MSAA > 1: k_shade writes image_big (W x H), Resize makes image (OW x OH)
+    uint32_t *image_big, *resize_tmp;
+    int16_t *rz_cx, *rz_cy;  // createWeights8 coefficients, both passes
+    int32_t *rz_ox, *rz_oy;  // and their start offsets
+    int rz_flx, rz_fly;
     float *linear, *bg_lin, *ao;
+    float* ao_tab;   // AO march tables (ao_tables)
+    int32_t ao_fast; // the tables hold for this frame size
     Counters* cnt;      // [2]: main, shadow
     Counters* h_cnt;    // pinned
     Capacities cap_main, cap_shadow;
@@ -475,16 +561,19 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!ctx || !s || !cam || !o || !out) return fail(PRH_E_INVALID, "prh_renderer_new: null argument");
     *out = nullptr;
     if (s->ctx != ctx) return fail(PRH_E_INVALID, "prh_renderer_new: scene belongs to another context");
-    if (o->width <= 0 || o->height <= 0 || o->width > 32768 || o->height > 32768)
+    const int msaa = o->msaa > 0 ? o->msaa : 1;
+    if (o->width <= 0 || o->height <= 0 || msaa > 16 || (int64_t)o->width * msaa > 32768 ||
+        (int64_t)o->height * msaa > 32768)
         return fail(PRH_E_INVALID, "framebuffer size out of range");
-    if (o->msaa > 1) return fail(PRH_E_UNSUPPORTED, "MSAA > 1 is not on the device path");
     if (o->shadow_size < 0 || o->shadow_size > 32768) return fail(PRH_E_INVALID, "shadow size out of range");
     if (cam->kind != PRH_CAMERA_PERSPECTIVE && cam->kind != PRH_CAMERA_ORTHOGRAPHIC)
         return fail(PRH_E_INVALID, "unknown camera kind");
     HIP_TRY(hipSetDevice(ctx->device));
     prh_renderer* r = new prh_renderer();
     r->ctx = ctx, r->sc = s, r->cam = *cam, r->opt = *o;
-    r->W = o->width, r->H = o->height;
+    // every pass renders the MSAA-sized buffer (render/raster.go:149-150)
+    r->msaa = msaa, r->OW = o->width, r->OH = o->height;
+    r->W = o->width * msaa, r->H = o->height * msaa;
     r->tiles_x = (r->W + TILE - 1) / TILE, r->tiles_y = (r->H + TILE - 1) / TILE;
     r->n_tiles = r->tiles_x * r->tiles_y;
     // shadow_size 0: framebuffer-sized maps as the reference (shadow.go:87-88)
@@ -507,7 +596,9 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     memcpy(vc.cam, cam->position, sizeof vc.cam);
     vc.W = r->W, vc.H = r->H, vc.tiles_x = r->tiles_x, vc.tiles_y = r->tiles_y;
     vc.perspect = cam->kind == PRH_CAMERA_PERSPECTIVE;
-    vc.vpbox_max_x = (float)r->W, vc.vpbox_max_y = (float)r->H;
+    // the viewport box and clip planes of draw scale the (already MSAA-sized)
+    // buffer by MSAA once more (render/raster.go:416-421,438-439)
+    vc.vpbox_max_x = (float)(msaa * r->W), vc.vpbox_max_y = (float)(msaa * r->H);
 
     const int O = s->d.n_objects, L = s->d.n_lights;
     std::vector<M4> trans(O), normal(O);
@@ -544,6 +635,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     shc.vp = viewport_matrix((float)r->SW, (float)r->SH);
     shc.SW = r->SW, shc.SH = r->SH, shc.tiles_x = r->stiles_x, shc.tiles_y = r->stiles_y;
     shc.n_slots = r->n_slots, shc.tiles_per_map = r->stiles_per_map;
+    shc.vpbox_max_x = (float)(msaa * r->SW), shc.vpbox_max_y = (float)(msaa * r->SH);
 
     ShadeConsts& sh = r->sh;
     memset(&sh, 0, sizeof sh);
@@ -554,6 +646,9 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     sh.gamma_correct = o->gamma_correct != 0;
     sh.n_lights = L;
     ao_dirs(sh.ao_dir);
+    std::vector<float> aotab;
+    const bool ao_tab_ok = ao_tables(sh.ao_dir, aotab);
+    r->ao_fast = ao_tab_ok && r->W <= 4096 && r->H <= 4096;
     sh.SW = r->SW, sh.SH = r->SH;
     sh.svp = shc.vp;
 
@@ -570,16 +665,18 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     auto chk = [&](int v) {
         if (!rc) rc = v;
     };
-    chk(upload(&r->d_trans, trans.data(), O));
+    if (!rc) chk(upload(&r->d_trans, trans.data(), O));
     if (!rc) r->allocs.push_back(r->d_trans);
-    chk(upload(&r->d_normal, normal.data(), O));
+    if (!rc) chk(upload(&r->d_normal, normal.data(), O));
     if (!rc) r->allocs.push_back(r->d_normal);
-    chk(upload(&r->d_strans, strans.data(), strans.size()));
+    if (!rc) chk(upload(&r->d_strans, strans.data(), strans.size()));
     if (!rc) r->allocs.push_back(r->d_strans);
-    chk(upload(&r->d_lview, lview.data(), lview.size()));
+    if (!rc) chk(upload(&r->d_lview, lview.data(), lview.size()));
     if (!rc) r->allocs.push_back(r->d_lview);
-    chk(upload(&r->d_lproj, lproj.data(), lproj.size()));
+    if (!rc) chk(upload(&r->d_lproj, lproj.data(), lproj.size()));
     if (!rc) r->allocs.push_back(r->d_lproj);
+    if (!rc) chk(upload(&r->ao_tab, aotab.data(), aotab.size()));
+    if (!rc) r->allocs.push_back(r->ao_tab);
     if (!rc) chk(ralloc(r, &r->tv_scr, Vt));
     if (!rc) chk(ralloc(r, &r->tv_nor, Vt));
     if (!rc) chk(ralloc(r, &r->tv_wp, Vt));
@@ -597,7 +694,26 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->vis, P));
     if (!rc) chk(ralloc(r, &r->depth, P));
     if (!rc) chk(ralloc(r, &r->ao, P));
-    if (!rc) chk(ralloc(r, &r->image, P));
+    if (!rc && msaa == 1) chk(ralloc(r, &r->image, P));
+    if (!rc && msaa > 1) {
+        // Resize(width, height, img): scale = old / new = MSAA (calcFactors,
+        // resize.go:121-138)
+        std::vector<int16_t> cx, cy;
+        std::vector<int32_t> ox, oy;
+        r->rz_flx = resize_weights(r->OW, (float)r->W / (float)r->OW, cx, ox);
+        r->rz_fly = resize_weights(r->OH, (float)r->H / (float)r->OH, cy, oy);
+        chk(ralloc(r, &r->image_big, P));
+        if (!rc) chk(ralloc(r, &r->resize_tmp, (size_t)r->OW * r->H));
+        if (!rc) chk(ralloc(r, &r->image, (size_t)r->OW * r->OH));
+        if (!rc) chk(upload(&r->rz_cx, cx.data(), cx.size()));
+        if (!rc) r->allocs.push_back(r->rz_cx);
+        if (!rc) chk(upload(&r->rz_cy, cy.data(), cy.size()));
+        if (!rc) r->allocs.push_back(r->rz_cy);
+        if (!rc) chk(upload(&r->rz_ox, ox.data(), ox.size()));
+        if (!rc) r->allocs.push_back(r->rz_ox);
+        if (!rc) chk(upload(&r->rz_oy, oy.data(), oy.size()));
+        if (!rc) r->allocs.push_back(r->rz_oy);
+    }
     if (!rc && o->keep_linear) chk(ralloc(r, &r->pregamma, P));
     if (!rc && o->keep_linear) chk(ralloc(r, &r->linear, 3 * P));
     if (!rc) chk(ralloc(r, &r->bg, 4));
@@ -700,13 +816,20 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
     b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
     b.light_view = r->d_lview, b.light_proj = r->d_lproj;
-    b.image = r->image, b.pregamma = r->pregamma, b.linear = r->linear, b.bg = r->bg, b.bg_lin = r->bg_lin;
+    b.image = r->msaa > 1 ? r->image_big : r->image, b.pregamma = r->pregamma, b.linear = r->linear, b.bg = r->bg, b.bg_lin = r->bg_lin;
     b.cnt = cm;
     b.ao = r->ao;
+    b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
     launch_ao(st, d, r->sh, b);
     (void)hipEventRecord(r->ev[5], st);
     launch_shade_bg(st, d, r->sh, b);
+    (void)hipEventRecord(r->ev[7], st);
     launch_shade(st, d, r->sh, b);
+    if (r->msaa > 1) {
+        // passAntialiasing (render/raster.go:361-378): Resize after gamma
+        launch_resize(st, r->image_big, r->W, r->resize_tmp, r->H, r->OW, r->rz_cx, r->rz_ox, r->rz_flx);
+        launch_resize(st, r->resize_tmp, r->H, r->image, r->OW, r->OH, r->rz_cy, r->rz_oy, r->rz_fly);
+    }
     (void)hipEventRecord(r->ev[6], st);
     (void)hipMemcpyAsync(r->h_cnt, r->cnt, 2 * sizeof(Counters), hipMemcpyDeviceToHost, st);
 }
@@ -757,8 +880,8 @@ int prh_render(prh_renderer* r) {
         float ms[6] = {0};
         for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
         (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
-        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4],
-        S.ms_shade = ms[5];
+        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
+        (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
         S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
         S.nan_rerun = nan_rerun ? 1 : 0;
         return PRH_OK;
@@ -767,7 +890,7 @@ int prh_render(prh_renderer* r) {
 
 int prh_read_image(prh_renderer* r, uint8_t* rgba, size_t bytes) {
     if (!r || !rgba) return fail(PRH_E_INVALID, "prh_read_image: null argument");
-    size_t need = (size_t)4 * r->W * r->H;
+    size_t need = (size_t)4 * r->OW * r->OH;
     if (bytes < need) return fail(PRH_E_INVALID, "prh_read_image: buffer too small");
     HIP_TRY(hipSetDevice(r->ctx->device));
     HIP_TRY(hipMemcpyAsync(rgba, r->image, need, hipMemcpyDeviceToHost, r->ctx->stream));

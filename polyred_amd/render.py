@@ -193,6 +193,7 @@ class Renderer:
         s.shadow_map, s.gamma, s.shadow_size = cfg.ShadowMap, cfg.GammaCorrect, cfg.ShadowSize
         s.background = cfg.Background
         s.camera = cfg.Camera
+        s.msaa = cfg.MSAA
         self.handle = None
         self.gscene = GPUScene(cfg.HIPDevice, s)
         cam = s.camera_struct()
@@ -207,7 +208,8 @@ class Renderer:
             raise
         self.handle = h
         self.gscene._renderers.add(self)
-        self.W, self.H = cfg.Width, cfg.Height
+        self.W, self.H = cfg.Width, cfg.Height  # the output image
+        self.msaa = max(1, cfg.MSAA)             # internal buffers are (H*msaa, W*msaa)
         self.n_shadow = sum(1 for l in s.lights if l.cast_shadow) if cfg.ShadowMap else 0
         self.S = (cfg.ShadowSize if cfg.ShadowSize > 0 else None)
 
@@ -231,7 +233,8 @@ class Renderer:
         return out
 
     def read(self, which: int) -> np.ndarray:
-        W, H = self.W, self.H
+        """An internal buffer, MSAA-sized (raster rows)."""
+        W, H = self.W * self.msaa, self.H * self.msaa
         if which == abi.PRH_BUF_VIS:
             out = np.empty((H, W), np.uint64)
         elif which == abi.PRH_BUF_DEPTH:
@@ -291,4 +294,5 @@ def renderer_for(scene, dev: Device, keep_linear: bool = False) -> Renderer:
     """A renderer configured from a generated scene's own settings."""
     return NewRenderer(Size(scene.width, scene.height), Camera(scene.camera), Scene(scene),
                        Background(scene.background), ShadowMap(scene.shadow_map), ShadowSize(scene.shadow_size),
-                       GammaCorrection(scene.gamma), KeepLinear(keep_linear), HIP(dev))
+                       GammaCorrection(scene.gamma), MSAA(getattr(scene, "msaa", 1)), KeepLinear(keep_linear),
+                       HIP(dev))

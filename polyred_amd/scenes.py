@@ -105,6 +105,7 @@ class Scene:
     gamma: bool = True
     shadow_size: int = 0
     background: tuple = (0, 0, 0, 0)
+    msaa: int = 1
     name: str = ""
     _keep: list = field(default_factory=list, repr=False)
 
@@ -217,7 +218,7 @@ class Scene:
 
     def options_struct(self, keep_linear: bool = False) -> abi.prh_options:
         o = abi.prh_options()
-        o.width, o.height, o.msaa = self.width, self.height, 1
+        o.width, o.height, o.msaa = self.width, self.height, self.msaa
         o.background[:] = self.background
         o.shadow_map = int(self.shadow_map)
         o.gamma_correct = int(self.gamma)

@@ -61,8 +61,11 @@ def assert_frame_equal(g, fr, mask=None, scene=None):
     assert same_bits(lin_g, lin_o), f"linear sums differ at {int((~((lin_g == lin_o) | (np.isnan(lin_g) & np.isnan(lin_o)))).any(-1).sum())} px"
     d = np.abs(g["pregamma"][sel].astype(int) - fr.pregamma[sel].astype(int))
     assert d.max() == 0, f"deferred colour differs at {int((d > 0).any(-1).sum())} px (max {d.max()})"
-    img_sel = sel[::-1]
-    d = np.abs(g["image"][img_sel].astype(int) - fr.image[img_sel].astype(int))
+    if g["image"].shape[:2] == sel.shape:
+        img_sel = sel[::-1]
+        d = np.abs(g["image"][img_sel].astype(int) - fr.image[img_sel].astype(int))
+    else:  # MSAA: the image is the resized frame, compared whole
+        d = np.abs(g["image"].astype(int) - fr.image.astype(int))
     assert d.max() == 0, f"image differs at {int((d > 0).any(-1).sum())} px"
 
 
@@ -149,12 +152,26 @@ def test_background_when_pixel_zero_uncovered(device, orc):
     assert_frame_equal(g, fr)
 
 
-def test_msaa_is_unsupported(device):
+@pytest.mark.parametrize("m", [2, 3])
+def test_msaa_bit_exact(device, orc, m):
+    """MSAA m: every pass at (W*m, H*m), the viewport-box quirk
+    (render/raster.go:416-439), then Resize after gamma: all buffers and the
+    resized image bit-exact."""
+    s = golden_cases.c1_shadow_ao() if m == 2 else scenes.tiny(77, width=45, height=31, n_tris=24)
+    s.width, s.height = (96, 72) if m == 2 else (s.width, s.height)
+    s.msaa = m
+    g = gpu_frame(device, s)
+    fr = orc.render(s)
+    assert g["image"].shape == (s.height, s.width, 4) and g["order"].shape == (s.height * m, s.width * m)
+    assert_frame_equal(g, fr, scene=s)
+
+
+def test_msaa_out_of_range_is_invalid(device):
     s = scenes.c1_plain(width=32, height=32)
     with pytest.raises(abi.PrhError) as e:
-        render.NewRenderer(render.Size(32, 32), render.Camera(s.camera), render.Scene(s), render.MSAA(2),
+        render.NewRenderer(render.Size(32, 32), render.Camera(s.camera), render.Scene(s), render.MSAA(17),
                            render.HIP(device))
-    assert e.value.code == abi.PRH_E_UNSUPPORTED
+    assert e.value.code == abi.PRH_E_INVALID
 
 
 def test_bad_scene_is_rejected(device):
