@@ -192,8 +192,43 @@ typedef struct prh_frame_stats {
     int32_t n_clipped;   /* triangles that went through the clipper         */
     int32_t n_bin_pairs; /* (primitive, tile) pairs                         */
     int32_t nan_rerun;   /* 1 if the NaN-depth exact-order pass ran         */
+    float ms_diff;       /* the last prh_diff_forward / _backward kernel    */
 } prh_frame_stats;
 int prh_last_stats(prh_renderer* r, prh_frame_stats* out);
+
+/* ---- differentiable shading (no counterpart in polyred) ---------------
+ * The forward and the adjoint of the deferred Blinn-Phong shading at the
+ * fixed visibility of the renderer's last frame, for gradient-based inverse
+ * rendering (the Go side exposes it as Renderer.Backward).  The shading is
+ * FragmentShader's sums before Round (shader/blinn_cpu.go:24-106) with
+ * real-valued parameters: a per-vertex albedo interpolated perspective-
+ * correctly (in place of the uniform texture colour), per-vertex normals,
+ * per-material Kd, Ks (0..255 scale) and shininess, per-light position and
+ * intensity.  The loss is the sum over shaded pixels and RGB of
+ * (linear - target)^2.  Buffers are MSAA-sized raster rows.  At most 8
+ * lights (PRH_E_UNSUPPORTED above). */
+typedef struct prh_diff_params {
+    const float* albedo;    /* [n_vertices][3] per transformed vertex            */
+    const float* normal;    /* [n_vertices][3], or NULL: the scene's normals
+                               transformed by each object's normal matrix       */
+    const float* kd;        /* [n_materials][3]                                  */
+    const float* ks;        /* [n_materials][3]                                  */
+    const float* shininess; /* [n_materials]                                     */
+    const float* light_pos; /* [n_lights][3] (directional lights: unused)        */
+    const float* light_int; /* [n_lights]                                        */
+} prh_diff_params;
+typedef struct prh_diff_grads { /* host outputs with prh_diff_params' shapes; any may be NULL */
+    float *albedo, *normal, *kd, *ks, *shininess, *light_pos, *light_int;
+} prh_diff_grads;
+
+/* Transformed vertices (objects x mesh vertices, object order), materials
+ * and lights of the renderer's scene. */
+int prh_diff_sizes(prh_renderer* r, int32_t* n_vertices, int32_t* n_materials, int32_t* n_lights);
+/* Linear shading of every pixel: float [H][W][3], NaN where not shaded. */
+int prh_diff_forward(prh_renderer* r, const prh_diff_params* p, float* linear, size_t bytes);
+/* Loss and its gradients; target is float [H][W][3]. */
+int prh_diff_backward(prh_renderer* r, const prh_diff_params* p, const float* target, size_t bytes, double* loss,
+                      const prh_diff_grads* grads);
 
 /* Library build identification ("libprh gfx950 ..."). */
 const char* prh_version(void);

@@ -72,6 +72,10 @@ def lib() -> C.CDLL:
                                              C.POINTER(abi.prh_light), C.c_int, f32p, u8p, f32p]
         L.oracle_ao_dirs.argtypes = [f32p]
         L.oracle_resize.argtypes = [u8p, C.c_int, C.c_int, u8p, C.c_int, C.c_int]
+        f64p = C.POINTER(C.c_double)
+        L.oracle_diff.argtypes = [C.POINTER(abi.prh_scene_desc), C.POINTER(abi.prh_camera),
+                                  C.POINTER(abi.prh_options), C.POINTER(C.c_uint32), C.POINTER(oracle_diff_params),
+                                  f64p, f64p, C.POINTER(oracle_diff_grads), f64p]
         _lib = L
     return _lib
 
@@ -132,6 +136,48 @@ def render(scene, mask=None) -> Frame:
     fr.pregamma = fr.pregamma.reshape(H, W, 4)
     fr.image = fr.image.reshape(scene.height, scene.width, 4)
     return fr
+
+
+class oracle_diff_params(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("albedo", "normal", "kd", "ks", "shininess", "light_pos",
+                                                     "light_int")]
+
+
+class oracle_diff_grads(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("albedo", "normal", "kd", "ks", "shininess", "light_pos",
+                                                     "light_int")]
+
+
+def _dptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def diff(scene, vis_order: np.ndarray, params, target=None, grads=True):
+    """The FP64 reference of the differentiable shading at the visibility
+    `vis_order` ((H*m, W*m) uint32, as Frame.vis_order): returns (loss,
+    linear (H*m, W*m, 3) float64, gradients as a dict of float64 arrays or
+    None).  `params` is a polyred_amd.diff.DiffParams."""
+    names = ("albedo", "normal", "kd", "ks", "shininess", "light_pos", "light_int")
+    p64 = {n: (None if getattr(params, n) is None else np.ascontiguousarray(getattr(params, n), np.float64))
+           for n in names}
+    ps = oracle_diff_params(**{n: _dptr(a) for n, a in p64.items()})
+    vis = np.ascontiguousarray(vis_order, np.uint32)
+    H, W = vis.shape
+    lin = np.empty((H, W, 3), np.float64)
+    tgt = None if target is None else np.ascontiguousarray(target, np.float64)
+    g = None
+    gs = None
+    if grads and tgt is not None:
+        g = {n: np.zeros_like(p64["albedo"] if n == "normal" and p64[n] is None else p64[n]) for n in names}
+        gs = oracle_diff_grads(**{n: _dptr(a) for n, a in g.items()})
+    loss = C.c_double()
+    desc, cam, opts = scene.desc(), scene.camera_struct(), scene.options_struct()
+    rc = lib().oracle_diff(C.byref(desc), C.byref(cam), C.byref(opts), vis.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           C.byref(ps), _dptr(tgt), _dptr(lin), C.byref(gs) if gs is not None else None,
+                           C.byref(loss))
+    if rc != 0:
+        raise RuntimeError(f"oracle_diff failed: {rc}")
+    return loss.value, lin, g
 
 
 def resize(img: np.ndarray, width: int, height: int) -> np.ndarray:

@@ -1010,6 +1010,217 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
     return rc;
 }
 
+/* --------------------------------------------------- differentiable shading
+ * The FP64 reference of libprh's adjoint (an extension: polyred has no
+ * derivative path).  At fixed visibility, each covered pixel with a material
+ * is shaded with the Blinn-Phong sums of shader/blinn_cpu.go:24-106 (before
+ * Round), except that its albedo is the perspective-correct interpolation of
+ * per-vertex albedo parameters (where the reference reads the uniform
+ * texture colour) and its normal the unit interpolation of per-vertex normal
+ * parameters; Kd, Ks and the shininess are real-valued.  A clipped piece
+ * takes its parent triangle's vertices.  The geometry (barycentrics at the
+ * pixel centre, perspective weights, world position) is float32 as in the
+ * forward pass; the shading, the loss sum((lin - target)^2) and the
+ * gradients are float64. */
+/* parameter layout: albedo [Vt][3] per transformed vertex; normal [Vt][3] or
+ * NULL (n.Apply(Normal) of the mesh normals); kd, ks [M][3]; shininess [M];
+ * light_pos [L][3]; light_int [L]. */
+static void dunit3(const double v[3], double out[3], double* len) {
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    *len = l;
+    for (int k = 0; k < 3; k++) out[k] = v[k] / l;
+}
+/* d(v/|v|) applied to an upstream gradient: (g - u (u.g)) / |v| */
+static void dunit3_back(const double u[3], double len, const double g[3], double out[3]) {
+    double ug = u[0] * g[0] + u[1] * g[1] + u[2] * g[2];
+    for (int k = 0; k < 3; k++) out[k] = (g[k] - u[k] * ug) / len;
+}
+
+int oracle_diff(const prh_scene_desc* s, const prh_camera* cam, const prh_options* o, const uint32_t* vis_order,
+                const oracle_diff_params* p, const double* target, double* linear, oracle_diff_grads* g,
+                double* loss) {
+    if (!s || !cam || !o || !vis_order || !p || !loss) return PRH_E_INVALID;
+    const int msaa = o->msaa > 0 ? o->msaa : 1, W = o->width * msaa, H = o->height * msaa;
+    const int perspect = cam->kind == PRH_CAMERA_PERSPECTIVE;
+    /* per transformed vertex: screen xy + recipw, transformed normal, back-transformed position */
+    int64_t nv = 0, nt = 0;
+    for (int ob = 0; ob < s->n_objects; ob++) {
+        int m = s->obj_mesh[ob];
+        nv += s->mesh_vtx_off[m + 1] - s->mesh_vtx_off[m];
+        nt += s->mesh_tri_off[m + 1] - s->mesh_tri_off[m];
+    }
+    v4* scr = malloc(sizeof(v4) * (size_t)nv);
+    v4* tnor = malloc(sizeof(v4) * (size_t)nv);
+    v4* wp = malloc(sizeof(v4) * (size_t)nv);
+    int64_t* tri_start = malloc(sizeof(int64_t) * (size_t)(s->n_objects + 1));
+    int64_t* vtx_start = malloc(sizeof(int64_t) * (size_t)(s->n_objects + 1));
+    if (!scr || !tnor || !wp || !tri_start || !vtx_start) {
+        free(scr), free(tnor), free(wp), free(tri_start), free(vtx_start);
+        return PRH_E_NOMEM;
+    }
+    m4 view = view_matrix(V3a(cam->position), V3a(cam->target), V3a(cam->up)), proj = cam_proj(cam);
+    m4 vp = viewport_matrix((float)W, (float)H);
+    m4 vinv = inv(&view), pinv = inv(&proj), vpinv = inv(&vp), pv = mulm(&proj, &view);
+    tri_start[0] = vtx_start[0] = 0;
+    for (int ob = 0; ob < s->n_objects; ob++) {
+        int m = s->obj_mesh[ob];
+        m4 model = obj_model(s, ob), mi = inv(&model), normal = transpose(&mi), trans = mulm(&pv, &model);
+        int64_t base = vtx_start[ob];
+        for (int v = s->mesh_vtx_off[m]; v < s->mesh_vtx_off[m + 1]; v++) {
+            vert_t in;
+            load_vert(s, v, &in);
+            v4 q = mulv(&trans, in.pos);
+            float rw = perspect ? -1 / q.w : 1.0f;
+            v4 sp = pos4(apply4(q, &vp));
+            int64_t id = base + (v - s->mesh_vtx_off[m]);
+            scr[id] = V4(sp.x, sp.y, sp.z, rw);
+            tnor[id] = apply4(in.nor, &normal);
+            wp[id] = apply4(apply4(apply4(sp, &vpinv), &pinv), &vinv);
+        }
+        vtx_start[ob + 1] = base + (s->mesh_vtx_off[m + 1] - s->mesh_vtx_off[m]);
+        tri_start[ob + 1] = tri_start[ob] + (s->mesh_tri_off[m + 1] - s->mesh_tri_off[m]);
+    }
+    double amb = 0;
+    for (int e = 0; e < s->n_ambient; e++) amb += s->ambient_intensity[e];
+    const double camp[3] = {cam->position[0], cam->position[1], cam->position[2]};
+    double L2 = 0;
+    for (int64_t i = 0; i < (int64_t)W * H; i++) {
+        const int x = (int)(i % W), y = (int)(i / W);
+        if (linear) linear[3 * i] = linear[3 * i + 1] = linear[3 * i + 2] = NAN;
+        const uint32_t ord = vis_order[i];
+        if (ord == 0xFFFFFFFFu) continue;
+        const int64_t gt = ord >> 3;
+        int ob = 0;
+        while (tri_start[ob + 1] <= gt) ob++;
+        const int mesh = s->obj_mesh[ob];
+        const int t = s->mesh_tri_off[mesh] + (int)(gt - tri_start[ob]);
+        int64_t mat = s->tri_mat[t];
+        if (mat >= 0) mat += s->obj_mat_base[ob];
+        if (mat < 0 || mat >= s->n_materials) continue;
+        const prh_material* mt = &s->materials[mat];
+        int64_t id[3];
+        for (int k = 0; k < 3; k++) id[k] = vtx_start[ob] + s->tri_v[3 * t + k];
+        float bc[3];
+        barycoord((float)x + 0.5f, (float)y + 0.5f, scr[id[0]].x, scr[id[0]].y, scr[id[1]].x, scr[id[1]].y,
+                  scr[id[2]].x, scr[id[2]].y, bc);
+        float wc[3] = {scr[id[0]].w * bc[0], scr[id[1]].w * bc[1], scr[id[2]].w * bc[2]};
+        float norm = perspect ? 1 / (wc[0] + wc[1] + wc[2]) : 1.0f;
+        double w[3], a[3] = {0, 0, 0}, nraw[3] = {0, 0, 0}, xw[3] = {0, 0, 0};
+        for (int k = 0; k < 3; k++) {
+            w[k] = (double)(wc[k] * norm);
+            const double* A = p->albedo + 3 * id[k];
+            double N[3];
+            if (p->normal) N[0] = p->normal[3 * id[k]], N[1] = p->normal[3 * id[k] + 1], N[2] = p->normal[3 * id[k] + 2];
+            else N[0] = tnor[id[k]].x, N[1] = tnor[id[k]].y, N[2] = tnor[id[k]].z;
+            for (int c2 = 0; c2 < 3; c2++) {
+                a[c2] += w[k] * A[c2];
+                nraw[c2] += (double)bc[k] * N[c2];
+            }
+        }
+        v4 xf = interp_wpos(bc, wp[id[0]], wp[id[1]], wp[id[2]]);
+        xw[0] = xf.x, xw[1] = xf.y, xw[2] = xf.z;
+        double n[3], nl;
+        if (mt->flat_shading) {
+            v4 m1 = wp[id[0]], m2 = wp[id[1]], m3 = wp[id[2]];
+            v4 fN = unit4(cross4(sub4(m2, m1), sub4(m3, m1)));
+            n[0] = fN.x, n[1] = fN.y, n[2] = fN.z, nl = 1;
+        } else {
+            dunit3(nraw, n, &nl);
+        }
+        double vv[3] = {camp[0] - xw[0], camp[1] - xw[1], camp[2] - xw[2]}, V[3], vl;
+        dunit3(vv, V, &vl);
+        const double* kd = p->kd + 3 * mat;
+        const double* ks = p->ks + 3 * mat;
+        const double sh = p->shininess[mat];
+        enum { MAXL = 64 };
+        double Ld[MAXL], Ls[MAXL], I[MAXL], Lv[MAXL][3], Hv[MAXL][3], hl[MAXL], dl[MAXL], dv[MAXL][3], nL[MAXL],
+            nH[MAXL];
+        const int NL = s->n_lights < MAXL ? s->n_lights : MAXL;
+        double sLd = 0, sLs[3] = {0, 0, 0};
+        for (int li = 0; li < NL; li++) {
+            const prh_light* l = &s->lights[li];
+            if (l->kind == PRH_LIGHT_POINT) {
+                for (int k = 0; k < 3; k++) dv[li][k] = p->light_pos[3 * li + k] - xw[k];
+                dunit3(dv[li], Lv[li], &dl[li]);
+                I[li] = p->light_int[li] / dl[li];
+            } else {
+                for (int k = 0; k < 3; k++) Lv[li][k] = -(double)l->direction[k];
+                dl[li] = 1;
+                I[li] = p->light_int[li];
+            }
+            double hv[3] = {Lv[li][0] + V[0], Lv[li][1] + V[1], Lv[li][2] + V[2]};
+            dunit3(hv, Hv[li], &hl[li]);
+            nL[li] = n[0] * Lv[li][0] + n[1] * Lv[li][1] + n[2] * Lv[li][2];
+            nH[li] = n[0] * Hv[li][0] + n[1] * Hv[li][1] + n[2] * Hv[li][2];
+            Ld[li] = nL[li] < 0 ? 0 : (nL[li] > 1 ? 1 : nL[li]);
+            const double cH = nH[li] < 0 ? 0 : (nH[li] > 1 ? 1 : nH[li]);
+            Ls[li] = pow(cH, sh);
+            sLd += Ld[li] * I[li];
+            for (int c2 = 0; c2 < 3; c2++) sLs[c2] += Ls[li] * (double)l->color[c2] * I[li];
+        }
+        double lin[3], gl[3];
+        for (int c2 = 0; c2 < 3; c2++) {
+            lin[c2] = amb * a[c2] + kd[c2] * sLd * a[c2] / 255.0 + ks[c2] * sLs[c2] / 255.0;
+            if (linear) linear[3 * i + c2] = lin[c2];
+            const double d = target ? lin[c2] - target[3 * i + c2] : 0;
+            L2 += d * d;
+            gl[c2] = 2 * d;
+        }
+        if (!g || !target) continue;
+        /* backward */
+        double ga[3], gn[3] = {0, 0, 0};
+        for (int c2 = 0; c2 < 3; c2++) {
+            if (g->kd) g->kd[3 * mat + c2] += gl[c2] * sLd * a[c2] / 255.0;
+            if (g->ks) g->ks[3 * mat + c2] += gl[c2] * sLs[c2] / 255.0;
+            ga[c2] = gl[c2] * (amb + kd[c2] * sLd / 255.0);
+        }
+        for (int li = 0; li < NL; li++) {
+            const prh_light* l = &s->lights[li];
+            double gLd = 0, gLs = 0, gI = 0;
+            for (int c2 = 0; c2 < 3; c2++) {
+                gLd += gl[c2] * kd[c2] * a[c2] * I[li] / 255.0;
+                gLs += gl[c2] * ks[c2] * (double)l->color[c2] * I[li] / 255.0;
+                gI += gl[c2] * (kd[c2] * Ld[li] * a[c2] + ks[c2] * Ls[li] * (double)l->color[c2]) / 255.0;
+            }
+            const double dLd = (nL[li] > 0 && nL[li] < 1) ? 1 : 0;
+            double dLs = 0;
+            if (nH[li] > 0 && nH[li] < 1) {
+                dLs = sh * pow(nH[li], sh - 1);
+                if (g->shininess) g->shininess[mat] += gLs * Ls[li] * log(nH[li]);
+            }
+            double gL[3], gH[3], ghv[3];
+            for (int k = 0; k < 3; k++) {
+                gn[k] += gLd * dLd * Lv[li][k] + gLs * dLs * Hv[li][k];
+                gL[k] = gLd * dLd * n[k];
+                gH[k] = gLs * dLs * n[k];
+            }
+            dunit3_back(Hv[li], hl[li], gH, ghv);
+            for (int k = 0; k < 3; k++) gL[k] += ghv[k];
+            if (l->kind == PRH_LIGHT_POINT) {
+                double gd[3];
+                dunit3_back(Lv[li], dl[li], gL, gd);
+                const double i0 = p->light_int[li], d3 = dl[li] * dl[li] * dl[li];
+                for (int k = 0; k < 3; k++) gd[k] += gI * (-i0 * dv[li][k] / d3);
+                if (g->light_pos)
+                    for (int k = 0; k < 3; k++) g->light_pos[3 * li + k] += gd[k];
+                if (g->light_int) g->light_int[li] += gI / dl[li];
+            } else if (g->light_int) {
+                g->light_int[li] += gI;
+            }
+        }
+        double gnr[3] = {0, 0, 0};
+        if (!mt->flat_shading) dunit3_back(n, nl, gn, gnr);
+        for (int k = 0; k < 3; k++)
+            for (int c2 = 0; c2 < 3; c2++) {
+                if (g->albedo) g->albedo[3 * id[k] + c2] += w[k] * ga[c2];
+                if (g->normal) g->normal[3 * id[k] + c2] += (double)bc[k] * gnr[c2];
+            }
+    }
+    *loss = L2;
+    free(scr), free(tnor), free(wp), free(tri_start), free(vtx_start);
+    return PRH_OK;
+}
+
 /* ----------------------------------------------------- known-answer hooks */
 void oracle_barycoord(const float p[2], const float a[2], const float b[2], const float c[2], float out[3]) {
     barycoord(p[0], p[1], a[0], a[1], b[0], b[1], c[0], c[1], out);

@@ -56,6 +56,25 @@ typedef struct oracle_out {
 int oracle_render(const prh_scene_desc* scene, const prh_camera* cam,
                   const prh_options* opts, oracle_out* out);
 
+/* The FP64 reference of libprh's adjoint of the shading pass at fixed
+ * visibility (oracle.c, "differentiable shading"); any gradient pointer
+ * may be NULL, target NULL computes the forward only. */
+typedef struct oracle_diff_params {
+    const double* albedo;
+    const double* normal;
+    const double* kd;
+    const double* ks;
+    const double* shininess;
+    const double* light_pos;
+    const double* light_int;
+} oracle_diff_params;
+typedef struct oracle_diff_grads {
+    double *albedo, *normal, *kd, *ks, *shininess, *light_pos, *light_int;
+} oracle_diff_grads;
+int oracle_diff(const prh_scene_desc* s, const prh_camera* cam, const prh_options* o, const uint32_t* vis_order,
+                const oracle_diff_params* p, const double* target, double* linear, oracle_diff_grads* g,
+                double* loss);
+
 /* imageutil.Resize (internal/imageutil/resize.go:16-66) of an iw x ih RGBA8
  * image to width x height. */
 int oracle_resize(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height);

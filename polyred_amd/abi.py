@@ -128,7 +128,16 @@ class prh_frame_stats(C.Structure):
         ("n_clipped", C.c_int32),
         ("n_bin_pairs", C.c_int32),
         ("nan_rerun", C.c_int32),
+        ("ms_diff", C.c_float),
     ]
+
+
+class prh_diff_params(C.Structure):
+    _fields_ = [(n, _f32p) for n in ("albedo", "normal", "kd", "ks", "shininess", "light_pos", "light_int")]
+
+
+class prh_diff_grads(C.Structure):
+    _fields_ = [(n, _f32p) for n in ("albedo", "normal", "kd", "ks", "shininess", "light_pos", "light_int")]
 
 
 # Every symbol include/prh.h declares, with its prototype.
@@ -146,6 +155,10 @@ PROTOTYPES = {
     "prh_read_buffer": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
     "prh_last_stats": (C.c_int, [C.c_void_p, C.POINTER(prh_frame_stats)]),
     "prh_version": (C.c_char_p, []),
+    "prh_diff_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "prh_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t]),
+    "prh_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t,
+                                    C.POINTER(C.c_double), C.POINTER(prh_diff_grads)]),
 }
 
 _lib = None

@@ -39,6 +39,29 @@ struct ShadeBuffers {
     int32_t ao_fast;           // the tables hold for this frame size
 };
 
+// The differentiable shading pass (diff.hip): parameters, fixed inputs and
+// gradient outputs, all in HBM.
+constexpr int DIFF_MAX_LIGHTS = 8;
+struct DiffBuffers {
+    const uint64_t* vis;       // fixed visibility (the last frame)
+    const float4* tv_scr;      // screen xy + recipw per transformed vertex
+    const float4* tv_nor;      // n.Apply(Normal) per transformed vertex (the default normals)
+    const float4* tv_wp;       // back-transformed position per transformed vertex
+    const float* albedo;       // [Vt][3]
+    const float* normal;       // [Vt][3], or null: tv_nor
+    const float* kd;           // [M][3]
+    const float* ks;           // [M][3]
+    const float* shininess;    // [M]
+    const float* light_pos;    // [L][3]
+    const float* light_int;    // [L]
+    float ambient;             // sum of the ambient intensities
+    const float* target;       // [H][W][3] raster rows (adjoint)
+    float* linear;             // [H][W][3] raster rows (forward), NaN where not shaded
+    float *g_albedo, *g_normal, *g_kd, *g_ks, *g_shin, *g_lpos, *g_lint;  // zeroed before the adjoint
+    double* loss;
+};
+void launch_diff(hipStream_t st, bool backward, const DevScene& s, const ViewConsts& v, const DiffBuffers& b);
+
 // Layout of the AO march tables, in float (4-byte) units.  Binade buckets are
 // frexp exponents AO_EXP_MIN .. AO_EXP_MIN + AO_EXP_N - 1.
 constexpr int AO_EXP_MIN = -10;

@@ -289,6 +289,7 @@ struct prh_scene {
     std::vector<int> light_slot;
     int n_slots;
     float center[3];
+    float ambient_sum;  // sum of the ambient intensities (differentiable shading)
 };
 
 struct prh_renderer {
@@ -332,6 +333,13 @@ struct prh_renderer {
     Capacities cap_main, cap_shadow;
     hipEvent_t ev[8];
     prh_frame_stats stats;
+    bool has_frame;  // the visibility of a frame is in vis
+    // differentiable shading, allocated on first use: parameters and
+    // gradients in one block each (albedo, normal, kd, ks, shininess,
+    // light_pos, light_int), the target, the forward output, the loss
+    float *diff_par, *diff_grad, *diff_target, *diff_lin;
+    double* diff_loss;
+    hipEvent_t dev[2];
 };
 
 namespace {
@@ -476,6 +484,8 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
     prh_scene* s = new prh_scene();
     s->ctx = ctx;
     s->n_slots = 0;
+    s->ambient_sum = 0.0f;
+    for (int e = 0; e < d->n_ambient; e++) s->ambient_sum += d->ambient_intensity[e];
     std::vector<DevLight> lights(d->n_lights);
     for (int i = 0; i < d->n_lights; i++) {
         const prh_light& l = d->lights[i];
@@ -756,6 +766,8 @@ int prh_renderer_free(prh_renderer* r) {
     if (r->h_cnt) (void)hipHostFree(r->h_cnt);
     for (int i = 0; i < 8; i++)
         if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
+    for (int i = 0; i < 2; i++)
+        if (r->dev[i]) (void)hipEventDestroy(r->dev[i]);
     r->sc->live_renderers--;
     delete r;
     return PRH_OK;
@@ -834,12 +846,104 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     (void)hipMemcpyAsync(r->h_cnt, r->cnt, 2 * sizeof(Counters), hipMemcpyDeviceToHost, st);
 }
 
+// Offsets (in floats) of the differentiable parameters / gradients block.
+struct DiffLayout {
+    size_t albedo, normal, kd, ks, shin, lpos, lint, total;
+};
+DiffLayout diff_layout(const prh_renderer* r) {
+    const size_t V = r->sc->d.n_tverts, M = r->sc->d.n_materials, L = r->sc->d.n_lights;
+    DiffLayout l;
+    l.albedo = 0, l.normal = 3 * V, l.kd = 6 * V, l.ks = l.kd + 3 * M, l.shin = l.ks + 3 * M;
+    l.lpos = l.shin + M, l.lint = l.lpos + 3 * L, l.total = l.lint + L + 1;
+    return l;
+}
+
+// One launch of the differentiable shading: forward (target == nullptr,
+// result in diff_lin) or adjoint (loss and gradients copied to the host).
+int diff_run(prh_renderer* r, const prh_diff_params* p, const float* target, double* loss,
+             const prh_diff_grads* grads) {
+    const DevScene& d = r->sc->d;
+    if (!r->has_frame) return fail(PRH_E_INVALID, "differentiable shading needs a rendered frame (prh_render)");
+    if (d.n_lights > DIFF_MAX_LIGHTS) return fail(PRH_E_UNSUPPORTED, "differentiable shading takes at most 8 lights");
+    if ((d.n_tverts && !p->albedo) || (d.n_materials && (!p->kd || !p->ks || !p->shininess)) ||
+        (d.n_lights && (!p->light_pos || !p->light_int)))
+        return fail(PRH_E_INVALID, "prh_diff_params: missing parameter array");
+    hipStream_t st = r->ctx->stream;
+    const DiffLayout l = diff_layout(r);
+    const size_t P = (size_t)r->W * r->H;
+    if (!r->diff_par) {
+        int rc = 0;
+        auto chk = [&](int v) {
+            if (!rc) rc = v;
+        };
+        chk(ralloc(r, &r->diff_par, l.total));
+        if (!rc) chk(ralloc(r, &r->diff_grad, l.total));
+        if (!rc) chk(ralloc(r, &r->diff_target, 3 * P));
+        if (!rc) chk(ralloc(r, &r->diff_lin, 3 * P));
+        if (!rc) chk(ralloc(r, &r->diff_loss, 1));
+        for (int i = 0; i < 2 && !rc; i++)
+            if (hipEventCreate(&r->dev[i]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
+        if (rc) {
+            r->diff_par = nullptr;  // (the allocations stay on the renderer's list)
+            return rc;
+        }
+    }
+    const size_t V = d.n_tverts, M = d.n_materials, L = d.n_lights;
+    auto up = [&](size_t off, const float* src, size_t n) -> hipError_t {
+        return n ? hipMemcpyAsync(r->diff_par + off, src, n * 4, hipMemcpyHostToDevice, st) : hipSuccess;
+    };
+    HIP_TRY(up(l.albedo, p->albedo, 3 * V));
+    if (p->normal) HIP_TRY(up(l.normal, p->normal, 3 * V));
+    HIP_TRY(up(l.kd, p->kd, 3 * M));
+    HIP_TRY(up(l.ks, p->ks, 3 * M));
+    HIP_TRY(up(l.shin, p->shininess, M));
+    HIP_TRY(up(l.lpos, p->light_pos, 3 * L));
+    HIP_TRY(up(l.lint, p->light_int, L));
+    DiffBuffers b;
+    b.vis = r->vis, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
+    b.albedo = r->diff_par + l.albedo, b.normal = p->normal ? r->diff_par + l.normal : nullptr;
+    b.kd = r->diff_par + l.kd, b.ks = r->diff_par + l.ks, b.shininess = r->diff_par + l.shin;
+    b.light_pos = r->diff_par + l.lpos, b.light_int = r->diff_par + l.lint;
+    b.ambient = r->sc->ambient_sum;
+    b.target = r->diff_target, b.linear = r->diff_lin;
+    b.g_albedo = r->diff_grad + l.albedo, b.g_normal = r->diff_grad + l.normal, b.g_kd = r->diff_grad + l.kd;
+    b.g_ks = r->diff_grad + l.ks, b.g_shin = r->diff_grad + l.shin, b.g_lpos = r->diff_grad + l.lpos;
+    b.g_lint = r->diff_grad + l.lint;
+    b.loss = r->diff_loss;
+    const bool backward = target != nullptr;
+    if (backward) {
+        HIP_TRY(hipMemcpyAsync(r->diff_target, target, 12 * P, hipMemcpyHostToDevice, st));
+        launch_fill_u32(st, reinterpret_cast<uint32_t*>(r->diff_grad), l.total, 0u);
+        launch_fill_u64(st, reinterpret_cast<uint64_t*>(r->diff_loss), 1, 0ull);
+    }
+    (void)hipEventRecord(r->dev[0], st);
+    launch_diff(st, backward, d, r->vc, b);
+    (void)hipEventRecord(r->dev[1], st);
+    if (backward) {
+        auto down = [&](float* dst, size_t off, size_t n) -> hipError_t {
+            return dst && n ? hipMemcpyAsync(dst, r->diff_grad + off, n * 4, hipMemcpyDeviceToHost, st) : hipSuccess;
+        };
+        HIP_TRY(down(grads->albedo, l.albedo, 3 * V));
+        HIP_TRY(down(grads->normal, l.normal, 3 * V));
+        HIP_TRY(down(grads->kd, l.kd, 3 * M));
+        HIP_TRY(down(grads->ks, l.ks, 3 * M));
+        HIP_TRY(down(grads->shininess, l.shin, M));
+        HIP_TRY(down(grads->light_pos, l.lpos, 3 * L));
+        HIP_TRY(down(grads->light_int, l.lint, L));
+        HIP_TRY(hipMemcpyAsync(loss, r->diff_loss, 8, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    (void)hipEventElapsedTime(&r->stats.ms_diff, r->dev[0], r->dev[1]);
+    return PRH_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int prh_render(prh_renderer* r) {
     if (!r) return fail(PRH_E_INVALID, "prh_render: null renderer");
+    r->has_frame = false;
     HIP_TRY(hipSetDevice(r->ctx->device));
     hipStream_t st = r->ctx->stream;
     bool nan_rerun = false;
@@ -884,6 +988,7 @@ int prh_render(prh_renderer* r) {
         (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
         S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
         S.nan_rerun = nan_rerun ? 1 : 0;
+        r->has_frame = true;
         return PRH_OK;
     }
 }
@@ -940,6 +1045,31 @@ int prh_last_stats(prh_renderer* r, prh_frame_stats* out) {
     if (!r || !out) return fail(PRH_E_INVALID, "prh_last_stats: null argument");
     *out = r->stats;
     return PRH_OK;
+}
+
+int prh_diff_sizes(prh_renderer* r, int32_t* n_vertices, int32_t* n_materials, int32_t* n_lights) {
+    if (!r || !n_vertices || !n_materials || !n_lights) return fail(PRH_E_INVALID, "prh_diff_sizes: null argument");
+    *n_vertices = r->sc->d.n_tverts, *n_materials = r->sc->d.n_materials, *n_lights = r->sc->d.n_lights;
+    return PRH_OK;
+}
+
+int prh_diff_forward(prh_renderer* r, const prh_diff_params* p, float* linear, size_t bytes) {
+    if (!r || !p || !linear) return fail(PRH_E_INVALID, "prh_diff_forward: null argument");
+    if (bytes < (size_t)12 * r->W * r->H) return fail(PRH_E_INVALID, "prh_diff_forward: buffer too small");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    int rc = diff_run(r, p, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(linear, r->diff_lin, (size_t)12 * r->W * r->H, hipMemcpyDeviceToHost, r->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+    return PRH_OK;
+}
+
+int prh_diff_backward(prh_renderer* r, const prh_diff_params* p, const float* target, size_t bytes, double* loss,
+                      const prh_diff_grads* grads) {
+    if (!r || !p || !target || !loss || !grads) return fail(PRH_E_INVALID, "prh_diff_backward: null argument");
+    if (bytes < (size_t)12 * r->W * r->H) return fail(PRH_E_INVALID, "prh_diff_backward: target too small");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    return diff_run(r, p, target, loss, grads);
 }
 
 }  // extern "C"

@@ -253,6 +253,19 @@ class Renderer:
         abi.check(self._lib.prh_read_buffer(self.handle, which, out.ctypes.data, out.nbytes), "prh_read_buffer")
         return out
 
+    # -------- differentiable shading (an extension: polyred has no adjoint)
+    def ForwardLinear(self, params) -> np.ndarray:
+        """Linear RGB (H*m, W*m, 3) of the shading with `params` (a
+        diff.DiffParams) at the last frame's visibility; NaN where unshaded."""
+        from . import diff
+        return diff.forward(self._lib, self.handle, params, self.H * self.msaa, self.W * self.msaa)
+
+    def Backward(self, params, target: np.ndarray):
+        """(loss, gradients): loss = sum over shaded pixels of
+        |ForwardLinear(params) - target|^2, gradients a diff.DiffParams."""
+        from . import diff
+        return diff.backward(self._lib, self.handle, params, target, self.H * self.msaa, self.W * self.msaa)
+
     def stats(self) -> abi.prh_frame_stats:
         st = abi.prh_frame_stats()
         abi.check(self._lib.prh_last_stats(self.handle, C.byref(st)), "prh_last_stats")

@@ -392,6 +392,40 @@ def c2_scale(seed: int = 2, width: int = 1920, height: int = 1080, n_spheres: in
                  name=f"c2_scale_{width}x{height}_{n_spheres}sph")
 
 
+def c4_adjoint(seed: int = 4, width: int = 512, height: int = 512, n_spheres: int = 104) -> Scene:
+    """Forward + adjoint: a 512x512 view of ~100k triangles (104 instanced
+    UV spheres of 960 triangles, scale log-uniform in [0.3,1.5] in a 12^3
+    box), 8 materials as the 1080p workload's, 4 point lights, no shadows or
+    AO.  Every sphere vertex has its own colour (the albedo parameters start
+    from them)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mats = []
+    for _ in range(8):
+        kd = rng.uniform(0.1, 0.9, 3)
+        ks = rng.uniform(0.0, 1.0)
+        shin = math.exp(rng.uniform(math.log(4), math.log(128)))
+        mats.append(_mat_from(kd, ks, shin, ao=False, shadow=False))
+    sphere = uv_sphere()
+    sphere.col[:, :3] = rng.integers(64, 256, (sphere.col.shape[0], 3)).astype(np.uint8)
+    models, bases = [], []
+    for _ in range(n_spheres):
+        s = math.exp(rng.uniform(math.log(0.3), math.log(1.5)))
+        t = rng.uniform(-6, 6, 3)
+        models.append(model_matrix(s, random_rotation(rng), t))
+        bases.append(int(rng.integers(0, 8)))
+    lights = []
+    for _ in range(4):
+        p = (rng.uniform(-8, 8), rng.uniform(2, 10), rng.uniform(-8, 8))
+        c = tuple(int(x) for x in rng.integers(200, 256, 3)) + (255,)
+        lights.append(Light(kind=abi.PRH_LIGHT_POINT, position=tuple(float(F32(x)) for x in p), intensity=4.0,
+                            color=c))
+    return Scene(meshes=[sphere], obj_mesh=np.zeros(n_spheres, np.int64), obj_model=np.stack(models),
+                 obj_mat_base=np.array(bases), materials=mats, lights=lights, ambient=[0.1],
+                 camera=dict(position=(0.0, 3.0, 18.0), target=(0.0, 0.0, 0.0), fov=45.0, near=0.1, far=100.0),
+                 width=width, height=height, shadow_map=False, gamma=True, shadow_size=0,
+                 name=f"c4_adjoint_{width}x{height}_{n_spheres}sph")
+
+
 def tiny(seed: int, width: int = 32, height: int = 24, n_tris: int = 6, vertex_color: bool = True,
          perspective: bool = True) -> Scene:
     """Small random-triangle scenes for brute-force raster checks: triangles
