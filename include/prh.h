@@ -164,6 +164,18 @@ int prh_renderer_free(prh_renderer* r);
  * complete on the device; the image stays in HBM until read. */
 int prh_render(prh_renderer* r);
 
+/* One frame of each of n renderers (all of one context), as n prh_render
+ * calls would produce them -- shadow passes included -- enqueued together
+ * over the context's batch streams with no host synchronisation between
+ * views; each view's counters are checked once at the end and a view that
+ * overflowed its work buffers is rendered again on its own.  With
+ * PRH_BATCH_GRAPH the whole batch is captured once into a HIP graph (kept
+ * until the batch changes) and replayed; per-pass statistics are then not
+ * recorded.  *ms (may be NULL) receives the batch's device time.  No
+ * counterpart in polyred (one renderer draws one view). */
+#define PRH_BATCH_GRAPH 1
+int prh_render_batch(prh_renderer* const* renderers, int32_t n, int32_t flags, float* ms);
+
 /* The rendered RGBA8 image, top row first, the layout of
  * FragmentBuffer.Image() (buffer/buffer.go:160-166).  bytes >= 4*W*H. */
 int prh_read_image(prh_renderer* r, uint8_t* rgba, size_t bytes);

@@ -25,6 +25,8 @@ PRH_LIGHT_DIRECTIONAL = 1
 PRH_CAMERA_PERSPECTIVE = 0
 PRH_CAMERA_ORTHOGRAPHIC = 1
 
+PRH_BATCH_GRAPH = 1
+
 PRH_BUF_VIS = 0
 PRH_BUF_DEPTH = 1
 PRH_BUF_LINEAR = 2
@@ -155,6 +157,7 @@ PROTOTYPES = {
     "prh_read_buffer": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
     "prh_last_stats": (C.c_int, [C.c_void_p, C.POINTER(prh_frame_stats)]),
     "prh_version": (C.c_char_p, []),
+    "prh_render_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_float)]),
     "prh_diff_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "prh_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t]),
     "prh_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t,

@@ -273,10 +273,22 @@ int upload(T** p, const T* src, size_t n) {
 
 }  // namespace
 
+struct prh_renderer;
+constexpr int PRH_BATCH_STREAMS = 4;  // GPU_MAX_HW_QUEUES is 4 by default
+
 struct prh_ctx {
     int device;
     hipStream_t stream;
     int live_scenes;  // a context outlives every scene uploaded to it
+    // prh_render_batch: its streams and events (created on first use) and
+    // the graph of the last captured batch, keyed by its renderers and their
+    // work-buffer generations
+    hipStream_t bst[PRH_BATCH_STREAMS];
+    hipEvent_t bev[2], bfork, bjoin[PRH_BATCH_STREAMS];
+    bool batch_ready;
+    hipGraph_t graph;
+    hipGraphExec_t graph_exec;
+    std::vector<std::pair<prh_renderer*, int>> graph_key;
 };
 
 struct prh_scene {
@@ -334,6 +346,7 @@ struct prh_renderer {
     hipEvent_t ev[8];
     prh_frame_stats stats;
     bool has_frame;  // the visibility of a frame is in vis
+    int work_gen;    // bumped when the work buffers are reallocated
     // differentiable shading, allocated on first use: parameters and
     // gradients in one block each (albedo, normal, kd, ks, shininess,
     // light_pos, light_int), the target, the forward output, the loss
@@ -343,6 +356,8 @@ struct prh_renderer {
 };
 
 namespace {
+
+void drop_graph(prh_ctx* ctx);
 
 template <class T>
 int ralloc(prh_renderer* r, T** p, size_t n) {
@@ -399,7 +414,8 @@ int prh_open(int32_t ordinal, prh_ctx** out) {
     HIP_TRY(hipSetDevice(ordinal));
     hipStream_t st;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    prh_ctx* c = new prh_ctx{ordinal, st, 0};
+    prh_ctx* c = new prh_ctx();
+    c->device = ordinal, c->stream = st, c->live_scenes = 0;
     *out = c;
     return PRH_OK;
 }
@@ -410,6 +426,15 @@ int prh_close(prh_ctx* ctx) {
         return fail(PRH_E_INVALID, "prh_close: " + std::to_string(ctx->live_scenes) + " scene(s) still alive");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    drop_graph(ctx);
+    if (ctx->batch_ready) {
+        for (int k = 0; k < PRH_BATCH_STREAMS; k++) {
+            (void)hipStreamSynchronize(ctx->bst[k]);
+            (void)hipStreamDestroy(ctx->bst[k]);
+            (void)hipEventDestroy(ctx->bjoin[k]);
+        }
+        (void)hipEventDestroy(ctx->bev[0]), (void)hipEventDestroy(ctx->bev[1]), (void)hipEventDestroy(ctx->bfork);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PRH_OK;
@@ -762,6 +787,11 @@ int prh_renderer_free(prh_renderer* r) {
     if (!r) return fail(PRH_E_INVALID, "prh_renderer_free: null renderer");
     (void)hipSetDevice(r->ctx->device);
     (void)hipStreamSynchronize(r->ctx->stream);
+    for (const auto& kv : r->ctx->graph_key)
+        if (kv.first == r) {
+            drop_graph(r->ctx);
+            break;
+        }
     free_list(r->allocs);
     if (r->h_cnt) (void)hipHostFree(r->h_cnt);
     for (int i = 0; i < 8; i++)
@@ -780,13 +810,14 @@ namespace {
 int d_n_tris(const prh_renderer* r) { return r->sc->d.n_tris; }
 
 // Enqueues one frame; nan_mode re-runs the visibility raster in exact order.
-void enqueue_frame(prh_renderer* r, bool nan_mode) {
-    hipStream_t st = r->ctx->stream;
+// `timed` records the per-pass events (not while a batch is captured into a
+// graph).
+void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     const DevScene& d = r->sc->d;
     Counters* cm = r->cnt;
     Counters* cs = r->cnt + 1;
     const int persistent = r->n_cu * 4;
-    (void)hipEventRecord(r->ev[0], st);
+    if (timed) (void)hipEventRecord(r->ev[0], st);
     launch_fill_u32(st, (uint32_t*)r->cnt, 2 * sizeof(Counters) / 4, 0u);
     if (r->n_slots > 0 && !nan_mode) {
         const int nt = r->n_slots * r->stiles_per_map;
@@ -801,9 +832,9 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
         launch_raster(st, 2, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                       r->tile_list, cs, rt);
     }
-    (void)hipEventRecord(r->ev[1], st);
+    if (timed) (void)hipEventRecord(r->ev[1], st);
     launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
-    (void)hipEventRecord(r->ev[2], st);
+    if (timed) (void)hipEventRecord(r->ev[2], st);
     launch_fill_u32(st, (uint32_t*)r->tile_count, r->n_tiles, 0u);
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
                       r->tile_count, r->blk_prims, cm, r->cap_main);
@@ -811,7 +842,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
                 (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
     launch_bin(st, r->pbox, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
                d.n_tris);
-    (void)hipEventRecord(r->ev[3], st);
+    if (timed) (void)hipEventRecord(r->ev[3], st);
     const size_t P = (size_t)r->W * r->H;
     launch_fill_u64(st, r->vis, P, VIS_EMPTY);
     if (nan_mode) {
@@ -823,7 +854,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                   r->tile_list, cm, rt);
     launch_resolve(st, nan_mode, rt, cm, r->depth);
-    (void)hipEventRecord(r->ev[4], st);
+    if (timed) (void)hipEventRecord(r->ev[4], st);
     ShadeBuffers b;
     b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
     b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
@@ -833,17 +864,127 @@ void enqueue_frame(prh_renderer* r, bool nan_mode) {
     b.ao = r->ao;
     b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
     launch_ao(st, d, r->sh, b);
-    (void)hipEventRecord(r->ev[5], st);
+    if (timed) (void)hipEventRecord(r->ev[5], st);
     launch_shade_bg(st, d, r->sh, b);
-    (void)hipEventRecord(r->ev[7], st);
+    if (timed) (void)hipEventRecord(r->ev[7], st);
     launch_shade(st, d, r->sh, b);
     if (r->msaa > 1) {
         // passAntialiasing (render/raster.go:361-378): Resize after gamma
         launch_resize(st, r->image_big, r->W, r->resize_tmp, r->H, r->OW, r->rz_cx, r->rz_ox, r->rz_flx);
         launch_resize(st, r->resize_tmp, r->H, r->image, r->OW, r->OH, r->rz_cy, r->rz_oy, r->rz_fly);
     }
-    (void)hipEventRecord(r->ev[6], st);
+    if (timed) (void)hipEventRecord(r->ev[6], st);
     (void)hipMemcpyAsync(r->h_cnt, r->cnt, 2 * sizeof(Counters), hipMemcpyDeviceToHost, st);
+}
+
+// After a frame's work has completed: after an overflow, grows the work
+// buffers and asks for another attempt (again = true); after a NaN depth,
+// re-runs the visibility in exact order; then fills the statistics (per-pass
+// times only if the frame recorded its events).
+int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
+    again = false;
+    hipStream_t st = r->ctx->stream;
+    const Counters cm = r->h_cnt[0], cs = r->h_cnt[1];
+    bool grow = false;
+    if (cs.overflow || cs.n_pairs > r->cap_shadow.pairs) {
+        r->cap_shadow.pairs = std::max(r->cap_shadow.pairs, cs.n_pairs + cs.n_pairs / 4 + 16);
+        grow = true;
+    }
+    if (cm.overflow || cm.n_pairs > r->cap_main.pairs || cm.n_clip_subs > r->cap_main.clip_subs ||
+        cm.n_clipped > r->cap_main.clip_tris) {
+        r->cap_main.pairs = std::max(r->cap_main.pairs, cm.n_pairs + cm.n_pairs / 4 + 16);
+        r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, cm.n_clip_subs + cm.n_clip_subs / 4 + 16);
+        r->cap_main.clip_tris = std::max(r->cap_main.clip_tris, cm.n_clipped + cm.n_clipped / 4 + 16);
+        r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, 7 * r->cap_main.clip_tris);
+        r->cap_main.prims = d_n_tris(r) + r->cap_main.clip_subs;
+        grow = true;
+    }
+    if (grow) {
+        if (attempt >= 3) return fail(PRH_E_NOMEM, "work buffers keep overflowing");
+        free_work(r);
+        int rc = alloc_work(r);
+        if (rc) return rc;
+        r->work_gen++;
+        again = true;
+        return PRH_OK;
+    }
+    if (cs.shadow_nan)
+        return fail(PRH_E_UNSUPPORTED, "NaN depth in a shadow pass: its result depends on sequential order");
+    bool nan_rerun = false;
+    if (cm.nan_flag) {
+        enqueue_frame(r, true, st, timed);
+        HIP_TRY(hipStreamSynchronize(st));
+        nan_rerun = true;
+    }
+    prh_frame_stats& S = r->stats;
+    if (timed) {
+        float ms[6] = {0};
+        for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
+        (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
+        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
+        (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
+    } else {
+        S.ms_total = S.ms_shadow = S.ms_vertex = S.ms_setup = S.ms_raster = S.ms_ao = S.ms_shade = 0.0f;
+    }
+    S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
+    S.nan_rerun = nan_rerun ? 1 : 0;
+    r->has_frame = true;
+    return PRH_OK;
+}
+
+void drop_graph(prh_ctx* ctx) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr, ctx->graph = nullptr;
+    ctx->graph_key.clear();
+}
+
+int batch_streams(prh_ctx* ctx) {
+    if (ctx->batch_ready) return PRH_OK;
+    for (int k = 0; k < PRH_BATCH_STREAMS; k++) {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->bst[k], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->bjoin[k], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreate(&ctx->bev[0]));
+    HIP_TRY(hipEventCreate(&ctx->bev[1]));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->bfork, hipEventDisableTiming));
+    ctx->batch_ready = true;
+    return PRH_OK;
+}
+
+// Captures one frame of every renderer, spread over the batch streams, into
+// a graph (kept until the batch, or a renderer's work buffers, change).
+int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n) {
+    std::vector<std::pair<prh_renderer*, int>> key(n);
+    for (int i = 0; i < n; i++) key[i] = {rs[i], rs[i]->work_gen};
+    if (ctx->graph_exec && key == ctx->graph_key) return PRH_OK;
+    drop_graph(ctx);
+    hipStream_t s0 = ctx->bst[0];
+    HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipEventRecord(ctx->bfork, s0);
+    for (int k = 1; k < PRH_BATCH_STREAMS && e == hipSuccess; k++) e = hipStreamWaitEvent(ctx->bst[k], ctx->bfork, 0);
+    if (e == hipSuccess) {
+        for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[i % PRH_BATCH_STREAMS], false);
+        for (int k = 1; k < PRH_BATCH_STREAMS && e == hipSuccess; k++) {
+            e = hipEventRecord(ctx->bjoin[k], ctx->bst[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s0, ctx->bjoin[k], 0);
+        }
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(s0, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(PRH_E_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        ctx->graph_exec = nullptr;
+        return fail(PRH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    ctx->graph = g;
+    ctx->graph_key = key;
+    return PRH_OK;
 }
 
 // Offsets (in floats) of the differentiable parameters / gradients block.
@@ -946,51 +1087,61 @@ int prh_render(prh_renderer* r) {
     r->has_frame = false;
     HIP_TRY(hipSetDevice(r->ctx->device));
     hipStream_t st = r->ctx->stream;
-    bool nan_rerun = false;
     for (int attempt = 0;; attempt++) {
-        enqueue_frame(r, false);
+        enqueue_frame(r, false, st, true);
         HIP_TRY(hipStreamSynchronize(st));
         HIP_TRY(hipGetLastError());
-        const Counters cm = r->h_cnt[0], cs = r->h_cnt[1];
-        bool grow = false;
-        if (cs.overflow || cs.n_pairs > r->cap_shadow.pairs) {
-            r->cap_shadow.pairs = std::max(r->cap_shadow.pairs, cs.n_pairs + cs.n_pairs / 4 + 16);
-            grow = true;
-        }
-        if (cm.overflow || cm.n_pairs > r->cap_main.pairs || cm.n_clip_subs > r->cap_main.clip_subs ||
-            cm.n_clipped > r->cap_main.clip_tris) {
-            r->cap_main.pairs = std::max(r->cap_main.pairs, cm.n_pairs + cm.n_pairs / 4 + 16);
-            r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, cm.n_clip_subs + cm.n_clip_subs / 4 + 16);
-            r->cap_main.clip_tris = std::max(r->cap_main.clip_tris, cm.n_clipped + cm.n_clipped / 4 + 16);
-            r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, 7 * r->cap_main.clip_tris);
-            r->cap_main.prims = d_n_tris(r) + r->cap_main.clip_subs;
-            grow = true;
-        }
-        if (grow) {
-            if (attempt >= 3) return fail(PRH_E_NOMEM, "work buffers keep overflowing");
-            free_work(r);
-            int rc = alloc_work(r);
-            if (rc) return rc;
-            continue;
-        }
-        if (cs.shadow_nan)
-            return fail(PRH_E_UNSUPPORTED, "NaN depth in a shadow pass: its result depends on sequential order");
-        if (cm.nan_flag) {
-            enqueue_frame(r, true);
-            HIP_TRY(hipStreamSynchronize(st));
-            nan_rerun = true;
-        }
-        prh_frame_stats& S = r->stats;
-        float ms[6] = {0};
-        for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
-        (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
-        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
-        (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
-        S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
-        S.nan_rerun = nan_rerun ? 1 : 0;
-        r->has_frame = true;
-        return PRH_OK;
+        bool again = false;
+        int rc = finish_frame(r, attempt, again);
+        if (rc || !again) return rc;
     }
+}
+
+int prh_render_batch(prh_renderer* const* rs, int32_t n, int32_t flags, float* ms) {
+    if (!rs || n < 0) return fail(PRH_E_INVALID, "prh_render_batch: null renderer list");
+    if (n == 0) return PRH_OK;
+    prh_ctx* ctx = rs[0] ? rs[0]->ctx : nullptr;
+    for (int i = 0; i < n; i++)
+        if (!rs[i] || rs[i]->ctx != ctx) return fail(PRH_E_INVALID, "prh_render_batch: renderers of one context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = batch_streams(ctx);
+    if (rc) return rc;
+    const bool graph = (flags & PRH_BATCH_GRAPH) != 0;
+    hipStream_t s0 = ctx->bst[0];
+    if (graph) {
+        rc = batch_graph(ctx, rs, n);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(ctx->bev[0], s0));
+    if (graph) {
+        HIP_TRY(hipGraphLaunch(ctx->graph_exec, s0));
+    } else {
+        // fork: every batch stream starts after the work already on s0
+        for (int k = 1; k < PRH_BATCH_STREAMS; k++) HIP_TRY(hipStreamWaitEvent(ctx->bst[k], ctx->bev[0], 0));
+        for (int i = 0; i < n; i++) {
+            rs[i]->has_frame = false;
+            enqueue_frame(rs[i], false, ctx->bst[i % PRH_BATCH_STREAMS], true);
+        }
+        for (int k = 1; k < PRH_BATCH_STREAMS; k++) {
+            HIP_TRY(hipEventRecord(ctx->bjoin[k], ctx->bst[k]));
+            HIP_TRY(hipStreamWaitEvent(s0, ctx->bjoin[k], 0));
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->bev[1], s0));
+    HIP_TRY(hipEventSynchronize(ctx->bev[1]));
+    HIP_TRY(hipGetLastError());
+    if (ms) (void)hipEventElapsedTime(ms, ctx->bev[0], ctx->bev[1]);
+    // per view: the same checks as prh_render; a view that overflowed its
+    // work buffers (or met a NaN depth) is rendered again on its own
+    for (int i = 0; i < n; i++) {
+        prh_renderer* r = rs[i];
+        if (graph) r->has_frame = false;
+        bool again = false;
+        rc = finish_frame(r, 0, again, !graph);
+        if (rc) return rc;
+        if (again && (rc = prh_render(r))) return rc;
+    }
+    return PRH_OK;
 }
 
 int prh_read_image(prh_renderer* r, uint8_t* rgba, size_t bytes) {

@@ -296,6 +296,25 @@ def _close_all() -> None:
             pass
 
 
+def render_batch(renderers, graph: bool = False) -> float:
+    """One frame of every renderer (all on one Device) in one batched
+    submission (prh_render_batch); with graph=True the batch is captured
+    into a HIP graph once and replayed.  Returns the batch's device time in
+    ms.  Each renderer then holds the frame a Render() would have drawn."""
+    rs = list(renderers)
+    if not rs:
+        return 0.0
+    lib = rs[0]._lib
+    arr = (C.c_void_p * len(rs))(*[r.handle for r in rs])
+    ms = C.c_float()
+    abi.check(lib.prh_render_batch(arr, len(rs), abi.PRH_BATCH_GRAPH if graph else 0, C.byref(ms)),
+              "prh_render_batch")
+    for r in rs:
+        for name in ("shadow", "forward", "deferred", "gamma"):
+            r.passGPU[name] = True
+    return ms.value
+
+
 def NewRenderer(*opts) -> Renderer:  # render/raster.go:84-143
     cfg = _Option()
     for opt in opts:

@@ -426,6 +426,55 @@ def c4_adjoint(seed: int = 4, width: int = 512, height: int = 512, n_spheres: in
                  name=f"c4_adjoint_{width}x{height}_{n_spheres}sph")
 
 
+def c3_views(seed: int = 3, n_views: int = 64, min_tris: int = 10_000, max_tris: int = 500_000,
+             min_px: int = 256 * 256, max_px: int = 1920 * 1080, shadow_size: int = 1024) -> List[Scene]:
+    """Batched forward: n_views independent views, each its own seeded
+    instanced-sphere scene.  Per view: pixel count log-uniform in
+    [256^2, 1920x1080] with aspect uniform in [0.75, 2]; triangles
+    log-uniform in [10k, 500k] (spheres of 960 triangles, scale log-uniform
+    in [0.1, 0.8] in a 16^3 box); 8 materials; 1-8 shadow-casting lights,
+    each point or directional, 1024^2 shadow maps; no AO; sRGB."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sphere = uv_sphere()
+    views = []
+    for v in range(n_views):
+        area = math.exp(rng.uniform(math.log(min_px), math.log(max_px)))
+        aspect = rng.uniform(0.75, 2.0)
+        w = max(16, int(round(math.sqrt(area * aspect))))
+        h = max(16, int(round(area / w)))
+        n_tris = math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))
+        n_sph = max(1, int(round(n_tris / sphere.tri.shape[0])))
+        mats = []
+        for _ in range(8):
+            kd = rng.uniform(0.1, 0.9, 3)
+            mats.append(_mat_from(kd, rng.uniform(0.0, 1.0), math.exp(rng.uniform(math.log(4), math.log(128))),
+                                  shadow=True))
+        models, bases = [], []
+        for _ in range(n_sph):
+            s = math.exp(rng.uniform(math.log(0.1), math.log(0.8)))
+            models.append(model_matrix(s, random_rotation(rng), rng.uniform(-8, 8, 3)))
+            bases.append(int(rng.integers(0, 8)))
+        lights = []
+        for _ in range(int(rng.integers(1, 9))):
+            c = tuple(int(x) for x in rng.integers(200, 256, 3)) + (255,)
+            if rng.uniform() < 0.75:
+                p = (rng.uniform(-8, 8), rng.uniform(0, 10), rng.uniform(-8, 8))
+                lights.append(Light(kind=abi.PRH_LIGHT_POINT, position=tuple(float(F32(x)) for x in p),
+                                    intensity=3.0, color=c, cast_shadow=True))
+            else:
+                d = unit3((rng.uniform(-0.5, 0.5), -1.0, rng.uniform(-0.5, 0.5)))
+                lights.append(Light(kind=abi.PRH_LIGHT_DIRECTIONAL, direction=d,
+                                    position=tuple(float(F32(-x * 30)) for x in d), intensity=0.6, color=c,
+                                    cast_shadow=True))
+        views.append(Scene(meshes=[sphere], obj_mesh=np.zeros(n_sph, np.int64), obj_model=np.stack(models),
+                           obj_mat_base=np.array(bases), materials=mats, lights=lights, ambient=[0.1],
+                           camera=dict(position=(0.0, 4.0, 22.0), target=(0.0, 0.0, 0.0), fov=45.0, near=0.1,
+                                       far=100.0),
+                           width=w, height=h, shadow_map=True, gamma=True, shadow_size=shadow_size,
+                           name=f"c3_view{v}_{w}x{h}_{n_sph}sph_{len(lights)}l"))
+    return views
+
+
 def tiny(seed: int, width: int = 32, height: int = 24, n_tris: int = 6, vertex_color: bool = True,
          perspective: bool = True) -> Scene:
     """Small random-triangle scenes for brute-force raster checks: triangles
