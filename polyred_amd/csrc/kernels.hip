@@ -33,6 +33,19 @@ __device__ __forceinline__ float gmax3(float a, float b, float c) {
 // Pixel box of a triangle, render/raster.go:473-477 (and shadow.go:187-191):
 // int(Round(min)-1) .. int(Round(max)+1) from the AABB of the three vertices,
 // then restricted to the buffer (FragmentBuffer.In).  Returns false if empty.
+//
+// For a small, well-shaped triangle the box is then narrowed to the pixels
+// whose centre lies within 0.24 of the AABB; the pixels dropped all fail the
+// inside test (no Barycoord >= -1e-7), so the result is the same.  Why: for a
+// centre p at least d = 0.23 outside the AABB along x (y alike), the exact
+// barycentrics l satisfy p.x - min.x = sum l_i (x_i - min.x) with at most two
+// l_i < 0, so some l_i <= -d / (2 W), W the AABB's larger side.  Over the box
+// (within W + 2 of the vertices) |l_i| <= 2 (W + 2) / (s W) when
+// |Sabc| >= s W^2, and Barycoord's float32 evaluation (the differences, the
+// fma64 areas, the division) stays within about 300 ulp * (1 + 2 / W) of l
+// for s = 1/4: under 2e-5 (W <= 64) or 4e-5 / W, far from closing the gap
+// from -0.115 / W to -1e-7.  Slivers, larger or degenerate triangles and
+// non-finite vertices keep the box.
 __device__ bool pixel_box(float ax, float ay, float az, float bx, float by, float bz, float cx, float cy, float cz,
                           int W, int H, int& x0, int& y0, int& x1, int& y1) {
     float mnx = gmin3(ax, bx, cx), mny = gmin3(ay, by, cy);
@@ -40,6 +53,15 @@ __device__ bool pixel_box(float ax, float ay, float az, float bx, float by, floa
     (void)az, (void)bz, (void)cz;
     int64_t xmin = f2i(round32(mnx) - 1), xmax = f2i(round32(mxx) + 1);
     int64_t ymin = f2i(round32(mny) - 1), ymax = f2i(round32(mxy) + 1);
+    const float wmax = fmaxf(mxx - mnx, mxy - mny);
+    if (wmax <= 64.0f && wmax >= 0x1p-20f) {  // false for NaN / Inf
+        const float sabc = tri_setup(ax, ay, bx, by, cx, cy).sabc;
+        if (fabsf(sabc) >= 0.25f * wmax * wmax) {
+            // centres x + 0.5 in [min - 0.24, max + 0.24]
+            xmin = max(xmin, (int64_t)ceilf(mnx - 0.74f)), xmax = min(xmax, (int64_t)floorf(mxx - 0.26f));
+            ymin = max(ymin, (int64_t)ceilf(mny - 0.74f)), ymax = min(ymax, (int64_t)floorf(mxy - 0.26f));
+        }
+    }
     if (xmin < 0) xmin = 0;
     if (ymin < 0) ymin = 0;
     if (xmax > W - 1) xmax = W - 1;
@@ -428,9 +450,12 @@ __global__ __launch_bounds__(1024) void k_scan_sums(const int32_t* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_apply(const int32_t* __restrict__ counts, int32_t n,
+// Also lists each work item's tile (item_tile, up to cap_items entries) and
+// leaves the counts at zero for the next pass.
+__global__ __launch_bounds__(1024) void k_scan_apply(int32_t* __restrict__ counts, int32_t n,
                                                      const int32_t* __restrict__ bsum, int32_t* __restrict__ offsets,
                                                      int32_t* __restrict__ chunk_off, int32_t* __restrict__ cursor,
+                                                     int32_t* __restrict__ item_tile, int32_t cap_items,
                                                      const int32_t* __restrict__ blk_prims, int32_t n_blk,
                                                      int32_t max_clip_subs, Counters* cnt) {
     __shared__ int32_t ws[16], wc[16];
@@ -467,6 +492,8 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const int32_t* __restrict__
             offsets[i0 + k] = run;
             cursor[i0 + k] = run;
             chunk_off[i0 + k] = runc;
+            for (int c = runc; c < runc + ch[k] && c < cap_items; c++) item_tile[c] = i0 + k;
+            counts[i0 + k] = 0;
             run += v[k];
             runc += ch[k];
         }
@@ -608,15 +635,7 @@ __device__ __forceinline__ bool eval_candidate(const LPrim& p, float px, float p
 // Work items of the rasters: (tile, up to ITEM_PRIMS primitives of its
 // list), so that a tile a scene crowds -- a shadow map fitted to the whole
 // camera frustum concentrates the scene in a few hundred of its tiles -- is
-// spread over many workgroups.  k_items lists each item's tile.
-__global__ __launch_bounds__(256) void k_items(const int32_t* __restrict__ chunk_off, int32_t n_tiles,
-                                               int32_t* __restrict__ item_tile, const Counters* cnt,
-                                               int32_t cap_items) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_tiles || cnt->n_chunks > cap_items) return;
-    for (int i = chunk_off[t]; i < chunk_off[t + 1]; i++) item_tile[i] = t;
-}
-
+// spread over many workgroups.  k_scan_apply lists each item's tile.
 // Persistent workgroups take items i = blockIdx.x, blockIdx.x + gridDim.x, ...
 // Each item evaluates its candidates (pixel box x tile) exactly as
 // drawClipped / drawDepth do, reduces them in LDS and writes the tile: with
@@ -1189,13 +1208,13 @@ void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& 
                        pbox, tile_count, cnt, cap);
 }
 int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK); }
-void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off,
-                 int32_t* cursor, int32_t* bsum, const int32_t* blk_prims, int32_t n_blk, int32_t max_clip_subs,
-                 Counters* cnt) {
+void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
+                 int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
+                 int32_t max_clip_subs, Counters* cnt) {
     const int nb = scan_blocks(n);
-    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(1024), 0, st, counts, n, bsum);
+    if (nb > 1) hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(1024), 0, st, counts, n, bsum);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, st, counts, n, bsum, offsets, chunk_off, cursor,
-                       blk_prims, n_blk, max_clip_subs, cnt);
+                       item_tile, cap_items, blk_prims, n_blk, max_clip_subs, cnt);
 }
 void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris) {
@@ -1207,8 +1226,6 @@ void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
                    Counters* cnt, const RasterTarget& rt) {
-    hipLaunchKernelGGL(k_items, dim3(blocks_for(rt.n_tiles, 256)), dim3(256), 0, st, chunk_off, rt.n_tiles, item_tile,
-                       cnt, cap_items);
     if (mode == MODE_SHADOW)
         hipLaunchKernelGGL(k_raster<MODE_SHADOW>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
                            item_tile, tile_list, cnt, rt, cap_items);

@@ -90,9 +90,10 @@ void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& 
 // the frame statistics (the main view)
 // bsum: 2 * scan_blocks(n) ints of scratch
 int scan_blocks(int32_t n);
-void launch_scan(hipStream_t st, const int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off,
-                 int32_t* cursor, int32_t* bsum, const int32_t* blk_prims, int32_t n_blk, int32_t max_clip_subs,
-                 Counters* cnt);
+// Also lists each work item's tile in item_tile and zeroes the counts.
+void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
+                 int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
+                 int32_t max_clip_subs, Counters* cnt);
 void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris);
 // mode: 0 visibility, 1 visibility in exact order (NaN rule), 2 shadow maps

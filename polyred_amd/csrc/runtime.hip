@@ -302,6 +302,7 @@ struct prh_scene {
     int n_slots;
     float center[3];
     float ambient_sum;  // sum of the ambient intensities (differentiable shading)
+    bool any_ao;        // some material has AmbientOcclusion
 };
 
 struct prh_renderer {
@@ -489,6 +490,7 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
         for (int32_t v = vtx_start[o]; v < vtx_start[o + 1]; v++) vtx_obj[v] = o;
     }
     std::vector<DevMaterial> mats(d->n_materials);
+    bool any_ao = false;
     for (int i = 0; i < d->n_materials; i++) {
         const prh_material& m = d->materials[i];
         DevMaterial& dm = mats[i];
@@ -504,11 +506,13 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
         memcpy(&dm.tex_rgba, m.texture, 4);
         dm.flat_shading = m.flat_shading != 0;
         dm.ambient_occlusion = m.ambient_occlusion != 0;
+        any_ao = any_ao || dm.ambient_occlusion;
         dm.receive_shadow = m.receive_shadow != 0;
     }
     prh_scene* s = new prh_scene();
     s->ctx = ctx;
     s->n_slots = 0;
+    s->any_ao = any_ao;
     s->ambient_sum = 0.0f;
     for (int e = 0; e < d->n_ambient; e++) s->ambient_sum += d->ambient_intensity[e];
     std::vector<DevLight> lights(d->n_lights);
@@ -770,6 +774,8 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     // Shadow maps start at zero once, here, as make([]float32) in initShadowMaps
     // (stored as ordered depth bits).
     launch_fill_u32(ctx->stream, r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH, depth_ord(0.0f));
+    // tile counts start at zero; every scan leaves them at zero
+    launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(r->tile_count), r->tile_slots, 0u);
     {
         int n_cu = 0;
         if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || n_cu <= 0)
@@ -822,10 +828,9 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     if (r->n_slots > 0 && !nan_mode) {
         const int nt = r->n_slots * r->stiles_per_map;
         launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
-        launch_fill_u32(st, (uint32_t*)r->tile_count, nt, 0u);
         launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->pbox, r->tile_count, cs, r->cap_shadow);
-        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, nullptr, 0, 0,
-                    cs);
+        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
+                    r->cap_items, nullptr, 0, 0, cs);
         launch_bin(st, r->pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
                    r->n_slots * d.n_tris, d.n_tris);
         RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
@@ -835,11 +840,10 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     if (timed) (void)hipEventRecord(r->ev[1], st);
     launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
     if (timed) (void)hipEventRecord(r->ev[2], st);
-    launch_fill_u32(st, (uint32_t*)r->tile_count, r->n_tiles, 0u);
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
                       r->tile_count, r->blk_prims, cm, r->cap_main);
-    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->blk_prims,
-                (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
+    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
+                r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
     launch_bin(st, r->pbox, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
                d.n_tris);
     if (timed) (void)hipEventRecord(r->ev[3], st);
@@ -863,7 +867,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     b.cnt = cm;
     b.ao = r->ao;
     b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
-    launch_ao(st, d, r->sh, b);
+    if (r->sc->any_ao) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
     if (timed) (void)hipEventRecord(r->ev[5], st);
     launch_shade_bg(st, d, r->sh, b);
     if (timed) (void)hipEventRecord(r->ev[7], st);
@@ -939,6 +943,25 @@ void drop_graph(prh_ctx* ctx) {
     ctx->graph_key.clear();
 }
 
+// Stream of each view of a batch: largest estimated cost first, each to the
+// least loaded stream (the cost: pixels plus triangles for every pass).
+std::vector<int> batch_lanes(prh_renderer* const* rs, int n) {
+    std::vector<int> order(n), lane(n);
+    std::vector<double> cost(n), load(PRH_BATCH_STREAMS, 0.0);
+    for (int i = 0; i < n; i++) {
+        order[i] = i;
+        cost[i] = (double)rs[i]->W * rs[i]->H + (double)rs[i]->sc->d.n_tris * (1 + rs[i]->n_slots) * 4.0 +
+                  (double)rs[i]->n_slots * rs[i]->SW * rs[i]->SH * 0.25;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    for (int i : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lane[i] = k;
+        load[k] += cost[i];
+    }
+    return lane;
+}
+
 int batch_streams(prh_ctx* ctx) {
     if (ctx->batch_ready) return PRH_OK;
     for (int k = 0; k < PRH_BATCH_STREAMS; k++) {
@@ -964,7 +987,8 @@ int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n) {
     hipError_t e = hipEventRecord(ctx->bfork, s0);
     for (int k = 1; k < PRH_BATCH_STREAMS && e == hipSuccess; k++) e = hipStreamWaitEvent(ctx->bst[k], ctx->bfork, 0);
     if (e == hipSuccess) {
-        for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[i % PRH_BATCH_STREAMS], false);
+        const std::vector<int> lane = batch_lanes(rs, n);
+        for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[lane[i]], false);
         for (int k = 1; k < PRH_BATCH_STREAMS && e == hipSuccess; k++) {
             e = hipEventRecord(ctx->bjoin[k], ctx->bst[k]);
             if (e == hipSuccess) e = hipStreamWaitEvent(s0, ctx->bjoin[k], 0);
@@ -1118,9 +1142,10 @@ int prh_render_batch(prh_renderer* const* rs, int32_t n, int32_t flags, float* m
     } else {
         // fork: every batch stream starts after the work already on s0
         for (int k = 1; k < PRH_BATCH_STREAMS; k++) HIP_TRY(hipStreamWaitEvent(ctx->bst[k], ctx->bev[0], 0));
+        const std::vector<int> lane = batch_lanes(rs, n);
         for (int i = 0; i < n; i++) {
             rs[i]->has_frame = false;
-            enqueue_frame(rs[i], false, ctx->bst[i % PRH_BATCH_STREAMS], true);
+            enqueue_frame(rs[i], false, ctx->bst[lane[i]], true);
         }
         for (int k = 1; k < PRH_BATCH_STREAMS; k++) {
             HIP_TRY(hipEventRecord(ctx->bjoin[k], ctx->bst[k]));
