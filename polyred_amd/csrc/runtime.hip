@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -274,7 +275,9 @@ int upload(T** p, const T* src, size_t n) {
 }  // namespace
 
 struct prh_renderer;
-constexpr int PRH_BATCH_STREAMS = 4;  // GPU_MAX_HW_QUEUES is 4 by default
+// prh_render_batch spreads a batch over one stream per hardware queue the
+// process has (GPU_MAX_HW_QUEUES, 4 unless set), at most PRH_BATCH_MAX.
+constexpr int PRH_BATCH_MAX = 8;
 
 struct prh_ctx {
     int device;
@@ -283,8 +286,9 @@ struct prh_ctx {
     // prh_render_batch: its streams and events (created on first use) and
     // the graph of the last captured batch, keyed by its renderers and their
     // work-buffer generations
-    hipStream_t bst[PRH_BATCH_STREAMS];
-    hipEvent_t bev[2], bfork, bjoin[PRH_BATCH_STREAMS];
+    int nbs;  // batch streams in use
+    hipStream_t bst[PRH_BATCH_MAX];
+    hipEvent_t bev[2], bfork, bjoin[PRH_BATCH_MAX];
     bool batch_ready;
     hipGraph_t graph;
     hipGraphExec_t graph_exec;
@@ -429,7 +433,7 @@ int prh_close(prh_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     drop_graph(ctx);
     if (ctx->batch_ready) {
-        for (int k = 0; k < PRH_BATCH_STREAMS; k++) {
+        for (int k = 0; k < ctx->nbs; k++) {
             (void)hipStreamSynchronize(ctx->bst[k]);
             (void)hipStreamDestroy(ctx->bst[k]);
             (void)hipEventDestroy(ctx->bjoin[k]);
@@ -947,7 +951,7 @@ void drop_graph(prh_ctx* ctx) {
 // least loaded stream (the cost: pixels plus triangles for every pass).
 std::vector<int> batch_lanes(prh_renderer* const* rs, int n) {
     std::vector<int> order(n), lane(n);
-    std::vector<double> cost(n), load(PRH_BATCH_STREAMS, 0.0);
+    std::vector<double> cost(n), load(rs[0]->ctx->nbs, 0.0);
     for (int i = 0; i < n; i++) {
         order[i] = i;
         cost[i] = (double)rs[i]->W * rs[i]->H + (double)rs[i]->sc->d.n_tris * (1 + rs[i]->n_slots) * 4.0 +
@@ -964,7 +968,9 @@ std::vector<int> batch_lanes(prh_renderer* const* rs, int n) {
 
 int batch_streams(prh_ctx* ctx) {
     if (ctx->batch_ready) return PRH_OK;
-    for (int k = 0; k < PRH_BATCH_STREAMS; k++) {
+    const char* q = getenv("GPU_MAX_HW_QUEUES");
+    ctx->nbs = std::min(PRH_BATCH_MAX, std::max(1, q ? atoi(q) : 4));
+    for (int k = 0; k < ctx->nbs; k++) {
         HIP_TRY(hipStreamCreateWithFlags(&ctx->bst[k], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&ctx->bjoin[k], hipEventDisableTiming));
     }
@@ -985,11 +991,11 @@ int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n) {
     hipStream_t s0 = ctx->bst[0];
     HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
     hipError_t e = hipEventRecord(ctx->bfork, s0);
-    for (int k = 1; k < PRH_BATCH_STREAMS && e == hipSuccess; k++) e = hipStreamWaitEvent(ctx->bst[k], ctx->bfork, 0);
+    for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) e = hipStreamWaitEvent(ctx->bst[k], ctx->bfork, 0);
     if (e == hipSuccess) {
         const std::vector<int> lane = batch_lanes(rs, n);
         for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[lane[i]], false);
-        for (int k = 1; k < PRH_BATCH_STREAMS && e == hipSuccess; k++) {
+        for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) {
             e = hipEventRecord(ctx->bjoin[k], ctx->bst[k]);
             if (e == hipSuccess) e = hipStreamWaitEvent(s0, ctx->bjoin[k], 0);
         }
@@ -1141,13 +1147,13 @@ int prh_render_batch(prh_renderer* const* rs, int32_t n, int32_t flags, float* m
         HIP_TRY(hipGraphLaunch(ctx->graph_exec, s0));
     } else {
         // fork: every batch stream starts after the work already on s0
-        for (int k = 1; k < PRH_BATCH_STREAMS; k++) HIP_TRY(hipStreamWaitEvent(ctx->bst[k], ctx->bev[0], 0));
+        for (int k = 1; k < ctx->nbs; k++) HIP_TRY(hipStreamWaitEvent(ctx->bst[k], ctx->bev[0], 0));
         const std::vector<int> lane = batch_lanes(rs, n);
         for (int i = 0; i < n; i++) {
             rs[i]->has_frame = false;
             enqueue_frame(rs[i], false, ctx->bst[lane[i]], true);
         }
-        for (int k = 1; k < PRH_BATCH_STREAMS; k++) {
+        for (int k = 1; k < ctx->nbs; k++) {
             HIP_TRY(hipEventRecord(ctx->bjoin[k], ctx->bst[k]));
             HIP_TRY(hipStreamWaitEvent(s0, ctx->bjoin[k], 0));
         }
