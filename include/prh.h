@@ -34,11 +34,22 @@ extern "C" {
 
 /* ---- scene contract (host-side, borrowed) ------------------------------ */
 
+/* A texture image (buffer.NewTexture, buffer/texture.go:35-68): RGBA8 rows,
+ * row 0 first as image.RGBA stores them.  With use_mipmap (TextureIsoMipmap,
+ * true for textures a model loads) the library builds the mip levels as the
+ * reference does, each an imageutil.Resize of the image. */
+typedef struct prh_texture {
+    int32_t width, height;
+    const uint8_t* rgba;  /* [height][width][4] */
+    int32_t use_mipmap;
+} prh_texture;
+
 /* Blinn-Phong material, one entry of the renderer's flat material table.
- * Mirrors material.BlinnPhong (material/material.go:44-52).  The texture is a
- * uniform 1x1 texture (buffer.NewUniformTexture, buffer/texture.go:17-24):
- * Texture.Query of a 1x1 texture returns its single texel for every
- * (lod, u, v) (buffer/texture.go:130-160), so the colour is all that matters. */
+ * Mirrors material.BlinnPhong (material/material.go:44-52).  Its texture is
+ * either textures[texture_id] of the scene or, with texture_id -1, the
+ * uniform 1x1 texture of colour `texture` (buffer.NewUniformTexture,
+ * buffer/texture.go:17-24), which Texture.Query returns for every
+ * (lod, u, v) (buffer/texture.go:83-129). */
 typedef struct prh_material {
     uint8_t diffuse[4];  /* BlinnPhong.Diffuse  RGBA8 */
     uint8_t specular[4]; /* BlinnPhong.Specular RGBA8 */
@@ -47,6 +58,7 @@ typedef struct prh_material {
     int32_t flat_shading;      /* Standard.FlatShading     */
     int32_t ambient_occlusion; /* Standard.AmbientOcclusion */
     int32_t receive_shadow;    /* Standard.ReceiveShadow   */
+    int32_t texture_id;        /* index into prh_scene_desc.textures, or -1 */
 } prh_material;
 
 #define PRH_LIGHT_POINT 0
@@ -100,6 +112,9 @@ typedef struct prh_scene_desc {
     const float* ambient_intensity; /* light.Ambient intensities in order   */
 
     float center[3];               /* Scene.Center() (scene/scene.go:49-52) */
+
+    int32_t n_textures;
+    const prh_texture* textures;   /* images the materials' texture_id name */
 } prh_scene_desc;
 
 #define PRH_CAMERA_PERSPECTIVE 0

@@ -72,6 +72,9 @@ def lib() -> C.CDLL:
                                              C.POINTER(abi.prh_light), C.c_int, f32p, u8p, f32p]
         L.oracle_ao_dirs.argtypes = [f32p]
         L.oracle_resize.argtypes = [u8p, C.c_int, C.c_int, u8p, C.c_int, C.c_int]
+        L.oracle_texture_query.argtypes = [C.POINTER(abi.prh_texture), C.c_float, C.c_float, C.c_float, u8p]
+        L.oracle_texture_level.argtypes = [C.POINTER(abi.prh_texture), C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), u8p, C.c_size_t]
         f64p = C.POINTER(C.c_double)
         L.oracle_diff.argtypes = [C.POINTER(abi.prh_scene_desc), C.POINTER(abi.prh_camera),
                                   C.POINTER(abi.prh_options), C.POINTER(C.c_uint32), C.POINTER(oracle_diff_params),
@@ -189,4 +192,41 @@ def resize(img: np.ndarray, width: int, height: int) -> np.ndarray:
                              out.ctypes.data_as(C.POINTER(C.c_uint8)), width, height)
     if rc != 0:
         raise RuntimeError(f"oracle_resize failed: {rc}")
+    return out
+
+
+def _texture(img: np.ndarray, use_mipmap: bool):
+    img = np.ascontiguousarray(img, np.uint8)
+    t = abi.prh_texture()
+    t.height, t.width = img.shape[0], img.shape[1]
+    t.rgba = img.ctypes.data_as(C.POINTER(C.c_uint8))
+    t.use_mipmap = int(use_mipmap)
+    return t, img
+
+
+def texture_query(img: np.ndarray, lod: float, u: float, v: float, use_mipmap: bool = True) -> tuple:
+    """Texture.Query(lod, u, v) of NewTexture(TextureImage(img),
+    TextureIsoMipmap(use_mipmap)) for an (h, w, 4) uint8 image."""
+    t, keep = _texture(img, use_mipmap)
+    out = (C.c_uint8 * 4)()
+    rc = lib().oracle_texture_query(C.byref(t), lod, u, v, out)
+    if rc != 0:
+        raise RuntimeError(f"oracle_texture_query failed: {rc}")
+    return tuple(out)
+
+
+def texture_levels(img: np.ndarray, use_mipmap: bool = True) -> list:
+    """The mip levels NewTexture builds from an (h, w, 4) uint8 image."""
+    t, keep = _texture(img, use_mipmap)
+    w, h = C.c_int(), C.c_int()
+    n = lib().oracle_texture_level(C.byref(t), 0, C.byref(w), C.byref(h), None, 0)
+    if n <= 0:
+        raise RuntimeError(f"oracle_texture_level failed: {n}")
+    out = []
+    for lv in range(n):
+        lib().oracle_texture_level(C.byref(t), lv, C.byref(w), C.byref(h), None, 0)
+        a = np.zeros((h.value, w.value, 4), np.uint8)
+        lib().oracle_texture_level(C.byref(t), lv, C.byref(w), C.byref(h),
+                                   a.ctypes.data_as(C.POINTER(C.c_uint8)), a.nbytes)
+        out.append(a)
     return out

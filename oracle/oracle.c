@@ -307,6 +307,284 @@ static float from_srgb_linear(float v) { /* color/srgb.go:34-50 with its LUT */
 /* shader/gamma.go:13-18 */
 static uint8_t gamma_u8(uint8_t c) { return gf2u8(from_linear_srgb((float)c / 0xff) * 0xff + 0.5f); }
 
+/* ----------------------------------------------------------------- resize */
+/* imageutil.Resize (internal/imageutil/resize.go:16-66), the MSAA
+ * antialiasing pass (render/raster.go:361-378): a separable bilinear filter
+ * with 8-bit fixed-point weights; the first pass filters the rows of the
+ * input into a transposed temporary, the second filters the rows of that. */
+static float resize_linear(float in) { /* resize.go:68-74 */
+    in = (float)fabs((double)in);
+    if (in <= 1) return 1 - in;
+    return 0;
+}
+static int resize_filter_len(float scale) { /* resize.go:149 with filterLength 2 */
+    return 2 * (int)gmax2((float)ceil((double)scale), 1.0f);
+}
+/* createWeights8, resize.go:148-164 */
+static void resize_weights(int dy, int fl, float scale, int16_t* coeffs, int* start) {
+    const float ff = gmin2(1.0f / scale, 1.0f);
+    for (int y = 0; y < dy; y++) {
+        float interp = scale * ((float)y + 0.5f) - 0.5f;
+        start[y] = (int)interp - fl / 2 + 1;
+        interp -= (float)start[y];
+        for (int i = 0; i < fl; i++) {
+            float in = (interp - (float)i) * ff;
+            coeffs[y * fl + i] = (int16_t)(int32_t)(resize_linear(in) * 256);
+        }
+    }
+}
+/* resizeRGBA, resize.go:76-118: `in` has in_rows rows of in_w RGBA pixels;
+ * out[y][x] (out_rows x out_w, row-major) = filter of input row x at y. */
+static void resize_rgba(const uint8_t* in, int in_w, uint8_t* out, int out_w, int out_rows, const int16_t* coeffs,
+                        const int* start, int fl) {
+    const int maxX = in_w - 1;
+    for (int x = 0; x < out_w; x++) {
+        const uint8_t* row = in + (size_t)x * in_w * 4;
+        for (int y = 0; y < out_rows; y++) {
+            int32_t rgba[4] = {0, 0, 0, 0}, sum = 0;
+            for (int i = 0; i < fl; i++) {
+                const int16_t coeff = coeffs[y * fl + i];
+                if (coeff == 0) continue;
+                int xi = start[y] + i;
+                if ((unsigned)xi < (unsigned)maxX) xi *= 4;
+                else if (xi >= maxX) xi = 4 * maxX;
+                else xi = 0;
+                for (int k = 0; k < 4; k++) rgba[k] += (int32_t)coeff * (int32_t)row[xi + k];
+                sum += coeff;
+            }
+            for (int k = 0; k < 4; k++) {
+                int v = rgba[k] / sum; /* Go's integer division truncates toward zero, as C's */
+                out[((size_t)y * out_w + x) * 4 + k] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+            }
+        }
+    }
+}
+/* calcFactors (resize.go:121-138) and the size Resize then makes
+ * (resize.go:17-23): a zero width or height follows the other's scale. */
+static void resize_plan(int width, int height, int iw, int ih, float* sx, float* sy, int* ow, int* oh) {
+    const float oldw = (float)iw, oldh = (float)ih;
+    if (width == 0) {
+        if (height == 0) {
+            *sx = *sy = 1.0f;
+        } else {
+            *sy = oldh / (float)height;
+            *sx = *sy;
+        }
+    } else {
+        *sx = oldw / (float)width;
+        *sy = height == 0 ? *sx : oldh / (float)height;
+    }
+    if (width == 0) width = (int)(0.7f + oldw / *sx);
+    if (height == 0) height = (int)(0.7f + oldh / *sy);
+    *ow = width, *oh = height;
+}
+/* Resize(width, height, img) with img (iw x ih RGBA8, top row first) into
+ * out (the size resize_plan gives); returns 0, or -1 on allocation failure.
+ * The identity when the sizes match. */
+static int resize_image(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height) {
+    float sx, sy;
+    resize_plan(width, height, iw, ih, &sx, &sy, &width, &height);
+    if ((iw == width && ih == height) || iw <= 0 || ih <= 0) {
+        memcpy(out, img, (size_t)iw * ih * 4);
+        return 0;
+    }
+    const int flx = resize_filter_len(sx), fly = resize_filter_len(sy);
+    uint8_t* temp = malloc((size_t)ih * width * 4); /* image.Rect(0, 0, ih, width): width rows of ih pixels */
+    int16_t* cx = malloc(sizeof(int16_t) * (size_t)width * flx);
+    int16_t* cy = malloc(sizeof(int16_t) * (size_t)height * fly);
+    int* ox = malloc(sizeof(int) * (size_t)width);
+    int* oy = malloc(sizeof(int) * (size_t)height);
+    int rc = -1;
+    if (temp && cx && cy && ox && oy) {
+        resize_weights(width, flx, sx, cx, ox);
+        resize_rgba(img, iw, temp, ih, width, cx, ox, flx);
+        resize_weights(height, fly, sy, cy, oy);
+        resize_rgba(temp, ih, out, width, height, cy, oy, fly);
+        rc = 0;
+    }
+    free(temp);
+    free(cx);
+    free(cy);
+    free(ox);
+    free(oy);
+    return rc;
+}
+
+/* --------------------------------------------------------------- textures */
+typedef struct {
+    int w, h;
+    uint8_t* px; /* [h][w][4] */
+} img_t;
+typedef struct {
+    int use_mipmap, size, n;
+    img_t lv[32];
+} tex_t;
+
+/* math.Log2 as Go computes it (Frexp, then Log of the fraction) */
+static double go_log2(double x) {
+    if (!(x < INFINITY)) return x; /* Log2(+Inf) = +Inf, Log2(NaN) = NaN */
+    int e;
+    double f = frexp(x, &e);
+    if (f == 0.5) return (double)(e - 1);
+    return log(f) * (1 / M_LN2) + (double)e;
+}
+
+/* buffer.NewTexture (buffer/texture.go:35-68): level i is
+ * Resize(dx / 2^i, dy / 2^i) of the image, for int(Log2(max(dx, dy))) + 1
+ * levels; a 1x1 image has one. */
+static int tex_build(const prh_texture* t, tex_t* out) {
+    memset(out, 0, sizeof *out);
+    out->use_mipmap = t->use_mipmap != 0;
+    out->size = t->width;
+    const int dx = t->width, dy = t->height;
+    out->lv[0].w = dx, out->lv[0].h = dy;
+    out->lv[0].px = malloc((size_t)dx * dy * 4);
+    if (!out->lv[0].px) return -1;
+    memcpy(out->lv[0].px, t->rgba, (size_t)dx * dy * 4);
+    out->n = 1;
+    if (dx == 1 && dy == 1) return 0;
+    int L = (int)(float)go_log2((double)gmax2((float)dx, (float)dy)) + 1;
+    if (L > 32) L = 32;
+    for (int i = 1; i < L; i++) {
+        const int d = (int)gpow(2.0f, (float)i);
+        float sx, sy;
+        int w, h;
+        resize_plan(dx / d, dy / d, dx, dy, &sx, &sy, &w, &h);
+        out->lv[i].w = w, out->lv[i].h = h;
+        out->lv[i].px = malloc((size_t)w * h * 4 + 4);
+        if (!out->lv[i].px || resize_image(t->rgba, dx, dy, out->lv[i].px, dx / d, dy / d)) return -1;
+        out->n = i + 1;
+    }
+    return 0;
+}
+static void tex_free(tex_t* t) {
+    for (int i = 0; i < t->n; i++) free(t->lv[i].px);
+    t->n = 0;
+}
+/* math.Modf as Go computes it: Modf(+-Inf) = +-Inf, NaN (C's modf gives 0) */
+static double go_modf(double x, double* ip) {
+    if (isinf(x)) {
+        *ip = x;
+        return NAN;
+    }
+    return modf(x, ip);
+}
+/* image.RGBA.RGBAAt: transparent black outside the image (indices are Go
+ * ints: int(Floor(NaN)) is the most negative one, outside) */
+static void rgba_at(const img_t* im, int64_t x, int64_t y, uint8_t o[4]) {
+    if (x < 0 || y < 0 || x >= im->w || y >= im->h) {
+        memset(o, 0, 4);
+        return;
+    }
+    memcpy(o, im->px + 4 * ((size_t)y * im->w + x), 4);
+}
+/* math.LerpC (math/interpolate.go:55-62) */
+static void lerp_c(const uint8_t a[4], const uint8_t b[4], float t, uint8_t o[4]) {
+    for (int k = 0; k < 4; k++) o[k] = gf2u8((float)a[k] + t * ((float)b[k] - (float)a[k]));
+}
+/* queryBilinear, buffer/texture.go:162-195 */
+static void tex_bilinear(const tex_t* t, int lod, float u, float v, uint8_t o[4]) {
+    const img_t* b = &t->lv[lod];
+    const int dx = b->w, dy = b->h;
+    if (dx == 1 && dy == 1) {
+        rgba_at(b, 0, 0, o);
+        return;
+    }
+    const float x = u * ((float)dx - 1), y = v * ((float)dy - 1);
+    const float x0 = (float)floor((double)x), y0 = (float)floor((double)y);
+    const int64_t i = gf2i(x0), j = gf2i(y0);
+    uint8_t p1[4], p2[4], p3[4], p4[4], q1[4], q2[4];
+    rgba_at(b, i, j, p1);
+    if (i < dx - 1) rgba_at(b, i + 1, j, p2);
+    else rgba_at(b, i, j, p2);
+    lerp_c(p1, p2, x - x0, q1);
+    if (j < dy - 1) rgba_at(b, i, j + 1, p3);
+    else rgba_at(b, i, j, p3);
+    if (i < dx - 1 && j < dy - 1) rgba_at(b, i + 1, j + 1, p4);
+    else rgba_at(b, i, j, p4);
+    lerp_c(p3, p4, x - x0, q2);
+    lerp_c(q1, q2, y - y0, o);
+}
+/* Texture.Query, buffer/texture.go:83-154 */
+static void tex_query(const tex_t* t, float lod, float u, float v, uint8_t o[4]) {
+    double iu, iv;
+    float fu = (float)go_modf((double)u, &iu), fv = (float)go_modf((double)v, &iv);
+    if ((float)iu != 0 && fu == 0) fu = 1;
+    if (fu < 0) fu = 1 - fu;
+    if ((float)iv != 0 && fv == 0) fv = 1;
+    if (fv < 0) fv = 1 - fv;
+    if (!t->use_mipmap) { /* queryL0, texture.go:130-141 */
+        const img_t* b = &t->lv[0];
+        if (b->w == 1 && b->h == 1) {
+            rgba_at(b, 0, 0, o);
+            return;
+        }
+        const int64_t x = gf2i((float)floor((double)(fu * ((float)b->w - 1))));
+        const int64_t y = gf2i((float)floor((double)(fv * ((float)b->h - 1))));
+        rgba_at(b, x, y, o);
+        return;
+    }
+    if (lod < 0) lod = 0;
+    else if (lod >= (float)t->n) lod = (float)(t->n - 1);
+    if (lod <= 1) {
+        tex_bilinear(t, 0, fu, fv, o);
+        return;
+    }
+    lod -= 1;
+    const int h = (int)gf2i((float)floor((double)lod)), l = h + 1;
+    if (l >= t->n) {
+        tex_bilinear(t, h, fu, fv, o);
+        return;
+    }
+    const float p = lod - (float)h;
+    if (approx_eq(p, 0, 1e-7f)) {
+        tex_bilinear(t, h, fu, fv, o);
+        return;
+    }
+    uint8_t a[4], b[4];
+    tex_bilinear(t, h, fu, fv, a);
+    tex_bilinear(t, l, fu, fv, b);
+    lerp_c(a, b, p, o);
+}
+/* FragmentShader's texture lookup (shader/blinn_cpu.go:28-35) */
+static void tex_fragment(const tex_t* t, float du, float dv, float u, float v, uint8_t o[4]) {
+    float lod = 0;
+    if (t->use_mipmap) {
+        float siz = (float)t->size * gsqrt(gmax2(du, dv));
+        if (siz < 1) siz = 1;
+        lod = (float)go_log2((double)siz);
+    }
+    tex_query(t, lod, u, 1 - v, o);
+}
+int oracle_texture_query(const prh_texture* tx, float lod, float u, float v, uint8_t out[4]) {
+    tex_t t;
+    if (tex_build(tx, &t)) {
+        tex_free(&t);
+        return PRH_E_NOMEM;
+    }
+    tex_query(&t, lod, u, v, out);
+    tex_free(&t);
+    return PRH_OK;
+}
+int oracle_texture_level(const prh_texture* tx, int level, int* w, int* h, uint8_t* out, size_t bytes) {
+    tex_t t;
+    if (tex_build(tx, &t)) {
+        tex_free(&t);
+        return PRH_E_NOMEM;
+    }
+    int rc = PRH_E_INVALID;
+    if (level >= 0 && level < t.n) {
+        *w = t.lv[level].w, *h = t.lv[level].h;
+        if (out && bytes >= (size_t)*w * *h * 4) memcpy(out, t.lv[level].px, (size_t)*w * *h * 4);
+        rc = t.n;
+    }
+    tex_free(&t);
+    return rc;
+}
+int oracle_resize(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height) {
+    return resize_image(img, iw, ih, out, width, height) ? PRH_E_NOMEM : PRH_OK;
+}
+
 /* ------------------------------------------------------------ frame state */
 typedef struct {
     int ok;
@@ -314,6 +592,7 @@ typedef struct {
     float depth;
     v4 nor, facenor, wpos;
     uint8_t col[4];
+    float u, v, du, dv;
     int64_t matid;
     uint32_t order; /* oracle bookkeeping: triangle*8+sub of the stored fragment */
 } frag_t;
@@ -342,6 +621,8 @@ typedef struct {
     float ao_dir[16];
     float* linear;
     double shade_err64;
+    tex_t* tex; /* the scene's textures with their mip levels */
+    int n_tex;
 } ctx_t;
 
 static inline frag_t* fat(ctx_t* c, int x, int y) { return &c->f[(size_t)(c->H - 1 - y) * c->W + x]; }
@@ -365,8 +646,7 @@ static inline v4 interp_wpos(const float bc[3], v4 m1, v4 m2, v4 m3) {
               bc[0] * m1.z + bc[1] * m2.z + bc[2] * m3.z, 1);
 }
 
-/* render/raster.go:462-572 (UV/Du/Dv only feed Texture.Query, which is
- * constant for uniform textures, so they are not materialised) */
+/* render/raster.go:462-572 */
 static void draw_clipped(ctx_t* c, const m4* vpinv, const m4* pinv, const m4* vinv, const vert_t* t1,
                          const vert_t* t2, const vert_t* t3, const float recipw[3], int64_t matid, uint32_t order) {
     v4 m1 = apply4(apply4(apply4(t1->pos, vpinv), pinv), vinv);
@@ -403,6 +683,27 @@ static void draw_clipped(ctx_t* c, const m4* vpinv, const m4* pinv, const m4* vi
                               bc[0] * t1->nor.z + bc[1] * t2->nor.z + bc[2] * t3->nor.z, 0));
             nf.facenor = fN;
             nf.wpos = interp_wpos(bc, m1, m2, m3);
+            /* UV and its screen derivatives (raster.go:511-535); the latter's
+             * normalisation is unconditional, unlike the UV's */
+            const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+            nf.u = (wc1 * t1->uv[0] + wc2 * t2->uv[0] + wc3 * t3->uv[0]) * norm;
+            nf.v = (wc1 * t1->uv[1] + wc2 * t2->uv[1] + wc3 * t3->uv[1]) * norm;
+            nf.du = nf.dv = 0;
+            if (matid >= 0) {
+                float bx[3], by[3];
+                barycoord(px + 1, py, t1->pos.x, t1->pos.y, t2->pos.x, t2->pos.y, t3->pos.x, t3->pos.y, bx);
+                float w1x = recipw[0] * bx[0], w2x = recipw[1] * bx[1], w3x = recipw[2] * bx[2];
+                float nx = 1 / (w1x + w2x + w3x);
+                barycoord(px, py + 1, t1->pos.x, t1->pos.y, t2->pos.x, t2->pos.y, t3->pos.x, t3->pos.y, by);
+                float w1y = recipw[0] * by[0], w2y = recipw[1] * by[1], w3y = recipw[2] * by[2];
+                float ny = 1 / (w1y + w2y + w3y);
+                float uU = (w1x * t1->uv[0] + w2x * t2->uv[0] + w3x * t3->uv[0]) * nx;
+                float uX = (w1x * t1->uv[1] + w2x * t2->uv[1] + w3x * t3->uv[1]) * nx;
+                float vV = (w1y * t1->uv[0] + w2y * t2->uv[0] + w3y * t3->uv[0]) * ny;
+                float vY = (w1y * t1->uv[1] + w2y * t2->uv[1] + w3y * t3->uv[1]) * ny;
+                nf.du = (uU - nf.u) * (uU - nf.u) + (uX - nf.v) * (uX - nf.v);
+                nf.dv = (vV - nf.u) * (vV - nf.u) + (vY - nf.v) * (vY - nf.v);
+            }
             for (int k = 0; k < 4; k++)
                 nf.col[k] = gf2u8(gclamp(
                     (wc1 * (float)t1->col[k] + wc2 * (float)t2->col[k] + wc3 * (float)t3->col[k]) * norm, 0, 0xff));
@@ -657,9 +958,9 @@ static const prh_material* mat_at(const prh_scene_desc* s, int64_t id) { /* rend
 
 /* shader/blinn_cpu.go:24-106; lin receives the pre-Round sums.  Returns 0 on
  * the no-lights early return (texture colour). */
-static int fragment_shader(const prh_material* m, v4 nor, v4 facenor, v4 x, v3 cam, int nl, const prh_light* ls,
-                           int na, const float* amb, uint8_t out[4], float lin[3], double lin64[3]) {
-    const uint8_t* col = m->texture;
+static int fragment_shader(const prh_material* m, const uint8_t* col, v4 nor, v4 facenor, v4 x, v3 cam, int nl,
+                           const prh_light* ls, int na, const float* amb, uint8_t out[4], float lin[3],
+                           double lin64[3]) {
     if (nl == 0) {
         memcpy(out, col, 4);
         return 0;
@@ -799,7 +1100,13 @@ static void shade(ctx_t* c, int x, int y) {
         if (mat) {
             float lin[3];
             double lin64[3];
-            int lit = fragment_shader(mat, info->nor, info->facenor, info->wpos, V3a(c->cam->position),
+            /* the texture colour: Query at the fragment's UV and mip level, or
+             * the uniform colour */
+            uint8_t tcol[4];
+            memcpy(tcol, mat->texture, 4);
+            if (mat->texture_id >= 0 && mat->texture_id < c->n_tex)
+                tex_fragment(&c->tex[mat->texture_id], info->du, info->dv, info->u, info->v, tcol);
+            int lit = fragment_shader(mat, tcol, info->nor, info->facenor, info->wpos, V3a(c->cam->position),
                                       c->s->n_lights, c->s->lights, c->s->n_ambient, c->s->ambient_intensity, col,
                                       lin, lin64);
             if (lit && c->linear) {
@@ -826,91 +1133,6 @@ static void shade(ctx_t* c, int x, int y) {
     memcpy(fr->col, col, 4);
 }
 
-/* ----------------------------------------------------------------- resize */
-/* imageutil.Resize (internal/imageutil/resize.go:16-66), the MSAA
- * antialiasing pass (render/raster.go:361-378): a separable bilinear filter
- * with 8-bit fixed-point weights; the first pass filters the rows of the
- * input into a transposed temporary, the second filters the rows of that. */
-static float resize_linear(float in) { /* resize.go:68-74 */
-    in = (float)fabs((double)in);
-    if (in <= 1) return 1 - in;
-    return 0;
-}
-static int resize_filter_len(float scale) { /* resize.go:149 with filterLength 2 */
-    return 2 * (int)gmax2((float)ceil((double)scale), 1.0f);
-}
-/* createWeights8, resize.go:148-164 */
-static void resize_weights(int dy, int fl, float scale, int16_t* coeffs, int* start) {
-    const float ff = gmin2(1.0f / scale, 1.0f);
-    for (int y = 0; y < dy; y++) {
-        float interp = scale * ((float)y + 0.5f) - 0.5f;
-        start[y] = (int)interp - fl / 2 + 1;
-        interp -= (float)start[y];
-        for (int i = 0; i < fl; i++) {
-            float in = (interp - (float)i) * ff;
-            coeffs[y * fl + i] = (int16_t)(int32_t)(resize_linear(in) * 256);
-        }
-    }
-}
-/* resizeRGBA, resize.go:76-118: `in` has in_rows rows of in_w RGBA pixels;
- * out[y][x] (out_rows x out_w, row-major) = filter of input row x at y. */
-static void resize_rgba(const uint8_t* in, int in_w, uint8_t* out, int out_w, int out_rows, const int16_t* coeffs,
-                        const int* start, int fl) {
-    const int maxX = in_w - 1;
-    for (int x = 0; x < out_w; x++) {
-        const uint8_t* row = in + (size_t)x * in_w * 4;
-        for (int y = 0; y < out_rows; y++) {
-            int32_t rgba[4] = {0, 0, 0, 0}, sum = 0;
-            for (int i = 0; i < fl; i++) {
-                const int16_t coeff = coeffs[y * fl + i];
-                if (coeff == 0) continue;
-                int xi = start[y] + i;
-                if ((unsigned)xi < (unsigned)maxX) xi *= 4;
-                else if (xi >= maxX) xi = 4 * maxX;
-                else xi = 0;
-                for (int k = 0; k < 4; k++) rgba[k] += (int32_t)coeff * (int32_t)row[xi + k];
-                sum += coeff;
-            }
-            for (int k = 0; k < 4; k++) {
-                int v = rgba[k] / sum; /* Go's integer division truncates toward zero, as C's */
-                out[((size_t)y * out_w + x) * 4 + k] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
-            }
-        }
-    }
-}
-/* Resize(width, height, img) with img (iw x ih RGBA8, top row first);
- * returns 0, or -1 on allocation failure.  The identity when the sizes match. */
-static int resize_image(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height) {
-    if (iw == width && ih == height) {
-        memcpy(out, img, (size_t)iw * ih * 4);
-        return 0;
-    }
-    const float sx = (float)iw / (float)width, sy = (float)ih / (float)height; /* calcFactors, resize.go:121-138 */
-    const int flx = resize_filter_len(sx), fly = resize_filter_len(sy);
-    uint8_t* temp = malloc((size_t)ih * width * 4); /* image.Rect(0, 0, ih, width): width rows of ih pixels */
-    int16_t* cx = malloc(sizeof(int16_t) * (size_t)width * flx);
-    int16_t* cy = malloc(sizeof(int16_t) * (size_t)height * fly);
-    int* ox = malloc(sizeof(int) * (size_t)width);
-    int* oy = malloc(sizeof(int) * (size_t)height);
-    int rc = -1;
-    if (temp && cx && cy && ox && oy) {
-        resize_weights(width, flx, sx, cx, ox);
-        resize_rgba(img, iw, temp, ih, width, cx, ox, flx);
-        resize_weights(height, fly, sy, cy, oy);
-        resize_rgba(temp, ih, out, width, height, cy, oy, fly);
-        rc = 0;
-    }
-    free(temp);
-    free(cx);
-    free(cy);
-    free(ox);
-    free(oy);
-    return rc;
-}
-int oracle_resize(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height) {
-    return resize_image(img, iw, ih, out, width, height) ? PRH_E_NOMEM : PRH_OK;
-}
-
 /* ----------------------------------------------------------------- render */
 int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_options* o, oracle_out* out) {
     if (!s || !cam || !o || !out) return PRH_E_INVALID;
@@ -928,6 +1150,17 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
     size_t np = (size_t)c.W * c.H;
     c.f = calloc(np, sizeof(frag_t));
     if (!c.f) return PRH_E_NOMEM;
+    if (s->n_textures > 0) {
+        c.tex = calloc((size_t)s->n_textures, sizeof(tex_t));
+        int bad = !c.tex;
+        for (int i = 0; i < s->n_textures && !bad; i++, c.n_tex++) bad = tex_build(&s->textures[i], &c.tex[i]) != 0;
+        if (bad) {
+            for (int i = 0; i < c.n_tex; i++) tex_free(&c.tex[i]);
+            free(c.tex);
+            free(c.f);
+            return PRH_E_NOMEM;
+        }
+    }
     c.linear = out->linear;
     if (c.linear)
         for (size_t i = 0; i < 3 * np; i++) c.linear[i] = NAN;
@@ -1003,6 +1236,8 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
         memcpy(out->shadow_maps, c.smaps, sizeof(float) * (size_t)s->n_lights * c.SW * c.SH);
     if (out->shade_err64) *out->shade_err64 = c.shade_err64;
     free(c.f);
+    for (int i = 0; i < c.n_tex; i++) tex_free(&c.tex[i]);
+    free(c.tex);
     free(c.lview);
     free(c.lproj);
     free(c.lactive);
@@ -1279,7 +1514,7 @@ void oracle_fragment_shader(const prh_material* m, const float nor[4], const flo
                             const float cam[3], int n_lights, const prh_light* lights, int n_amb, const float* amb,
                             uint8_t out[4], float lin[3]) {
     float l[3] = {NAN, NAN, NAN};
-    fragment_shader(m, V4(nor[0], nor[1], nor[2], nor[3]), V4(facenor[0], facenor[1], facenor[2], facenor[3]),
+    fragment_shader(m, m->texture, V4(nor[0], nor[1], nor[2], nor[3]), V4(facenor[0], facenor[1], facenor[2], facenor[3]),
                     V4(wpos[0], wpos[1], wpos[2], wpos[3]), V3a(cam), n_lights, lights, n_amb, amb, out, l, NULL);
     if (lin) memcpy(lin, l, sizeof l);
 }

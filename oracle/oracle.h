@@ -79,6 +79,13 @@ int oracle_diff(const prh_scene_desc* s, const prh_camera* cam, const prh_option
  * image to width x height. */
 int oracle_resize(const uint8_t* img, int iw, int ih, uint8_t* out, int width, int height);
 
+/* Texture.Query (buffer/texture.go:83-154) of a texture built as
+ * buffer.NewTexture builds it (buffer/texture.go:35-68). */
+int oracle_texture_query(const prh_texture* t, float lod, float u, float v, uint8_t out[4]);
+/* Mip level `level` of that texture: its size into *w, *h and, when `out`
+ * holds bytes >= w*h*4, its texels; returns the number of levels. */
+int oracle_texture_level(const prh_texture* t, int level, int* w, int* h, uint8_t* out, size_t bytes);
+
 /* Known-answer entry points (row-major Mat4 X00..X33). */
 void oracle_barycoord(const float p[2], const float a[2], const float b[2],
                       const float c[2], float out[3]);

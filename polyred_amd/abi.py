@@ -34,6 +34,15 @@ PRH_BUF_SHADOW = 3
 PRH_BUF_PREGAMMA = 4
 
 
+class prh_texture(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("rgba", C.POINTER(C.c_uint8)),
+        ("use_mipmap", C.c_int32),
+    ]
+
+
 class prh_material(C.Structure):
     _fields_ = [
         ("diffuse", C.c_uint8 * 4),
@@ -43,6 +52,7 @@ class prh_material(C.Structure):
         ("flat_shading", C.c_int32),
         ("ambient_occlusion", C.c_int32),
         ("receive_shadow", C.c_int32),
+        ("texture_id", C.c_int32),
     ]
 
 
@@ -84,6 +94,8 @@ class prh_scene_desc(C.Structure):
         ("n_ambient", C.c_int32),
         ("ambient_intensity", _f32p),
         ("center", C.c_float * 3),
+        ("n_textures", C.c_int32),
+        ("textures", C.POINTER(prh_texture)),
     ]
 
 

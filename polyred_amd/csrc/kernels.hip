@@ -335,6 +335,9 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
     uint32_t c0 = s.vcol[mvb + s.tri_v[3 * t]], c1 = s.vcol[mvb + s.tri_v[3 * t + 1]],
              c2 = s.vcol[mvb + s.tri_v[3 * t + 2]];
     V4 n0 = ld(tv_nor[i0]), n1 = ld(tv_nor[i1]), n2 = ld(tv_nor[i2]);
+    float2 u0 = make_float2(0.0f, 0.0f), u1 = u0, u2 = u0;  // UVs exist in scenes with textures only
+    if (s.n_textures > 0)
+        u0 = s.vuv[mvb + s.tri_v[3 * t]], u1 = s.vuv[mvb + s.tri_v[3 * t + 1]], u2 = s.vuv[mvb + s.tri_v[3 * t + 2]];
     float rw0 = A.w, rw1 = B.w, rw2 = Cc.w;
     TriSetup par = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
     for (int i = 2; i < nc; i++) {
@@ -356,6 +359,7 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
             rec.pos[k] = make_float4(q.x, q.y, q.z, rws[k]);
             rec.nor[k] = make_float4(nq.x, nq.y, nq.z, bitsf(u8x4(cc[0], cc[1], cc[2], cc[3])));
             rec.wp[k] = make_float4(m.x, m.y, m.z, 0.0f);
+            rec.uv[k] = make_float2(b[0] * u0.x + b[1] * u1.x + b[2] * u2.x, b[0] * u0.y + b[1] * u1.y + b[2] * u2.y);
             sv[k] = q;
         }
         clips[base + i - 2] = rec;
@@ -753,13 +757,16 @@ struct GFrag {
     V4 nor, facenor, wpos;
     uint32_t col;
     int mat;  // flat material id (-1 when matAt gives nil)
+    float u, v, du, dv;  // texture coordinates and their screen derivatives (scenes with textures)
 };
 
 // Rebuilds the G-buffer fragment of pixel (x, y) from its visibility key,
 // as drawClipped wrote it (render/raster.go:487-568): Barycoord at the pixel
 // centre, linear depth, perspective-correct vertex colour, unit
 // interpolated normal, face normal and interpWorldPos of the back-transformed
-// vertices.  UV/Du/Dv only feed Texture.Query, constant for 1x1 textures.
+// vertices; in a scene with textures also the perspective-correct UV and its
+// squared screen derivatives Du, Dv from Barycoords one pixel right and one
+// down (render/raster.go:511-535; their normalisation is unconditional).
 __device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int x, int y,
                                 uint64_t key) {
     const uint32_t ord = (uint32_t)key;
@@ -770,6 +777,8 @@ __device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const S
     V4 P[3], N[3], M[3];
     float rw[3];
     uint32_t C3[3];
+    float2 UV[3];
+    const bool tex = s.n_textures > 0;
     if (piece == (int)SUB_DIRECT) {
         const int vb = s.obj_vtx_start[o], mvb = s.mesh_vtx_off[mesh];
         for (int k = 0; k < 3; k++) {
@@ -780,6 +789,7 @@ __device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const S
             N[k] = ld(b.tv_nor[vb + lv]);
             M[k] = ld(b.tv_wp[vb + lv]);
             C3[k] = s.vcol[mvb + lv];
+            UV[k] = tex ? s.vuv[mvb + lv] : make_float2(0.0f, 0.0f);
         }
     } else {
         const ClipRec& r = b.clips[b.tri_clip_slot[g] + piece];
@@ -789,6 +799,7 @@ __device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const S
             N[k] = v4(r.nor[k].x, r.nor[k].y, r.nor[k].z, 0);
             C3[k] = fbits(r.nor[k].w);
             M[k] = ld(r.wp[k]);
+            UV[k] = r.uv[k];
         }
     }
     TriSetup ts = tri_setup(P[0].x, P[0].y, P[1].x, P[1].y, P[2].x, P[2].y);
@@ -812,16 +823,121 @@ __device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const S
     int mat = s.tri_mat[t];
     if (mat >= 0) mat += s.obj_mat_base[o];
     f.mat = (mat >= 0 && mat < s.n_materials) ? mat : -1;
+    f.u = f.v = f.du = f.dv = 0.0f;
+    if (tex) {
+        f.u = (wc1 * UV[0].x + wc2 * UV[1].x + wc3 * UV[2].x) * norm;
+        f.v = (wc1 * UV[0].y + wc2 * UV[1].y + wc3 * UV[2].y) * norm;
+        if (mat >= 0) {
+            float bx[3], by[3];
+            const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+            barycoord(ts, px + 1, py, bx);
+            const float w1x = rw[0] * bx[0], w2x = rw[1] * bx[1], w3x = rw[2] * bx[2];
+            const float nx = 1 / (w1x + w2x + w3x);
+            barycoord(ts, px, py + 1, by);
+            const float w1y = rw[0] * by[0], w2y = rw[1] * by[1], w3y = rw[2] * by[2];
+            const float ny = 1 / (w1y + w2y + w3y);
+            const float uU = (w1x * UV[0].x + w2x * UV[1].x + w3x * UV[2].x) * nx;
+            const float uX = (w1x * UV[0].y + w2x * UV[1].y + w3x * UV[2].y) * nx;
+            const float vV = (w1y * UV[0].x + w2y * UV[1].x + w3y * UV[2].x) * ny;
+            const float vY = (w1y * UV[0].y + w2y * UV[1].y + w3y * UV[2].y) * ny;
+            f.du = (uU - f.u) * (uU - f.u) + (uX - f.v) * (uX - f.v);
+            f.dv = (vV - f.u) * (vV - f.u) + (vY - f.v) * (vY - f.v);
+        }
+    }
     return f;
+}
+
+// ---- textures (buffer/texture.go:83-195)
+// image.RGBA.RGBAAt of one level: transparent black outside it (Go ints:
+// int(Floor(NaN)) is the most negative one, outside)
+__device__ __forceinline__ uint32_t texel(const DevScene& s, const DevTexture& t, int lv, int64_t x, int64_t y) {
+    if (x < 0 || y < 0 || x >= t.w[lv] || y >= t.h[lv]) return 0u;
+    return s.tex_data[t.off[lv] + y * t.w[lv] + x];
+}
+// math.LerpC (math/interpolate.go:55-62)
+__device__ __forceinline__ uint32_t lerp_c(uint32_t a, uint32_t b, float t) {
+    uint32_t o = 0;
+    for (int k = 0; k < 4; k++) o |= (uint32_t)f2u8(chan(a, k) + t * (chan(b, k) - chan(a, k))) << (8 * k);
+    return o;
+}
+// queryBilinear, buffer/texture.go:162-195
+__device__ uint32_t tex_bilinear(const DevScene& s, const DevTexture& t, int lv, float u, float v) {
+    const int dx = t.w[lv], dy = t.h[lv];
+    if (dx == 1 && dy == 1) return texel(s, t, lv, 0, 0);
+    const float x = u * ((float)dx - 1), y = v * ((float)dy - 1);
+    const float x0 = floorf(x), y0 = floorf(y);
+    const int64_t i = f2i(x0), j = f2i(y0);
+    const uint32_t p1 = texel(s, t, lv, i, j);
+    const uint32_t p2 = i < dx - 1 ? texel(s, t, lv, i + 1, j) : p1;
+    const uint32_t q1 = lerp_c(p1, p2, x - x0);
+    const uint32_t p3 = j < dy - 1 ? texel(s, t, lv, i, j + 1) : p1;
+    const uint32_t p4 = (i < dx - 1 && j < dy - 1) ? texel(s, t, lv, i + 1, j + 1) : p1;
+    const uint32_t q2 = lerp_c(p3, p4, x - x0);
+    return lerp_c(q1, q2, y - y0);
+}
+// Texture.Query, buffer/texture.go:83-154
+__device__ uint32_t tex_query(const DevScene& s, const DevTexture& t, float lod, float u, float v) {
+    double iu, iv;  // math.Modf: Modf(+-Inf) = +-Inf, NaN
+    float fu = isinf(u) ? __builtin_nanf("") : (float)modf((double)u, &iu);
+    float fv = isinf(v) ? __builtin_nanf("") : (float)modf((double)v, &iv);
+    if (isinf(u)) iu = u;
+    if (isinf(v)) iv = v;
+    if ((float)iu != 0 && fu == 0) fu = 1;
+    if (fu < 0) fu = 1 - fu;
+    if ((float)iv != 0 && fv == 0) fv = 1;
+    if (fv < 0) fv = 1 - fv;
+    if (!t.use_mipmap) {  // queryL0, texture.go:130-141
+        if (t.w[0] == 1 && t.h[0] == 1) return texel(s, t, 0, 0, 0);
+        return texel(s, t, 0, f2i(floorf(fu * ((float)t.w[0] - 1))), f2i(floorf(fv * ((float)t.h[0] - 1))));
+    }
+    if (lod < 0) lod = 0;
+    else if (lod >= (float)t.n) lod = (float)(t.n - 1);
+    if (lod <= 1) return tex_bilinear(s, t, 0, fu, fv);
+    lod -= 1;
+    const int h = (int)f2i(floorf(lod)), l = h + 1;  // lod is within [1, n-1] here
+    if (l >= t.n) return tex_bilinear(s, t, h, fu, fv);
+    const float p = lod - (float)h;
+    if (approx_eq(p, 0, 1e-7f)) return tex_bilinear(s, t, h, fu, fv);
+    return lerp_c(tex_bilinear(s, t, h, fu, fv), tex_bilinear(s, t, l, fu, fv), p);
+}
+// math.Log2 as Go computes it: Frexp, then Log of the fraction
+__device__ __forceinline__ double go_log2(double x) {
+    if (!(x < INFINITY)) return x;  // Log2(+Inf) = +Inf, Log2(NaN) = NaN
+    int e;
+    const double f = frexp(x, &e);
+    if (f == 0.5) return (double)(e - 1);
+    return log(f) * (1 / 0.693147180559945309417232121458176568) + (double)e;
+}
+// FragmentShader's texture colour (shader/blinn_cpu.go:28-35)
+__device__ uint32_t tex_fragment(const DevScene& s, const DevTexture& t, const GFrag& f) {
+    float lod = 0.0f;
+    if (t.use_mipmap) {
+        float siz = (float)t.size * sqrt32(max2(f.du, f.dv));
+        if (siz < 1) siz = 1;
+        lod = (float)go_log2((double)siz);
+    }
+    return tex_query(s, t, lod, f.u, 1 - f.v);
 }
 
 // shader.FragmentShader (shader/blinn_cpu.go:24-106).  Returns false on the
 // no-lights early return (texture colour, no linear sums).
 __device__ bool fragment_shader(const DevScene& s, const ShadeConsts& c, const DevMaterial& m, const GFrag& f,
                                 uint32_t& out, float lin[3]) {
+    // the texture colour: the uniform one, or Query at the fragment's UV
+    const uint32_t tc = m.tex_id >= 0 ? tex_fragment(s, s.textures[m.tex_id], f) : m.tex_rgba;
     if (s.n_lights == 0) {
-        out = m.tex_rgba;
+        out = tc;
         return false;
+    }
+    float tex[3], la[3];
+    if (m.tex_id >= 0) {  // La = sum_e Intensity_e * col (blinn_cpu.go:44-52)
+        for (int k = 0; k < 3; k++) {
+            tex[k] = chan(tc, k);
+            la[k] = 0.0f;
+            for (int e = 0; e < s.n_ambient; e++) la[k] += s.ambient[e] * tex[k];
+        }
+    } else {
+        for (int k = 0; k < 3; k++) tex[k] = m.tex[k], la[k] = m.la[k];
     }
     float LdR = 0, LdG = 0, LdB = 0, LsR = 0, LsG = 0, LsB = 0;
     const V4 n = m.flat_shading ? f.facenor : f.nor;
@@ -842,19 +958,19 @@ __device__ bool fragment_shader(const DevScene& s, const ShadeConsts& c, const D
         V4 H = unit(add(L, V));
         float Ld = clamp(dot(n, L), 0, 1);
         float Ls = (float)pow64((double)clamp(dot(n, H), 0, 1), (double)m.shininess);
-        LdR += Ld * m.tex[0] * I;
-        LdG += Ld * m.tex[1] * I;
-        LdB += Ld * m.tex[2] * I;
+        LdR += Ld * tex[0] * I;
+        LdG += Ld * tex[1] * I;
+        LdB += Ld * tex[2] * I;
         LsR += Ls * l.color[0] * I;
         LsG += Ls * l.color[1] * I;
         LsB += Ls * l.color[2] * I;
     }
-    float r = m.la[0] + (m.diffuse[0] * LdR / 255.0f) + (m.specular[0] * LsR / 255.0f);
-    float g = m.la[1] + (m.diffuse[1] * LdG / 255.0f) + (m.specular[1] * LsG / 255.0f);
-    float bb = m.la[2] + (m.diffuse[2] * LdB / 255.0f) + (m.specular[2] * LsB / 255.0f);
+    float r = la[0] + (m.diffuse[0] * LdR / 255.0f) + (m.specular[0] * LsR / 255.0f);
+    float g = la[1] + (m.diffuse[1] * LdG / 255.0f) + (m.specular[1] * LsG / 255.0f);
+    float bb = la[2] + (m.diffuse[2] * LdB / 255.0f) + (m.specular[2] * LsB / 255.0f);
     lin[0] = r, lin[1] = g, lin[2] = bb;
     out = u8x4(f2u8(clamp(round32(r), 0, 0xff)), f2u8(clamp(round32(g), 0, 0xff)), f2u8(clamp(round32(bb), 0, 0xff)),
-               (uint8_t)(m.tex_rgba >> 24));
+               (uint8_t)(tc >> 24));
     return true;
 }
 

@@ -25,6 +25,15 @@ struct DevMaterial {
     float shininess;
     uint32_t tex_rgba;  // texture RGBA8 (alpha is the shaded alpha)
     int32_t flat_shading, ambient_occlusion, receive_shadow;
+    int32_t tex_id;     // index into DevScene.textures, or -1: the uniform colour above
+};
+
+// A texture with its mip levels (buffer.NewTexture, buffer/texture.go:35-68)
+// in DevScene.tex_data: level i is w[i] x h[i] RGBA8 texels at off[i].
+constexpr int TEX_MAX_LEVELS = 16;
+struct DevTexture {
+    int32_t n, use_mipmap, size, pad;
+    int32_t off[TEX_MAX_LEVELS], w[TEX_MAX_LEVELS], h[TEX_MAX_LEVELS];
 };
 
 struct DevLight {
@@ -60,6 +69,12 @@ struct DevScene {
     const DevMaterial* mats;
     int32_t n_lights;
     const DevLight* lights;
+    const float2* vuv;            // [Vm] texture coordinates (mesh vertices)
+    int32_t n_textures;
+    const DevTexture* textures;
+    const uint32_t* tex_data;     // every level of every texture, RGBA8
+    int32_t n_ambient;
+    const float* ambient;         // ambient intensities in Scene order
 };
 
 // Per-view constants of the main camera (passed by value).
@@ -92,6 +107,7 @@ struct ClipRec {
     float4 pos[3];  // screen x y z, parent recipw
     float4 nor[3];  // normal x y z, colour RGBA8 bits
     float4 wp[3];   // back-transformed m_i x y z (render/raster.go:467-469)
+    float2 uv[3];   // texture coordinates (render/clipping.go:88-91)
 };
 
 struct Counters {

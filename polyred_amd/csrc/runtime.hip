@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/prh.h"
@@ -256,6 +257,31 @@ int resize_weights(int dy, float scale, std::vector<int16_t>& coeffs, std::vecto
     return fl;
 }
 
+// calcFactors (internal/imageutil/resize.go:121-138) and the size Resize then
+// makes (resize.go:17-23): a zero width or height follows the other's scale.
+void resize_plan(int width, int height, int iw, int ih, float* sx, float* sy, int* ow, int* oh) {
+    const float oldw = (float)iw, oldh = (float)ih;
+    if (width == 0) {
+        *sy = height == 0 ? 1.0f : oldh / (float)height;
+        *sx = *sy;
+    } else {
+        *sx = oldw / (float)width;
+        *sy = height == 0 ? *sx : oldh / (float)height;
+    }
+    if (width == 0) width = (int)(0.7f + oldw / *sx);
+    if (height == 0) height = (int)(0.7f + oldh / *sy);
+    *ow = width, *oh = height;
+}
+
+// math.Log2 as Go computes it (Frexp, then Log of the fraction)
+double go_log2(double x) {
+    if (!(x < INFINITY)) return x;
+    int e;
+    const double f = std::frexp(x, &e);
+    if (f == 0.5) return (double)(e - 1);
+    return std::log(f) * (1 / 0.693147180559945309417232121458176568) + (double)e;
+}
+
 template <class T>
 int dalloc(T** p, size_t n) {
     *p = nullptr;
@@ -445,6 +471,107 @@ int prh_close(prh_ctx* ctx) {
     return PRH_OK;
 }
 
+// The textures and their mip chains (buffer.NewTexture, buffer/texture.go:
+// 35-68): level 0 is the image, level i = imageutil.Resize(dx / 2^i, dy / 2^i)
+// of it for int(Log2(max(dx, dy))) + 1 levels (one for a 1x1 image), each
+// resize run on the device as two separable passes of k_resize.
+static int upload_textures(prh_ctx* ctx, const prh_scene_desc* d, prh_scene* s, DevScene* ds) {
+    if (d->n_textures == 0) return PRH_OK;
+    struct Pass {
+        int src_tex, level, iw, ih, w, h, flx, fly;
+        size_t cx, ox, cy, oy;  // offsets into the coefficient / start tables
+    };
+    std::vector<DevTexture> tex(d->n_textures);
+    std::vector<Pass> passes;
+    std::vector<int16_t> coeffs;
+    std::vector<int32_t> starts;
+    int64_t total = 0, tmp_max = 1;
+    for (int i = 0; i < d->n_textures; i++) {
+        const prh_texture& t = d->textures[i];
+        DevTexture& dt = tex[i];
+        memset(&dt, 0, sizeof dt);
+        dt.use_mipmap = t.use_mipmap != 0;
+        dt.size = t.width;
+        const int dx = t.width, dy = t.height;
+        const int L = dx == 1 && dy == 1 ? 1 : (int)(float)go_log2((double)max2((float)dx, (float)dy)) + 1;
+        if (L > TEX_MAX_LEVELS) return fail(PRH_E_UNSUPPORTED, "texture has too many mip levels");
+        dt.n = L;
+        dt.w[0] = dx, dt.h[0] = dy, dt.off[0] = (int32_t)total;
+        total += (int64_t)dx * dy;
+        for (int l = 1; l < L; l++) {
+            float sx, sy;
+            int w, h;
+            resize_plan(dx >> l, dy >> l, dx, dy, &sx, &sy, &w, &h);
+            dt.w[l] = w, dt.h[l] = h, dt.off[l] = (int32_t)total;
+            total += (int64_t)w * h;
+            if (total >= (1ll << 31)) return fail(PRH_E_UNSUPPORTED, "textures exceed 2^31 texels");
+            Pass p{i, l, dx, dy, w, h, 0, 0, coeffs.size(), starts.size(), 0, 0};
+            if (w == dx && h == dy) {  // Resize returns the image itself
+                p.flx = 0;
+                passes.push_back(p);
+                continue;
+            }
+            std::vector<int16_t> cx, cy;
+            std::vector<int32_t> ox, oy;
+            p.flx = resize_weights(w, sx, cx, ox);
+            p.fly = resize_weights(h, sy, cy, oy);
+            // Go divides by the row's coefficient sum: it must not be zero
+            for (auto [c, fl, n] : {std::tuple{&cx, p.flx, w}, std::tuple{&cy, p.fly, h}})
+                for (int y = 0; y < n; y++) {
+                    int32_t sum = 0;
+                    for (int k = 0; k < fl; k++) sum += (*c)[(size_t)y * fl + k];
+                    if (sum == 0) return fail(PRH_E_UNSUPPORTED, "texture resize with a zero filter sum");
+                }
+            coeffs.insert(coeffs.end(), cx.begin(), cx.end());
+            starts.insert(starts.end(), ox.begin(), ox.end());
+            p.cy = coeffs.size(), p.oy = starts.size();
+            coeffs.insert(coeffs.end(), cy.begin(), cy.end());
+            starts.insert(starts.end(), oy.begin(), oy.end());
+            tmp_max = std::max<int64_t>(tmp_max, (int64_t)dy * w);
+            passes.push_back(p);
+        }
+    }
+    uint32_t* data = nullptr;
+    DevTexture* dtex = nullptr;
+    int rc = dalloc(&data, (size_t)total);
+    if (!rc) s->allocs.push_back(data);
+    if (!rc) rc = upload(&dtex, tex.data(), tex.size());
+    if (!rc) s->allocs.push_back(dtex);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < d->n_textures && e == hipSuccess; i++)
+        e = hipMemcpyAsync(data + tex[i].off[0], d->textures[i].rgba, (size_t)tex[i].w[0] * tex[i].h[0] * 4,
+                           hipMemcpyHostToDevice, ctx->stream);
+    int16_t* dco = nullptr;
+    int32_t* dst = nullptr;
+    uint32_t* tmp = nullptr;
+    rc = upload(&dco, coeffs.data(), coeffs.size());
+    if (!rc) rc = upload(&dst, starts.data(), starts.size());
+    if (!rc) rc = dalloc(&tmp, (size_t)tmp_max);
+    for (size_t k = 0; !rc && e == hipSuccess && k < passes.size(); k++) {
+        const Pass& p = passes[k];
+        const DevTexture& t = tex[p.src_tex];
+        const uint32_t* img = data + t.off[0];
+        uint32_t* dst_level = data + t.off[p.level];
+        if (p.flx == 0) {
+            e = hipMemcpyAsync(dst_level, img, (size_t)p.w * p.h * 4, hipMemcpyDeviceToDevice, ctx->stream);
+            continue;
+        }
+        // rows of the image -> tmp (w rows of ih), then tmp's rows -> the level
+        launch_resize(ctx->stream, img, p.iw, tmp, p.ih, p.w, dco + p.cx, dst + p.ox, p.flx);
+        launch_resize(ctx->stream, tmp, p.ih, dst_level, p.w, p.h, dco + p.cy, dst + p.oy, p.fly);
+    }
+    if (!rc) {
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(PRH_E_HIP, std::string("texture mip levels: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(dco), (void)hipFree(dst), (void)hipFree(tmp);
+    if (rc) return rc;
+    ds->n_textures = d->n_textures, ds->textures = dtex, ds->tex_data = data;
+    return PRH_OK;
+}
+
 int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
     if (!ctx || !d || !out) return fail(PRH_E_INVALID, "prh_scene_upload: null argument");
     *out = nullptr;
@@ -493,6 +620,15 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
         for (int32_t g = tri_start[o]; g < tri_start[o + 1]; g++) tri_obj[g] = o;
         for (int32_t v = vtx_start[o]; v < vtx_start[o + 1]; v++) vtx_obj[v] = o;
     }
+    if (d->n_textures < 0 || (d->n_textures > 0 && !d->textures)) return fail(PRH_E_INVALID, "bad texture list");
+    for (int i = 0; i < d->n_textures; i++) {
+        const prh_texture& t = d->textures[i];
+        // NewTexture panics below 1x1 (buffer/texture.go:204-206); the bound
+        // keeps the resize sums (<= 255 * 256 * scale) inside int32 as Go's
+        if (t.width < 1 || t.height < 1 || !t.rgba) return fail(PRH_E_INVALID, "texture image is empty");
+        if (t.width > 16384 || t.height > 16384) return fail(PRH_E_UNSUPPORTED, "texture larger than 16384");
+    }
+    if (d->n_textures > 0 && !d->vuv) return fail(PRH_E_INVALID, "textured scene without texture coordinates");
     std::vector<DevMaterial> mats(d->n_materials);
     bool any_ao = false;
     for (int i = 0; i < d->n_materials; i++) {
@@ -512,6 +648,9 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
         dm.ambient_occlusion = m.ambient_occlusion != 0;
         any_ao = any_ao || dm.ambient_occlusion;
         dm.receive_shadow = m.receive_shadow != 0;
+        if (m.texture_id < -1 || m.texture_id >= d->n_textures)
+            return fail(PRH_E_INVALID, "material texture_id out of range");
+        dm.tex_id = m.texture_id;
     }
     prh_scene* s = new prh_scene();
     s->ctx = ctx;
@@ -572,6 +711,15 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
     up(&dvo, vtx_obj.data(), vtx_obj.size());
     up(&dmats, mats.data(), mats.size());
     up(&dlights, lights.data(), lights.size());
+    float* damb;
+    up(&damb, d->ambient_intensity, d->n_ambient);
+    float2* dvuv = nullptr;
+    if (d->n_textures > 0) {
+        std::vector<float2> vuv(Vm);
+        for (int i = 0; i < Vm; i++) vuv[i] = make_float2(d->vuv[2 * i], d->vuv[2 * i + 1]);
+        up(&dvuv, vuv.data(), Vm);
+    }
+    if (!rc) rc = upload_textures(ctx, d, s, &ds);
     if (rc) {
         free_list(s->allocs);
         delete s;
@@ -583,6 +731,7 @@ int prh_scene_upload(prh_ctx* ctx, const prh_scene_desc* d, prh_scene** out) {
     ds.n_objects = O, ds.n_tris = (int32_t)tg, ds.n_tverts = (int32_t)vt;
     ds.n_materials = d->n_materials, ds.mats = dmats;
     ds.n_lights = d->n_lights, ds.lights = dlights;
+    ds.n_ambient = d->n_ambient, ds.ambient = damb, ds.vuv = dvuv;
     ctx->live_scenes++;
     *out = s;
     return PRH_OK;

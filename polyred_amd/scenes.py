@@ -65,6 +65,15 @@ class Material:
     flat_shading: bool = False
     ambient_occlusion: bool = False
     receive_shadow: bool = False
+    texture_id: int = -1        # index into Scene.textures; -1: the uniform `texture` colour
+
+
+@dataclass
+class Texture:
+    """buffer.Texture: an (h, w, 4) uint8 image, row 0 first, and whether
+    Query uses its mip levels (TextureIsoMipmap; NewTexture's default)."""
+    rgba: np.ndarray
+    use_mipmap: bool = True
 
 
 @dataclass
@@ -106,6 +115,7 @@ class Scene:
     shadow_size: int = 0
     background: tuple = (0, 0, 0, 0)
     msaa: int = 1
+    textures: list = field(default_factory=list)  # Texture images the materials name
     name: str = ""
     _keep: list = field(default_factory=list, repr=False)
 
@@ -168,6 +178,13 @@ class Scene:
             mats[i].flat_shading = int(m.flat_shading)
             mats[i].ambient_occlusion = int(m.ambient_occlusion)
             mats[i].receive_shadow = int(m.receive_shadow)
+            mats[i].texture_id = int(m.texture_id)
+        texs = (abi.prh_texture * max(1, len(self.textures)))()
+        for i, t in enumerate(self.textures):
+            px = np.ascontiguousarray(t.rgba, np.uint8)
+            texs[i].height, texs[i].width = px.shape[0], px.shape[1]
+            texs[i].rgba = ptr(px, C.c_uint8)
+            texs[i].use_mipmap = int(t.use_mipmap)
         lights = (abi.prh_light * max(1, len(self.lights)))()
         for i, l in enumerate(self.lights):
             lights[i].kind = l.kind
@@ -177,7 +194,7 @@ class Scene:
             lights[i].color[:] = l.color
             lights[i].cast_shadow = int(l.cast_shadow)
         amb = _f32(self.ambient if self.ambient else [0.0])
-        keep += [mats, lights]
+        keep += [mats, lights, texs]
         d = abi.prh_scene_desc()
         d.n_meshes = len(self.meshes)
         d.mesh_vtx_off = ptr(_i32(vtx_off), C.c_int32)
@@ -199,6 +216,8 @@ class Scene:
         d.n_ambient = len(self.ambient)
         d.ambient_intensity = ptr(amb, C.c_float)
         d.center[:] = [float(c) for c in self.center()]
+        d.n_textures = len(self.textures)
+        d.textures = texs
         self._keep = keep
         return d
 
@@ -243,13 +262,15 @@ def _orient(pos: np.ndarray, tri: np.ndarray, outward) -> np.ndarray:
 
 def uv_sphere(n_lon: int = 32, n_lat: int = 16, color=(255, 255, 255, 255)) -> Mesh:
     """Unit UV sphere with single-vertex poles and smooth (position) normals."""
-    verts = [(0.0, 1.0, 0.0)]
+    verts, uvs = [(0.0, 1.0, 0.0)], [(0.5, 1.0)]
     for i in range(1, n_lat):
         th = math.pi * i / n_lat
         for j in range(n_lon):
             ph = 2 * math.pi * j / n_lon
             verts.append((math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph)))
+            uvs.append((j / n_lon, 1.0 - i / n_lat))
     verts.append((0.0, -1.0, 0.0))
+    uvs.append((0.5, 0.0))
     v = np.array(verts, dtype=np.float64)
     south = len(verts) - 1
 
@@ -272,7 +293,7 @@ def uv_sphere(n_lon: int = 32, n_lat: int = 16, color=(255, 255, 255, 255)) -> M
     nor = np.concatenate([nrm, np.zeros((len(v), 1))], axis=1).astype(F32)
     tri = _orient(pos, tri, lambda a, b, c: (a + b + c) / 3.0)
     col = np.tile(np.array(color, np.uint8), (len(v), 1))
-    uv = np.zeros((len(v), 2), F32)
+    uv = np.array(uvs, F32)  # longitude, latitude (the seam wraps through Query)
     return Mesh(pos, nor, col, uv, tri, np.zeros(len(tri), np.int32))
 
 
@@ -502,3 +523,68 @@ def tiny(seed: int, width: int = 32, height: int = 24, n_tris: int = 6, vertex_c
                  lights=[Light(position=(1.0, 2.0, 3.0)),
                          Light(kind=abi.PRH_LIGHT_DIRECTIONAL, direction=unit3((0.2, -1.0, -0.5)), intensity=0.7)],
                  ambient=[0.15], camera=cam, width=width, height=height, gamma=True, name=f"tiny_{seed}")
+
+
+# ---------------------------------------------------------------- textures
+def procedural_texture(width: int, height: int, seed: int = 0, alpha: bool = False) -> np.ndarray:
+    """An (height, width, 4) uint8 image with structure at every scale (so
+    that each mip level differs): an 8 x 8 checker over two gradients, plus
+    per-texel noise; alpha 255 or, with `alpha`, noisy too."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y, x = np.mgrid[0:height, 0:width]
+    cx, cy = max(1, width // 8), max(1, height // 8)
+    checker = ((x // cx + y // cy) % 2) * 110
+    img = np.zeros((height, width, 4), np.int64)
+    img[..., 0] = x * 255 // max(1, width - 1) // 2 + checker
+    img[..., 1] = y * 255 // max(1, height - 1) // 2 + 110 - checker
+    img[..., 2] = rng.integers(0, 256, (height, width))
+    img[..., 3] = rng.integers(96, 256, (height, width)) if alpha else 255
+    return np.clip(img, 0, 255).astype(np.uint8)
+
+
+def textured(seed: int = 6, width: int = 160, height: int = 120, perspective: bool = True,
+             use_mipmap: bool = True, tex_sizes=((256, 256), (96, 40)), uv_repeat: float = 3.0,
+             n_tris: int = 8, msaa: int = 1) -> Scene:
+    """Textured materials (buffer.NewTexture images, mip-mapped unless
+    `use_mipmap` is off): a ground plane whose u repeats `uv_repeat` times
+    (the wrap of Query), a UV sphere (non-power-of-two texture, alpha), and
+    random triangles with UVs in [-1.5, 2.5] of which several cross the
+    viewport border (the clipped path), next to a uniform-colour material
+    and vertex-colour triangles; two lights and two ambient sources."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    texs = [Texture(procedural_texture(w, h, seed=seed + i, alpha=(i == 1)), use_mipmap=use_mipmap)
+            for i, (w, h) in enumerate(tex_sizes)]
+    sphere = uv_sphere()
+    ground = grid_plane(2, 5.0, -1.0, zrange=(-6.0, 2.5), material=0)
+    # u repeats (Query wraps it); v stays in [0, 1]: Query reads 1 - v, and
+    # maps a negative fraction f to 1 - f > 1, outside the image (transparent)
+    ground.uv[:, 0] = (ground.uv[:, 0] * F32(uv_repeat)).astype(F32)
+    pos = rng.uniform(-2.2, 2.2, (3 * n_tris, 3))
+    pos[:, 1] = rng.uniform(-1.0, 2.0, 3 * n_tris)
+    pos[:, 2] = rng.uniform(-1.5, 1.0, 3 * n_tris)
+    P = np.concatenate([pos, np.ones((3 * n_tris, 1))], 1).astype(F32)
+    nor = rng.normal(size=(3 * n_tris, 3))
+    nor[:, 2] = np.abs(nor[:, 2]) + 0.5
+    nor /= np.linalg.norm(nor, axis=1, keepdims=True)
+    N = np.concatenate([nor, np.zeros((3 * n_tris, 1))], 1).astype(F32)
+    tri = np.arange(3 * n_tris, dtype=np.int32).reshape(n_tris, 3)
+    tri = _orient(P, tri, lambda a, b, c: np.tile([0.0, 0.0, 1.0], (len(a), 1)))
+    mat = rng.choice(np.array([-1, 0, 1, 2], np.int32), n_tris)
+    mat[:2] = (0, 1)
+    col = rng.integers(0, 256, (3 * n_tris, 4)).astype(np.uint8)
+    uv = rng.uniform(-1.5, 2.5, (3 * n_tris, 2)).astype(F32)
+    loose = Mesh(P, N, col, uv, tri, mat)
+    mats = [_mat_from((0.8, 0.8, 0.8), 0.3, 16), _mat_from(rng.uniform(0.3, 0.9, 3), 0.5, 32),
+            _mat_from(rng.uniform(0.3, 0.9, 3), 0.2, 8)]
+    mats[0].texture_id, mats[1].texture_id = 0, 1
+    ident = np.eye(4, dtype=F32).reshape(16)
+    cam = dict(position=(0.0, 1.5, 4.0), fov=50.0, near=0.1, far=100.0)
+    if not perspective:
+        cam = dict(kind=abi.PRH_CAMERA_ORTHOGRAPHIC, position=(0.0, 1.5, 4.0), near=1.0, far=-20.0,
+                   frustum=(-2.5, 2.5, -1.9, 1.9))
+    return Scene(meshes=[sphere, ground, loose], obj_mesh=np.array([0, 1, 2]),
+                 obj_model=np.stack([ident, ident, ident]), obj_mat_base=np.array([1, 0, 0]), materials=mats,
+                 lights=[Light(position=(2.0, 3.0, 2.0), intensity=1.0),
+                         Light(kind=abi.PRH_LIGHT_DIRECTIONAL, direction=unit3((0.3, -1.0, -0.6)), intensity=0.6)],
+                 ambient=[0.1, 0.05], camera=cam, width=width, height=height, gamma=True, msaa=msaa,
+                 textures=texs, name=f"textured_{seed}")
