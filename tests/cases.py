@@ -1,5 +1,7 @@
 """Hand-built edge-case scenes shared by the oracle tests and the GPU parity
 tests."""
+import os
+
 import numpy as np
 
 from polyred_amd import abi, scenes
@@ -69,3 +71,25 @@ def no_materials(width=40, height=30):
     s = scenes.tiny(8, width=width, height=height, n_tris=10)
     s.materials = []
     return s
+
+
+TESTDATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "testdata")
+
+
+def reference_bunny(width=1920, height=1080, msaa=2, root=TESTDATA):
+    """render/raster_test.go:32-53 (newscene) as TestRender configures it
+    (:55-64): bunny.obj rotated -pi/6 about y, scaled 4, translated
+    (0.1, 0, -0.2); a point light (intensity 5, black colour) at
+    (-2, 2.5, 6) and an ambient light 0.5; the camera at (0, 1.5, 1) looking
+    at (0, 0, -0.5), fov 45, near 0.1, far 3; MSAA 2, background
+    (0, 127, 255, 255), no gamma correction (the option's default)."""
+    from polyred_amd import model
+    g = model.load_obj(os.path.join(root, "bunny.obj"))
+    g.rotate((0, 1, 0), float(np.float32(-np.pi / 6)))
+    g.scale(4, 4, 4)
+    g.translate(0.1, 0, -0.2)
+    cam = dict(position=(0.0, 1.5, 1.0), target=(0.0, 0.0, -0.5), up=(0.0, 1.0, 0.0), fov=45.0,
+               aspect=float(np.float32(width) / np.float32(height)), near=0.1, far=3.0)
+    return model.scene_from_groups([g], [scenes.Light(position=(-2.0, 2.5, 6.0), intensity=5.0, color=(0, 0, 0, 255))],
+                                   [0.5], cam, width, height, msaa=msaa, gamma=False,
+                                   background=(0, 127, 255, 255), name=f"reference_bunny_{width}x{height}")

@@ -25,6 +25,10 @@ def cases_dict():
         d[f"tiny_{seed}"] = scenes.tiny(seed, width=40, height=30, n_tris=12)
     for seed in range(2):
         d[f"tiny_ortho_{seed}"] = scenes.tiny(100 + seed, width=40, height=30, n_tris=12, perspective=False)
+    d["textured"] = scenes.textured(seed=6, width=96, height=72)
+    d["textured_ortho_nearest"] = scenes.textured(seed=7, width=96, height=72, perspective=False, use_mipmap=False)
+    # render/raster_test.go's TestRender scene: the textured bunny, 1920x1080, MSAA 2
+    d["reference_bunny"] = edge.reference_bunny()
     return d
 
 
