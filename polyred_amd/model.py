@@ -18,9 +18,10 @@ accumulates a quaternion, Scale and Translate left-multiply the internal
 matrix, and the model matrix is internal . R(q), all in float32 with Go's
 rounding (Dot and Cross through float32(fma64)).
 
-Quads and polygons (faces of 4+ vertices) are not on the device path: the
-reference draws quads as two triangles only through a QuadMesh, and a file
-mixing them goes through a PolygonMesh.  `load_obj` rejects them.
+Quads and polygons become triangles as the reference's meshes split them
+(QuadMesh: (1,2,3), (1,3,4); PolygonMesh: a fan), with its quirks: a mesh of
+triangles or of quads alone drops the object's polygons, and a mixed mesh's
+triangles lose their material (-1: vertex colours).
 """
 from __future__ import annotations
 
@@ -404,35 +405,49 @@ def load_obj(path: str) -> Group:
     for ob in fi.objs:
         local: Dict[int, int] = {}
         gm: List[tuple] = []
-        pos, nor, uv, mid = [], [], [], []
+        tris, quads, polys = [], [], []
         for face in ob.faces:
             m = mats.get(face.material) or default
             if id(m) not in local:
                 local[id(m)] = len(gm)
                 gm.append(m)
-            if len(face.vertices) != 3:
-                raise ObjError(f"{len(face.vertices)}-vertex faces are not supported on the device path")
+            k = len(face.vertices)
+            (tris if k == 3 else quads if k == 4 else polys).append((face, local[id(m)]))
+        # the mesh loadObj makes (model/load.go:120-141): triangles only, quads
+        # only (polygons dropped in both), or a PolygonMesh of triangles, then
+        # quads, then polygons whose primitives carry material -1
+        # (primitive.NewPolygon, geometry/primitive/polygon.go:21-25)
+        if tris and not quads:
+            faces, hybrid = tris, False
+        elif quads and not tris:
+            faces, hybrid = quads, False
+        elif tris and quads:
+            faces, hybrid = tris + quads + polys, True
+        else:
+            raise ObjError(f"object {ob.name!r} has no triangle or quad faces: the reference builds no mesh for it")
+        pos, nor, uv, tri, mid = [], [], [], [], []
+        for face, lm in faces:
+            b, k = len(pos), len(face.vertices)
             p = [np.array([*V[i], 1], F32) for i in face.vertices]
             if len(N):
                 n = [np.array([*N[i], 0], F32) for i in face.normals]
-                # loadObj replaces a zero normal by the face normal -- always
-                # into V1 (model/load.go:176-184)
-                for k in range(3):
-                    if _is_zero(n[k]):
-                        n[0] = _face_normal(*p)
+                if k == 3:  # newTrianglePrimitive replaces a zero normal by the
+                    # face normal -- always into V1 (model/load.go:176-184)
+                    for j in range(3):
+                        if _is_zero(n[j]):
+                            n[0] = _face_normal(*p)
             else:
-                n = [np.zeros(4, F32)] * 3
-            t = [np.array(UV[i], F32) for i in face.uvs] if len(UV) else [np.zeros(2, F32)] * 3
+                n = [np.zeros(4, F32)] * k
             pos += p
             nor += n
-            uv += t
-            mid.append(local[id(m)])
-        if not pos:
-            continue
-        T = len(mid)
-        mesh = scenes.Mesh(np.array(pos, F32), np.array(nor, F32), np.full((3 * T, 4), 255, np.uint8),
-                           np.array(uv, F32), np.arange(3 * T, dtype=np.int32).reshape(T, 3),
-                           np.array(mid, np.int32))
+            uv += [np.array(UV[i], F32) for i in face.uvs] if len(UV) else [np.zeros(2, F32)] * k
+            # Quad.Triangles is (1,2,3), (1,3,4); Polygon.Triangles the fan from
+            # vertex 1 (geometry/primitive/quad.go:55-63, polygon.go:71-78)
+            for j in range(k - 2):
+                tri.append((b, b + j + 1, b + j + 2))
+                mid.append(-1 if hybrid else lm)
+        mesh = scenes.Mesh(np.array(pos, F32), np.array(nor, F32), np.full((len(pos), 4), 255, np.uint8),
+                           np.array(uv, F32), np.array(tri, np.int32), np.array(mid, np.int32))
         geoms.append(Geometry(mesh, [m for m, _ in gm], [img for _, img in gm]))
     return Group(geoms)
 

@@ -93,3 +93,70 @@ def reference_bunny(width=1920, height=1080, msaa=2, root=TESTDATA):
     return model.scene_from_groups([g], [scenes.Light(position=(-2.0, 2.5, 6.0), intensity=5.0, color=(0, 0, 0, 255))],
                                    [0.5], cam, width, height, msaa=msaa, gamma=False,
                                    background=(0, 127, 255, 255), name=f"reference_bunny_{width}x{height}")
+
+
+def _load(name, scale, translate=None):
+    from polyred_amd import model
+    g = model.load_obj(os.path.join(TESTDATA, name))
+    g.scale(scale, scale, scale)
+    if translate is not None:
+        g.translate(*translate)
+    return g
+
+
+def _ref_scene(groups, lights, ambient, cam, w, h, **opts):
+    from polyred_amd import model
+    return model.scene_from_groups(groups, lights, ambient, cam, w, h, background=(0, 127, 255, 255), gamma=False,
+                                   **opts)
+
+
+def reference_shadow(two_lights=False):
+    """render/gpudeferred_shadow_test.go:28-60 (and gpudeferred_shadow2_test.go
+    with the second light): bunny and ground scaled 2, every material
+    ReceiveShadow, point light(s) of intensity 3 casting shadows, ambient 0.7,
+    200x200, framebuffer-sized shadow maps."""
+    lights = [scenes.Light(position=(4.0, 4.0, 2.0), intensity=3.0, cast_shadow=True)]
+    if two_lights:
+        lights.append(scenes.Light(position=(-6.0, 4.0, 2.0), intensity=3.0, cast_shadow=True))
+    cam = dict(position=(0.0, 0.6, 0.9), fov=45.0, aspect=1.0, near=0.1, far=2.0)
+    s = _ref_scene([_load("bunny.obj", 2), _load("ground.obj", 2)], lights, [0.7], cam, 200, 200, shadow_map=True,
+                   name="reference_shadow2" if two_lights else "reference_shadow")
+    for m in s.materials:
+        m.receive_shadow = True
+    return s
+
+
+def reference_multimaterial():
+    """render/gpudeferred_multimat_test.go:24-50: the bunny and the gopher (7
+    materials; its objects mixing triangles and quads keep no material) side
+    by side, 160x160."""
+    lights = [scenes.Light(position=(-2.0, 2.5, 6.0), intensity=3.0, color=(0, 0, 0, 255))]
+    cam = dict(position=(0.0, 1.5, 1.0), target=(0.0, 0.0, -0.5), fov=45.0, aspect=1.0, near=0.1, far=3.0)
+    return _ref_scene([_load("bunny.obj", 4, (-0.3, 0, -0.2)), _load("gopher.obj", 4, (0.4, 0, -0.2))], lights, [0.5],
+                      cam, 160, 160, name="reference_multimaterial")
+
+
+def reference_ao():
+    """render/gpudeferred_ao_test.go:25-50: the bunny scaled 2 with
+    AmbientOcclusion on, 150x150."""
+    cam = dict(position=(0.0, 0.6, 0.9), fov=45.0, aspect=1.0, near=0.1, far=2.0)
+    s = _ref_scene([_load("bunny.obj", 2)], [scenes.Light(position=(2.0, 3.0, 4.0), intensity=3.0)], [0.5], cam, 150,
+                   150, name="reference_ao")
+    for m in s.materials:
+        m.ambient_occlusion = True
+    return s
+
+
+def reference_directional():
+    """render/gpudeferred_directional_test.go:23-42: TestRender's bunny under
+    a directional light (intensity 0.9, colour (255,240,220), direction
+    (-1,-1,-1)) and ambient 0.3, 150x150."""
+    from polyred_amd import model
+    g = model.load_obj(os.path.join(TESTDATA, "bunny.obj"))
+    g.rotate((0, 1, 0), float(np.float32(-np.pi / 6)))
+    g.scale(4, 4, 4)
+    g.translate(0.1, 0, -0.2)
+    light = scenes.Light(kind=abi.PRH_LIGHT_DIRECTIONAL, direction=scenes.unit3((-1.0, -1.0, -1.0)), intensity=0.9,
+                         color=(255, 240, 220, 255))
+    cam = dict(position=(0.0, 1.5, 1.0), target=(0.0, 0.0, -0.5), fov=45.0, aspect=1.0, near=0.1, far=3.0)
+    return _ref_scene([g], [light], [0.3], cam, 150, 150, name="reference_directional")

@@ -29,6 +29,12 @@ def cases_dict():
     d["textured_ortho_nearest"] = scenes.textured(seed=7, width=96, height=72, perspective=False, use_mipmap=False)
     # render/raster_test.go's TestRender scene: the textured bunny, 1920x1080, MSAA 2
     d["reference_bunny"] = edge.reference_bunny()
+    # the scenes of the reference's CPU-vs-GPU deferred tests (render/gpudeferred_*_test.go)
+    d["reference_shadow"] = edge.reference_shadow()
+    d["reference_shadow2"] = edge.reference_shadow(two_lights=True)
+    d["reference_multimaterial"] = edge.reference_multimaterial()
+    d["reference_ao"] = edge.reference_ao()
+    d["reference_directional"] = edge.reference_directional()
     return d
 
 

@@ -71,9 +71,8 @@ def test_bunny_geometry_is_the_file():
 def test_obj_forms(tmp_path):
     """Relative indices, the `v x y z w` divisor, faces before any object or
     material (material.Default(): Kd 0.7, Ks 0.5, Ns 30, Blue), a usemtl
-    name no library defines,
-    a zero normal (replaced by the face normal, into V1 whichever vertex had
-    it: model/load.go:176-184) and a quad (not on the device path)."""
+    name no library defines, and a zero normal (replaced by the face normal,
+    into V1 whichever vertex had it: model/load.go:176-184)."""
     (tmp_path / "t.obj").write_text(
         "v 0 0 0\nv 2 0 0 2\nv 0 1 0\nvn 0 0 1\nvn 0 0 0\n"
         "f 1//1 2//1 3//1\n"
@@ -94,9 +93,28 @@ def test_obj_forms(tmp_path):
     assert (z.diffuse, z.specular, z.shininess, z.texture) == ((0, 0, 0, 0), (0, 0, 0, 0), 0.0, (0, 0, 255, 255))
     # second triangle: its second vertex had the zero normal; V1 gets the face normal
     assert np.array_equal(m.nor[3], [0, 0, 1, 0]) and np.array_equal(m.nor[4], [0, 0, 0, 0])
-    (tmp_path / "q.obj").write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n")
+
+
+def test_quads_and_polygons(tmp_path):
+    """QuadMesh splits (1,2,3), (1,3,4) and keeps the material; a mesh mixing
+    triangles and quads lists triangles, then quads, then polygons (fans),
+    all with material -1; a triangle-only or quad-only mesh drops polygons;
+    polygons alone leave no mesh (model/load.go:120-141,
+    geometry/primitive/quad.go:55-63, polygon.go:21-25,71-78)."""
+    head = "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0.5 1.5 0\n"
+    (tmp_path / "q.obj").write_text(head + "f 1 2 3 4\nf 1 2 3 4 5\n")
+    m = model.load_obj(str(tmp_path / "q.obj")).geometries[0].mesh
+    assert np.array_equal(m.tri, [[0, 1, 2], [0, 2, 3]]) and np.array_equal(m.mat, [0, 0])
+    assert np.array_equal(m.pos[:, :2], [[0, 0], [1, 0], [1, 1], [0, 1]])
+    (tmp_path / "h.obj").write_text(head + "f 1 2 3 4 5\nf 1 2 3 4\nf 2 3 4\n")
+    m = model.load_obj(str(tmp_path / "h.obj")).geometries[0].mesh
+    assert m.pos.shape[0] == 3 + 4 + 5
+    assert np.array_equal(m.pos[:3, :2], [[1, 0], [1, 1], [0, 1]])  # the triangle first
+    assert np.array_equal(m.tri, [[0, 1, 2], [3, 4, 5], [3, 5, 6], [7, 8, 9], [7, 9, 10], [7, 10, 11]])
+    assert np.all(m.mat == -1)
+    (tmp_path / "p.obj").write_text(head + "f 1 2 3 4 5\n")
     with pytest.raises(model.ObjError):
-        model.load_obj(str(tmp_path / "q.obj"))
+        model.load_obj(str(tmp_path / "p.obj"))
 
 
 def test_parse_f32_rounds_once():
