@@ -734,7 +734,7 @@ __global__ __launch_bounds__(256) void k_raster(const Prim* __restrict__ prims, 
 // ao.go reads an empty fragment); after the exact-order pass, a NaN-depth
 // fragment that came first at its pixel takes it.
 template <bool EXACT>
-__global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, const Counters* cnt, float* __restrict__ depth) {
+__global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, Counters* cnt, float* __restrict__ depth) {
     const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (o >= (size_t)rt.W * rt.H || cnt->overflow) return;
     unsigned long long v = rt.vis[o];
@@ -746,7 +746,9 @@ __global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, const Counters
         }
     }
     const uint32_t hi = (uint32_t)(v >> 32);
-    depth[o] = v == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
+    const float d = v == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
+    depth[o] = d;
+    if (!(__builtin_fabsf(d) <= 3.40282347e38f)) atomicOr(&cnt->depth_nonfinite, 1);  // lets k_ao skip NaN checks
 }
 
 // ---------------------------------------------------------------- shade
@@ -1097,11 +1099,17 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
     if (__ballot(need) != 0) {
         const float dx = c.ao_dir[2 * wv], dy = c.ao_dir[2 * wv + 1];
         const float X = (float)x, Y = (float)y, dp = s_dp[lane];
-        // the tap with the largest exact elevation/distance quotient, tracked
-        // with exact float64 cross products (e * d_best > e_best * d: both
-        // products of two float32 are exact); its rounded quotient is the
-        // largest rounded quotient
-        double e_best = -__builtin_inf(), d_best = 1.0;
+        // the tap with the largest exact elevation/distance quotient; its
+        // rounded quotient is the largest rounded quotient.  The general
+        // path compares exact float64 cross products (e * d_best > e_best * d:
+        // products of two float32 are exact).  The table paths compare
+        // q = e * fl64(1 / d) instead, within 2^-52 of e / d: two distinct
+        // quotients of float32 e and d (d a float32 >= 1) differ relatively by
+        // at least 2^-47, and e / d lies at least 2^-47 (relatively) from any
+        // float32 rounding boundary, so q orders distinct quotients exactly
+        // (equal ones may order either way, and round alike) and float(q)
+        // is the correctly rounded float32 quotient.
+        double e_best = -__builtin_inf(), d_best = 1.0, q_best = -__builtin_inf();
         bool nan = false;
         // depth taps through a buffer descriptor: an out-of-range offset
         // reads 0 instead of faulting, so a masked lane needs no clamped
@@ -1121,7 +1129,48 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
             d_best = better ? d64 : d_best;
             nan |= v & (e != e);
         };
-        if (ao_fast && (wu & 1) == 0) {
+        auto take_q = [&](bool v, float e, double rinv) {
+            const double q = (double)e * rinv;
+            q_best = (v & (q > q_best)) ? q : q_best;
+            nan |= v & (e != e);
+        };
+        // every depth finite: no quotient is NaN (an infinite e gives an
+        // infinite q), so the NaN test can go (uniform)
+        const bool finite = cnt->depth_nonfinite == 0;
+        const int* axis = reinterpret_cast<const int*>(ao_tab + AO_TAB_AXIS) + (wu >> 1) * 4;
+        const int bx0 = (blk % bx) * 8, by0 = (blk / bx) * 8;
+        const double* rinv_t = reinterpret_cast<const double*>(ao_tab + AO_TAB_T64);  // fl64(1 / t) (uniform)
+        if (ao_fast && (wu & 1) == 0 && finite && (axis[0] ? by0 : bx0) >= axis[2]) {
+            // Axis direction, the whole block away from the small minor
+            // coordinates: the major offset is exactly +-t and the minor one
+            // leaves the tap on the pixel's row (column), so tap t is the
+            // pixel t steps along the row and lies in the buffer for
+            // t < tlim.  Up to the wave's smallest tlim no lane needs the
+            // mask (lanes without AO compute values nobody reads): a plain
+            // running max of q.
+            const bool along_x = axis[0] != 0;
+            const int sgn = axis[1];
+            const int maj = along_x ? x : y, n_maj = along_x ? W : H;
+            const int tlim = need ? (sgn > 0 ? n_maj - maj : maj + 1) : 100;
+            const int step = sgn * (along_x ? 4 : 4 * W);
+            const uint32_t off0 = (__umul24((unsigned)y, (unsigned)W) + (unsigned)x) * 4u;
+            int tmin = tlim;  // the wave's smallest
+            for (int k = 32; k >= 1; k >>= 1) tmin = min(tmin, __shfl_xor(tmin, k));
+            tmin = min(__builtin_amdgcn_readfirstlane(tmin), 100);
+            int ti = 1;
+#pragma unroll 4
+            for (; ti < tmin; ti++) {
+                const float e = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(off0 + (uint32_t)(ti * step)), 0, 0)) - dp;
+                q_best = __builtin_fmax(q_best, (double)e * rinv_t[ti]);
+            }
+            for (; ti < 100; ti++) {
+                const float e = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(off0 + (uint32_t)(ti * step)), 0, 0)) - dp;
+                const double q = (double)e * rinv_t[ti];
+                q_best = (need & (ti < tlim) & (q > q_best)) ? q : q_best;
+            }
+        } else if (ao_fast && (wu & 1) == 0) {
             // Axis directions (a = 0, pi/2, pi, 3pi/2 in float32): one offset
             // is exactly +-t, the other the tiny cos/sin residue (<= 8.8e-8 t)
             // rounded onto the coordinate grid, under 2e-5 for coordinates
@@ -1134,36 +1183,77 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
                 const int ix = (int)cx, iy = (int)cy;
                 const bool v = need & ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
                 const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
-                take(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, (double)ti);
+                take_q(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, rinv_t[ti]);
             }
         } else if (ao_fast) {
             // Diagonal directions: X - fl(X + fl(dir.x * t)) is fixed by t and
             // the binade of the tap coordinate (checked when the table is
             // built, for every coordinate below 4096), so the distance
-            // Len = sqrt32(fma64(ex, ex, ey * ey)) is a table over
-            // (binade of cx, binade of cy, t), held as float64.
+            // Len = sqrt32(fma64(ex, ex, ey * ey)) is fixed by (binade of cx,
+            // binade of cy, t); the table holds fl64(1 / Len).
             const double* dtab =
                 reinterpret_cast<const double*>(ao_tab + AO_TAB_DIST) + ((wu - 1) >> 1) * AO_DIST_PER_DIR;
             const __amdgpu_buffer_rsrc_t trs =
                 __builtin_amdgcn_make_buffer_rsrc((void*)dtab, (short)0, AO_DIST_PER_DIR * 8, BUF_DWORD3);
-            constexpr int KMAX = AO_EXP_MIN + AO_EXP_N - 1;
             // byte offset of (kx, ky, t) = kx * ROW_X + ky * ROW_Y + t * 8 + BIAS
             constexpr int ROW_Y = 100 * 8, ROW_X = AO_EXP_N * ROW_Y, BIAS = -AO_EXP_MIN * (ROW_X + ROW_Y);
-            auto tap = [&](int ti, bool first) {
+            struct Tap {
+                double rinv;
+                float e;
+                bool in;  // in the buffer
+            };
+            auto tap = [&](int ti) {
                 const float cx = X + off_x[ti], cy = Y + off_y[ti];
                 const int ix = (int)cx, iy = (int)cy;
-                const int kx = min(max(__builtin_amdgcn_frexp_expf(cx), AO_EXP_MIN), KMAX);
-                const int ky = min(max(__builtin_amdgcn_frexp_expf(cy), AO_EXP_MIN), KMAX);
-                const int toff = kx * ROW_X + ky * ROW_Y + (ti * 8 + BIAS);
-                const double dist = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(trs, toff, 0, 0));
-                bool v = need & ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
-                if (first) v = v & (dist >= 1.0);  // distance < 1 only occurs at t = 1
+                // an exponent outside the table only occurs for a tap outside
+                // the buffer (checked when the table is built); its offset
+                // then reads 0 or another entry, masked either way
+                const int kx = __builtin_amdgcn_frexp_expf(cx), ky = __builtin_amdgcn_frexp_expf(cy);
+                const int toff = __mul24(kx, ROW_X) + __mul24(ky, ROW_Y) + (ti * 8 + BIAS);
+                Tap r;
+                r.rinv = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(trs, toff, 0, 0));
+                r.in = ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
                 const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
-                take(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, dist);
+                r.e = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
+                return r;
             };
-            tap(1, true);
-#pragma unroll 3
-            for (int ti = 2; ti < 100; ti++) tap(ti, false);
+            {
+                const Tap a = tap(1);
+                take_q(need & a.in & (a.rinv <= 1.0), a.e, a.rinv);  // distance < 1 only occurs at t = 1
+            }
+            int ti = 2;
+            if (finite) {
+                // Taps with t <= r / (|dir| * 1.001), r the room to the border
+                // the direction heads for, stay in the buffer whatever the
+                // rounding of fl(X + fl(dir * t)) (off by far less than the
+                // 0.1 % margin): up to the wave's smallest such bound no lane
+                // needs the mask.
+                int tl = 100;
+                if (need) {
+                    const float rx = dx > 0 ? (float)(W - 1 - x) : X, ry = dy > 0 ? (float)(H - 1 - y) : Y;
+                    const float lim = fminf(rx / (__builtin_fabsf(dx) * 1.001f), ry / (__builtin_fabsf(dy) * 1.001f));
+                    tl = min((int)lim + 1, 100);
+                }
+                for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
+                const int tmin = __builtin_amdgcn_readfirstlane(tl);
+#pragma unroll 4
+                for (; ti < tmin; ti++) {
+                    const Tap a = tap(ti);
+                    q_best = __builtin_fmax(q_best, (double)a.e * a.rinv);
+                }
+#pragma unroll 2
+                for (; ti < 100; ti++) {
+                    const Tap a = tap(ti);
+                    const double q = (double)a.e * a.rinv;
+                    q_best = (need & a.in & (q > q_best)) ? q : q_best;
+                }
+            } else {
+#pragma unroll 2
+                for (; ti < 100; ti++) {
+                    const Tap a = tap(ti);
+                    take_q(need & a.in, a.e, a.rinv);
+                }
+            }
         } else {
             float t = 1.0f;  // t = 0 is the pixel itself: distance 0 < 1, skipped
 #pragma unroll 3
@@ -1179,7 +1269,7 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
                 take(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, (double)dist);
             }
         }
-        const float maxr = (float)e_best / (float)d_best;
+        const float maxr = ao_fast ? (float)q_best : (float)e_best / (float)d_best;
         m = nan ? __builtin_nanf("") : (maxr > 0 ? (float)atan64((double)maxr) : 0.0f);
     }
     s_m[wv][lane] = m;
@@ -1352,7 +1442,7 @@ void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims
         hipLaunchKernelGGL(k_raster<MODE_VIS>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
                            item_tile, tile_list, cnt, rt, cap_items);
 }
-void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, const Counters* cnt, float* depth) {
+void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth) {
     const size_t P = (size_t)rt.W * rt.H;
     if (exact) hipLaunchKernelGGL(k_resolve<true>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth);
     else hipLaunchKernelGGL(k_resolve<false>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth);

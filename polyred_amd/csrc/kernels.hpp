@@ -68,9 +68,13 @@ constexpr int AO_EXP_MIN = -10;
 constexpr int AO_EXP_N = 24;
 constexpr int AO_DIST_PER_DIR = AO_EXP_N * AO_EXP_N * 100;  // doubles
 constexpr int AO_TAB_OFF = 0;           // float [8 directions][2][100 t]: fl(dir * t)
-constexpr int AO_TAB_T64 = 8 * 200;     // double [100]: t
-constexpr int AO_TAB_DIST = AO_TAB_T64 + 200;  // double [4 diagonals][AO_EXP_N][AO_EXP_N][100 t]
-constexpr int AO_TAB_SIZE = AO_TAB_DIST + 2 * 4 * AO_DIST_PER_DIR;
+constexpr int AO_TAB_T64 = 8 * 200;     // double [100]: fl64(1 / t)
+constexpr int AO_TAB_DIST = AO_TAB_T64 + 200;  // double [4 diagonals][AO_EXP_N][AO_EXP_N][100 t]: fl64(1 / distance)
+// int32 [4 axis directions][4]: major axis (1 = x), sign of the major
+// offset (+-1, exactly +-t), the smallest minor coordinate from which the
+// minor offset never moves the tap off its row/column (INT32_MAX: none)
+constexpr int AO_TAB_AXIS = AO_TAB_DIST + 2 * 4 * AO_DIST_PER_DIR;
+constexpr int AO_TAB_SIZE = AO_TAB_AXIS + 16;
 // Third descriptor word of a raw (unstrided, untyped) buffer on gfx9 parts.
 constexpr uint32_t BUF_DWORD3 = 0x00020000u;
 
@@ -100,7 +104,7 @@ void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
                    Counters* cnt, const RasterTarget& rt);
-void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, const Counters* cnt, float* depth);
+void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth);
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);

@@ -120,7 +120,8 @@ struct Counters {
     int32_t shadow_nan;
     int32_t n_chunks;       // raster work items: (tile, up to ITEM_PRIMS primitives)
     int32_t next_item;      // persistent raster work queue head
-    int32_t pad[7];
+    int32_t depth_nonfinite;  // some pixel's resolved depth is NaN or infinite
+    int32_t pad[6];
 };
 
 struct Capacities {

@@ -200,9 +200,28 @@ bool ao_tables(const float dirs[16], std::vector<float>& tab) {
             tab[AO_TAB_OFF + d * 200 + t] = dirs[2 * d] * (float)t;
             tab[AO_TAB_OFF + d * 200 + 100 + t] = dirs[2 * d + 1] * (float)t;
         }
-    double* t64 = reinterpret_cast<double*>(tab.data() + AO_TAB_T64);
-    for (int t = 0; t < 100; t++) t64[t] = (double)t;
-    double* dist = reinterpret_cast<double*>(tab.data() + AO_TAB_DIST);
+    double* rinv_t = reinterpret_cast<double*>(tab.data() + AO_TAB_T64);
+    for (int t = 0; t < 100; t++) rinv_t[t] = 1.0 / (double)t;  // correctly rounded; t = 0 unused
+    // axis directions: is the major offset exactly +-t, and from which minor
+    // coordinate M on is fl(M + minor offset) == M for every t
+    int32_t* axis = reinterpret_cast<int32_t*>(tab.data() + AO_TAB_AXIS);
+    for (int a = 0; a < 4; a++) {
+        const int d = 2 * a;
+        const bool along_x = std::fabs(dirs[2 * d]) >= std::fabs(dirs[2 * d + 1]);
+        const float maj = dirs[2 * d + (along_x ? 0 : 1)], mnr = dirs[2 * d + (along_x ? 1 : 0)];
+        const int sgn = maj > 0 ? 1 : -1;
+        bool exact = true;
+        for (int t = 0; t < 100; t++) exact = exact && maj * (float)t == (float)(sgn * t);
+        int mmin = 0;
+        for (int M = 0; M < 4096; M++)
+            for (int t = 0; t < 100; t++)
+                if ((float)M + mnr * (float)t != (float)M) mmin = M + 1;
+        axis[4 * a + 0] = along_x ? 1 : 0;
+        axis[4 * a + 1] = sgn;
+        axis[4 * a + 2] = exact ? mmin : 0x7fffffff;
+        axis[4 * a + 3] = 0;
+    }
+    double* rinv_d = reinterpret_cast<double*>(tab.data() + AO_TAB_DIST);
     for (int q = 0; q < 4; q++) {
         const int d = 2 * q + 1;
         // e[comp][t][bucket]: the offset for each binade bucket (NaN: unused)
@@ -231,7 +250,7 @@ bool ao_tables(const float dirs[16], std::vector<float>& tab) {
                     // from t = 2 on every distance is >= 1 (checked): the
                     // march tests distance < 1 at t = 1 only
                     if (t >= 2 && len < 1.0f) return false;
-                    dist[q * AO_DIST_PER_DIR + (bx * AO_EXP_N + by) * 100 + t] = (double)len;
+                    rinv_d[q * AO_DIST_PER_DIR + (bx * AO_EXP_N + by) * 100 + t] = 1.0 / (double)len;
                 }
     }
     return true;
