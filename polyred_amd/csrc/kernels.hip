@@ -1048,8 +1048,9 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 
 // Screen-space AO (material/ao.go:20-73) for every pixel whose material has
 // it, plus pixel (0,0) (the uncovered pixels are shaded from it).  One
-// workgroup = one 8x8 pixel block x the 8 march directions, one wave per
-// direction.  maxElevationAngle returns at the first tap outside the buffer;
+// workgroup = one 64-pixel row segment x the 8 march directions, one wave per
+// direction; lanes hold consecutive pixels of the row, so each tap of every
+// direction reads contiguous depths across the wave.  maxElevationAngle returns at the first tap outside the buffer;
 // the tap coordinates move monotonically along the ray, so a tap never
 // re-enters once out and masking every later tap gives the same result --
 // which turns the march into a fixed 99-step loop with no divergence.  The
@@ -1063,14 +1064,64 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
     __shared__ float s_m[8][64];
     __shared__ float s_dp[64];
     __shared__ int32_t s_need[64];
+    __shared__ float s_win[2][192];  // the row around the segment for the two x-axis marches
+    __shared__ int32_t s_dbase[4][100];  // diagonal marches, tap t: byte offset of lane 0's tap
+    __shared__ double s_drinv[4][100];   // and fl64(1 / distance), where every lane's tap shares them
     if (cnt->overflow) return;
     const int W = c.v.W, H = c.v.H;
-    const int bx = (W + 7) / 8, nb = bx * ((H + 7) / 8);
+    const int nbx = (W + 63) / 64, nb = nbx * H;
     const int per_xcd = (nb + 7) / 8;
     const int blk = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     if (blk >= nb) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x = (blk % bx) * 8 + (lane & 7), y = (blk / bx) * 8 + (lane >> 3);
+    const int x0 = (blk % nbx) * 64, y = blk / nbx;
+    const int x = x0 + lane;
+    // wave w marches direction 2w (an axis) for w < 4 and 2(w-4)+1 (a
+    // diagonal) after: the waves sharing a SIMD (w, w+4) then pair a cheap
+    // axis march with a costlier diagonal one
+    const int wu = __builtin_amdgcn_readfirstlane(wv < 4 ? 2 * wv : 2 * (wv - 4) + 1);
+    const int* axis = reinterpret_cast<const int*>(ao_tab + AO_TAB_AXIS) + (wu >> 1) * 4;
+    if (ao_fast && (wu & 3) == 0 && axis[0]) {
+        // directions 0 and 4 march along x: depths x0+1 .. x0+162 (or
+        // x0-99 .. x0+62) of this row, 0 outside it
+        const int ws = axis[1] > 0 ? x0 + 1 : x0 - 99;
+        for (int j = lane; j < 192; j += 64) {
+            const int px = ws + j;
+            s_win[wu >> 2][j] = (unsigned)px < (unsigned)W ? depth[(size_t)y * W + px] : 0.0f;
+        }
+    }
+    // Diagonal marches: where lanes 0 and 63 (so every lane) put tap t's x
+    // in one binade, fl(X + fl(dir.x * t)) - X is the same for the whole
+    // segment (the table's invariant), so every lane's tap is lane 0's
+    // shifted by the lane, at one distance: offset and 1/distance per t,
+    // and a mask of those t, for the march below.
+    uint64_t uni_lo = 0, uni_hi = 0;
+    constexpr int ROW_Y = 100 * 8, ROW_X = AO_EXP_N * ROW_Y, BIAS = -AO_EXP_MIN * (ROW_X + ROW_Y);
+    if (ao_fast && (wu & 1)) {
+        const int k = wu >> 1;
+        const float* ox_t = ao_tab + AO_TAB_OFF + wu * 200;
+        const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(ao_tab + AO_TAB_DIST + 2 * k * AO_DIST_PER_DIR), (short)0, AO_DIST_PER_DIR * 8, BUF_DWORD3);
+        const float X0 = (float)x0, X63 = (float)(x0 + 63), Yf = (float)y;
+        bool u0 = false, u1 = false;
+        for (int h = 0; h < 2; h++) {
+            const int t = lane + 64 * h;
+            bool uni = false;
+            if (t < 100) {
+                const float lo = X0 + ox_t[t], hi = X63 + ox_t[t], cy = Yf + ox_t[100 + t];
+                const int kl = __builtin_amdgcn_frexp_expf(lo), kh = __builtin_amdgcn_frexp_expf(hi);
+                uni = (lo > 0.0f) & (kl == kh);
+                const int iy = (int)cy, ky = __builtin_amdgcn_frexp_expf(cy);
+                s_dbase[k][t] = (iy * W + (int)lo) * 4;
+                const int toff = kl * ROW_X + ky * ROW_Y + t * 8 + BIAS;
+                s_drinv[k][t] =
+                    uni ? __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(trs, toff, 0, 0)) : 0.0;
+            }
+            (h ? u1 : u0) = uni;
+        }
+        uni_lo = __ballot(u0);
+        uni_hi = __ballot(u1);
+    }
     if (wv == 0) {
         bool need = false;
         float dp = -1.0f;
@@ -1097,8 +1148,9 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
     const bool need = s_need[lane] != 0;
     float m = 0.0f;
     if (__ballot(need) != 0) {
-        const float dx = c.ao_dir[2 * wv], dy = c.ao_dir[2 * wv + 1];
+        const float dx = c.ao_dir[2 * wu], dy = c.ao_dir[2 * wu + 1];
         const float X = (float)x, Y = (float)y, dp = s_dp[lane];
+        (void)dx;
         // the tap with the largest exact elevation/distance quotient; its
         // rounded quotient is the largest rounded quotient.  The general
         // path compares exact float64 cross products (e * d_best > e_best * d:
@@ -1116,7 +1168,6 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
         // address (its value is never used)
         const __amdgpu_buffer_rsrc_t drs =
             __builtin_amdgcn_make_buffer_rsrc((void*)depth, (short)0, W * H * 4, BUF_DWORD3);
-        const int wu = __builtin_amdgcn_readfirstlane(wv);
         const float* off_x = ao_tab + AO_TAB_OFF + wu * 200;  // fl(dir.x * t), t = 0..99 (uniform)
         const float* off_y = off_x + 100;                     // fl(dir.y * t)
         // one tap: the running argmax of elevation / distance, exact
@@ -1137,38 +1188,40 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
         // every depth finite: no quotient is NaN (an infinite e gives an
         // infinite q), so the NaN test can go (uniform)
         const bool finite = cnt->depth_nonfinite == 0;
-        const int* axis = reinterpret_cast<const int*>(ao_tab + AO_TAB_AXIS) + (wu >> 1) * 4;
-        const int bx0 = (blk % bx) * 8, by0 = (blk / bx) * 8;
         const double* rinv_t = reinterpret_cast<const double*>(ao_tab + AO_TAB_T64);  // fl64(1 / t) (uniform)
-        if (ao_fast && (wu & 1) == 0 && finite && (axis[0] ? by0 : bx0) >= axis[2]) {
-            // Axis direction, the whole block away from the small minor
-            // coordinates: the major offset is exactly +-t and the minor one
-            // leaves the tap on the pixel's row (column), so tap t is the
-            // pixel t steps along the row and lies in the buffer for
-            // t < tlim.  Up to the wave's smallest tlim no lane needs the
-            // mask (lanes without AO compute values nobody reads): a plain
-            // running max of q.
-            const bool along_x = axis[0] != 0;
+        // Axis direction, the segment away from the small minor coordinates:
+        // the major offset is exactly +-t and the minor one leaves the tap on
+        // the pixel's row (column), so tap t is the pixel t steps along it,
+        // in the buffer while t < tlim.  Up to the wave's smallest tlim no
+        // lane needs the mask (lanes without AO compute values nobody reads):
+        // a plain running max of q.
+        const bool row_fast = ao_fast && (wu & 1) == 0 && finite && (axis[0] ? y : x0) >= axis[2];
+        if (row_fast && axis[0] && (wu & 3) == 0) {  // (directions 0 and 4, whose windows were loaded)
+            // along x: tap t of lane i is window entry i - 1 + t (i + 99 - t)
             const int sgn = axis[1];
-            const int maj = along_x ? x : y, n_maj = along_x ? W : H;
-            const int tlim = need ? (sgn > 0 ? n_maj - maj : maj + 1) : 100;
-            const int step = sgn * (along_x ? 4 : 4 * W);
-            const uint32_t off0 = (__umul24((unsigned)y, (unsigned)W) + (unsigned)x) * 4u;
-            int tmin = tlim;  // the wave's smallest
-            for (int k = 32; k >= 1; k >>= 1) tmin = min(tmin, __shfl_xor(tmin, k));
-            tmin = min(__builtin_amdgcn_readfirstlane(tmin), 100);
+            const float* win = s_win[wu >> 2];
+            const int base = sgn > 0 ? lane - 1 : lane + 99;
+            const int tlim = need ? (sgn > 0 ? W - x : x + 1) : 100;
+            int tl = tlim;  // the wave's smallest
+            for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
+            const int tmin = min(__builtin_amdgcn_readfirstlane(tl), 100);
             int ti = 1;
 #pragma unroll 4
-            for (; ti < tmin; ti++) {
-                const float e = __uint_as_float(
-                    __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(off0 + (uint32_t)(ti * step)), 0, 0)) - dp;
-                q_best = __builtin_fmax(q_best, (double)e * rinv_t[ti]);
-            }
+            for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(win[base + sgn * ti] - dp) * rinv_t[ti]);
             for (; ti < 100; ti++) {
-                const float e = __uint_as_float(
-                    __builtin_amdgcn_raw_buffer_load_b32(drs, (int)(off0 + (uint32_t)(ti * step)), 0, 0)) - dp;
-                const double q = (double)e * rinv_t[ti];
+                const double q = (double)(win[base + sgn * ti] - dp) * rinv_t[ti];
                 q_best = (need & (ti < tlim) & (q > q_best)) ? q : q_best;
+            }
+        } else if (row_fast && !axis[0]) {
+            // along y: tap t is row y +- t, the same for every lane, in the
+            // buffer for t < tl; the march ends there for the whole wave
+            const int sgn = axis[1];
+            const int tl = min(sgn > 0 ? H - y : y + 1, 100);
+#pragma unroll 4
+            for (int ti = 1; ti < tl; ti++) {
+                const uint32_t off = ((uint32_t)((y + sgn * ti) * W) + (uint32_t)x) * 4u;
+                const float e = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
+                q_best = __builtin_fmax(q_best, (double)e * rinv_t[ti]);
             }
         } else if (ao_fast && (wu & 1) == 0) {
             // Axis directions (a = 0, pi/2, pi, 3pi/2 in float32): one offset
@@ -1196,7 +1249,6 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
             const __amdgpu_buffer_rsrc_t trs =
                 __builtin_amdgcn_make_buffer_rsrc((void*)dtab, (short)0, AO_DIST_PER_DIR * 8, BUF_DWORD3);
             // byte offset of (kx, ky, t) = kx * ROW_X + ky * ROW_Y + t * 8 + BIAS
-            constexpr int ROW_Y = 100 * 8, ROW_X = AO_EXP_N * ROW_Y, BIAS = -AO_EXP_MIN * (ROW_X + ROW_Y);
             struct Tap {
                 double rinv;
                 float e;
@@ -1236,7 +1288,27 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
                 }
                 for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
                 const int tmin = __builtin_amdgcn_readfirstlane(tl);
-#pragma unroll 4
+                const int kd = wu >> 1;
+                const uint32_t lane4 = (uint32_t)lane * 4u;
+                for (; ti + 3 < tmin; ti += 4) {
+                    const uint64_t bits = ti < 64 ? (uni_lo >> ti) | (ti > 60 ? uni_hi << (64 - ti) : 0ull)
+                                                  : uni_hi >> (ti - 64);
+                    if ((bits & 0xFull) == 0xFull) {  // four taps of one shape across the wave
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const uint32_t off = (uint32_t)s_dbase[kd][ti + j] + lane4;
+                            const float e =
+                                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
+                            q_best = __builtin_fmax(q_best, (double)e * s_drinv[kd][ti + j]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const Tap a = tap(ti + j);
+                            q_best = __builtin_fmax(q_best, (double)a.e * a.rinv);
+                        }
+                    }
+                }
                 for (; ti < tmin; ti++) {
                     const Tap a = tap(ti);
                     q_best = __builtin_fmax(q_best, (double)a.e * a.rinv);
@@ -1272,7 +1344,7 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
         const float maxr = ao_fast ? (float)q_best : (float)e_best / (float)d_best;
         m = nan ? __builtin_nanf("") : (maxr > 0 ? (float)atan64((double)maxr) : 0.0f);
     }
-    s_m[wv][lane] = m;
+    s_m[wu][lane] = m;
     __syncthreads();
     if (wv == 0 && need) {
         float total = 0.0f;
@@ -1448,7 +1520,7 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters
     else hipLaunchKernelGGL(k_resolve<false>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth);
 }
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
-    const int nb = ((c.v.W + 7) / 8) * ((c.v.H + 7) / 8);
+    const int nb = ((c.v.W + 63) / 64) * c.v.H;
     hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(512), 0, st, s, c, b.vis, b.depth, b.ao, b.ao_tab,
                        b.ao_fast, b.cnt);
 }
