@@ -1217,7 +1217,7 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
             // buffer for t < tl; the march ends there for the whole wave
             const int sgn = axis[1];
             const int tl = min(sgn > 0 ? H - y : y + 1, 100);
-#pragma unroll 4
+#pragma unroll 8
             for (int ti = 1; ti < tl; ti++) {
                 const uint32_t off = ((uint32_t)((y + sgn * ti) * W) + (uint32_t)x) * 4u;
                 const float e = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
@@ -1290,20 +1290,20 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
                 const int tmin = __builtin_amdgcn_readfirstlane(tl);
                 const int kd = wu >> 1;
                 const uint32_t lane4 = (uint32_t)lane * 4u;
-                for (; ti + 3 < tmin; ti += 4) {
-                    const uint64_t bits = ti < 64 ? (uni_lo >> ti) | (ti > 60 ? uni_hi << (64 - ti) : 0ull)
+                for (; ti + 7 < tmin; ti += 8) {
+                    const uint64_t bits = ti < 64 ? (uni_lo >> ti) | (ti > 56 ? uni_hi << (64 - ti) : 0ull)
                                                   : uni_hi >> (ti - 64);
-                    if ((bits & 0xFull) == 0xFull) {  // four taps of one shape across the wave
+                    if ((bits & 0xFFull) == 0xFFull) {  // eight taps of one shape across the wave
 #pragma unroll
-                        for (int j = 0; j < 4; j++) {
+                        for (int j = 0; j < 8; j++) {
                             const uint32_t off = (uint32_t)s_dbase[kd][ti + j] + lane4;
                             const float e =
                                 __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
                             q_best = __builtin_fmax(q_best, (double)e * s_drinv[kd][ti + j]);
                         }
                     } else {
-#pragma unroll
-                        for (int j = 0; j < 4; j++) {
+#pragma unroll 4
+                        for (int j = 0; j < 8; j++) {
                             const Tap a = tap(ti + j);
                             q_best = __builtin_fmax(q_best, (double)a.e * a.rinv);
                         }
