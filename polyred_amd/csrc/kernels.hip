@@ -734,7 +734,8 @@ __global__ __launch_bounds__(256) void k_raster(const Prim* __restrict__ prims, 
 // ao.go reads an empty fragment); after the exact-order pass, a NaN-depth
 // fragment that came first at its pixel takes it.
 template <bool EXACT>
-__global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, Counters* cnt, float* __restrict__ depth) {
+__global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, Counters* cnt, float* __restrict__ depth, DevScene s,
+                                                 uint8_t* __restrict__ ao_need) {
     const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (o >= (size_t)rt.W * rt.H || cnt->overflow) return;
     unsigned long long v = rt.vis[o];
@@ -749,6 +750,23 @@ __global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, Counters* cnt,
     const float d = v == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
     depth[o] = d;
     if (!(__builtin_fabsf(d) <= 3.40282347e38f)) atomicOr(&cnt->depth_nonfinite, 1);  // lets k_ao skip NaN checks
+    if (ao_need) {
+        // the pixels k_ao computes: those whose material has AmbientOcclusion,
+        // and pixel (0,0) when covered (uncovered pixels are shaded from it)
+        bool need = false;
+        if (v != VIS_EMPTY) {
+            if (o == 0) {
+                need = true;
+            } else {
+                const int g = (int)((uint32_t)v >> 3);
+                const int ob = s.tri_obj[g];
+                int mat = s.tri_mat[s.mesh_tri_off[s.obj_mesh[ob]] + (g - s.obj_tri_start[ob])];
+                if (mat >= 0) mat += s.obj_mat_base[ob];
+                need = mat >= 0 && mat < s.n_materials && s.mats[mat].ambient_occlusion;
+            }
+        }
+        ao_need[o] = need;
+    }
 }
 
 // ---------------------------------------------------------------- shade
@@ -1058,12 +1076,10 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 // elevation/distance ratio (atan and its rounding to float32 are monotone);
 // a NaN ratio poisons it as math.Max does.  The per-pixel factor
 // pow(total, 10000) (ao.go:31-33) is written for k_shade.
-__global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uint64_t* __restrict__ vis,
+__global__ __launch_bounds__(512) void k_ao(ShadeConsts c, const uint8_t* __restrict__ ao_need,
                                             const float* __restrict__ depth, float* __restrict__ ao,
                                             const float* __restrict__ ao_tab, int32_t ao_fast, const Counters* cnt) {
     __shared__ float s_m[8][64];
-    __shared__ float s_dp[64];
-    __shared__ int32_t s_need[64];
     __shared__ float s_win[2][192];  // the row around the segment for the two x-axis marches
     __shared__ int32_t s_dbase[4][100];  // diagonal marches, tap t: byte offset of lane 0's tap
     __shared__ double s_drinv[4][100];   // and fl64(1 / distance), where every lane's tap shares them
@@ -1076,6 +1092,11 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int x0 = (blk % nbx) * 64, y = blk / nbx;
     const int x = x0 + lane;
+    // every wave reads the segment's need flags (k_resolve wrote them): a
+    // segment without AO pixels ends here, for all its waves alike
+    const bool need = x < W && ao_need[(size_t)y * W + x] != 0;
+    if (__ballot(need) == 0) return;
+    const float dp = need ? depth[(size_t)y * W + x] : -1.0f;
     // wave w marches direction 2w (an axis) for w < 4 and 2(w-4)+1 (a
     // diagonal) after: the waves sharing a SIMD (w, w+4) then pair a cheap
     // axis march with a costlier diagonal one
@@ -1122,34 +1143,15 @@ __global__ __launch_bounds__(512) void k_ao(DevScene s, ShadeConsts c, const uin
         uni_lo = __ballot(u0);
         uni_hi = __ballot(u1);
     }
-    if (wv == 0) {
-        bool need = false;
-        float dp = -1.0f;
-        if (x < W && y < H) {
-            const uint64_t key = vis[(size_t)y * W + x];
-            if (key != VIS_EMPTY) {
-                dp = depth[(size_t)y * W + x];
-                if (x == 0 && y == 0) {
-                    need = true;
-                } else {
-                    const int g = (int)((uint32_t)key >> 3);
-                    const int o = s.tri_obj[g];
-                    const int mesh = s.obj_mesh[o];
-                    int mat = s.tri_mat[s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o])];
-                    if (mat >= 0) mat += s.obj_mat_base[o];
-                    need = mat >= 0 && mat < s.n_materials && s.mats[mat].ambient_occlusion;
-                }
-            }
-        }
-        s_need[lane] = need;
-        s_dp[lane] = dp;
-    }
-    __syncthreads();
-    const bool need = s_need[lane] != 0;
+    // the window and the diagonal tables above are this wave's own LDS:
+    // its writes are visible to its lanes past a wave-level fence
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float m = 0.0f;
-    if (__ballot(need) != 0) {
+    {
         const float dx = c.ao_dir[2 * wu], dy = c.ao_dir[2 * wu + 1];
-        const float X = (float)x, Y = (float)y, dp = s_dp[lane];
+        const float X = (float)x, Y = (float)y;
         (void)dx;
         // the tap with the largest exact elevation/distance quotient; its
         // rounded quotient is the largest rounded quotient.  The general
@@ -1514,14 +1516,15 @@ void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims
         hipLaunchKernelGGL(k_raster<MODE_VIS>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
                            item_tile, tile_list, cnt, rt, cap_items);
 }
-void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth) {
+void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
+                    uint8_t* ao_need) {
     const size_t P = (size_t)rt.W * rt.H;
-    if (exact) hipLaunchKernelGGL(k_resolve<true>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth);
-    else hipLaunchKernelGGL(k_resolve<false>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth);
+    if (exact) hipLaunchKernelGGL(k_resolve<true>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth, s, ao_need);
+    else hipLaunchKernelGGL(k_resolve<false>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth, s, ao_need);
 }
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const int nb = ((c.v.W + 63) / 64) * c.v.H;
-    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(512), 0, st, s, c, b.vis, b.depth, b.ao, b.ao_tab,
+    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(512), 0, st, c, b.ao_need, b.depth, b.ao, b.ao_tab,
                        b.ao_fast, b.cnt);
 }
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {

@@ -35,6 +35,7 @@ struct ShadeBuffers {
     float* bg_lin;             // [3]
     const Counters* cnt;       // main-pass counters: nothing is shaded after an overflow
     float* ao;                 // [H][W] AO factor of the pixels that need it
+    const uint8_t* ao_need;    // [H][W] 1 where k_ao computes the factor
     const float* ao_tab;       // AO march tables (tap offsets, diagonal distances)
     int32_t ao_fast;           // the tables hold for this frame size
 };
@@ -104,7 +105,8 @@ void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
                    Counters* cnt, const RasterTarget& rt);
-void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth);
+void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
+                    uint8_t* ao_need);
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);

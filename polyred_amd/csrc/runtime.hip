@@ -388,6 +388,7 @@ struct prh_renderer {
     int32_t *rz_ox, *rz_oy;  // and their start offsets
     int rz_flx, rz_fly;
     float *linear, *bg_lin, *ao;
+    uint8_t* ao_need;  // scenes with AO only
     float* ao_tab;   // AO march tables (ao_tables)
     int32_t ao_fast; // the tables hold for this frame size
     Counters* cnt;      // [2]: main, shadow
@@ -905,6 +906,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->vis, P));
     if (!rc) chk(ralloc(r, &r->depth, P));
     if (!rc) chk(ralloc(r, &r->ao, P));
+    if (!rc && s->any_ao) chk(ralloc(r, &r->ao_need, P));
     if (!rc && msaa == 1) chk(ralloc(r, &r->image, P));
     if (!rc && msaa > 1) {
         // Resize(width, height, img): scale = old / new = MSAA (calcFactors,
@@ -1029,7 +1031,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
                     r->first_nan, nullptr};
     launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                   r->tile_list, cm, rt);
-    launch_resolve(st, nan_mode, rt, cm, r->depth);
+    launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr);
     if (timed) (void)hipEventRecord(r->ev[4], st);
     ShadeBuffers b;
     b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
@@ -1038,6 +1040,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     b.image = r->msaa > 1 ? r->image_big : r->image, b.pregamma = r->pregamma, b.linear = r->linear, b.bg = r->bg, b.bg_lin = r->bg_lin;
     b.cnt = cm;
     b.ao = r->ao;
+    b.ao_need = r->ao_need;
     b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
     if (r->sc->any_ao) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
     if (timed) (void)hipEventRecord(r->ev[5], st);
