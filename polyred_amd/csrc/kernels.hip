@@ -1066,17 +1066,18 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 
 // Screen-space AO (material/ao.go:20-73) for every pixel whose material has
 // it, plus pixel (0,0) (the uncovered pixels are shaded from it).  One
-// workgroup = one 64-pixel row segment x the 8 march directions, one wave per
-// direction; lanes hold consecutive pixels of the row, so each tap of every
-// direction reads contiguous depths across the wave.  maxElevationAngle returns at the first tap outside the buffer;
-// the tap coordinates move monotonically along the ray, so a tap never
-// re-enters once out and masking every later tap gives the same result --
-// which turns the march into a fixed 99-step loop with no divergence.  The
-// running Max of atan over the taps equals atan of the largest
-// elevation/distance ratio (atan and its rounding to float32 are monotone);
-// a NaN ratio poisons it as math.Max does.  The per-pixel factor
-// pow(total, 10000) (ao.go:31-33) is written for k_shade.
-__global__ __launch_bounds__(512) void k_ao(ShadeConsts c, const uint8_t* __restrict__ ao_need,
+// workgroup = one 64-pixel row segment x the 8 march directions, four waves of
+// two directions each; lanes hold consecutive pixels of the row, so each tap
+// of every direction reads contiguous depths across the wave.
+// maxElevationAngle returns at the first tap outside the buffer; the tap
+// coordinates move monotonically along the ray, so a tap never re-enters once
+// out and masking every later tap gives the same result -- which turns the
+// march into a fixed 99-step loop with no divergence.  The running Max of atan
+// over the taps equals atan of the largest elevation/distance ratio (atan and
+// its rounding to float32 are monotone); a NaN ratio poisons it as math.Max
+// does.  The per-pixel factor pow(total, 10000) (ao.go:31-33) is written for
+// k_shade.
+__global__ __launch_bounds__(256) void k_ao(ShadeConsts c, const uint8_t* __restrict__ ao_need,
                                             const float* __restrict__ depth, float* __restrict__ ao,
                                             const float* __restrict__ ao_tab, int32_t ao_fast, const Counters* cnt) {
     __shared__ float s_m[8][64];
@@ -1097,10 +1098,10 @@ __global__ __launch_bounds__(512) void k_ao(ShadeConsts c, const uint8_t* __rest
     const bool need = x < W && ao_need[(size_t)y * W + x] != 0;
     if (__ballot(need) == 0) return;
     const float dp = need ? depth[(size_t)y * W + x] : -1.0f;
-    // wave w marches direction 2w (an axis) for w < 4 and 2(w-4)+1 (a
-    // diagonal) after: the waves sharing a SIMD (w, w+4) then pair a cheap
-    // axis march with a costlier diagonal one
-    const int wu = __builtin_amdgcn_readfirstlane(wv < 4 ? 2 * wv : 2 * (wv - 4) + 1);
+    // wave w marches direction 2w (an axis, cheap) then 2w+1 (a diagonal,
+    // costlier): four waves of equal work, none idling at the barrier
+    for (int half = 0; half < 2; half++) {
+    const int wu = __builtin_amdgcn_readfirstlane(2 * wv + half);
     const int* axis = reinterpret_cast<const int*>(ao_tab + AO_TAB_AXIS) + (wu >> 1) * 4;
     if (ao_fast && (wu & 3) == 0 && axis[0]) {
         // directions 0 and 4 march along x: depths x0+1 .. x0+162 (or
@@ -1347,6 +1348,7 @@ __global__ __launch_bounds__(512) void k_ao(ShadeConsts c, const uint8_t* __rest
         m = nan ? __builtin_nanf("") : (maxr > 0 ? (float)atan64((double)maxr) : 0.0f);
     }
     s_m[wu][lane] = m;
+    }
     __syncthreads();
     if (wv == 0 && need) {
         float total = 0.0f;
@@ -1524,7 +1526,7 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters
 }
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const int nb = ((c.v.W + 63) / 64) * c.v.H;
-    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(512), 0, st, c, b.ao_need, b.depth, b.ao, b.ao_tab,
+    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(256), 0, st, c, b.ao_need, b.depth, b.ao, b.ao_tab,
                        b.ao_fast, b.cnt);
 }
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
