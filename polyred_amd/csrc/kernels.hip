@@ -378,10 +378,18 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
 
 // render/shadow.go:152-176 (drawDepth up to the pixel loop) for every
 // (triangle, shadow slot): back-face cull, cullViewFrustum, pixel box.
+// A shadow-map triangle whose pixel box holds at most this many texel
+// centres is drawn here: each centre is tested and merged into the map with a
+// global atomic max, exactly as k_raster<SHADOW> would (the same Barycoord,
+// depth and max over ordered bits; order-independent), and it is neither
+// stored nor binned.  Most triangles of a map fitted to the whole frustum
+// cover a texel or two.
+constexpr int SHADOW_DIRECT_MAX = 4;
+
 __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts sc, const float4* __restrict__ ts,
                                                       Prim* __restrict__ prims, uint2* __restrict__ pbox,
                                                       int32_t* __restrict__ tile_count,
-                                                      Counters* cnt, Capacities cap) {
+                                                      Counters* cnt, Capacities cap, uint32_t* __restrict__ smaps) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
     bool emit = false;
@@ -402,6 +410,22 @@ __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts s
                     emit = pixel_box(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, sc.SW, sc.SH, x0, y0, x1, y1);
             }
         }
+    }
+    if (emit && (x1 - x0 + 1) * (y1 - y0 + 1) <= SHADOW_DIRECT_MAX) {
+        emit = false;
+        const TriSetup t = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+        uint32_t* map = smaps + (size_t)slot * sc.SW * sc.SH;
+        bool nan = false;
+        for (int py = y0; py <= y1; py++)
+            for (int px = x0; px <= x1; px++) {
+                float bc[3];
+                barycoord(t, (float)px + 0.5f, (float)py + 0.5f, bc);
+                if (!inside(bc)) continue;
+                const float z = bc[0] * p1.z + bc[1] * p2.z + bc[2] * p3.z;  // raster.go:496
+                if (z != z) nan = true;
+                else if (z > 0.0f) atomicMax(&map[(size_t)py * sc.SW + px], depth_ord(z));  // the map starts at 0
+            }
+        if (nan) atomicOr(&cnt->shadow_nan, 1);
     }
     if (g < s.n_tris) {
         const size_t ps = (size_t)slot * s.n_tris + g;
@@ -1484,10 +1508,10 @@ void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, 
                        prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
 }
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
-                         uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap) {
+                         uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap, uint32_t* smaps) {
     if (!s.n_tris || !sc.n_slots) return;
     hipLaunchKernelGGL(k_setup_shadow, dim3(blocks_for(s.n_tris, 256), sc.n_slots), dim3(256), 0, st, s, sc, ts, prims,
-                       pbox, tile_count, cnt, cap);
+                       pbox, tile_count, cnt, cap, smaps);
 }
 int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK); }
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,

@@ -90,7 +90,8 @@ void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, 
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap);
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
-                         Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap);
+                         Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
+                         uint32_t* smaps);
 // n_blk > 0: also sum the per-workgroup primitive counts of the setup into
 // the frame statistics (the main view)
 // bsum: 2 * scan_blocks(n) ints of scratch

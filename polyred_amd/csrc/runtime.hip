@@ -1002,7 +1002,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     if (r->n_slots > 0 && !nan_mode) {
         const int nt = r->n_slots * r->stiles_per_map;
         launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
-        launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->pbox, r->tile_count, cs, r->cap_shadow);
+        launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->pbox, r->tile_count, cs, r->cap_shadow, r->smaps);
         launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
                     r->cap_items, nullptr, 0, 0, cs);
         launch_bin(st, r->pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
