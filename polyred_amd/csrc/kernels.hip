@@ -383,8 +383,8 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
 // global atomic max, exactly as k_raster<SHADOW> would (the same Barycoord,
 // depth and max over ordered bits; order-independent), and it is neither
 // stored nor binned.  Most triangles of a map fitted to the whole frustum
-// cover a texel or two.
-constexpr int SHADOW_DIRECT_MAX = 4;
+// cover a few texels; past a few dozen the binned raster is no slower.
+constexpr int SHADOW_DIRECT_MAX = 48;
 
 __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts sc, const float4* __restrict__ ts,
                                                       Prim* __restrict__ prims, uint2* __restrict__ pbox,
