@@ -254,13 +254,21 @@ __global__ __launch_bounds__(256) void k_vertex_shadow(DevScene s, const M4* __r
 // cull, viewport AABB test, then either one primitive in the triangle's own
 // slot (all vertices inside) or a place in the clipper's queue.  Each
 // primitive is counted into every 16x16 tile its pixel box touches.
+// An unclipped triangle whose pixel box holds at most this many pixel
+// centres is drawn by the setup itself outside the exact-order pass: each
+// centre is tested as k_raster<VIS> tests it and merged into the visibility
+// buffer with a global 64-bit atomic min of the same key (order-independent),
+// and the triangle is neither stored nor binned.
+constexpr int MAIN_DIRECT_MAX = 48;
+
 __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
                                                     Prim* __restrict__ prims, uint2* __restrict__ pbox,
                                                     int32_t* __restrict__ clip_list,
                                                     int32_t* __restrict__ tile_count, int32_t* __restrict__ blk_prims,
-                                                    Counters* cnt, Capacities cap) {
+                                                    Counters* cnt, Capacities cap, unsigned long long* __restrict__ vis,
+                                                    int32_t draw_small) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    bool direct = false, doclip = false;
+    bool direct = false, doclip = false, drawn = false;
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
@@ -284,10 +292,26 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
                 }
             }
         }
-        if (direct) store_prim(prims, pbox, g, p1, p2, p3, (uint32_t)g * 8u + SUB_DIRECT, x0, y0, x1, y1);
+        const uint32_t key = (uint32_t)g * 8u + SUB_DIRECT;
+        if (direct && draw_small && (x1 - x0 + 1) * (y1 - y0 + 1) <= MAIN_DIRECT_MAX) {
+            drawn = true;
+            const TriSetup ts = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+            bool nan = false;
+            for (int py = y0; py <= y1; py++)
+                for (int px = x0; px <= x1; px++) {
+                    float bc[3];
+                    barycoord(ts, (float)px + 0.5f, (float)py + 0.5f, bc);
+                    if (!inside(bc)) continue;
+                    const float z = bc[0] * p1.z + bc[1] * p2.z + bc[2] * p3.z;  // raster.go:496
+                    if (z != z) nan = true;
+                    else atomicMin(&vis[(size_t)py * vc.W + px], ((unsigned long long)depth_key_hi(z) << 32) | key);
+                }
+            if (nan) atomicOr(&cnt->nan_flag, 1);
+        }
+        if (direct && !drawn) store_prim(prims, pbox, g, p1, p2, p3, key, x0, y0, x1, y1);
         else store_empty(pbox, g);
     }
-    count_tiles(tile_count, 0, vc.tiles_x, direct, x0, y0, x1, y1);
+    count_tiles(tile_count, 0, vc.tiles_x, direct && !drawn, x0, y0, x1, y1);
     const int pos = wave_alloc(&cnt->n_clipped, 0, doclip);
     if (doclip) {
         if (pos < cap.clip_tris) clip_list[pos] = g;
@@ -742,8 +766,14 @@ __global__ __launch_bounds__(256) void k_raster(const Prim* __restrict__ prims, 
             }
         } else if (x < rt.W && y < rt.H) {
             const size_t o = (size_t)y * rt.W + x;
-            if (!shared_tile) rt.vis[o] = svis[tid];
-            else if (svis[tid] != VIS_EMPTY) atomicMin(&rt.vis[o], svis[tid]);
+            // a single-item tile is this item's alone here, but the setup may
+            // have drawn small triangles into it already: keep the smaller key
+            if (!shared_tile) {
+                const unsigned long long cur = rt.vis[o];
+                rt.vis[o] = svis[tid] < cur ? svis[tid] : cur;
+            } else if (svis[tid] != VIS_EMPTY) {
+                atomicMin(&rt.vis[o], svis[tid]);
+            }
             if (EXACT) {
                 if (sfirst[tid] != 0xffffffffu) atomicMin(&rt.first_all[o], sfirst[tid]);
                 if (snan[tid] != 0xffffffffu) atomicMin(&rt.first_nan[o], snan[tid]);
@@ -1500,10 +1530,11 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, c
 }
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
-                       int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap) {
+                       int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
+                       unsigned long long* vis, bool draw_small) {
     if (!s.n_tris) return;
     hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, prims, pbox,
-                       clip_list, tile_count, blk_prims, cnt, cap);
+                       clip_list, tile_count, blk_prims, cnt, cap, vis, (int32_t)draw_small);
     hipLaunchKernelGGL(k_clip, dim3(blocks_for(cap.clip_tris, 64)), dim3(64), 0, st, s, vc, tv_scr, tv_nor, clip_list,
                        prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
 }

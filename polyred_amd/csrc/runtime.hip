@@ -1014,15 +1014,16 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     if (timed) (void)hipEventRecord(r->ev[1], st);
     launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
     if (timed) (void)hipEventRecord(r->ev[2], st);
+    const size_t P = (size_t)r->W * r->H;
+    launch_fill_u64(st, r->vis, P, VIS_EMPTY);  // before the setup, which draws small triangles into it
+    // (outside the exact-order pass, whose NaN rule needs every fragment in order)
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
-                      r->tile_count, r->blk_prims, cm, r->cap_main);
+                      r->tile_count, r->blk_prims, cm, r->cap_main, (unsigned long long*)r->vis, !nan_mode);
     launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
                 r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
     launch_bin(st, r->pbox, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
                d.n_tris);
     if (timed) (void)hipEventRecord(r->ev[3], st);
-    const size_t P = (size_t)r->W * r->H;
-    launch_fill_u64(st, r->vis, P, VIS_EMPTY);
     if (nan_mode) {
         launch_fill_u32(st, r->first_all, P, 0xffffffffu);
         launch_fill_u32(st, r->first_nan, P, 0xffffffffu);
