@@ -254,6 +254,73 @@ __global__ __launch_bounds__(256) void k_vertex_shadow(DevScene s, const M4* __r
 // cull, viewport AABB test, then either one primitive in the triangle's own
 // slot (all vertices inside) or a place in the clipper's queue.  Each
 // primitive is counted into every 16x16 tile its pixel box touches.
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ void block_scan_256(int v, int32_t* start /*[257]*/, int32_t* wsum /*[4]*/) {
+    int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < wv; k++) before += wsum[k];
+    start[tid] = before + incl - v;
+    if (tid == 255) start[256] = before + incl;
+    __syncthreads();
+}
+
+// Small triangles drawn by the setup workgroups themselves: each thread's
+// candidate pixel centres (its pixel box) are spread over the 256 threads of
+// the workgroup, each centre tested as the rasters test it (the same
+// Barycoord, inside test and depth, raster.go:487-496) and handed to `put`
+// with its triangle's staged record.
+struct DPrim {
+    float ax, ay, bx, by, abx, aby, acx, acy, bcx, bcy, sabc;  // TriSetup
+    float az, bz, cz;
+    int32_t xy0;  // x0 | y0 << 16
+    int32_t w;    // box width
+    uint32_t key;
+};
+template <class F>
+__device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, uint32_t key, int x0, int y0, int x1,
+                                           int y1, DPrim* sp, int32_t* start, int32_t* wsum, F&& put) {
+    const int tid = threadIdx.x;
+    int c = 0;
+    if (mine) {
+        const TriSetup t = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+        DPrim& d = sp[tid];
+        d.ax = t.ax, d.ay = t.ay, d.bx = t.bx, d.by = t.by, d.abx = t.abx, d.aby = t.aby;
+        d.acx = t.acx, d.acy = t.acy, d.bcx = t.bcx, d.bcy = t.bcy, d.sabc = t.sabc;
+        d.az = p1.z, d.bz = p2.z, d.cz = p3.z;
+        d.xy0 = x0 | (y0 << 16), d.w = x1 - x0 + 1, d.key = key;
+        c = (x1 - x0 + 1) * (y1 - y0 + 1);
+    }
+    block_scan_256(c, start, wsum);
+    const int total = start[256];
+    for (int idx = tid; idx < total; idx += 256) {
+        int lo = 0, hi = 255;  // the thread whose range holds idx (empty ranges are skipped)
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= idx) lo = mid;
+            else hi = mid - 1;
+        }
+        const DPrim& d = sp[lo];
+        const int l = idx - start[lo], w = d.w;
+        // l / w by multiply-shift, exact for l, w <= 64
+        const int q = (int)(((uint32_t)l * ((65536u + (uint32_t)w - 1u) / (uint32_t)w)) >> 16);
+        const int px = (d.xy0 & 0xffff) + (l - q * w), py = (d.xy0 >> 16) + q;
+        TriSetup t;
+        t.ax = d.ax, t.ay = d.ay, t.bx = d.bx, t.by = d.by, t.abx = d.abx, t.aby = d.aby;
+        t.acx = d.acx, t.acy = d.acy, t.bcx = d.bcx, t.bcy = d.bcy, t.sabc = d.sabc;
+        float bc[3];
+        barycoord(t, (float)px + 0.5f, (float)py + 0.5f, bc);
+        if (!inside(bc)) continue;
+        put(d, px, py, bc[0] * d.az + bc[1] * d.bz + bc[2] * d.cz);  // raster.go:496
+    }
+}
+
 // An unclipped triangle whose pixel box holds at most this many pixel
 // centres is drawn by the setup itself outside the exact-order pass: each
 // centre is tested as k_raster<VIS> tests it and merged into the visibility
@@ -294,6 +361,7 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
         }
         const uint32_t key = (uint32_t)g * 8u + SUB_DIRECT;
         if (direct && draw_small && (x1 - x0 + 1) * (y1 - y0 + 1) <= MAIN_DIRECT_MAX) {
+            // (a thread draws its own: the main view's boxes vary little)
             drawn = true;
             const TriSetup ts = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
             bool nan = false;
@@ -408,7 +476,7 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
 // depth and max over ordered bits; order-independent), and it is neither
 // stored nor binned.  Most triangles of a map fitted to the whole frustum
 // cover a few texels; past a few dozen the binned raster is no slower.
-constexpr int SHADOW_DIRECT_MAX = 48;
+constexpr int SHADOW_DIRECT_MAX = 64;
 
 __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts sc, const float4* __restrict__ ts,
                                                       Prim* __restrict__ prims, uint2* __restrict__ pbox,
@@ -435,20 +503,17 @@ __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts s
             }
         }
     }
-    if (emit && (x1 - x0 + 1) * (y1 - y0 + 1) <= SHADOW_DIRECT_MAX) {
-        emit = false;
-        const TriSetup t = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+    const bool small = emit && (x1 - x0 + 1) * (y1 - y0 + 1) <= SHADOW_DIRECT_MAX;
+    if (small) emit = false;
+    {
+        __shared__ DPrim sp[256];
+        __shared__ int32_t start[257], wsum[4];
         uint32_t* map = smaps + (size_t)slot * sc.SW * sc.SH;
         bool nan = false;
-        for (int py = y0; py <= y1; py++)
-            for (int px = x0; px <= x1; px++) {
-                float bc[3];
-                barycoord(t, (float)px + 0.5f, (float)py + 0.5f, bc);
-                if (!inside(bc)) continue;
-                const float z = bc[0] * p1.z + bc[1] * p2.z + bc[2] * p3.z;  // raster.go:496
-                if (z != z) nan = true;
-                else if (z > 0.0f) atomicMax(&map[(size_t)py * sc.SW + px], depth_ord(z));  // the map starts at 0
-            }
+        draw_small_tris(small, p1, p2, p3, 0u, x0, y0, x1, y1, sp, start, wsum, [&](const DPrim&, int px, int py, float z) {
+            if (z != z) nan = true;
+            else if (z > 0.0f) atomicMax(&map[(size_t)py * sc.SW + px], depth_ord(z));  // the map starts at 0
+        });
         if (nan) atomicOr(&cnt->shadow_nan, 1);
     }
     if (g < s.n_tris) {
@@ -634,23 +699,6 @@ __device__ __forceinline__ int stage_prim(const Prim& p, int px0, int py0, LPrim
     int w = x1 - x0 + 1, h = y1 - y0 + 1;
     lp.rect = (uint32_t)(x0 - px0) | ((uint32_t)(y0 - py0) << 4) | ((uint32_t)(w - 1) << 8) | ((uint32_t)(h - 1) << 12);
     return w * h;
-}
-
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ __forceinline__ void block_scan_256(int v, int32_t* start /*[257]*/, int32_t* wsum /*[4]*/) {
-    int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    int incl = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        int u = __shfl_up(incl, off);
-        if (lane >= off) incl += u;
-    }
-    if (lane == 63) wsum[wv] = incl;
-    __syncthreads();
-    int before = 0;
-    for (int k = 0; k < wv; k++) before += wsum[k];
-    start[tid] = before + incl - v;
-    if (tid == 255) start[256] = before + incl;
-    __syncthreads();
 }
 
 // Candidate idx -> (primitive j, tile-local pixel).  l / w by multiply-shift
