@@ -193,6 +193,28 @@ GM_HD void barycoord(const TriSetup& t, float px, float py, float bc[3]) {
 GM_HD bool inside(const float bc[3]) {  // render/raster.go:491
     return !(bc[0] < -1e-7f || bc[1] < -1e-7f || bc[2] < -1e-7f);
 }
+// The same inside test and the depth (render/raster.go:487-496) with the
+// divisions only for covered points: fl(S / Sabc) < float32(-1e-7) exactly
+// when S / Sabc < M, M the midpoint between float32(-1e-7) and the float32
+// below it (a quotient of two float32 is never such a midpoint), and for a
+// finite non-zero Sabc, S < M * Sabc (S > M * Sabc when Sabc < 0) decides that
+// exactly, the product of M (25 bits) and Sabc (24 bits) being exact in
+// float64.  Other Sabc take the divisions.
+constexpr double BC_EDGE_MID = -0x1.ad7f2bp-24;
+GM_HD bool cover_z(const TriSetup& t, float px, float py, float az, float bz, float cz, float& z) {
+    float s[3];
+    tri_areas(t, px, py, s);
+    if (t.sabc != 0.0f && t.sabc - t.sabc == 0.0f) {  // finite, non-zero
+        const double ms = BC_EDGE_MID * (double)t.sabc;
+        const bool out = t.sabc > 0.0f ? ((double)s[0] < ms) | ((double)s[1] < ms) | ((double)s[2] < ms)
+                                       : ((double)s[0] > ms) | ((double)s[1] > ms) | ((double)s[2] > ms);
+        if (out) return false;
+    }
+    const float bc[3] = {s[0] / t.sabc, s[1] / t.sabc, s[2] / t.sabc};
+    if (!inside(bc)) return false;
+    z = bc[0] * az + bc[1] * bz + bc[2] * cz;
+    return true;
+}
 
 // Depth ordering for a 64-bit atomic-min visibility key: larger depth gives a
 // smaller key; -0 and +0 tie (they compare equal under '>').  NaN is handled

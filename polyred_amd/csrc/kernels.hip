@@ -314,10 +314,9 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
         TriSetup t;
         t.ax = d.ax, t.ay = d.ay, t.bx = d.bx, t.by = d.by, t.abx = d.abx, t.aby = d.aby;
         t.acx = d.acx, t.acy = d.acy, t.bcx = d.bcx, t.bcy = d.bcy, t.sabc = d.sabc;
-        float bc[3];
-        barycoord(t, (float)px + 0.5f, (float)py + 0.5f, bc);
-        if (!inside(bc)) continue;
-        put(d, px, py, bc[0] * d.az + bc[1] * d.bz + bc[2] * d.cz);  // raster.go:496
+        float z;
+        if (!cover_z(t, (float)px + 0.5f, (float)py + 0.5f, d.az, d.bz, d.cz, z)) continue;
+        put(d, px, py, z);
     }
 }
 
@@ -367,10 +366,8 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
             bool nan = false;
             for (int py = y0; py <= y1; py++)
                 for (int px = x0; px <= x1; px++) {
-                    float bc[3];
-                    barycoord(ts, (float)px + 0.5f, (float)py + 0.5f, bc);
-                    if (!inside(bc)) continue;
-                    const float z = bc[0] * p1.z + bc[1] * p2.z + bc[2] * p3.z;  // raster.go:496
+                    float z;
+                    if (!cover_z(ts, (float)px + 0.5f, (float)py + 0.5f, p1.z, p2.z, p3.z, z)) continue;
                     if (z != z) nan = true;
                     else atomicMin(&vis[(size_t)py * vc.W + px], ((unsigned long long)depth_key_hi(z) << 32) | key);
                 }
@@ -724,11 +721,7 @@ __device__ __forceinline__ bool eval_candidate(const LPrim& p, float px, float p
     t.ax = p.ax, t.ay = p.ay, t.bx = p.bx, t.by = p.by;
     t.abx = p.abx, t.aby = p.aby, t.acx = p.acx, t.acy = p.acy;
     t.bcx = p.bcx, t.bcy = p.bcy, t.sabc = p.sabc;
-    float bc[3];
-    barycoord(t, px, py, bc);
-    if (!inside(bc)) return false;
-    z = bc[0] * p.az + bc[1] * p.bz + bc[2] * p.cz;  // raster.go:496
-    return true;
+    return cover_z(t, px, py, p.az, p.bz, p.cz, z);  // raster.go:487-496
 }
 }  // namespace
 
