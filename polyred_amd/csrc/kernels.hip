@@ -392,15 +392,13 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
 // Sutherland-Hodgman against the six screen-space planes, a fan of pieces
 // with screen-barycentric attributes that keep the parent's recipw
 // (raster.go:440-443), appended after the per-triangle primitive slots.
-__global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
-                                             const float4* __restrict__ tv_nor, const int32_t* __restrict__ clip_list,
-                                             Prim* __restrict__ prims, uint2* __restrict__ pbox,
-                                             ClipRec* __restrict__ clips,
-                                             int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
-                                             Counters* cnt, Capacities cap) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= min(cnt->n_clipped, cap.clip_tris) || cnt->overflow) return;
-    const int g = clip_list[q];
+__device__ __forceinline__ void clip_one(int qi, const DevScene& s, const ViewConsts& vc,
+                                         const float4* __restrict__ tv_scr, const float4* __restrict__ tv_nor,
+                                         const int32_t* __restrict__ clip_list, Prim* __restrict__ prims,
+                                         uint2* __restrict__ pbox, ClipRec* __restrict__ clips,
+                                         int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
+                                         Counters* cnt, const Capacities& cap) {
+    const int g = clip_list[qi];
     const int o = s.tri_obj[g];
     const int mesh = s.obj_mesh[o];
     const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
@@ -463,6 +461,21 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
         for (int ty = by0 / TILE; ty <= by1 / TILE; ty++)
             for (int tx = bx0 / TILE; tx <= bx1 / TILE; tx++) atomicAdd(&tile_count[ty * vc.tiles_x + tx], 1);
     }
+}
+
+// One queued triangle per thread, the grid striding over the queue (its
+// length is known on the device only; the grid stays small).
+constexpr int CLIP_GRID = 128;
+__global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
+                                             const float4* __restrict__ tv_nor, const int32_t* __restrict__ clip_list,
+                                             Prim* __restrict__ prims, uint2* __restrict__ pbox,
+                                             ClipRec* __restrict__ clips,
+                                             int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
+                                             Counters* cnt, Capacities cap) {
+    if (cnt->overflow) return;
+    const int n = min(cnt->n_clipped, cap.clip_tris);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        clip_one(q, s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
 }
 
 // render/shadow.go:152-176 (drawDepth up to the pixel loop) for every
@@ -1576,7 +1589,8 @@ void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, 
     if (!s.n_tris) return;
     hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, prims, pbox,
                        clip_list, tile_count, blk_prims, cnt, cap, vis, (int32_t)draw_small);
-    hipLaunchKernelGGL(k_clip, dim3(blocks_for(cap.clip_tris, 64)), dim3(64), 0, st, s, vc, tv_scr, tv_nor, clip_list,
+    hipLaunchKernelGGL(k_clip, dim3(std::min<size_t>(blocks_for(cap.clip_tris, 64), CLIP_GRID)), dim3(64), 0, st, s, vc,
+                       tv_scr, tv_nor, clip_list,
                        prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
 }
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
