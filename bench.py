@@ -174,6 +174,50 @@ def batch_bench(dev, reps: int) -> dict:
             "mpix_per_s": round(n_px / best / 1e3, 1)}
 
 
+def fit_bench(dev, steps: int = 200) -> dict:
+    """Batched inverse rendering: 32 views (256^2 to 1024^2) of one 250k-
+    triangle scene with 8 materials and 4 shadow-casting point lights
+    (1024^2 maps) at fixed visibility.  Targets from the scene's own
+    materials and lights; start 30 % off (Gaussian, per value); 200 Adam
+    steps at lr 1e-2 on Kd, Ks, shininess, light positions and intensities,
+    each step the adjoint of all 32 views in one launch plus the update (a
+    replayed HIP graph).  Run twice: per-pixel state rebuilt from the
+    visibility buffer at every step, and read from a stored G-buffer."""
+    from polyred_amd import diff, render, scenes
+
+    s, cams = scenes.c5_inverse()
+    gs, rs = render.views_of(s, cams, dev)
+    t0 = time.perf_counter()
+    for r in rs:
+        r.render_device()
+    ms_render = 1000.0 * (time.perf_counter() - t0)
+    truth = diff.scene_params(s)
+    rng = np.random.default_rng(5)
+    start = truth.copy()
+    groups = ("kd", "ks", "shininess", "light_pos", "light_int")
+    for n in groups:
+        a = getattr(start, n)
+        a *= (1 + 0.3 * rng.standard_normal(a.shape)).astype(a.dtype)
+    out = {"workload": s.name, "views": len(rs), "triangles": s.n_triangles,
+           "pixels": int(sum(w * h for _, w, h in cams)), "steps": steps, "lr": 1e-2,
+           "ms_render_views": round(ms_render, 2), "rmse_start": diff.rmse(start, truth),
+           "gauge_free_start": diff.gauge_free_error(start, truth)}
+    for mode, stored in (("rebuilt", False), ("stored", True)):
+        fit = diff.Fit(rs, stored_gbuffer=stored)
+        fit.targets(truth)
+        fit.run(start, steps=2)  # warm-up
+        p, hist = fit.run(start, steps=steps)
+        st = fit.stats()
+        out[mode] = {"ms_per_step": round(st.ms_steps / steps, 4), "ms_total": round(st.ms_steps, 2),
+                     "ms_gbuffer": round(st.ms_gbuffer, 3),
+                     "visibility_mb": round(st.visibility_bytes / 2 ** 20, 1),
+                     "loss_start": float(hist[0]), "loss_end": float(hist[-1]),
+                     "rmse_end": diff.rmse(p, truth), "gauge_free_end": diff.gauge_free_error(p, truth)}
+        fit.free()
+    gs.free()
+    return out
+
+
 def cpu_baseline(scene, frames: int) -> dict:
     import oracle  # the parity oracle, timed as the CPU baseline only
 
@@ -199,6 +243,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batched measurement")
+    ap.add_argument("--no-fit", action="store_true", help="skip the batched inverse-rendering measurement")
     ap.add_argument("--small", action="store_true", help="reduced scene for a quick check")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run 3 elementwise copies of MIB MiB (torch.neg), the known-size kernel that "
@@ -295,6 +340,8 @@ def main() -> int:
         out["adjoint"] = adjoint_bench(dev, max(5, args.steps // 2))
     if not args.no_batch:
         out["batched"] = batch_bench(dev, 3)
+    if not args.no_fit:
+        out["inverse"] = fit_bench(dev)
     dev.close()
     if not args.no_cpu_baseline:
         cb = cpu_baseline(scene, args.cpu_frames)

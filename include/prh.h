@@ -200,6 +200,9 @@ int prh_read_image(prh_renderer* r, uint8_t* rgba, size_t bytes);
 #define PRH_BUF_LINEAR 2    /* float  [H][W][3] pre-Round shading sums         */
 #define PRH_BUF_SHADOW 3    /* float  [n_lights][S][S] shadow maps             */
 #define PRH_BUF_PREGAMMA 4  /* uint8  [H][W][4] deferred colour before gamma   */
+#define PRH_BUF_SHADOW_COUNT 5 /* uint8 [H][W] lights whose shadow test reported
+                                  true at covered pixels whose material receives
+                                  shadows (the colour is scaled by 2^-count)   */
 
 /* Copy one internal buffer to host memory (parity tests, debugging). */
 int prh_read_buffer(prh_renderer* r, int32_t which, void* dst, size_t bytes);
@@ -256,6 +259,61 @@ int prh_diff_forward(prh_renderer* r, const prh_diff_params* p, float* linear, s
 /* Loss and its gradients; target is float [H][W][3]. */
 int prh_diff_backward(prh_renderer* r, const prh_diff_params* p, const float* target, size_t bytes, double* loss,
                       const prh_diff_grads* grads);
+
+/* ---- batched inverse rendering (no counterpart in polyred) -------------
+ * Gradient descent on the differentiable shading above over many views of
+ * one scene at once.  Every view's last frame fixes its visibility and its
+ * shadow visibility: the shading of a pixel is scaled by 2^-count, count the
+ * lights whose shadow test (render/shadow.go:230-283) reported true there,
+ * as the reference scales the shaded colour (render/raster.go:339-347).  One
+ * step is one launch of the adjoint over every view (gradients summed over
+ * views) and one Adam update (Kingma & Ba 2015) of the chosen parameter
+ * groups, parametrised as p = start * (1 + theta), theta starting at 0; the
+ * steps of a run are one captured HIP graph replayed.  The loss is the sum
+ * over views of prh_diff_backward's loss.  kd, ks, shininess, light
+ * intensities and albedo are kept >= 0. */
+typedef struct prh_fit prh_fit;
+#define PRH_FIT_ALBEDO 1 /* (normals are not fitted) */
+#define PRH_FIT_KD 4
+#define PRH_FIT_KS 8
+#define PRH_FIT_SHININESS 16
+#define PRH_FIT_LIGHT_POS 32
+#define PRH_FIT_LIGHT_INT 64
+/* prh_fit_new flag: read the adjoint's per-pixel state from a G-buffer
+ * stored once per run (64 bytes a pixel) instead of rebuilding it from the
+ * visibility buffer at every step; albedo gradients need the rebuild
+ * (PRH_E_UNSUPPORTED).  Start and truth parameters take no normals. */
+#define PRH_FIT_STORED_GBUFFER 256
+typedef struct prh_fit_options {
+    int32_t steps;
+    int32_t groups; /* PRH_FIT_ALBEDO .. PRH_FIT_LIGHT_INT to optimise */
+    float lr, beta1, beta2, eps;
+} prh_fit_options;
+typedef struct prh_fit_stats {
+    double visibility_bytes; /* the fixed visibility an adjoint step reads: the
+                                visibility buffer and shadow counts, or the
+                                stored G-buffer, summed over views          */
+    float ms_steps;          /* the run's steps (HIP events)                  */
+    float ms_gbuffer;        /* building the stored G-buffer                  */
+    int32_t steps;
+    int32_t n_views;
+} prh_fit_stats;
+/* Views of one scene, each with a rendered frame (prh_render or
+ * prh_render_batch), each at most once; rendering a view again invalidates
+ * the fit, and a view cannot be freed before the fits that hold it. */
+int prh_fit_new(prh_renderer* const* views, int32_t n, int32_t flags, prh_fit** out);
+int prh_fit_free(prh_fit* f);
+/* Every view's target := its forward shading (prh_diff_forward, shadow
+ * scale included) at `truth`, on the device. */
+int prh_fit_targets(prh_fit* f, const prh_diff_params* truth);
+/* View `view`'s target from the host: float [H][W][3]. */
+int prh_fit_set_target(prh_fit* f, int32_t view, const float* target, size_t bytes);
+/* o->steps steps from `start`; `result` (arrays of prh_diff_params'
+ * shapes, any may be NULL) receives the final parameters and
+ * loss_history[i] (may be NULL) the loss before step i. */
+int prh_fit_run(prh_fit* f, const prh_diff_params* start, const prh_fit_options* o, const prh_diff_grads* result,
+                double* loss_history);
+int prh_fit_last_stats(prh_fit* f, prh_fit_stats* out);
 
 /* Library build identification ("libprh gfx950 ..."). */
 const char* prh_version(void);

@@ -29,6 +29,7 @@ class oracle_out(C.Structure):
         ("shadow_maps", C.POINTER(C.c_float)),
         ("shade_mask", C.POINTER(C.c_uint8)),
         ("shade_err64", C.POINTER(C.c_double)),
+        ("shadow_count", C.POINTER(C.c_uint8)),
     ]
 
 
@@ -108,6 +109,7 @@ class Frame:
         self.shadow_maps = np.zeros(nl * (S * S if S else n), np.float32) if scene.shadow_map else None
         self.err64 = C.c_double(0.0)
         self.mask = None if mask is None else np.ascontiguousarray(mask.astype(np.uint8).reshape(-1))
+        self.shadow_count = np.zeros(n, np.uint8)
 
 
 def render(scene, mask=None) -> Frame:
@@ -125,6 +127,7 @@ def render(scene, mask=None) -> Frame:
     if fr.mask is not None:
         o.shade_mask = fr.mask.ctypes.data_as(C.POINTER(C.c_uint8))
     o.shade_err64 = C.pointer(fr.err64)
+    o.shadow_count = fr.shadow_count.ctypes.data_as(C.POINTER(C.c_uint8))
     desc = scene.desc()
     cam = scene.camera_struct()
     opts = scene.options_struct()
@@ -138,12 +141,13 @@ def render(scene, mask=None) -> Frame:
     fr.linear = fr.linear.reshape(H, W, 3)
     fr.pregamma = fr.pregamma.reshape(H, W, 4)
     fr.image = fr.image.reshape(scene.height, scene.width, 4)
+    fr.shadow_count = fr.shadow_count.reshape(H, W)
     return fr
 
 
 class oracle_diff_params(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in ("albedo", "normal", "kd", "ks", "shininess", "light_pos",
-                                                     "light_int")]
+                                                     "light_int")] + [("shadow_count", C.POINTER(C.c_uint8))]
 
 
 class oracle_diff_grads(C.Structure):
@@ -155,15 +159,19 @@ def _dptr(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def diff(scene, vis_order: np.ndarray, params, target=None, grads=True):
+def diff(scene, vis_order: np.ndarray, params, target=None, grads=True, shadow_count=None):
     """The FP64 reference of the differentiable shading at the visibility
     `vis_order` ((H*m, W*m) uint32, as Frame.vis_order): returns (loss,
     linear (H*m, W*m, 3) float64, gradients as a dict of float64 arrays or
-    None).  `params` is a polyred_amd.diff.DiffParams."""
+    None).  `params` is a polyred_amd.diff.DiffParams; `shadow_count` (as
+    Frame.shadow_count) scales each pixel's shading by 2^-count."""
     names = ("albedo", "normal", "kd", "ks", "shininess", "light_pos", "light_int")
     p64 = {n: (None if getattr(params, n) is None else np.ascontiguousarray(getattr(params, n), np.float64))
            for n in names}
     ps = oracle_diff_params(**{n: _dptr(a) for n, a in p64.items()})
+    cnt = None if shadow_count is None else np.ascontiguousarray(shadow_count, np.uint8)
+    if cnt is not None:
+        ps.shadow_count = cnt.ctypes.data_as(C.POINTER(C.c_uint8))
     vis = np.ascontiguousarray(vis_order, np.uint32)
     H, W = vis.shape
     lin = np.empty((H, W, 3), np.float64)

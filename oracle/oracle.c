@@ -620,6 +620,7 @@ typedef struct {
     m4 view, proj, vp, viewinv, projinv, vpinv, s2w;
     float ao_dir[16];
     float* linear;
+    uint8_t* shadow_count;
     double shade_err64;
     tex_t* tex; /* the scene's textures with their mip levels */
     int n_tex;
@@ -1122,6 +1123,7 @@ static void shade(ctx_t* c, int x, int y) {
                 float vis = 0.0f;
                 for (int i = 0; i < c->s->n_lights; i++)
                     if (shading_visibility(c, i, info)) vis++;
+                if (c->shadow_count && frag.ok) c->shadow_count[lid] = (uint8_t)vis;
                 float w = gpow(0.5f, vis);
                 for (int k = 0; k < 3; k++) col[k] = gf2u8((float)col[k] * w);
             }
@@ -1164,6 +1166,8 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
     c.linear = out->linear;
     if (c.linear)
         for (size_t i = 0; i < 3 * np; i++) c.linear[i] = NAN;
+    c.shadow_count = out->shadow_count;
+    if (c.shadow_count) memset(c.shadow_count, 0, np);
     ao_dirs(c.ao_dir);
     c.view = view_matrix(V3a(cam->position), V3a(cam->target), V3a(cam->up));
     c.proj = cam_proj(cam);
@@ -1394,12 +1398,15 @@ int oracle_diff(const prh_scene_desc* s, const prh_camera* cam, const prh_option
             for (int c2 = 0; c2 < 3; c2++) sLs[c2] += Ls[li] * (double)l->color[c2] * I[li];
         }
         double lin[3], gl[3];
+        /* fixed shadow visibility: the reference halves the shaded colour
+         * per reporting light (render/raster.go:339-347) */
+        const double sw = p->shadow_count ? ldexp(1.0, -(int)p->shadow_count[i]) : 1.0;
         for (int c2 = 0; c2 < 3; c2++) {
-            lin[c2] = amb * a[c2] + kd[c2] * sLd * a[c2] / 255.0 + ks[c2] * sLs[c2] / 255.0;
+            lin[c2] = sw * (amb * a[c2] + kd[c2] * sLd * a[c2] / 255.0 + ks[c2] * sLs[c2] / 255.0);
             if (linear) linear[3 * i + c2] = lin[c2];
             const double d = target ? lin[c2] - target[3 * i + c2] : 0;
             L2 += d * d;
-            gl[c2] = 2 * d;
+            gl[c2] = 2 * d * sw;
         }
         if (!g || !target) continue;
         /* backward */

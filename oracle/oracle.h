@@ -49,6 +49,9 @@ typedef struct oracle_out {
     float* shadow_maps;    /* [n_lights][S][S]                                          */
     const uint8_t* shade_mask; /* optional [H][W] raster rows: deferred-shade only these */
     double* shade_err64;   /* optional: max |f32 linear - f64 re-evaluation|            */
+    uint8_t* shadow_count; /* optional [H][W] raster rows: the lights whose shadow test
+                              reported true, at covered pixels whose material receives
+                              shadows (the colour is scaled by 2^-count); 0 elsewhere  */
 } oracle_out;
 
 /* With MSAA m > 1 every buffer is (height*m) x (width*m) except the final
@@ -67,6 +70,7 @@ typedef struct oracle_diff_params {
     const double* shininess;
     const double* light_pos;
     const double* light_int;
+    const uint8_t* shadow_count; /* optional [H][W]: the shading is scaled by 2^-count */
 } oracle_diff_params;
 typedef struct oracle_diff_grads {
     double *albedo, *normal, *kd, *ks, *shininess, *light_pos, *light_int;

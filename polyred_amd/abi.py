@@ -32,6 +32,7 @@ PRH_BUF_DEPTH = 1
 PRH_BUF_LINEAR = 2
 PRH_BUF_SHADOW = 3
 PRH_BUF_PREGAMMA = 4
+PRH_BUF_SHADOW_COUNT = 5
 
 
 class prh_texture(C.Structure):
@@ -154,6 +155,20 @@ class prh_diff_grads(C.Structure):
     _fields_ = [(n, _f32p) for n in ("albedo", "normal", "kd", "ks", "shininess", "light_pos", "light_int")]
 
 
+PRH_FIT_ALBEDO, PRH_FIT_KD, PRH_FIT_KS, PRH_FIT_SHININESS, PRH_FIT_LIGHT_POS, PRH_FIT_LIGHT_INT = 1, 4, 8, 16, 32, 64
+PRH_FIT_STORED_GBUFFER = 256
+
+
+class prh_fit_options(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("groups", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class prh_fit_stats(C.Structure):
+    _fields_ = [("visibility_bytes", C.c_double), ("ms_steps", C.c_float), ("ms_gbuffer", C.c_float),
+                ("steps", C.c_int32), ("n_views", C.c_int32)]
+
+
 # Every symbol include/prh.h declares, with its prototype.
 PROTOTYPES = {
     "prh_last_error": (C.c_char_p, []),
@@ -174,6 +189,13 @@ PROTOTYPES = {
     "prh_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t]),
     "prh_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t,
                                     C.POINTER(C.c_double), C.POINTER(prh_diff_grads)]),
+    "prh_fit_new": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "prh_fit_free": (C.c_int, [C.c_void_p]),
+    "prh_fit_targets": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params)]),
+    "prh_fit_set_target": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
+    "prh_fit_run": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.POINTER(prh_fit_options),
+                              C.POINTER(prh_diff_grads), C.c_void_p]),
+    "prh_fit_last_stats": (C.c_int, [C.c_void_p, C.POINTER(prh_fit_stats)]),
 }
 
 _lib = None

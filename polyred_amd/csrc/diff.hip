@@ -18,6 +18,12 @@
 // and light colours are fixed.  The loss is sum over shaded pixels and RGB of
 // (lin - target)^2.  Gradients are accumulated with one atomic per distinct
 // triangle / material and per light in each wave.
+//
+// Shadows enter at fixed visibility, as the reference applies them after the
+// shading (render/raster.go:339-347): the pixel's sums are scaled by
+// 2^-count, count the lights for which shadingVisibility reported true in the
+// frame that fixed the visibility.  A view's pixel state is either rebuilt
+// from the visibility buffer at every launch or read from a stored G-buffer.
 #include "kernels.hpp"
 
 using namespace gm;
@@ -56,10 +62,30 @@ struct DPix {
     float w[3], bc[3];  // perspective weights (albedo) and barycentrics (normal)
     F3 a, nraw, n, x, V;
     float nl;
+    float sw;  // shadow scale 2^-count
     bool flat;
 };
 
+// G-buffer planes: (x, material), (n, shadow scale), (V, 0), (albedo, 0).
+__device__ DPix stored_pixel(const ViewConsts& v, const DiffBuffers& b, int x, int y, bool inb) {
+    DPix p;
+    p.act = false;
+    if (!inb) return p;
+    const size_t P = (size_t)v.W * v.H, o = (size_t)y * v.W + x;
+    const float4 g0 = b.gbuf[o];
+    const int mat = __float_as_int(g0.w);
+    if (mat < 0) return p;
+    const float4 g1 = b.gbuf[P + o], g2 = b.gbuf[2 * P + o], g3 = b.gbuf[3 * P + o];
+    p.act = true;
+    p.g = -1, p.mat = mat;
+    p.x = xyz(g0), p.n = xyz(g1), p.sw = g1.w, p.V = xyz(g2), p.a = xyz(g3);
+    p.nraw = p.n, p.nl = 1.0f, p.flat = true;  // no normal gradient from a stored pixel
+    for (int k = 0; k < 3; k++) p.vid[k] = 0, p.w[k] = p.bc[k] = 0.0f;
+    return p;
+}
+
 __device__ DPix diff_pixel(const DevScene& s, const ViewConsts& v, const DiffBuffers& b, int x, int y, bool inb) {
+    if (b.gbuf) return stored_pixel(v, b, x, y, inb);
     DPix p;
     p.act = false;
     if (!inb) return p;
@@ -107,6 +133,7 @@ __device__ DPix diff_pixel(const DevScene& s, const ViewConsts& v, const DiffBuf
     }
     const F3 vv = f3(v.cam[0], v.cam[1], v.cam[2]) - p.x;
     p.V = (1.0f / len3(vv)) * vv;
+    p.sw = b.shadow_count ? __builtin_ldexpf(1.0f, -(int)b.shadow_count[(size_t)y * v.W + x]) : 1.0f;
     return p;
 }
 
@@ -141,11 +168,12 @@ __device__ __forceinline__ DLight light_terms(const DevLight& l, const DiffBuffe
 
 }  // namespace
 
+// One workgroup (`bid` of the view's grid) of the forward or the adjoint.
 template <bool BACKWARD>
-__global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuffers b) {
+__device__ __forceinline__ void diff_body(const DevScene& s, const ViewConsts& v, const DiffBuffers& b, int bid) {
     const int tiles_x = v.tiles_x, n_tiles = tiles_x * v.tiles_y;
     const int per_xcd = (n_tiles + 7) / 8;
-    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int tile = (bid % 8) * per_xcd + bid / 8;
     if (tile >= n_tiles) return;  // whole workgroup
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int x = (tile % tiles_x) * TILE + (wv & 1) * 8 + (lane & 7);
@@ -165,7 +193,8 @@ __global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuff
             for (int c = 0; c < 3; c++) sLs[c] += t.Ls * l.color[c] * t.I;
         }
         const float a[3] = {p.a.x, p.a.y, p.a.z};
-        for (int c = 0; c < 3; c++) lin[c] = b.ambient * a[c] + kd[c] * sLd * a[c] / 255.0f + ks[c] * sLs[c] / 255.0f;
+        for (int c = 0; c < 3; c++)
+            lin[c] = p.sw * (b.ambient * a[c] + kd[c] * sLd * a[c] / 255.0f + ks[c] * sLs[c] / 255.0f);
     }
     if (!BACKWARD) {
         if (inb && b.linear) {
@@ -182,7 +211,7 @@ __global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuff
         const size_t o = (size_t)y * v.W + x;
         for (int c = 0; c < 3; c++) {
             const float d = lin[c] - b.target[3 * o + c];
-            gl[c] = 2.0f * d;
+            gl[c] = 2.0f * d * p.sw;  // through the fixed shadow scale
             l2 += (double)d * (double)d;
         }
     }
@@ -228,7 +257,7 @@ __global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuff
     const F3 gnr = (p.act && !p.flat) ? unit_back(p.n, p.nl, gn) : f3(0, 0, 0);
 
     // per light: one reduction over the wave, lane j adds value j
-    {
+    if (b.want & (DIFF_LIGHT_POS | DIFF_LIGHT_INT)) {
         float mine = 0.0f;
         for (int j = 0; j < 4 * DIFF_MAX_LIGHTS; j++) {
             if (j >= 4 * NL) break;
@@ -241,7 +270,7 @@ __global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuff
         }
     }
     // per material: one reduction per distinct material of the wave
-    {
+    if (b.want & (DIFF_KD | DIFF_KS | DIFF_SHININESS)) {
         uint64_t pending = __ballot(p.act);
         while (pending) {
             const int leader = __ffsll((unsigned long long)pending) - 1;
@@ -258,8 +287,9 @@ __global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuff
             pending &= ~__ballot(same);
         }
     }
-    // per triangle: its three vertices' albedo and normal gradients
-    {
+    // per triangle: its three vertices' albedo and normal gradients (a
+    // stored G-buffer keeps no vertices: the host refuses that combination)
+    if (!b.gbuf && (b.want & (DIFF_ALBEDO | DIFF_NORMAL))) {
         const int tri = p.act ? p.g : -1;
         uint64_t pending = __ballot(p.act);
         while (pending) {
@@ -289,11 +319,106 @@ __global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuff
     if (lane == 0) atomicAdd(b.loss, tot);
 }
 
+template <bool BACKWARD>
+__global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuffers b) {
+    diff_body<BACKWARD>(s, v, b, blockIdx.x);
+}
+
+// Many views of one scene in one launch: view i owns the workgroups
+// [block0, block0 + nblocks), block0 ascending and a multiple of 8 (so a
+// view's workgroups keep the XCD spread of its own launch).
+template <bool BACKWARD>
+__global__ __launch_bounds__(256) void k_diff_views(DevScene s, const DiffView* __restrict__ views,
+                                                    int32_t n_views) {
+    const int bid = blockIdx.x;
+    int lo = 0, hi = n_views - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].block0 <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    const DiffView& w = views[lo];
+    if (bid - w.block0 >= w.nblocks) return;
+    diff_body<BACKWARD>(s, w.v, w.b, bid - w.block0);
+}
+
+__global__ __launch_bounds__(256) void k_diff_gbuffer(DevScene s, ViewConsts v, DiffBuffers b, float4* __restrict__ g) {
+    const size_t P = (size_t)v.W * v.H;
+    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (o >= P) return;
+    const int x = (int)(o % (size_t)v.W), y = (int)(o / (size_t)v.W);
+    const DPix p = diff_pixel(s, v, b, x, y, true);
+    if (!p.act) {
+        g[o] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+        return;
+    }
+    g[o] = make_float4(p.x.x, p.x.y, p.x.z, __int_as_float(p.mat));
+    g[P + o] = make_float4(p.n.x, p.n.y, p.n.z, p.sw);
+    g[2 * P + o] = make_float4(p.V.x, p.V.y, p.V.z, 0.0f);
+    g[3 * P + o] = make_float4(p.a.x, p.a.y, p.a.z, 0.0f);
+}
+
+// One workgroup: every listed parameter, then the step counter and the loss
+// history (the loss is that of the parameters before this step).
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ par, const float* __restrict__ grad,
+                                              const float* __restrict__ base, float* __restrict__ theta,
+                                              float* __restrict__ m, float* __restrict__ v,
+                                              const int32_t* __restrict__ idx, const float* __restrict__ lower,
+                                              AdamConsts a, int32_t* step, const double* loss, double* hist) {
+    const int t = *step + 1;
+    const float c1 = (float)(1.0 - pow64((double)a.beta1, (double)t));
+    const float c2 = (float)(1.0 - pow64((double)a.beta2, (double)t));
+    for (int k = threadIdx.x; k < a.n; k += blockDim.x) {
+        const int i = idx[k];
+        const float b0 = base[i];
+        const float g = grad[i] * b0;  // d loss / d theta for p = base * (1 + theta)
+        const float mk = a.beta1 * m[k] + (1.0f - a.beta1) * g;
+        const float vk = a.beta2 * v[k] + (1.0f - a.beta2) * (g * g);
+        m[k] = mk, v[k] = vk;
+        float th = theta[k] - a.lr * (mk / c1) / (__builtin_sqrtf(vk / c2) + a.eps);
+        float pv = b0 * (1.0f + th);
+        if (pv < lower[k]) {
+            pv = lower[k];
+            if (b0 != 0.0f) th = pv / b0 - 1.0f;
+        }
+        theta[k] = th, par[i] = pv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *step = t;
+        if (t <= a.hist_cap) hist[t - 1] = *loss;
+    }
+}
+
 void launch_diff(hipStream_t st, bool backward, const DevScene& s, const ViewConsts& v, const DiffBuffers& b) {
-    const int n_tiles = v.tiles_x * v.tiles_y;
-    const int grid = ((n_tiles + 7) / 8) * 8;
+    const int grid = diff_blocks(v);
     if (backward) hipLaunchKernelGGL(k_diff<true>, dim3(grid), dim3(256), 0, st, s, v, b);
     else hipLaunchKernelGGL(k_diff<false>, dim3(grid), dim3(256), 0, st, s, v, b);
+}
+
+int32_t diff_blocks(const ViewConsts& v) {
+    const int n_tiles = v.tiles_x * v.tiles_y;
+    return ((n_tiles + 7) / 8) * 8;
+}
+
+void launch_diff_views(hipStream_t st, bool backward, const DevScene& s, const DiffView* views, int32_t n_views,
+                       int32_t total_blocks) {
+    if (n_views <= 0 || total_blocks <= 0) return;
+    if (backward) hipLaunchKernelGGL(k_diff_views<true>, dim3(total_blocks), dim3(256), 0, st, s, views, n_views);
+    else hipLaunchKernelGGL(k_diff_views<false>, dim3(total_blocks), dim3(256), 0, st, s, views, n_views);
+}
+
+void launch_diff_store_gbuffer(hipStream_t st, const DevScene& s, const ViewConsts& v, const DiffBuffers& b,
+                               float4* gbuf) {
+    const size_t P = (size_t)v.W * v.H;
+    if (P) hipLaunchKernelGGL(k_diff_gbuffer, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, s, v, b, gbuf);
+}
+
+void launch_adam(hipStream_t st, float* par, const float* grad, const float* base, float* theta, float* m, float* v,
+                 const int32_t* idx, const float* lower, AdamConsts a, int32_t* step, const double* loss,
+                 double* hist) {
+    hipLaunchKernelGGL(k_adam, dim3(1), dim3(256), 0, st, par, grad, base, theta, m, v, idx, lower, a, step, loss,
+                       hist);
 }
 
 }  // namespace prh

@@ -211,6 +211,12 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 __global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
+// The frame's start: counters to zero and the visibility buffer empty.
+__global__ void k_frame_clear(uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_t n_vis) {
+    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i0 < (size_t)n_cnt) cnt[i0] = 0u;
+    for (size_t i = i0; i < n_vis; i += (size_t)gridDim.x * blockDim.x) vis[i] = VIS_EMPTY;
+}
 
 // ------------------------------------------------------- vertex transform
 // render/raster.go:381-410 per vertex: Pos = trans.MulV(p) with
@@ -1472,7 +1478,13 @@ __global__ __launch_bounds__(256) void k_ao(ShadeConsts c, const uint8_t* __rest
 // fragment with material table[0]; if pixel (0,0) is uncovered it is the
 // Background.  With no material 0 the colour is pixel (0,0)'s own final one.
 __global__ void k_shade_bg(DevScene s, ShadeConsts c, ShadeBuffers b) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || b.cnt->overflow) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // the frame's counters are final here: hand them to the host's pinned
+    // copy (main then shadow), in place of a copy after the frame
+    if (b.cnt_host)
+        for (int i = 0; i < (int)(2 * sizeof(Counters) / 4); i++)
+            reinterpret_cast<int32_t*>(b.cnt_host)[i] = reinterpret_cast<const int32_t*>(b.cnt)[i];
+    if (b.cnt->overflow) return;
     const uint64_t k00 = b.vis[0];
     float lin[3] = {__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf("")};
     uint32_t col = c.background;
@@ -1529,6 +1541,25 @@ __global__ __launch_bounds__(256) void k_shade(DevScene s, ShadeConsts c, ShadeB
     b.image[(size_t)(c.v.H - 1 - y) * c.v.W + x] = gamma(c, col);
 }
 
+// The fixed shadow visibility of the differentiable shading: per pixel of the
+// last frame, the number k_shade halves the colour by -- the lights for which
+// shadingVisibility reports true (render/shadow.go:230-283, summed as
+// render/raster.go:339-347 sums it) at a covered pixel whose material
+// receives shadows; 0 elsewhere.
+__global__ __launch_bounds__(256) void k_shadow_count(DevScene s, ShadeConsts c, ShadeBuffers b,
+                                                      uint8_t* __restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)c.v.W * c.v.H) return;
+    const int x = (int)(i % (size_t)c.v.W), y = (int)(i / (size_t)c.v.W);
+    const uint64_t key = b.vis[i];
+    int occ = 0;
+    if (key != VIS_EMPTY && c.shadow_map) {
+        const GFrag f = fetch_fragment(s, c, b, x, y, key);
+        if (f.mat >= 0 && s.mats[f.mat].receive_shadow) occ = occluded(s, c, b, x, y, f.depth);
+    }
+    out[i] = (uint8_t)occ;
+}
+
 // ------------------------------------------------------------- launchers
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -1566,6 +1597,10 @@ __global__ __launch_bounds__(256) void k_resize(const uint32_t* __restrict__ in,
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min<size_t>(blocks_for(n, 256), 4096)), dim3(256), 0, st, p, n, v);
+}
+void launch_frame_clear(hipStream_t st, uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_t n_vis) {
+    hipLaunchKernelGGL(k_frame_clear, dim3(std::min<size_t>(blocks_for(std::max<size_t>(n_vis, n_cnt), 256), 4096)),
+                       dim3(256), 0, st, cnt, n_cnt, vis, n_vis);
 }
 void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
     if (!n) return;
@@ -1648,6 +1683,11 @@ void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const
     hipLaunchKernelGGL(k_shade, dim3(per_xcd * 8), dim3(256), 0, st, s, c, b);
 }
 
+void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
+                         uint8_t* out) {
+    const size_t P = (size_t)c.v.W * c.v.H;
+    if (P) hipLaunchKernelGGL(k_shadow_count, dim3(blocks_for(P, 256)), dim3(256), 0, st, s, c, b, out);
+}
 void launch_resize(hipStream_t st, const uint32_t* in, int32_t in_w, uint32_t* out, int32_t out_w, int32_t out_rows,
                    const int16_t* coeffs, const int32_t* start, int32_t fl) {
     if (out_w <= 0 || out_rows <= 0) return;

@@ -33,7 +33,8 @@ struct ShadeBuffers {
     float* linear;             // optional [H][W][3]
     uint32_t* bg;              // [2] uncovered colour, pixel (0,0)'s colour
     float* bg_lin;             // [3]
-    const Counters* cnt;       // main-pass counters: nothing is shaded after an overflow
+    const Counters* cnt;       // main-pass counters, then the shadow ones: nothing is shaded after an overflow
+    Counters* cnt_host;        // pinned host copy of both, written once they are final
     float* ao;                 // [H][W] AO factor of the pixels that need it
     const uint8_t* ao_need;    // [H][W] 1 where k_ao computes the factor
     const float* ao_tab;       // AO march tables (tap offsets, diagonal distances)
@@ -56,12 +57,41 @@ struct DiffBuffers {
     const float* light_pos;    // [L][3]
     const float* light_int;    // [L]
     float ambient;             // sum of the ambient intensities
+    const uint8_t* shadow_count;  // [H][W] fixed shadow visibility: the shading is scaled by 2^-count; or null
+    const float4* gbuf;        // [4][H][W] stored G-buffer (diff_store_gbuffer), or null: rebuilt from vis
     const float* target;       // [H][W][3] raster rows (adjoint)
     float* linear;             // [H][W][3] raster rows (forward), NaN where not shaded
     float *g_albedo, *g_normal, *g_kd, *g_ks, *g_shin, *g_lpos, *g_lint;  // zeroed before the adjoint
     double* loss;
+    uint32_t want;             // DIFF_* gradient groups to accumulate
 };
+// Gradient groups (the bits of prh.h's PRH_FIT_* flags).
+constexpr uint32_t DIFF_ALBEDO = 1, DIFF_NORMAL = 2, DIFF_KD = 4, DIFF_KS = 8, DIFF_SHININESS = 16,
+                   DIFF_LIGHT_POS = 32, DIFF_LIGHT_INT = 64, DIFF_ALL = 127;
 void launch_diff(hipStream_t st, bool backward, const DevScene& s, const ViewConsts& v, const DiffBuffers& b);
+// One view of a multi-view launch: its blocks are [block0, block0 + nblocks).
+struct DiffView {
+    ViewConsts v;
+    DiffBuffers b;
+    int32_t block0, nblocks;
+};
+int32_t diff_blocks(const ViewConsts& v);
+void launch_diff_views(hipStream_t st, bool backward, const DevScene& s, const DiffView* views, int32_t n_views,
+                       int32_t total_blocks);
+// The stored G-buffer of one view (world position + material, normal +
+// shadow scale, view vector, albedo: 4 float4 planes of H*W).
+void launch_diff_store_gbuffer(hipStream_t st, const DevScene& s, const ViewConsts& v, const DiffBuffers& b,
+                               float4* gbuf);
+// Adam (Kingma & Ba 2015, Algorithm 1) on the parameters p = base * (1 + theta)
+// listed in idx; t is the device step counter, loss the step's loss, kept
+// in hist[t - 1].
+struct AdamConsts {
+    float lr, beta1, beta2, eps;
+    int32_t n, hist_cap;
+};
+void launch_adam(hipStream_t st, float* par, const float* grad, const float* base, float* theta, float* m, float* v,
+                 const int32_t* idx, const float* lower, AdamConsts a, int32_t* step, const double* loss,
+                 double* hist);
 
 // Layout of the AO march tables, in float (4-byte) units.  Binade buckets are
 // frexp exponents AO_EXP_MIN .. AO_EXP_MIN + AO_EXP_N - 1.
@@ -81,6 +111,7 @@ constexpr uint32_t BUF_DWORD3 = 0x00020000u;
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
+void launch_frame_clear(hipStream_t st, uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_t n_vis);
 void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, const gm::M4* normal,
                         const ViewConsts& vc, float4* tv_scr, float4* tv_nor, float4* tv_wp);
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
@@ -112,6 +143,8 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
+void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
+                         uint8_t* out);
 // out[y][x] (out_rows x out_w) = 8-bit bilinear filter of row x of `in` at y
 void launch_resize(hipStream_t st, const uint32_t* in, int32_t in_w, uint32_t* out, int32_t out_w, int32_t out_rows,
                    const int16_t* coeffs, const int32_t* start, int32_t fl);

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -322,7 +323,7 @@ int upload(T** p, const T* src, size_t n) {
 struct prh_renderer;
 // prh_render_batch spreads a batch over one stream per hardware queue the
 // process has (GPU_MAX_HW_QUEUES, 4 unless set), at most PRH_BATCH_MAX.
-constexpr int PRH_BATCH_MAX = 8;
+constexpr int PRH_BATCH_MAX = 16;
 
 struct prh_ctx {
     int device;
@@ -393,6 +394,7 @@ struct prh_renderer {
     int32_t ao_fast; // the tables hold for this frame size
     Counters* cnt;      // [2]: main, shadow
     Counters* h_cnt;    // pinned
+    Counters* d_h_cnt;  // its device address
     Capacities cap_main, cap_shadow;
     hipEvent_t ev[8];
     prh_frame_stats stats;
@@ -404,6 +406,10 @@ struct prh_renderer {
     float *diff_par, *diff_grad, *diff_target, *diff_lin;
     double* diff_loss;
     hipEvent_t dev[2];
+    uint8_t* diff_count;    // the fixed shadow visibility of frame count_serial
+    uint32_t frame_serial;  // bumped by every completed frame
+    uint32_t count_serial;
+    int live_fits;  // a renderer outlives every prh_fit holding it as a view
 };
 
 namespace {
@@ -937,6 +943,8 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) {
         hipError_t e = hipHostMalloc((void**)&r->h_cnt, 2 * sizeof(Counters), hipHostMallocDefault);
         if (e != hipSuccess) rc = fail(PRH_E_NOMEM, "hipHostMalloc failed");
+        else if (hipHostGetDevicePointer((void**)&r->d_h_cnt, r->h_cnt, 0) != hipSuccess)
+            rc = fail(PRH_E_HIP, "hipHostGetDevicePointer failed");
     }
     for (int i = 0; i < 8 && !rc; i++)
         if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
@@ -965,6 +973,8 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
 
 int prh_renderer_free(prh_renderer* r) {
     if (!r) return fail(PRH_E_INVALID, "prh_renderer_free: null renderer");
+    if (r->live_fits > 0)
+        return fail(PRH_E_INVALID, "prh_renderer_free: " + std::to_string(r->live_fits) + " fit(s) still use it");
     (void)hipSetDevice(r->ctx->device);
     (void)hipStreamSynchronize(r->ctx->stream);
     for (const auto& kv : r->ctx->graph_key)
@@ -989,6 +999,21 @@ namespace {
 
 int d_n_tris(const prh_renderer* r) { return r->sc->d.n_tris; }
 
+ShadeBuffers shade_buffers(const prh_renderer* r) {
+    ShadeBuffers b;
+    b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
+    b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
+    b.light_view = r->d_lview, b.light_proj = r->d_lproj;
+    b.image = r->msaa > 1 ? r->image_big : r->image, b.pregamma = r->pregamma, b.linear = r->linear;
+    b.bg = r->bg, b.bg_lin = r->bg_lin;
+    b.cnt = r->cnt;
+    b.cnt_host = r->d_h_cnt;
+    b.ao = r->ao;
+    b.ao_need = r->ao_need;
+    b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
+    return b;
+}
+
 // Enqueues one frame; nan_mode re-runs the visibility raster in exact order.
 // `timed` records the per-pass events (not while a batch is captured into a
 // graph).
@@ -998,7 +1023,11 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     Counters* cs = r->cnt + 1;
     const int persistent = r->n_cu * 4;
     if (timed) (void)hipEventRecord(r->ev[0], st);
-    launch_fill_u32(st, (uint32_t*)r->cnt, 2 * sizeof(Counters) / 4, 0u);
+    const size_t P = (size_t)r->W * r->H;
+    // counters zeroed, visibility emptied (before the setup, which draws
+    // small triangles into it, outside the exact-order pass, whose NaN rule
+    // needs every fragment through the ordered raster)
+    launch_frame_clear(st, (uint32_t*)r->cnt, (int32_t)(2 * sizeof(Counters) / 4), (uint64_t*)r->vis, P);
     if (r->n_slots > 0 && !nan_mode) {
         const int nt = r->n_slots * r->stiles_per_map;
         launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
@@ -1014,9 +1043,6 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     if (timed) (void)hipEventRecord(r->ev[1], st);
     launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
     if (timed) (void)hipEventRecord(r->ev[2], st);
-    const size_t P = (size_t)r->W * r->H;
-    launch_fill_u64(st, r->vis, P, VIS_EMPTY);  // before the setup, which draws small triangles into it
-    // (outside the exact-order pass, whose NaN rule needs every fragment in order)
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
                       r->tile_count, r->blk_prims, cm, r->cap_main, (unsigned long long*)r->vis, !nan_mode);
     launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
@@ -1034,15 +1060,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
                   r->tile_list, cm, rt);
     launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr);
     if (timed) (void)hipEventRecord(r->ev[4], st);
-    ShadeBuffers b;
-    b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
-    b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
-    b.light_view = r->d_lview, b.light_proj = r->d_lproj;
-    b.image = r->msaa > 1 ? r->image_big : r->image, b.pregamma = r->pregamma, b.linear = r->linear, b.bg = r->bg, b.bg_lin = r->bg_lin;
-    b.cnt = cm;
-    b.ao = r->ao;
-    b.ao_need = r->ao_need;
-    b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
+    const ShadeBuffers b = shade_buffers(r);
     if (r->sc->any_ao) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
     if (timed) (void)hipEventRecord(r->ev[5], st);
     launch_shade_bg(st, d, r->sh, b);
@@ -1054,7 +1072,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
         launch_resize(st, r->resize_tmp, r->H, r->image, r->OW, r->OH, r->rz_cy, r->rz_oy, r->rz_fly);
     }
     if (timed) (void)hipEventRecord(r->ev[6], st);
-    (void)hipMemcpyAsync(r->h_cnt, r->cnt, 2 * sizeof(Counters), hipMemcpyDeviceToHost, st);
+    // (k_shade_bg writes the counters into r->h_cnt)
 }
 
 // After a frame's work has completed: after an overflow, grows the work
@@ -1109,6 +1127,7 @@ int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
     S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
     S.nan_rerun = nan_rerun ? 1 : 0;
     r->has_frame = true;
+    r->frame_serial++;
     return PRH_OK;
 }
 
@@ -1201,16 +1220,78 @@ DiffLayout diff_layout(const prh_renderer* r) {
     return l;
 }
 
+// The fixed shadow visibility of the renderer's last frame (k_shadow_count),
+// computed once per frame; null when the frame draws no shadow maps.
+int shadow_counts(prh_renderer* r, hipStream_t st, const uint8_t** out) {
+    *out = nullptr;
+    if (!r->sh.shadow_map) return PRH_OK;
+    if (!r->diff_count) {
+        int rc = ralloc(r, &r->diff_count, (size_t)r->W * r->H);
+        if (rc) return rc;
+        r->count_serial = r->frame_serial - 1;
+    }
+    if (r->count_serial != r->frame_serial) {
+        launch_shadow_count(st, r->sc->d, r->sh, shade_buffers(r), r->diff_count);
+        HIP_TRY(hipGetLastError());
+        r->count_serial = r->frame_serial;
+    }
+    *out = r->diff_count;
+    return PRH_OK;
+}
+
+// A view's differentiable-shading buffers over a parameter block `par` and a
+// gradient block `grad` (both laid out by diff_layout).
+DiffBuffers diff_buffers(const prh_renderer* r, const DiffLayout& l, const float* par, float* grad, double* loss,
+                         bool normal, const uint8_t* count) {
+    DiffBuffers b;
+    b.vis = r->vis, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
+    b.albedo = par + l.albedo, b.normal = normal ? par + l.normal : nullptr;
+    b.kd = par + l.kd, b.ks = par + l.ks, b.shininess = par + l.shin;
+    b.light_pos = par + l.lpos, b.light_int = par + l.lint;
+    b.ambient = r->sc->ambient_sum;
+    b.shadow_count = count, b.gbuf = nullptr;
+    b.target = nullptr, b.linear = nullptr;
+    b.g_albedo = grad + l.albedo, b.g_normal = grad + l.normal, b.g_kd = grad + l.kd;
+    b.g_ks = grad + l.ks, b.g_shin = grad + l.shin, b.g_lpos = grad + l.lpos, b.g_lint = grad + l.lint;
+    b.loss = loss;
+    b.want = DIFF_ALL;
+    return b;
+}
+
+// Uploads the parameter arrays of `p` into a block laid out by diff_layout.
+int upload_params(const prh_renderer* r, const DiffLayout& l, const prh_diff_params* p, float* par, hipStream_t st) {
+    const DevScene& d = r->sc->d;
+    const size_t V = d.n_tverts, M = d.n_materials, L = d.n_lights;
+    auto up = [&](size_t off, const float* src, size_t n) -> hipError_t {
+        return n ? hipMemcpyAsync(par + off, src, n * 4, hipMemcpyHostToDevice, st) : hipSuccess;
+    };
+    HIP_TRY(up(l.albedo, p->albedo, 3 * V));
+    if (p->normal) HIP_TRY(up(l.normal, p->normal, 3 * V));
+    HIP_TRY(up(l.kd, p->kd, 3 * M));
+    HIP_TRY(up(l.ks, p->ks, 3 * M));
+    HIP_TRY(up(l.shin, p->shininess, M));
+    HIP_TRY(up(l.lpos, p->light_pos, 3 * L));
+    HIP_TRY(up(l.lint, p->light_int, L));
+    return PRH_OK;
+}
+
+int check_params(const prh_renderer* r, const prh_diff_params* p) {
+    const DevScene& d = r->sc->d;
+    if (d.n_lights > DIFF_MAX_LIGHTS) return fail(PRH_E_UNSUPPORTED, "differentiable shading takes at most 8 lights");
+    if ((d.n_tverts && !p->albedo) || (d.n_materials && (!p->kd || !p->ks || !p->shininess)) ||
+        (d.n_lights && (!p->light_pos || !p->light_int)))
+        return fail(PRH_E_INVALID, "prh_diff_params: missing parameter array");
+    return PRH_OK;
+}
+
 // One launch of the differentiable shading: forward (target == nullptr,
 // result in diff_lin) or adjoint (loss and gradients copied to the host).
 int diff_run(prh_renderer* r, const prh_diff_params* p, const float* target, double* loss,
              const prh_diff_grads* grads) {
     const DevScene& d = r->sc->d;
     if (!r->has_frame) return fail(PRH_E_INVALID, "differentiable shading needs a rendered frame (prh_render)");
-    if (d.n_lights > DIFF_MAX_LIGHTS) return fail(PRH_E_UNSUPPORTED, "differentiable shading takes at most 8 lights");
-    if ((d.n_tverts && !p->albedo) || (d.n_materials && (!p->kd || !p->ks || !p->shininess)) ||
-        (d.n_lights && (!p->light_pos || !p->light_int)))
-        return fail(PRH_E_INVALID, "prh_diff_params: missing parameter array");
+    int rc0 = check_params(r, p);
+    if (rc0) return rc0;
     hipStream_t st = r->ctx->stream;
     const DiffLayout l = diff_layout(r);
     const size_t P = (size_t)r->W * r->H;
@@ -1232,27 +1313,12 @@ int diff_run(prh_renderer* r, const prh_diff_params* p, const float* target, dou
         }
     }
     const size_t V = d.n_tverts, M = d.n_materials, L = d.n_lights;
-    auto up = [&](size_t off, const float* src, size_t n) -> hipError_t {
-        return n ? hipMemcpyAsync(r->diff_par + off, src, n * 4, hipMemcpyHostToDevice, st) : hipSuccess;
-    };
-    HIP_TRY(up(l.albedo, p->albedo, 3 * V));
-    if (p->normal) HIP_TRY(up(l.normal, p->normal, 3 * V));
-    HIP_TRY(up(l.kd, p->kd, 3 * M));
-    HIP_TRY(up(l.ks, p->ks, 3 * M));
-    HIP_TRY(up(l.shin, p->shininess, M));
-    HIP_TRY(up(l.lpos, p->light_pos, 3 * L));
-    HIP_TRY(up(l.lint, p->light_int, L));
-    DiffBuffers b;
-    b.vis = r->vis, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
-    b.albedo = r->diff_par + l.albedo, b.normal = p->normal ? r->diff_par + l.normal : nullptr;
-    b.kd = r->diff_par + l.kd, b.ks = r->diff_par + l.ks, b.shininess = r->diff_par + l.shin;
-    b.light_pos = r->diff_par + l.lpos, b.light_int = r->diff_par + l.lint;
-    b.ambient = r->sc->ambient_sum;
+    int rc1 = upload_params(r, l, p, r->diff_par, st);
+    const uint8_t* count = nullptr;
+    if (!rc1) rc1 = shadow_counts(r, st, &count);
+    if (rc1) return rc1;
+    DiffBuffers b = diff_buffers(r, l, r->diff_par, r->diff_grad, r->diff_loss, p->normal != nullptr, count);
     b.target = r->diff_target, b.linear = r->diff_lin;
-    b.g_albedo = r->diff_grad + l.albedo, b.g_normal = r->diff_grad + l.normal, b.g_kd = r->diff_grad + l.kd;
-    b.g_ks = r->diff_grad + l.ks, b.g_shin = r->diff_grad + l.shin, b.g_lpos = r->diff_grad + l.lpos;
-    b.g_lint = r->diff_grad + l.lint;
-    b.loss = r->diff_loss;
     const bool backward = target != nullptr;
     if (backward) {
         HIP_TRY(hipMemcpyAsync(r->diff_target, target, 12 * P, hipMemcpyHostToDevice, st));
@@ -1278,6 +1344,88 @@ int diff_run(prh_renderer* r, const prh_diff_params* p, const float* target, dou
     HIP_TRY(hipStreamSynchronize(st));
     (void)hipEventElapsedTime(&r->stats.ms_diff, r->dev[0], r->dev[1]);
     return PRH_OK;
+}
+
+}  // namespace
+
+// ---- batched inverse rendering (prh_fit_*)
+struct prh_fit {
+    prh_ctx* ctx;
+    prh_scene* sc;
+    std::vector<prh_renderer*> views;
+    std::vector<uint32_t> serials;  // each view's frame at creation
+    bool stored;
+    DiffLayout l;
+    std::vector<void*> allocs;
+    float *par, *grad, *base, *theta, *m, *v, *lower;
+    int32_t* idx;
+    double *loss, *hist;
+    int32_t hist_cap;
+    int32_t* step;
+    std::vector<float*> target;
+    std::vector<const uint8_t*> count;
+    std::vector<float4*> gbuf;
+    DiffView* d_views;
+    int32_t total_blocks;
+    hipEvent_t ev[4];
+    prh_fit_stats stats;
+};
+
+namespace {
+
+template <class T>
+int falloc(prh_fit* f, T** p, size_t n) {
+    int rc = dalloc(p, n);
+    if (!rc) f->allocs.push_back((void*)*p);
+    return rc;
+}
+
+int fit_check(prh_fit* f) {
+    for (size_t i = 0; i < f->views.size(); i++)
+        if (!f->views[i]->has_frame || f->views[i]->frame_serial != f->serials[i])
+            return fail(PRH_E_INVALID, "prh_fit: a view was rendered again after prh_fit_new");
+    return PRH_OK;
+}
+
+// Every view's buffers, launch blocks in view order, uploaded to d_views.
+int fit_views(prh_fit* f, bool backward, uint32_t want, hipStream_t st) {
+    std::vector<DiffView> h(f->views.size());
+    int32_t blk = 0;
+    for (size_t i = 0; i < f->views.size(); i++) {
+        prh_renderer* r = f->views[i];
+        DiffBuffers b = diff_buffers(r, f->l, f->par, f->grad, f->loss, false, f->count[i]);
+        if (backward) {
+            b.target = f->target[i];
+            b.gbuf = f->stored ? f->gbuf[i] : nullptr;
+        } else {
+            b.linear = f->target[i];
+        }
+        b.want = want;
+        h[i].v = r->vc, h[i].b = b;
+        h[i].block0 = blk, h[i].nblocks = diff_blocks(r->vc);
+        blk += h[i].nblocks;
+    }
+    f->total_blocks = blk;
+    HIP_TRY(hipMemcpyAsync(f->d_views, h.data(), h.size() * sizeof(DiffView), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));  // h is a local
+    return PRH_OK;
+}
+
+// The optimised indices of the parameter block and their lower bounds.
+void fit_groups(const prh_fit* f, int32_t groups, std::vector<int32_t>& idx, std::vector<float>& lower) {
+    const DevScene& d = f->sc->d;
+    const size_t V = d.n_tverts, M = d.n_materials, L = d.n_lights;
+    const float inf = std::numeric_limits<float>::infinity();
+    auto add = [&](int32_t bit, size_t off, size_t n, float lo) {
+        if (!(groups & bit)) return;
+        for (size_t k = 0; k < n; k++) idx.push_back((int32_t)(off + k)), lower.push_back(lo);
+    };
+    add(PRH_FIT_ALBEDO, f->l.albedo, 3 * V, 0.0f);
+    add(PRH_FIT_KD, f->l.kd, 3 * M, 0.0f);
+    add(PRH_FIT_KS, f->l.ks, 3 * M, 0.0f);
+    add(PRH_FIT_SHININESS, f->l.shin, M, 0.0f);
+    add(PRH_FIT_LIGHT_POS, f->l.lpos, 3 * L, -inf);
+    add(PRH_FIT_LIGHT_INT, f->l.lint, L, 0.0f);
 }
 
 }  // namespace
@@ -1386,10 +1534,22 @@ int prh_read_buffer(prh_renderer* r, int32_t which, void* dst, size_t bytes) {
             if (!r->pregamma) return fail(PRH_E_INVALID, "renderer was created without keep_linear");
             src = r->pregamma, need = 4 * P;
             break;
+        case PRH_BUF_SHADOW_COUNT: need = P; break;
         default: return fail(PRH_E_INVALID, "unknown buffer id");
     }
     if (bytes < need) return fail(PRH_E_INVALID, "prh_read_buffer: buffer too small");
     HIP_TRY(hipSetDevice(r->ctx->device));
+    if (which == PRH_BUF_SHADOW_COUNT) {
+        if (!r->has_frame) return fail(PRH_E_INVALID, "prh_read_buffer: no rendered frame");
+        const uint8_t* c = nullptr;
+        int rc = shadow_counts(r, r->ctx->stream, &c);
+        if (rc) return rc;
+        if (!c) {
+            std::memset(dst, 0, need);
+            return PRH_OK;
+        }
+        src = c;
+    }
     if (need) HIP_TRY(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, r->ctx->stream));
     HIP_TRY(hipStreamSynchronize(r->ctx->stream));
     return PRH_OK;
@@ -1424,6 +1584,226 @@ int prh_diff_backward(prh_renderer* r, const prh_diff_params* p, const float* ta
     if (bytes < (size_t)12 * r->W * r->H) return fail(PRH_E_INVALID, "prh_diff_backward: target too small");
     HIP_TRY(hipSetDevice(r->ctx->device));
     return diff_run(r, p, target, loss, grads);
+}
+
+int prh_fit_new(prh_renderer* const* views, int32_t n, int32_t flags, prh_fit** out) {
+    if (!views || n <= 0 || !out) return fail(PRH_E_INVALID, "prh_fit_new: no views");
+    *out = nullptr;
+    if (flags & ~PRH_FIT_STORED_GBUFFER) return fail(PRH_E_INVALID, "prh_fit_new: unknown flags");
+    prh_scene* sc = views[0] ? views[0]->sc : nullptr;
+    for (int32_t i = 0; i < n; i++) {
+        if (!views[i] || views[i]->sc != sc) return fail(PRH_E_INVALID, "prh_fit_new: views of one scene");
+        if (!views[i]->has_frame) return fail(PRH_E_INVALID, "prh_fit_new: every view needs a rendered frame");
+    }
+    if (sc->d.n_lights > DIFF_MAX_LIGHTS)
+        return fail(PRH_E_UNSUPPORTED, "differentiable shading takes at most 8 lights");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    hipStream_t st = sc->ctx->stream;
+    prh_fit* f = new prh_fit();
+    f->ctx = sc->ctx, f->sc = sc, f->stored = (flags & PRH_FIT_STORED_GBUFFER) != 0;
+    f->views.assign(views, views + n);
+    std::sort(f->views.begin(), f->views.end());
+    if (std::adjacent_find(f->views.begin(), f->views.end()) != f->views.end()) {
+        delete f;
+        return fail(PRH_E_INVALID, "prh_fit_new: a view appears twice");
+    }
+    f->views.assign(views, views + n);
+    for (prh_renderer* r : f->views) r->live_fits++;
+    f->l = diff_layout(views[0]);
+    const size_t T = f->l.total;
+    int rc = 0;
+    auto chk = [&](int v) {
+        if (!rc) rc = v;
+    };
+    chk(falloc(f, &f->par, T));
+    chk(falloc(f, &f->grad, T));
+    chk(falloc(f, &f->base, T));
+    chk(falloc(f, &f->theta, T));
+    chk(falloc(f, &f->m, T));
+    chk(falloc(f, &f->v, T));
+    chk(falloc(f, &f->lower, T));
+    chk(falloc(f, &f->idx, T));
+    chk(falloc(f, &f->loss, 1));
+    chk(falloc(f, &f->step, 1));
+    chk(falloc(f, &f->d_views, (size_t)n));
+    for (int k = 0; k < 4 && !rc; k++)
+        if (hipEventCreate(&f->ev[k]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
+    double vis_bytes = 0;
+    for (int32_t i = 0; i < n && !rc; i++) {
+        prh_renderer* r = views[i];
+        const size_t P = (size_t)r->W * r->H;
+        float* t = nullptr;
+        chk(falloc(f, &t, 3 * P));
+        f->target.push_back(t);
+        const uint8_t* c = nullptr;
+        if (!rc) chk(shadow_counts(r, st, &c));
+        f->count.push_back(c);
+        float4* g = nullptr;
+        if (f->stored && !rc) chk(falloc(f, &g, 4 * P));
+        f->gbuf.push_back(g);
+        f->serials.push_back(r->frame_serial);
+        vis_bytes += f->stored ? 64.0 * (double)P : (8.0 + (c ? 1.0 : 0.0)) * (double)P;
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(PRH_E_HIP, "prh_fit_new: shadow counts failed");
+    if (rc) {
+        prh_fit_free(f);
+        return rc;
+    }
+    f->stats = prh_fit_stats{};
+    f->stats.visibility_bytes = vis_bytes;
+    f->stats.n_views = n;
+    *out = f;
+    return PRH_OK;
+}
+
+int prh_fit_free(prh_fit* f) {
+    if (!f) return PRH_OK;
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    for (int k = 0; k < 4; k++)
+        if (f->ev[k]) (void)hipEventDestroy(f->ev[k]);
+    free_list(f->allocs);
+    for (prh_renderer* r : f->views) r->live_fits--;
+    delete f;
+    return PRH_OK;
+}
+
+int prh_fit_targets(prh_fit* f, const prh_diff_params* truth) {
+    if (!f || !truth) return fail(PRH_E_INVALID, "prh_fit_targets: null argument");
+    if (truth->normal) return fail(PRH_E_UNSUPPORTED, "prh_fit: the views' own normals only");
+    int rc = fit_check(f);
+    if (!rc) rc = check_params(f->views[0], truth);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t st = f->ctx->stream;
+    if ((rc = upload_params(f->views[0], f->l, truth, f->par, st))) return rc;
+    if ((rc = fit_views(f, false, DIFF_ALL, st))) return rc;
+    launch_diff_views(st, false, f->sc->d, f->d_views, (int32_t)f->views.size(), f->total_blocks);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return PRH_OK;
+}
+
+int prh_fit_set_target(prh_fit* f, int32_t view, const float* target, size_t bytes) {
+    if (!f || !target || view < 0 || view >= (int32_t)f->views.size())
+        return fail(PRH_E_INVALID, "prh_fit_set_target: bad argument");
+    const prh_renderer* r = f->views[view];
+    const size_t need = (size_t)12 * r->W * r->H;
+    if (bytes < need) return fail(PRH_E_INVALID, "prh_fit_set_target: target too small");
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    HIP_TRY(hipMemcpyAsync(f->target[view], target, need, hipMemcpyHostToDevice, f->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(f->ctx->stream));
+    return PRH_OK;
+}
+
+int prh_fit_run(prh_fit* f, const prh_diff_params* start, const prh_fit_options* o, const prh_diff_grads* result,
+                double* loss_history) {
+    if (!f || !start || !o) return fail(PRH_E_INVALID, "prh_fit_run: null argument");
+    const int32_t known = PRH_FIT_ALBEDO | PRH_FIT_KD | PRH_FIT_KS | PRH_FIT_SHININESS | PRH_FIT_LIGHT_POS |
+                          PRH_FIT_LIGHT_INT;
+    if (o->steps < 0 || (o->groups & ~known)) return fail(PRH_E_INVALID, "prh_fit_run: bad steps or parameter groups");
+    if (!(o->lr > 0.0f) || !(o->beta1 >= 0.0f && o->beta1 < 1.0f) || !(o->beta2 >= 0.0f && o->beta2 < 1.0f) ||
+        !(o->eps > 0.0f))
+        return fail(PRH_E_INVALID, "prh_fit_run: bad Adam constants");
+    if (f->stored && (o->groups & PRH_FIT_ALBEDO))
+        return fail(PRH_E_UNSUPPORTED, "prh_fit_run: albedo gradients need the rebuilt G-buffer");
+    if (start->normal) return fail(PRH_E_UNSUPPORTED, "prh_fit: the views' own normals only");
+    int rc = fit_check(f);
+    if (!rc) rc = check_params(f->views[0], start);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(f->ctx->device));
+    hipStream_t st = f->ctx->stream;
+    const DiffLayout& l = f->l;
+    // parameters and the Adam state
+    if ((rc = upload_params(f->views[0], l, start, f->par, st))) return rc;
+    HIP_TRY(hipMemcpyAsync(f->base, f->par, l.total * 4, hipMemcpyDeviceToDevice, st));
+    std::vector<int32_t> idx;
+    std::vector<float> lower;
+    fit_groups(f, o->groups, idx, lower);
+    const int32_t n_idx = (int32_t)idx.size();
+    if (n_idx) {
+        HIP_TRY(hipMemcpyAsync(f->idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(f->lower, lower.data(), lower.size() * 4, hipMemcpyHostToDevice, st));
+    }
+    launch_fill_u32(st, reinterpret_cast<uint32_t*>(f->theta), l.total, 0u);
+    launch_fill_u32(st, reinterpret_cast<uint32_t*>(f->m), l.total, 0u);
+    launch_fill_u32(st, reinterpret_cast<uint32_t*>(f->v), l.total, 0u);
+    launch_fill_u32(st, reinterpret_cast<uint32_t*>(f->step), 1, 0u);
+    HIP_TRY(hipStreamSynchronize(st));  // idx and lower are locals
+    if (o->steps > f->hist_cap) {
+        if (f->hist) {
+            auto it = std::find(f->allocs.begin(), f->allocs.end(), (void*)f->hist);
+            if (it != f->allocs.end()) f->allocs.erase(it);
+            (void)hipFree(f->hist);
+            f->hist = nullptr, f->hist_cap = 0;
+        }
+        if ((rc = falloc(f, &f->hist, (size_t)o->steps))) return rc;
+        f->hist_cap = o->steps;
+    }
+    // the stored G-buffer, from the start parameters
+    HIP_TRY(hipEventRecord(f->ev[2], st));
+    if (f->stored) {
+        for (size_t i = 0; i < f->views.size(); i++) {
+            prh_renderer* r = f->views[i];
+            const DiffBuffers b = diff_buffers(r, l, f->par, f->grad, f->loss, false, f->count[i]);
+            launch_diff_store_gbuffer(st, f->sc->d, r->vc, b, f->gbuf[i]);
+        }
+    }
+    HIP_TRY(hipEventRecord(f->ev[3], st));
+    if ((rc = fit_views(f, true, (uint32_t)o->groups, st))) return rc;
+    HIP_TRY(hipGetLastError());
+    const AdamConsts a{o->lr, o->beta1, o->beta2, o->eps, n_idx, f->hist_cap};
+    // one step: clear, adjoint over every view, Adam; captured once, replayed
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (o->steps > 0) {
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        launch_fill_u32(st, reinterpret_cast<uint32_t*>(f->grad), l.total, 0u);
+        launch_fill_u64(st, reinterpret_cast<uint64_t*>(f->loss), 1, 0ull);
+        launch_diff_views(st, true, f->sc->d, f->d_views, (int32_t)f->views.size(), f->total_blocks);
+        launch_adam(st, f->par, f->grad, f->base, f->theta, f->m, f->v, f->idx, f->lower, a, f->step, f->loss,
+                    f->hist);
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (e != hipSuccess) return fail(PRH_E_HIP, std::string("prh_fit_run capture: ") + hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e2 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(PRH_E_HIP, std::string("prh_fit_run instantiate: ") + hipGetErrorString(e2));
+        }
+    }
+    hipError_t e = hipEventRecord(f->ev[0], st);
+    for (int32_t k = 0; k < o->steps && e == hipSuccess; k++) e = hipGraphLaunch(ge, st);
+    if (e == hipSuccess) e = hipEventRecord(f->ev[1], st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(PRH_E_HIP, std::string("prh_fit_run: ") + hipGetErrorString(e));
+    f->stats.steps = o->steps;
+    (void)hipEventElapsedTime(&f->stats.ms_steps, f->ev[0], f->ev[1]);
+    f->stats.ms_gbuffer = 0.0f;
+    if (f->stored) (void)hipEventElapsedTime(&f->stats.ms_gbuffer, f->ev[2], f->ev[3]);
+    if (loss_history && o->steps > 0)
+        HIP_TRY(hipMemcpy(loss_history, f->hist, (size_t)o->steps * 8, hipMemcpyDeviceToHost));
+    if (result) {
+        const DevScene& d = f->sc->d;
+        const size_t V = d.n_tverts, M = d.n_materials, L = d.n_lights;
+        auto down = [&](float* dst, size_t off, size_t n) -> hipError_t {
+            return dst && n ? hipMemcpy(dst, f->par + off, n * 4, hipMemcpyDeviceToHost) : hipSuccess;
+        };
+        HIP_TRY(down(result->albedo, l.albedo, 3 * V));
+        HIP_TRY(down(result->kd, l.kd, 3 * M));
+        HIP_TRY(down(result->ks, l.ks, 3 * M));
+        HIP_TRY(down(result->shininess, l.shin, M));
+        HIP_TRY(down(result->light_pos, l.lpos, 3 * L));
+        HIP_TRY(down(result->light_int, l.lint, L));
+    }
+    return PRH_OK;
+}
+
+int prh_fit_last_stats(prh_fit* f, prh_fit_stats* out) {
+    if (!f || !out) return fail(PRH_E_INVALID, "prh_fit_last_stats: null argument");
+    *out = f->stats;
+    return PRH_OK;
 }
 
 }  // extern "C"

@@ -126,4 +126,111 @@ def backward(lib, handle, p: DiffParams, target: np.ndarray, H: int, W: int):
     return loss.value, g
 
 
-__all__ = ["DiffParams", "scene_params", "perturb", "forward", "backward", "sizes"]
+__all__ = ["DiffParams", "scene_params", "perturb", "forward", "backward", "sizes", "Fit", "rmse", "gauge_free_error"]
+
+
+# ------------------------------------------------ batched inverse rendering
+GROUPS = {"albedo": abi.PRH_FIT_ALBEDO, "kd": abi.PRH_FIT_KD, "ks": abi.PRH_FIT_KS,
+          "shininess": abi.PRH_FIT_SHININESS, "light_pos": abi.PRH_FIT_LIGHT_POS, "light_int": abi.PRH_FIT_LIGHT_INT}
+
+
+class Fit:
+    """Gradient descent on the shading parameters over many views of one
+    scene (prh_fit_*): each view's last frame fixes its visibility and its
+    shadow visibility; one step is the adjoint of every view in one launch
+    plus an Adam update on the device, the steps one replayed HIP graph.
+
+        gs, views = render.views_of(scene, cams, dev)
+        for r in views: r.render_device()
+        fit = diff.Fit(views)
+        fit.targets(truth)                       # or fit.set_target(i, image)
+        p, hist = fit.run(start, steps=200, lr=1e-2, groups=("kd", "ks", ...))
+
+    polyred has no derivative path; the shading is prh_diff_*'s with each
+    pixel scaled by 2^-count, count the lights whose shadow test reported
+    true (render/shadow.go:230-283) as the reference scales the colour
+    (render/raster.go:339-347)."""
+
+    def __init__(self, renderers, stored_gbuffer: bool = False):
+        self.views = list(renderers)
+        if not self.views:
+            raise ValueError("Fit needs at least one view")
+        self._lib = self.views[0]._lib
+        self.sizes = sizes(self._lib, self.views[0].handle)
+        arr = (C.c_void_p * len(self.views))(*[r.handle for r in self.views])
+        h = C.c_void_p()
+        abi.check(self._lib.prh_fit_new(arr, len(self.views), abi.PRH_FIT_STORED_GBUFFER if stored_gbuffer else 0,
+                                        C.byref(h)), "prh_fit_new")
+        self.handle = h
+        gs = self.views[0].gscene
+        if gs is not None:
+            gs._fits.add(self)  # freed before the scene's renderers
+
+    def targets(self, truth: DiffParams) -> None:
+        """Every view's target := its forward shading at `truth`, on the device."""
+        check_shapes(truth, *self.sizes)
+        b = _Bound(truth, abi.prh_diff_params)
+        abi.check(self._lib.prh_fit_targets(self.handle, C.byref(b.s)), "prh_fit_targets")
+
+    def set_target(self, i: int, target: np.ndarray) -> None:
+        r = self.views[i]
+        t = np.ascontiguousarray(target, F32)
+        if t.shape != (r.H * r.msaa, r.W * r.msaa, 3):
+            raise ValueError(f"target has shape {t.shape}")
+        abi.check(self._lib.prh_fit_set_target(self.handle, i, t.ctypes.data, t.nbytes), "prh_fit_set_target")
+
+    def run(self, start: DiffParams, steps: int = 200, lr: float = 1e-2, groups=("kd", "ks", "shininess",
+            "light_pos", "light_int"), beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        """Adam from `start`; returns (final parameters, loss before each step)."""
+        nv, nm, nl = self.sizes
+        check_shapes(start, nv, nm, nl)
+        b = _Bound(start, abi.prh_diff_params)
+        o = abi.prh_fit_options(steps=steps, groups=sum(GROUPS[g] for g in groups), lr=lr, beta1=beta1,
+                                beta2=beta2, eps=eps)
+        out = DiffParams(albedo=np.zeros((nv, 3), F32), normal=None, kd=np.zeros((nm, 3), F32),
+                         ks=np.zeros((nm, 3), F32), shininess=np.zeros(nm, F32), light_pos=np.zeros((nl, 3), F32),
+                         light_int=np.zeros(nl, F32))
+        gs = abi.prh_diff_grads(**{n: (_f32p(getattr(out, n)) if getattr(out, n) is not None else None)
+                                   for n in DiffParams.NAMES})
+        hist = np.zeros(max(1, steps), np.float64)
+        abi.check(self._lib.prh_fit_run(self.handle, C.byref(b.s), C.byref(o), C.byref(gs), hist.ctypes.data),
+                  "prh_fit_run")
+        return out, hist[:steps]
+
+    def stats(self) -> abi.prh_fit_stats:
+        st = abi.prh_fit_stats()
+        abi.check(self._lib.prh_fit_last_stats(self.handle, C.byref(st)), "prh_fit_last_stats")
+        return st
+
+    def free(self) -> None:
+        if getattr(self, "handle", None):
+            abi.check(self._lib.prh_fit_free(self.handle), "prh_fit_free")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rmse(a: DiffParams, b: DiffParams, names=("kd", "ks", "shininess", "light_pos", "light_int")) -> dict:
+    """Root-mean-square error of `a` against `b` per parameter group."""
+    return {n: float(np.sqrt(np.mean((np.asarray(getattr(a, n), np.float64) -
+                                      np.asarray(getattr(b, n), np.float64)) ** 2))) for n in names}
+
+
+def gauge_free_error(a: DiffParams, b: DiffParams) -> dict:
+    """Errors of `a` against `b` in what the images determine.  The shading
+    is unchanged when every light intensity is multiplied by s and every Kd
+    and Ks divided by s, so the products Kd[m] I[l] and Ks[m] I[l] are
+    compared (relative RMSE) rather than the factors; shininess and light
+    positions as they are (RMSE)."""
+    def rel(x, y):
+        x, y = np.asarray(x, np.float64), np.asarray(y, np.float64)
+        return float(np.sqrt(np.mean(((x - y) / np.where(y != 0, y, 1)) ** 2)))
+    ia, ib = np.asarray(a.light_int, np.float64), np.asarray(b.light_int, np.float64)
+    out = {"kd_x_intensity": rel(np.multiply.outer(a.kd, ia), np.multiply.outer(b.kd, ib)),
+           "ks_x_intensity": rel(np.multiply.outer(a.ks, ia), np.multiply.outer(b.ks, ib))}
+    out.update(rmse(a, b, ("shininess", "light_pos")))
+    return out

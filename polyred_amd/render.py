@@ -66,6 +66,7 @@ class GPUScene:
         self.scene = scene
         self.handle = None
         self._renderers: "weakref.WeakSet" = weakref.WeakSet()
+        self._fits: "weakref.WeakSet" = weakref.WeakSet()  # diff.Fit objects over its renderers
         desc = scene.desc()
         h = C.c_void_p()
         abi.check(self._lib.prh_scene_upload(dev.handle, C.byref(desc), C.byref(h)), "prh_scene_upload")
@@ -74,6 +75,8 @@ class GPUScene:
 
     def free(self) -> None:
         if self.handle:
+            for f in list(self._fits):
+                f.free()
             for r in list(self._renderers):
                 r.free()
             abi.check(self._lib.prh_scene_free(self.handle), "prh_scene_free")
@@ -195,7 +198,9 @@ class Renderer:
         s.camera = cfg.Camera
         s.msaa = cfg.MSAA
         self.handle = None
-        self.gscene = GPUScene(cfg.HIPDevice, s)
+        shared = cfg.extra.get("gscene")
+        self._owns_scene = shared is None
+        self.gscene = shared if shared is not None else GPUScene(cfg.HIPDevice, s)
         cam = s.camera_struct()
         opts = s.options_struct(keep_linear=cfg.KeepLinear)
         opts.msaa = cfg.MSAA
@@ -204,7 +209,8 @@ class Renderer:
             abi.check(self._lib.prh_renderer_new(cfg.HIPDevice.handle, self.gscene.handle, C.byref(cam),
                                                  C.byref(opts), C.byref(h)), "prh_renderer_new")
         except abi.PrhError:
-            self.gscene.free()
+            if self._owns_scene:
+                self.gscene.free()
             raise
         self.handle = h
         self.gscene._renderers.add(self)
@@ -243,6 +249,8 @@ class Renderer:
             out = np.empty((H, W, 3), np.float32)
         elif which == abi.PRH_BUF_PREGAMMA:
             out = np.empty((H, W, 4), np.uint8)
+        elif which == abi.PRH_BUF_SHADOW_COUNT:
+            out = np.empty((H, W), np.uint8)
         elif which == abi.PRH_BUF_SHADOW:
             sw, sh = (self.S, self.S) if self.S else (W, H)
             out = np.empty((max(1, self.n_shadow), sh, sw), np.float32)
@@ -273,12 +281,17 @@ class Renderer:
 
     def free(self) -> None:
         if getattr(self, "handle", None):
+            gs = getattr(self, "gscene", None)
+            for f in list(gs._fits if gs is not None else ()):
+                if self in f.views:
+                    f.free()
             abi.check(self._lib.prh_renderer_free(self.handle), "prh_renderer_free")
             self.handle = None
         gs = getattr(self, "gscene", None)
         if gs is not None:
             self.gscene = None
-            gs.free()
+            if getattr(self, "_owns_scene", True):
+                gs.free()
 
     def __del__(self):
         try:
@@ -320,6 +333,24 @@ def NewRenderer(*opts) -> Renderer:  # render/raster.go:84-143
     for opt in opts:
         opt(cfg)
     return Renderer(cfg)
+
+
+def views_of(scene, views, dev: Device):
+    """Renderers of one scene uploaded once (prh_scene_upload), one per
+    (camera dict, width, height) of `views`, with the scene's other
+    settings.  Returns (GPUScene, renderers); freeing the GPUScene frees
+    them."""
+    gs = GPUScene(dev, scene)
+
+    def share(o: _Option):
+        o.extra["gscene"] = gs
+
+    rs = []
+    for cam, w, h in views:
+        rs.append(NewRenderer(Size(w, h), Camera(cam), Scene(scene), Background(scene.background),
+                              ShadowMap(scene.shadow_map), ShadowSize(scene.shadow_size),
+                              GammaCorrection(scene.gamma), MSAA(getattr(scene, "msaa", 1)), HIP(dev), share))
+    return gs, rs
 
 
 def renderer_for(scene, dev: Device, keep_linear: bool = False) -> Renderer:

@@ -496,6 +496,55 @@ def c3_views(seed: int = 3, n_views: int = 64, min_tris: int = 10_000, max_tris:
     return views
 
 
+def c5_inverse(seed: int = 5, n_views: int = 32, n_spheres: int = 260, min_side: int = 256, max_side: int = 1024,
+               shadow_size: int = 1024) -> tuple:
+    """Batched inverse rendering: one scene seen from n_views cameras.  The
+    scene is n_spheres instanced UV spheres (960 triangles each; scale
+    log-uniform in [0.4, 1.6] in a 14^3 box) on a 20x20-quad ground, 250k
+    triangles at the defaults, 8 materials (kd U[0.1,0.9]^3, ks U[0.1,1]
+    grey, shininess log-U[4,64]) that receive shadows, 4 shadow-casting
+    point lights in the box's upper half with 1024^2 maps, ambient 0.1.
+    Views: eye on a ring of radius U[20,26] at height U[3,12] around the
+    box, looking at its centre, fov 45; pixel count log-uniform in
+    [256^2, 1024^2] with aspect U[0.75, 1.5].  Returns (scene, views), views
+    a list of (camera dict, width, height)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mats = []
+    for _ in range(8):
+        kd = rng.uniform(0.1, 0.9, 3)
+        mats.append(_mat_from(kd, rng.uniform(0.1, 1.0), math.exp(rng.uniform(math.log(4), math.log(64))),
+                              shadow=True))
+    sphere = uv_sphere()
+    ground = grid_plane(20, 10.0, -8.0)
+    models, bases = [], []
+    for _ in range(n_spheres):
+        s = math.exp(rng.uniform(math.log(0.4), math.log(1.6)))
+        models.append(model_matrix(s, random_rotation(rng), rng.uniform(-7, 7, 3)))
+        bases.append(int(rng.integers(0, 8)))
+    models.append(np.eye(4, dtype=F32).reshape(16))
+    bases.append(0)
+    lights = []
+    for _ in range(4):
+        p = (rng.uniform(-7, 7), rng.uniform(4, 12), rng.uniform(-7, 7))
+        c = tuple(int(x) for x in rng.integers(200, 256, 3)) + (255,)
+        lights.append(Light(kind=abi.PRH_LIGHT_POINT, position=tuple(float(F32(x)) for x in p), intensity=4.0,
+                            color=c, cast_shadow=True))
+    views = []
+    for _ in range(n_views):
+        area = math.exp(rng.uniform(math.log(min_side * min_side), math.log(max_side * max_side)))
+        aspect = rng.uniform(0.75, 1.5)
+        w = max(16, int(round(math.sqrt(area * aspect))))
+        h = max(16, int(round(area / w)))
+        a, r, y = rng.uniform(0, 2 * math.pi), rng.uniform(20, 26), rng.uniform(3, 12)
+        eye = tuple(float(F32(v)) for v in (r * math.cos(a), y, r * math.sin(a)))
+        views.append((dict(position=eye, target=(0.0, -1.0, 0.0), fov=45.0, near=0.1, far=100.0), w, h))
+    scene = Scene(meshes=[sphere, ground], obj_mesh=np.array([0] * n_spheres + [1]), obj_model=np.stack(models),
+                  obj_mat_base=np.array(bases), materials=mats, lights=lights, ambient=[0.1],
+                  camera=views[0][0], width=views[0][1], height=views[0][2], shadow_map=True, gamma=True,
+                  shadow_size=shadow_size, name=f"c5_inverse_{n_views}views_{n_spheres}sph")
+    return scene, views
+
+
 def tiny(seed: int, width: int = 32, height: int = 24, n_tris: int = 6, vertex_color: bool = True,
          perspective: bool = True) -> Scene:
     """Small random-triangle scenes for brute-force raster checks: triangles
