@@ -168,9 +168,45 @@ __device__ __forceinline__ DLight light_terms(const DevLight& l, const DiffBuffe
 
 }  // namespace
 
-// One workgroup (`bid` of the view's grid) of the forward or the adjoint.
+// Per-workgroup gradient accumulators in LDS: the lights' (4 per light)
+// and, for up to DIFF_LDS_MATS materials, the materials' (7 each); the
+// waves add their reductions here and the workgroup flushes them to the
+// global gradients once, so the global atomics to these few addresses are
+// per workgroup, not per wave.
+constexpr int DIFF_LDS_MATS = 256;
+constexpr int DIFF_ACC = 4 * DIFF_MAX_LIGHTS + 7 * DIFF_LDS_MATS;
+struct DiffAcc {
+    float g[DIFF_ACC];
+    double loss;
+};
+
+__device__ __forceinline__ void acc_clear(DiffAcc& a) {
+    for (int i = threadIdx.x; i < DIFF_ACC; i += blockDim.x) a.g[i] = 0.0f;
+    if (threadIdx.x == 0) a.loss = 0.0;
+    __syncthreads();
+}
+
+// After the workgroup's last tile: every nonzero accumulator to its gradient.
+__device__ __forceinline__ void acc_flush(const DevScene& s, const DiffBuffers& b, DiffAcc& a) {
+    __syncthreads();
+    const int NL = s.n_lights, NM = s.n_materials < DIFF_LDS_MATS ? s.n_materials : DIFF_LDS_MATS;
+    for (int i = threadIdx.x; i < 4 * NL; i += blockDim.x) {
+        const float v = a.g[i];
+        if (v != 0.0f) atomicAdd((i & 3) < 3 ? &b.g_lpos[3 * (i >> 2) + (i & 3)] : &b.g_lint[i >> 2], v);
+    }
+    for (int i = threadIdx.x; i < 7 * NM; i += blockDim.x) {
+        const float v = a.g[4 * DIFF_MAX_LIGHTS + i];
+        const int m = i / 7, j = i % 7;
+        if (v != 0.0f) atomicAdd(j < 3 ? &b.g_kd[3 * m + j] : (j < 6 ? &b.g_ks[3 * m + j - 3] : &b.g_shin[m]), v);
+    }
+    if (threadIdx.x == 0 && a.loss != 0.0) atomicAdd(b.loss, a.loss);
+}
+
+// One 16x16 tile (`bid` of the view's grid) of the forward or the adjoint;
+// the adjoint adds its light and material gradients and its loss to `acc`.
 template <bool BACKWARD>
-__device__ __forceinline__ void diff_body(const DevScene& s, const ViewConsts& v, const DiffBuffers& b, int bid) {
+__device__ __forceinline__ void diff_body(const DevScene& s, const ViewConsts& v, const DiffBuffers& b, int bid,
+                                          DiffAcc& acc) {
     const int tiles_x = v.tiles_x, n_tiles = tiles_x * v.tiles_y;
     const int per_xcd = (n_tiles + 7) / 8;
     const int tile = (bid % 8) * per_xcd + bid / 8;
@@ -264,10 +300,7 @@ __device__ __forceinline__ void diff_body(const DevScene& s, const ViewConsts& v
             const float sj = wsum(glight[j]);
             if (lane == j) mine = sj;
         }
-        if (lane < 4 * NL && lane < 4 * DIFF_MAX_LIGHTS) {
-            const int li = lane >> 2, k = lane & 3;
-            atomicAdd(k < 3 ? &b.g_lpos[3 * li + k] : &b.g_lint[li], mine);
-        }
+        if (lane < 4 * NL && lane < 4 * DIFF_MAX_LIGHTS && mine != 0.0f) atomicAdd(&acc.g[lane], mine);
     }
     // per material: one reduction per distinct material of the wave
     if (b.want & (DIFF_KD | DIFF_KS | DIFF_SHININESS)) {
@@ -281,9 +314,15 @@ __device__ __forceinline__ void diff_body(const DevScene& s, const ViewConsts& v
                 const float sj = wsum(same ? gm_[j] : 0.0f);
                 if (lane == j) mine = sj;
             }
-            if (lane < 3) atomicAdd(&b.g_kd[3 * lm + lane], mine);
-            else if (lane < 6) atomicAdd(&b.g_ks[3 * lm + lane - 3], mine);
-            else if (lane == 6) atomicAdd(&b.g_shin[lm], mine);
+            if (lm < DIFF_LDS_MATS) {
+                if (lane < 7 && mine != 0.0f) atomicAdd(&acc.g[4 * DIFF_MAX_LIGHTS + 7 * lm + lane], mine);
+            } else if (lane < 3) {
+                atomicAdd(&b.g_kd[3 * lm + lane], mine);
+            } else if (lane < 6) {
+                atomicAdd(&b.g_ks[3 * lm + lane - 3], mine);
+            } else if (lane == 6) {
+                atomicAdd(&b.g_shin[lm], mine);
+            }
             pending &= ~__ballot(same);
         }
     }
@@ -316,30 +355,38 @@ __device__ __forceinline__ void diff_body(const DevScene& s, const ViewConsts& v
         }
     }
     const double tot = wsum(l2);
-    if (lane == 0) atomicAdd(b.loss, tot);
+    if (lane == 0 && tot != 0.0) atomicAdd(&acc.loss, tot);
 }
 
 template <bool BACKWARD>
 __global__ __launch_bounds__(256) void k_diff(DevScene s, ViewConsts v, DiffBuffers b) {
-    diff_body<BACKWARD>(s, v, b, blockIdx.x);
+    __shared__ DiffAcc acc;
+    if (BACKWARD) acc_clear(acc);
+    diff_body<BACKWARD>(s, v, b, blockIdx.x, acc);
+    if (BACKWARD) acc_flush(s, b, acc);
 }
 
-// Many views of one scene in one launch: view i owns the workgroups
-// [block0, block0 + nblocks), block0 ascending and a multiple of 8 (so a
-// view's workgroups keep the XCD spread of its own launch).
+// Many views of one scene in one launch: view i owns the tiles
+// [block0, block0 + nblocks) of a virtual grid, block0 ascending and a
+// multiple of 8 (so a view's tiles keep the XCD spread of its own launch);
+// the workgroups stride over that grid, accumulating in LDS, and flush once.
+// The views' gradient and loss buffers are the same.
 template <bool BACKWARD>
-__global__ __launch_bounds__(256) void k_diff_views(DevScene s, const DiffView* __restrict__ views,
-                                                    int32_t n_views) {
-    const int bid = blockIdx.x;
-    int lo = 0, hi = n_views - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (views[mid].block0 <= bid) lo = mid;
-        else hi = mid - 1;
+__global__ __launch_bounds__(256) void k_diff_views(DevScene s, const DiffView* __restrict__ views, int32_t n_views,
+                                                    int32_t total_blocks) {
+    __shared__ DiffAcc acc;
+    if (BACKWARD) acc_clear(acc);
+    for (int bid = blockIdx.x; bid < total_blocks; bid += gridDim.x) {
+        int lo = 0, hi = n_views - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (views[mid].block0 <= bid) lo = mid;
+            else hi = mid - 1;
+        }
+        const DiffView& w = views[lo];
+        if (bid - w.block0 < w.nblocks) diff_body<BACKWARD>(s, w.v, w.b, bid - w.block0, acc);
     }
-    const DiffView& w = views[lo];
-    if (bid - w.block0 >= w.nblocks) return;
-    diff_body<BACKWARD>(s, w.v, w.b, bid - w.block0);
+    if (BACKWARD) acc_flush(s, views[0].b, acc);
 }
 
 __global__ __launch_bounds__(256) void k_diff_gbuffer(DevScene s, ViewConsts v, DiffBuffers b, float4* __restrict__ g) {
@@ -404,8 +451,11 @@ int32_t diff_blocks(const ViewConsts& v) {
 void launch_diff_views(hipStream_t st, bool backward, const DevScene& s, const DiffView* views, int32_t n_views,
                        int32_t total_blocks) {
     if (n_views <= 0 || total_blocks <= 0) return;
-    if (backward) hipLaunchKernelGGL(k_diff_views<true>, dim3(total_blocks), dim3(256), 0, st, s, views, n_views);
-    else hipLaunchKernelGGL(k_diff_views<false>, dim3(total_blocks), dim3(256), 0, st, s, views, n_views);
+    // (a fixed grid of 2048 workgroups, 8 per CU, strides over the views' tiles)
+    const int grid = total_blocks < 2048 ? total_blocks : 2048;
+    if (backward)
+        hipLaunchKernelGGL(k_diff_views<true>, dim3(grid), dim3(256), 0, st, s, views, n_views, total_blocks);
+    else hipLaunchKernelGGL(k_diff_views<false>, dim3(grid), dim3(256), 0, st, s, views, n_views, total_blocks);
 }
 
 void launch_diff_store_gbuffer(hipStream_t st, const DevScene& s, const ViewConsts& v, const DiffBuffers& b,
