@@ -141,16 +141,19 @@ def batch_bench(dev, reps: int) -> dict:
     """Batched forward: 64 independent views (each its own scene: 256^2 to
     1920x1080, 10k-500k triangles, 1-8 shadow-casting lights with 1024^2
     maps), rendered as 64 sequential single-view prh_render calls, as one
-    prh_render_batch over the batch streams, and as that batch captured in a
-    HIP graph and replayed.  Wall-clock per batch, scenes resident."""
+    prh_render_batch of wide launches (one per pass over every view), as
+    that batch captured in a HIP graph and replayed, and (for comparison)
+    as each view's own launches over the batch streams, eager and as a
+    graph.  Wall-clock per batch, scenes resident."""
     from polyred_amd import render, scenes
 
     views = scenes.c3_views()
     rs = [render.renderer_for(s, dev) for s in views]
     for r in rs:
         r.render_device()
-    render.render_batch(rs)
-    render.render_batch(rs, graph=True)
+    for streams in (False, True):
+        render.render_batch(rs, streams=streams)
+        render.render_batch(rs, graph=True, streams=streams)
 
     def timed(fn):
         t = time.perf_counter()
@@ -161,6 +164,8 @@ def batch_bench(dev, reps: int) -> dict:
     seq = timed(lambda: [r.render_device() for r in rs])
     bat = timed(lambda: render.render_batch(rs))
     gr = timed(lambda: render.render_batch(rs, graph=True))
+    sbat = timed(lambda: render.render_batch(rs, streams=True))
+    sgr = timed(lambda: render.render_batch(rs, graph=True, streams=True))
     for r in rs:
         r.free()
     n_tri = sum(s.n_triangles for s in views)
@@ -169,6 +174,8 @@ def batch_bench(dev, reps: int) -> dict:
     return {"views": len(views), "triangles": n_tri, "pixels": n_px,
             "shadow_maps": sum(len(s.lights) for s in views), "shadow_size": views[0].shadow_size,
             "ms_sequential": round(seq, 3), "ms_batch": round(bat, 3), "ms_graph": round(gr, 3),
+            "ms_streams": round(sbat, 3), "ms_streams_graph": round(sgr, 3),
+            "speedup_streams_graph": round(seq / sgr, 3),
             "speedup_batch": round(seq / bat, 3), "speedup_graph": round(seq / gr, 3),
             "views_per_s": round(1000.0 * len(views) / best, 1), "mtris_per_s": round(n_tri / best / 1e3, 1),
             "mpix_per_s": round(n_px / best / 1e3, 1)}

@@ -180,15 +180,18 @@ int prh_renderer_free(prh_renderer* r);
 int prh_render(prh_renderer* r);
 
 /* One frame of each of n renderers (all of one context), as n prh_render
- * calls would produce them -- shadow passes included -- enqueued together
- * over the context's batch streams with no host synchronisation between
- * views; each view's counters are checked once at the end and a view that
- * overflowed its work buffers is rendered again on its own.  With
- * PRH_BATCH_GRAPH the whole batch is captured once into a HIP graph (kept
- * until the batch changes) and replayed; per-pass statistics are then not
- * recorded.  *ms (may be NULL) receives the batch's device time.  No
+ * calls would produce them -- shadow passes included -- with no host
+ * synchronisation between views: by default one launch per pass covers
+ * every view (each view's workgroups find its arguments in a table built
+ * once per batch), with PRH_BATCH_STREAMS each view's own launches over the
+ * context's batch streams.  Each view's counters are checked once at the
+ * end and a view that overflowed its work buffers is rendered again on its
+ * own.  With PRH_BATCH_GRAPH the whole batch is captured once into a HIP
+ * graph (kept until the batch changes) and replayed.  Per-pass statistics
+ * are not recorded.  *ms (may be NULL) receives the batch's device time.  No
  * counterpart in polyred (one renderer draws one view). */
 #define PRH_BATCH_GRAPH 1
+#define PRH_BATCH_STREAMS 2
 int prh_render_batch(prh_renderer* const* renderers, int32_t n, int32_t flags, float* ms);
 
 /* The rendered RGBA8 image, top row first, the layout of

@@ -26,6 +26,7 @@ PRH_CAMERA_PERSPECTIVE = 0
 PRH_CAMERA_ORTHOGRAPHIC = 1
 
 PRH_BATCH_GRAPH = 1
+PRH_BATCH_STREAMS = 2
 
 PRH_BUF_VIS = 0
 PRH_BUF_DEPTH = 1

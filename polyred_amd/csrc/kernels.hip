@@ -6,6 +6,10 @@
 // shader/blinn_cpu.go, material/ao.go, shader/gamma.go) through gomath.hpp.
 #include "kernels.hpp"
 
+#include <functional>
+#include <map>
+#include <vector>
+
 using namespace gm;
 
 namespace prh {
@@ -204,6 +208,23 @@ __device__ __forceinline__ float chan(uint32_t c, int k) { return (float)((c >> 
 
 }  // namespace
 
+// A kernel body's workgroup: its index and the grid's size in x and y --
+// blockIdx/gridDim for a single-view launch, the view-local values in a
+// wide launch.
+struct VB {
+    int bx, by, gx, gy;
+};
+// The view of workgroup b in a wide launch: the last i with start[i] <= b.
+__device__ __forceinline__ int wide_view(const int32_t* __restrict__ start, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------- fills
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -212,10 +233,33 @@ __global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 // The frame's start: counters to zero and the visibility buffer empty.
-__global__ void k_frame_clear(uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_t n_vis) {
-    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+struct k_frame_clear_args {
+    uint32_t* cnt;
+    int32_t n_cnt;
+    uint64_t* vis;
+    size_t n_vis;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_frame_clear_body(const k_frame_clear_args& a_, const VB vb) {
+    uint32_t* cnt = a_.cnt;
+    const int32_t n_cnt = a_.n_cnt;
+    uint64_t* vis = a_.vis;
+    const size_t n_vis = a_.n_vis;
+
+    const size_t i0 = vb.bx * (size_t)blockDim.x + threadIdx.x;
     if (i0 < (size_t)n_cnt) cnt[i0] = 0u;
-    for (size_t i = i0; i < n_vis; i += (size_t)gridDim.x * blockDim.x) vis[i] = VIS_EMPTY;
+    for (size_t i = i0; i < n_vis; i += (size_t)vb.gx * blockDim.x) vis[i] = VIS_EMPTY;
+}
+__global__ __launch_bounds__(256) void k_frame_clear(k_frame_clear_args a) {
+    k_frame_clear_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_frame_clear_wide(const k_frame_clear_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_frame_clear_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_frame_clear_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // ------------------------------------------------------- vertex transform
@@ -223,11 +267,26 @@ __global__ void k_frame_clear(uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_
 // trans = Proj.View.Model, recipw = -1/W (perspective), Pos.Apply(Viewport).Pos();
 // Nor = n.Apply(Normal); and drawClipped's back transform
 // m = Pos.Apply(ViewportInv).Apply(ProjInv).Apply(ViewInv) (raster.go:467-469).
-__global__ __launch_bounds__(256) void k_vertex_main(DevScene s, const M4* __restrict__ trans,
-                                                     const M4* __restrict__ normal, ViewConsts vc,
-                                                     float4* __restrict__ tv_scr, float4* __restrict__ tv_nor,
-                                                     float4* __restrict__ tv_wp) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+struct k_vertex_main_args {
+    DevScene s;
+    const M4* trans;
+    const M4* normal;
+    ViewConsts vc;
+    float4* tv_scr;
+    float4* tv_nor;
+    float4* tv_wp;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_vertex_main_body(const k_vertex_main_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const M4* __restrict__ trans = a_.trans;
+    const M4* __restrict__ normal = a_.normal;
+    const ViewConsts& vc = a_.vc;
+    float4* __restrict__ tv_scr = a_.tv_scr;
+    float4* __restrict__ tv_nor = a_.tv_nor;
+    float4* __restrict__ tv_wp = a_.tv_wp;
+
+    int i = vb.bx * blockDim.x + threadIdx.x;
     if (i >= s.n_tverts) return;
     int o = s.vtx_obj[i];
     int mv = s.mesh_vtx_off[s.obj_mesh[o]] + (i - s.obj_vtx_start[o]);
@@ -241,18 +300,51 @@ __global__ __launch_bounds__(256) void k_vertex_main(DevScene s, const M4* __res
     tv_nor[i] = make_float4(nn.x, nn.y, nn.z, 0.0f);
     tv_wp[i] = make_float4(m.x, m.y, m.z, 0.0f);
 }
+__global__ __launch_bounds__(256) void k_vertex_main(k_vertex_main_args a) {
+    k_vertex_main_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_vertex_main_wide(const k_vertex_main_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_vertex_main_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_vertex_main_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
 
 // render/shadow.go:155-173: light-space screen position of every vertex for
 // every shadow slot (grid.y = slot).
-__global__ __launch_bounds__(256) void k_vertex_shadow(DevScene s, const M4* __restrict__ strans, ShadowConsts sc,
-                                                       float4* __restrict__ ts) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int slot = blockIdx.y;
+struct k_vertex_shadow_args {
+    DevScene s;
+    const M4* strans;
+    ShadowConsts sc;
+    float4* ts;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_vertex_shadow_body(const k_vertex_shadow_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const M4* __restrict__ strans = a_.strans;
+    const ShadowConsts& sc = a_.sc;
+    float4* __restrict__ ts = a_.ts;
+
+    int i = vb.bx * blockDim.x + threadIdx.x;
+    int slot = vb.by;
     if (i >= s.n_tverts) return;
     int o = s.vtx_obj[i];
     int mv = s.mesh_vtx_off[s.obj_mesh[o]] + (i - s.obj_vtx_start[o]);
     V4 q = pos(apply(mulv(strans[(size_t)slot * s.n_objects + o], ld(s.vpos[mv])), sc.vp));
     ts[(size_t)slot * s.n_tverts + i] = make_float4(q.x, q.y, q.z, 0.0f);
+}
+__global__ __launch_bounds__(256) void k_vertex_shadow(k_vertex_shadow_args a) {
+    k_vertex_shadow_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_vertex_shadow_wide(const k_vertex_shadow_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_vertex_shadow_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_vertex_shadow_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // ------------------------------------------------------- triangle setup
@@ -333,13 +425,36 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
 // and the triangle is neither stored nor binned.
 constexpr int MAIN_DIRECT_MAX = 48;
 
-__global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
-                                                    Prim* __restrict__ prims, uint2* __restrict__ pbox,
-                                                    int32_t* __restrict__ clip_list,
-                                                    int32_t* __restrict__ tile_count, int32_t* __restrict__ blk_prims,
-                                                    Counters* cnt, Capacities cap, unsigned long long* __restrict__ vis,
-                                                    int32_t draw_small) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+struct k_setup_main_args {
+    DevScene s;
+    ViewConsts vc;
+    const float4* tv_scr;
+    Prim* prims;
+    uint2* pbox;
+    int32_t* clip_list;
+    int32_t* tile_count;
+    int32_t* blk_prims;
+    Counters* cnt;
+    Capacities cap;
+    unsigned long long* vis;
+    int32_t draw_small;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const ViewConsts& vc = a_.vc;
+    const float4* __restrict__ tv_scr = a_.tv_scr;
+    Prim* __restrict__ prims = a_.prims;
+    uint2* __restrict__ pbox = a_.pbox;
+    int32_t* __restrict__ clip_list = a_.clip_list;
+    int32_t* __restrict__ tile_count = a_.tile_count;
+    int32_t* __restrict__ blk_prims = a_.blk_prims;
+    Counters* cnt = a_.cnt;
+    const Capacities& cap = a_.cap;
+    unsigned long long* __restrict__ vis = a_.vis;
+    const int32_t draw_small = a_.draw_small;
+
+    const int g = vb.bx * blockDim.x + threadIdx.x;
     bool direct = false, doclip = false, drawn = false;
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
@@ -391,7 +506,18 @@ __global__ __launch_bounds__(256) void k_setup_main(DevScene s, ViewConsts vc, c
     // the primitive count of the frame statistics, summed by k_scan (one
     // global counter hit by every workgroup would serialise them)
     const int nd = __syncthreads_count(direct);
-    if (threadIdx.x == 0) blk_prims[blockIdx.x] = nd;
+    if (threadIdx.x == 0) blk_prims[vb.bx] = nd;
+}
+__global__ __launch_bounds__(256) void k_setup_main(k_setup_main_args a) {
+    k_setup_main_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_setup_main_wide(const k_setup_main_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_setup_main_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_setup_main_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // The clipper (clipTriangle, clipping.go:67-157) for the queued triangles:
@@ -472,16 +598,50 @@ __device__ __forceinline__ void clip_one(int qi, const DevScene& s, const ViewCo
 // One queued triangle per thread, the grid striding over the queue (its
 // length is known on the device only; the grid stays small).
 constexpr int CLIP_GRID = 128;
-__global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const float4* __restrict__ tv_scr,
-                                             const float4* __restrict__ tv_nor, const int32_t* __restrict__ clip_list,
-                                             Prim* __restrict__ prims, uint2* __restrict__ pbox,
-                                             ClipRec* __restrict__ clips,
-                                             int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
-                                             Counters* cnt, Capacities cap) {
+struct k_clip_args {
+    DevScene s;
+    ViewConsts vc;
+    const float4* tv_scr;
+    const float4* tv_nor;
+    const int32_t* clip_list;
+    Prim* prims;
+    uint2* pbox;
+    ClipRec* clips;
+    int32_t* tri_clip_slot;
+    int32_t* tile_count;
+    Counters* cnt;
+    Capacities cap;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_clip_body(const k_clip_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const ViewConsts& vc = a_.vc;
+    const float4* __restrict__ tv_scr = a_.tv_scr;
+    const float4* __restrict__ tv_nor = a_.tv_nor;
+    const int32_t* __restrict__ clip_list = a_.clip_list;
+    Prim* __restrict__ prims = a_.prims;
+    uint2* __restrict__ pbox = a_.pbox;
+    ClipRec* __restrict__ clips = a_.clips;
+    int32_t* __restrict__ tri_clip_slot = a_.tri_clip_slot;
+    int32_t* __restrict__ tile_count = a_.tile_count;
+    Counters* cnt = a_.cnt;
+    const Capacities& cap = a_.cap;
+
     if (cnt->overflow) return;
     const int n = min(cnt->n_clipped, cap.clip_tris);
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    for (int q = vb.bx * blockDim.x + threadIdx.x; q < n; q += vb.gx * blockDim.x)
         clip_one(q, s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
+}
+__global__ __launch_bounds__(64) void k_clip(k_clip_args a) {
+    k_clip_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(64) void k_clip_wide(const k_clip_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_clip_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_clip_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // render/shadow.go:152-176 (drawDepth up to the pixel loop) for every
@@ -494,12 +654,31 @@ __global__ __launch_bounds__(64) void k_clip(DevScene s, ViewConsts vc, const fl
 // cover a few texels; past a few dozen the binned raster is no slower.
 constexpr int SHADOW_DIRECT_MAX = 64;
 
-__global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts sc, const float4* __restrict__ ts,
-                                                      Prim* __restrict__ prims, uint2* __restrict__ pbox,
-                                                      int32_t* __restrict__ tile_count,
-                                                      Counters* cnt, Capacities cap, uint32_t* __restrict__ smaps) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int slot = blockIdx.y;
+struct k_setup_shadow_args {
+    DevScene s;
+    ShadowConsts sc;
+    const float4* ts;
+    Prim* prims;
+    uint2* pbox;
+    int32_t* tile_count;
+    Counters* cnt;
+    Capacities cap;
+    uint32_t* smaps;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const ShadowConsts& sc = a_.sc;
+    const float4* __restrict__ ts = a_.ts;
+    Prim* __restrict__ prims = a_.prims;
+    uint2* __restrict__ pbox = a_.pbox;
+    int32_t* __restrict__ tile_count = a_.tile_count;
+    Counters* cnt = a_.cnt;
+    const Capacities& cap = a_.cap;
+    uint32_t* __restrict__ smaps = a_.smaps;
+
+    const int g = vb.bx * blockDim.x + threadIdx.x;
+    const int slot = vb.by;
     bool emit = false;
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
@@ -539,6 +718,17 @@ __global__ __launch_bounds__(256) void k_setup_shadow(DevScene s, ShadowConsts s
     }
     count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, emit, x0, y0, x1, y1);
 }
+__global__ __launch_bounds__(256) void k_setup_shadow(k_setup_shadow_args a) {
+    k_setup_shadow_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_setup_shadow_wide(const k_setup_shadow_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_setup_shadow_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_setup_shadow_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
 
 // ------------------------------------------------------------ tile scan
 // Exclusive prefix sums of the per-tile primitive counts and of the per-tile
@@ -561,11 +751,20 @@ __device__ __forceinline__ int wave_sum(int v) {
 }
 __device__ __forceinline__ int item_count(int v) { return (v + ITEM_PRIMS - 1) / ITEM_PRIMS; }
 
-__global__ __launch_bounds__(1024) void k_scan_sums(const int32_t* __restrict__ counts, int32_t n,
-                                                    int32_t* __restrict__ bsum) {
+struct k_scan_sums_args {
+    const int32_t* counts;
+    int32_t n;
+    int32_t* bsum;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_scan_sums_body(const k_scan_sums_args& a_, const VB vb) {
+    const int32_t* __restrict__ counts = a_.counts;
+    const int32_t n = a_.n;
+    int32_t* __restrict__ bsum = a_.bsum;
+
     __shared__ int32_t ws[16], wc[16];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int i0 = blockIdx.x * SCAN_BLOCK + 4 * tid;
+    const int i0 = vb.bx * SCAN_BLOCK + 4 * tid;
     int sv = 0, sc = 0;
     for (int k = 0; k < 4; k++) {
         const int v = i0 + k < n ? counts[i0 + k] : 0;
@@ -578,23 +777,57 @@ __global__ __launch_bounds__(1024) void k_scan_sums(const int32_t* __restrict__ 
     if (tid == 0) {
         int a = 0, c = 0;
         for (int k = 0; k < 16; k++) a += ws[k], c += wc[k];
-        bsum[2 * blockIdx.x] = a;
-        bsum[2 * blockIdx.x + 1] = c;
+        bsum[2 * vb.bx] = a;
+        bsum[2 * vb.bx + 1] = c;
     }
+}
+__global__ __launch_bounds__(1024) void k_scan_sums(k_scan_sums_args a) {
+    k_scan_sums_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(1024) void k_scan_sums_wide(const k_scan_sums_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_scan_sums_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_scan_sums_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // Also lists each work item's tile (item_tile, up to cap_items entries) and
 // leaves the counts at zero for the next pass.
-__global__ __launch_bounds__(1024) void k_scan_apply(int32_t* __restrict__ counts, int32_t n,
-                                                     const int32_t* __restrict__ bsum, int32_t* __restrict__ offsets,
-                                                     int32_t* __restrict__ chunk_off, int32_t* __restrict__ cursor,
-                                                     int32_t* __restrict__ item_tile, int32_t cap_items,
-                                                     const int32_t* __restrict__ blk_prims, int32_t n_blk,
-                                                     int32_t max_clip_subs, Counters* cnt) {
+struct k_scan_apply_args {
+    int32_t* counts;
+    int32_t n;
+    const int32_t* bsum;
+    int32_t* offsets;
+    int32_t* chunk_off;
+    int32_t* cursor;
+    int32_t* item_tile;
+    int32_t cap_items;
+    const int32_t* blk_prims;
+    int32_t n_blk;
+    int32_t max_clip_subs;
+    Counters* cnt;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, const VB vb) {
+    int32_t* __restrict__ counts = a_.counts;
+    const int32_t n = a_.n;
+    const int32_t* __restrict__ bsum = a_.bsum;
+    int32_t* __restrict__ offsets = a_.offsets;
+    int32_t* __restrict__ chunk_off = a_.chunk_off;
+    int32_t* __restrict__ cursor = a_.cursor;
+    int32_t* __restrict__ item_tile = a_.item_tile;
+    const int32_t cap_items = a_.cap_items;
+    const int32_t* __restrict__ blk_prims = a_.blk_prims;
+    const int32_t n_blk = a_.n_blk;
+    const int32_t max_clip_subs = a_.max_clip_subs;
+    Counters* cnt = a_.cnt;
+
     __shared__ int32_t ws[16], wc[16];
     __shared__ int32_t carry_s, carry_c;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int b = blockIdx.x;
+    const int b = vb.bx;
     // sums of the blocks before this one
     int ps = 0, pc = 0;
     for (int k = tid; k < b; k += 1024) ps += bsum[2 * k], pc += bsum[2 * k + 1];
@@ -630,7 +863,7 @@ __global__ __launch_bounds__(1024) void k_scan_apply(int32_t* __restrict__ count
             run += v[k];
             runc += ch[k];
         }
-    if (b != gridDim.x - 1) return;
+    if (b != vb.gx - 1) return;
     // the last block writes the totals and the frame statistics: unclipped
     // primitives (per-workgroup counts of the setup) plus the clipper's pieces
     if (tid == 1023) {
@@ -651,15 +884,47 @@ __global__ __launch_bounds__(1024) void k_scan_apply(int32_t* __restrict__ count
         cnt->n_prims = a + min(cnt->n_clip_subs, max_clip_subs);
     }
 }
+__global__ __launch_bounds__(1024) void k_scan_apply(k_scan_apply_args a) {
+    k_scan_apply_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(1024) void k_scan_apply_wide(const k_scan_apply_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_scan_apply_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_scan_apply_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
 
 // Scatter each primitive into the lists of the tiles its pixel box touches.
 // Slots [0, n_fixed) are one per triangle (per shadow slot); the main view's
 // clipped pieces follow them.
-__global__ __launch_bounds__(256) void k_bin(const uint2* __restrict__ pbox, Counters* cnt, int32_t tiles_x,
-                                             int32_t tiles_per_view, int32_t shadow, int32_t* __restrict__ cursor,
-                                             int32_t* __restrict__ list, Capacities cap, int32_t n_fixed,
-                                             int32_t n_tris) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+struct k_bin_args {
+    const uint2* pbox;
+    Counters* cnt;
+    int32_t tiles_x;
+    int32_t tiles_per_view;
+    int32_t shadow;
+    int32_t* cursor;
+    int32_t* list;
+    Capacities cap;
+    int32_t n_fixed;
+    int32_t n_tris;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_bin_body(const k_bin_args& a_, const VB vb) {
+    const uint2* __restrict__ pbox = a_.pbox;
+    Counters* cnt = a_.cnt;
+    const int32_t tiles_x = a_.tiles_x;
+    const int32_t tiles_per_view = a_.tiles_per_view;
+    const int32_t shadow = a_.shadow;
+    int32_t* __restrict__ cursor = a_.cursor;
+    int32_t* __restrict__ list = a_.list;
+    const Capacities& cap = a_.cap;
+    const int32_t n_fixed = a_.n_fixed;
+    const int32_t n_tris = a_.n_tris;
+
+    const int i = vb.bx * blockDim.x + threadIdx.x;
     if (cnt->n_pairs > cap.pairs) {
         if (i == 0) atomicOr(&cnt->overflow, 1);
         return;
@@ -684,6 +949,17 @@ __global__ __launch_bounds__(256) void k_bin(const uint2* __restrict__ pbox, Cou
         const int pos = wave_alloc(cursor, key, act);
         if (act) list[pos] = i;
     }
+}
+__global__ __launch_bounds__(256) void k_bin(k_bin_args a) {
+    k_bin_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_bin_wide(const k_bin_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_bin_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_bin_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // ------------------------------------------------------------- rasters
@@ -764,12 +1040,28 @@ __device__ __forceinline__ bool eval_candidate(const LPrim& p, float px, float p
 //               once, never cleared) as a max over ordered depth bits.
 enum { MODE_VIS = 0, MODE_EXACT = 1, MODE_SHADOW = 2 };
 
+struct k_raster_args {
+    const Prim* prims;
+    const int32_t* tile_off;
+    const int32_t* chunk_off;
+    const int32_t* item_tile;
+    const int32_t* tile_list;
+    Counters* cnt;
+    RasterTarget rt;
+    int32_t cap_items;
+    int32_t gx, gy;  // the grid of this launch
+};
 template <int MODE>
-__global__ __launch_bounds__(256) void k_raster(const Prim* __restrict__ prims, const int32_t* __restrict__ tile_off,
-                                                const int32_t* __restrict__ chunk_off,
-                                                const int32_t* __restrict__ item_tile,
-                                                const int32_t* __restrict__ tile_list, Counters* cnt, RasterTarget rt,
-                                                int32_t cap_items) {
+__device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB vb) {
+    const Prim* __restrict__ prims = a_.prims;
+    const int32_t* __restrict__ tile_off = a_.tile_off;
+    const int32_t* __restrict__ chunk_off = a_.chunk_off;
+    const int32_t* __restrict__ item_tile = a_.item_tile;
+    const int32_t* __restrict__ tile_list = a_.tile_list;
+    Counters* cnt = a_.cnt;
+    const RasterTarget& rt = a_.rt;
+    const int32_t cap_items = a_.cap_items;
+
     constexpr bool SHADOW = MODE == MODE_SHADOW, EXACT = MODE == MODE_EXACT;
     __shared__ LPrim sp[CHUNK];
     __shared__ int32_t start[CHUNK + 1];
@@ -782,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_raster(const Prim* __restrict__ prims, 
     const int n_items = cnt->n_chunks;
     if (cnt->overflow || n_items > cap_items) return;
     bool saw_nan = false;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    for (int item = vb.bx; item < n_items; item += vb.gx) {
         const int tile = item_tile[item];
         const int slot = SHADOW ? tile / rt.tiles_per_map : 0;
         const int t = SHADOW ? tile % rt.tiles_per_map : tile;
@@ -843,14 +1135,40 @@ __global__ __launch_bounds__(256) void k_raster(const Prim* __restrict__ prims, 
     }
     if (saw_nan) atomicOr(SHADOW ? &cnt->shadow_nan : &cnt->nan_flag, 1);
 }
+template <int MODE>
+__global__ __launch_bounds__(256) void k_raster(k_raster_args a) {
+    k_raster_body<MODE>(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+template <int MODE>
+__global__ __launch_bounds__(256) void k_raster_wide(const k_raster_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_raster_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_raster_body<MODE>(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
 
 // Fragment depth per pixel from the visibility key (-1 where uncovered, as
 // ao.go reads an empty fragment); after the exact-order pass, a NaN-depth
 // fragment that came first at its pixel takes it.
+struct k_resolve_args {
+    RasterTarget rt;
+    Counters* cnt;
+    float* depth;
+    DevScene s;
+    uint8_t* ao_need;
+    int32_t gx, gy;  // the grid of this launch
+};
 template <bool EXACT>
-__global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, Counters* cnt, float* __restrict__ depth, DevScene s,
-                                                 uint8_t* __restrict__ ao_need) {
-    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_resolve_body(const k_resolve_args& a_, const VB vb) {
+    const RasterTarget& rt = a_.rt;
+    Counters* cnt = a_.cnt;
+    float* __restrict__ depth = a_.depth;
+    const DevScene& s = a_.s;
+    uint8_t* __restrict__ ao_need = a_.ao_need;
+
+    const size_t o = vb.bx * (size_t)blockDim.x + threadIdx.x;
     if (o >= (size_t)rt.W * rt.H || cnt->overflow) return;
     unsigned long long v = rt.vis[o];
     if (EXACT) {
@@ -881,6 +1199,19 @@ __global__ __launch_bounds__(256) void k_resolve(RasterTarget rt, Counters* cnt,
         }
         ao_need[o] = need;
     }
+}
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_resolve(k_resolve_args a) {
+    k_resolve_body<EXACT>(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_resolve_wide(const k_resolve_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_resolve_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_resolve_body<EXACT>(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // ---------------------------------------------------------------- shade
@@ -1191,9 +1522,25 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 // its rounding to float32 are monotone); a NaN ratio poisons it as math.Max
 // does.  The per-pixel factor pow(total, 10000) (ao.go:31-33) is written for
 // k_shade.
-__global__ __launch_bounds__(256) void k_ao(ShadeConsts c, const uint8_t* __restrict__ ao_need,
-                                            const float* __restrict__ depth, float* __restrict__ ao,
-                                            const float* __restrict__ ao_tab, int32_t ao_fast, const Counters* cnt) {
+struct k_ao_args {
+    ShadeConsts c;
+    const uint8_t* ao_need;
+    const float* depth;
+    float* ao;
+    const float* ao_tab;
+    int32_t ao_fast;
+    const Counters* cnt;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
+    const ShadeConsts& c = a_.c;
+    const uint8_t* __restrict__ ao_need = a_.ao_need;
+    const float* __restrict__ depth = a_.depth;
+    float* __restrict__ ao = a_.ao;
+    const float* __restrict__ ao_tab = a_.ao_tab;
+    const int32_t ao_fast = a_.ao_fast;
+    const Counters* cnt = a_.cnt;
+
     __shared__ float s_m[8][64];
     __shared__ float s_win[2][192];  // the row around the segment for the two x-axis marches
     __shared__ int32_t s_dbase[4][100];  // diagonal marches, tap t: byte offset of lane 0's tap
@@ -1202,7 +1549,7 @@ __global__ __launch_bounds__(256) void k_ao(ShadeConsts c, const uint8_t* __rest
     const int W = c.v.W, H = c.v.H;
     const int nbx = (W + 63) / 64, nb = nbx * H;
     const int per_xcd = (nb + 7) / 8;
-    const int blk = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int blk = (vb.bx % 8) * per_xcd + vb.bx / 8;
     if (blk >= nb) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int x0 = (blk % nbx) * 64, y = blk / nbx;
@@ -1471,14 +1818,35 @@ __global__ __launch_bounds__(256) void k_ao(ShadeConsts c, const uint8_t* __rest
         ao[(size_t)y * W + x] = (float)pow64((double)total, 10000.0);
     }
 }
+__global__ __launch_bounds__(256) void k_ao(k_ao_args a) {
+    k_ao_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_ao_wide(const k_ao_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_ao_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_ao_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
 
 // The colour of every uncovered pixel (render/raster.go:324-328 with
 // raster_screen.go:75-87): DrawFragment hands `shade` the zero fragment, whose
 // X = Y = 0 and MaterialID = 0, so it is shaded from pixel (0,0)'s stored
 // fragment with material table[0]; if pixel (0,0) is uncovered it is the
 // Background.  With no material 0 the colour is pixel (0,0)'s own final one.
-__global__ void k_shade_bg(DevScene s, ShadeConsts c, ShadeBuffers b) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+struct k_shade_bg_args {
+    DevScene s;
+    ShadeConsts c;
+    ShadeBuffers b;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_shade_bg_body(const k_shade_bg_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const ShadeConsts& c = a_.c;
+    const ShadeBuffers& b = a_.b;
+
+    if (threadIdx.x != 0 || vb.bx != 0) return;
     // the frame's counters are final here: hand them to the host's pinned
     // copy (main then shadow), in place of a copy after the frame
     if (b.cnt_host)
@@ -1503,16 +1871,37 @@ __global__ void k_shade_bg(DevScene s, ShadeConsts c, ShadeBuffers b) {
     b.bg[0] = col;
     b.bg_lin[0] = lin[0], b.bg_lin[1] = lin[1], b.bg_lin[2] = lin[2];
 }
+__global__ __launch_bounds__(64) void k_shade_bg(k_shade_bg_args a) {
+    k_shade_bg_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(64) void k_shade_bg_wide(const k_shade_bg_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_shade_bg_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_shade_bg_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
 
 // Deferred pass (DrawFragments(shade) then DrawFragments(GammaCorrection),
 // render/raster.go:300-314,369-376): one thread per pixel, each wave an 8x8
 // block; workgroups are spread so that consecutive workgroups (which the
 // dispatcher deals round-robin to the 8 XCDs) land in one XCD's screen band.
-__global__ __launch_bounds__(256) void k_shade(DevScene s, ShadeConsts c, ShadeBuffers b) {
+struct k_shade_args {
+    DevScene s;
+    ShadeConsts c;
+    ShadeBuffers b;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_shade_body(const k_shade_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const ShadeConsts& c = a_.c;
+    const ShadeBuffers& b = a_.b;
+
     if (b.cnt->overflow) return;
     const int tiles_x = c.v.tiles_x, n_tiles = tiles_x * c.v.tiles_y;
     const int per_xcd = (n_tiles + 7) / 8;
-    const int bid = blockIdx.x;
+    const int bid = vb.bx;
     const int tile = (bid % 8) * per_xcd + bid / 8;
     if (tile >= n_tiles) return;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -1539,6 +1928,17 @@ __global__ __launch_bounds__(256) void k_shade(DevScene s, ShadeConsts c, ShadeB
         b.linear[3 * o + 2] = hl ? lin[2] : __builtin_nanf("");
     }
     b.image[(size_t)(c.v.H - 1 - y) * c.v.W + x] = gamma(c, col);
+}
+__global__ __launch_bounds__(256) void k_shade(k_shade_args a) {
+    k_shade_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_shade_wide(const k_shade_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_shade_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_shade_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // The fixed shadow visibility of the differentiable shading: per pixel of the
@@ -1568,11 +1968,28 @@ static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n 
 // thread: out[y][x] filters row x of `in` (rows of in_w RGBA8 pixels) at
 // output position y with createWeights8's 8-bit weights.  Two launches make
 // imageutil.Resize: the first writes its result transposed.
-__global__ __launch_bounds__(256) void k_resize(const uint32_t* __restrict__ in, int32_t in_w,
-                                                uint32_t* __restrict__ out, int32_t out_w, int32_t out_rows,
-                                                const int16_t* __restrict__ coeffs, const int32_t* __restrict__ start,
-                                                int32_t fl) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+struct k_resize_args {
+    const uint32_t* in;
+    int32_t in_w;
+    uint32_t* out;
+    int32_t out_w;
+    int32_t out_rows;
+    const int16_t* coeffs;
+    const int32_t* start;
+    int32_t fl;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_resize_body(const k_resize_args& a_, const VB vb) {
+    const uint32_t* __restrict__ in = a_.in;
+    const int32_t in_w = a_.in_w;
+    uint32_t* __restrict__ out = a_.out;
+    const int32_t out_w = a_.out_w;
+    const int32_t out_rows = a_.out_rows;
+    const int16_t* __restrict__ coeffs = a_.coeffs;
+    const int32_t* __restrict__ start = a_.start;
+    const int32_t fl = a_.fl;
+
+    const int x = vb.bx * blockDim.x + threadIdx.x, y = vb.by;
     if (x >= out_w || y >= out_rows) return;
     const uint32_t* row = in + (size_t)x * in_w;
     const int maxX = in_w - 1;
@@ -1593,14 +2010,109 @@ __global__ __launch_bounds__(256) void k_resize(const uint32_t* __restrict__ in,
     }
     out[(size_t)y * out_w + x] = o;
 }
+__global__ __launch_bounds__(256) void k_resize(k_resize_args a) {
+    k_resize_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_resize_wide(const k_resize_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_resize_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_resize_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
+
+// ------------------------------------------------------------- wide launches
+// While a WideBatch records (wide_record), every launch_* call appends its
+// argument block and grid to the group of its stage instead of launching;
+// wide_launch then issues one *_wide launch per group, whose workgroups find
+// their view's arguments by a binary search over the group's block offsets
+// (each view's range padded to a multiple of 8 so that its workgroups keep
+// the XCD spread of its own launch).  Stage numbers come from the caller
+// (wide_stage), in the order a frame issues them, so views that skip a stage
+// simply have no entry in its group.
+struct WideGroup {
+    const void* kernel = nullptr;  // the single-view kernel of every entry
+    bool mixed = false;
+    int bs = 0;
+    std::vector<uint8_t> args;
+    std::vector<int32_t> start{0};
+    std::function<void(hipStream_t, const uint8_t*, const int32_t*, int32_t, int32_t)> launch;
+    size_t off_args = 0, off_start = 0;
+};
+struct WideBatch {
+    std::map<int, WideGroup> groups;
+    int stage = 0, sub = 0;
+};
+static WideBatch* g_wide = nullptr;
+
+WideBatch* wide_new() { return new WideBatch(); }
+void wide_free(WideBatch* w) { delete w; }
+void wide_record(WideBatch* w) { g_wide = w; }
+void wide_stage(int stage) {
+    if (g_wide) g_wide->stage = stage, g_wide->sub = 0;
+}
+size_t wide_layout(WideBatch* w) {
+    size_t off = 0;
+    for (auto& kv : w->groups) {
+        WideGroup& g = kv.second;
+        if (g.mixed) return 0;
+        g.off_args = off;
+        off += (g.args.size() + 255) & ~(size_t)255;
+        g.off_start = off;
+        off += (g.start.size() * 4 + 255) & ~(size_t)255;
+    }
+    return off;
+}
+int wide_upload(const WideBatch* w, uint8_t* dev) {
+    for (const auto& kv : w->groups) {
+        const WideGroup& g = kv.second;
+        if (hipMemcpy(dev + g.off_args, g.args.data(), g.args.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dev + g.off_start, g.start.data(), g.start.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return -1;
+    }
+    return 0;
+}
+void wide_launch(const WideBatch* w, hipStream_t st, const uint8_t* dev) {
+    for (const auto& kv : w->groups) {
+        const WideGroup& g = kv.second;
+        const int32_t n = (int32_t)g.start.size() - 1;
+        if (n > 0 && g.start.back() > 0)
+            g.launch(st, dev + g.off_args, reinterpret_cast<const int32_t*>(dev + g.off_start), n, g.start.back());
+    }
+}
+
+// One launch of `single` over `grid`, or its entry in the recording batch.
+template <class A>
+static void go(void (*single)(A), void (*wide)(const A*, const int32_t*, int32_t), dim3 grid, int bs,
+               hipStream_t st, A a) {
+    a.gx = (int32_t)grid.x, a.gy = (int32_t)grid.y;
+    if (!g_wide) {
+        hipLaunchKernelGGL(single, grid, dim3(bs), 0, st, a);
+        return;
+    }
+    WideGroup& g = g_wide->groups[g_wide->stage * 8 + g_wide->sub++];
+    if (!g.kernel) {
+        g.kernel = (const void*)single, g.bs = bs;
+        g.launch = [wide, bs](hipStream_t s, const uint8_t* args, const int32_t* start, int32_t n, int32_t total) {
+            hipLaunchKernelGGL(wide, dim3(total), dim3(bs), 0, s, reinterpret_cast<const A*>(args), start, n);
+        };
+    } else if (g.kernel != (const void*)single || g.bs != bs) {
+        g.mixed = true;
+    }
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(&a);
+    g.args.insert(g.args.end(), p, p + sizeof(A));
+    const int32_t blocks = (int32_t)(grid.x * grid.y);
+    g.start.push_back(g.start.back() + ((blocks + 7) & ~7));
+}
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min<size_t>(blocks_for(n, 256), 4096)), dim3(256), 0, st, p, n, v);
 }
 void launch_frame_clear(hipStream_t st, uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_t n_vis) {
-    hipLaunchKernelGGL(k_frame_clear, dim3(std::min<size_t>(blocks_for(std::max<size_t>(n_vis, n_cnt), 256), 4096)),
-                       dim3(256), 0, st, cnt, n_cnt, vis, n_vis);
+    go(k_frame_clear, k_frame_clear_wide, dim3(std::min<size_t>(blocks_for(std::max<size_t>(n_vis, n_cnt), 256), 4096)),
+       256, st, k_frame_clear_args{cnt, n_cnt, vis, n_vis, 0, 0});
 }
 void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
     if (!n) return;
@@ -1609,78 +2121,78 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
 void launch_vertex_main(hipStream_t st, const DevScene& s, const M4* trans, const M4* normal, const ViewConsts& vc,
                         float4* tv_scr, float4* tv_nor, float4* tv_wp) {
     if (!s.n_tverts) return;
-    hipLaunchKernelGGL(k_vertex_main, dim3(blocks_for(s.n_tverts, 256)), dim3(256), 0, st, s, trans, normal, vc, tv_scr,
-                       tv_nor, tv_wp);
+    go(k_vertex_main, k_vertex_main_wide, dim3(blocks_for(s.n_tverts, 256)), 256, st,
+       k_vertex_main_args{s, trans, normal, vc, tv_scr, tv_nor, tv_wp, 0, 0});
 }
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float4* ts) {
     if (!s.n_tverts || !sc.n_slots) return;
-    hipLaunchKernelGGL(k_vertex_shadow, dim3(blocks_for(s.n_tverts, 256), sc.n_slots), dim3(256), 0, st, s, strans, sc,
-                       ts);
+    go(k_vertex_shadow, k_vertex_shadow_wide, dim3(blocks_for(s.n_tverts, 256), sc.n_slots), 256, st,
+       k_vertex_shadow_args{s, strans, sc, ts, 0, 0});
 }
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
                        unsigned long long* vis, bool draw_small) {
     if (!s.n_tris) return;
-    hipLaunchKernelGGL(k_setup_main, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, vc, tv_scr, prims, pbox,
-                       clip_list, tile_count, blk_prims, cnt, cap, vis, (int32_t)draw_small);
-    hipLaunchKernelGGL(k_clip, dim3(std::min<size_t>(blocks_for(cap.clip_tris, 64), CLIP_GRID)), dim3(64), 0, st, s, vc,
-                       tv_scr, tv_nor, clip_list,
-                       prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
+    go(k_setup_main, k_setup_main_wide, dim3(blocks_for(s.n_tris, 256)), 256, st,
+       k_setup_main_args{s, vc, tv_scr, prims, pbox, clip_list, tile_count, blk_prims, cnt, cap, vis,
+                         (int32_t)draw_small, 0, 0});
+    go(k_clip, k_clip_wide, dim3(std::min<size_t>(blocks_for(cap.clip_tris, 64), CLIP_GRID)), 64, st,
+       k_clip_args{s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap, 0, 0});
 }
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
                          uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap, uint32_t* smaps) {
     if (!s.n_tris || !sc.n_slots) return;
-    hipLaunchKernelGGL(k_setup_shadow, dim3(blocks_for(s.n_tris, 256), sc.n_slots), dim3(256), 0, st, s, sc, ts, prims,
-                       pbox, tile_count, cnt, cap, smaps);
+    go(k_setup_shadow, k_setup_shadow_wide, dim3(blocks_for(s.n_tris, 256), sc.n_slots), 256, st,
+       k_setup_shadow_args{s, sc, ts, prims, pbox, tile_count, cnt, cap, smaps, 0, 0});
 }
 int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK); }
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
                  int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
                  int32_t max_clip_subs, Counters* cnt) {
     const int nb = scan_blocks(n);
-    if (nb > 1) hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(1024), 0, st, counts, n, bsum);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, st, counts, n, bsum, offsets, chunk_off, cursor,
-                       item_tile, cap_items, blk_prims, n_blk, max_clip_subs, cnt);
+    // (the sums stage always records, so the apply stage keeps its place)
+    if (g_wide) g_wide->sub = 0;
+    if (nb > 1) go(k_scan_sums, k_scan_sums_wide, dim3(nb), 1024, st, k_scan_sums_args{counts, n, bsum, 0, 0});
+    if (g_wide) g_wide->sub = 1;
+    go(k_scan_apply, k_scan_apply_wide, dim3(nb), 1024, st,
+       k_scan_apply_args{counts, n, bsum, offsets, chunk_off, cursor, item_tile, cap_items, blk_prims, n_blk,
+                         max_clip_subs, cnt, 0, 0});
 }
 void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris) {
     const int max_prims = n_fixed + (shadow ? 0 : cap.clip_subs);
     if (max_prims <= 0) return;
-    hipLaunchKernelGGL(k_bin, dim3(blocks_for(max_prims, 256)), dim3(256), 0, st, pbox, cnt, tiles_x, tiles_per_view,
-                       shadow, cursor, list, cap, n_fixed, n_tris);
+    go(k_bin, k_bin_wide, dim3(blocks_for(max_prims, 256)), 256, st,
+       k_bin_args{pbox, cnt, tiles_x, tiles_per_view, shadow, cursor, list, cap, n_fixed, n_tris, 0, 0});
 }
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
                    Counters* cnt, const RasterTarget& rt) {
-    if (mode == MODE_SHADOW)
-        hipLaunchKernelGGL(k_raster<MODE_SHADOW>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
-                           item_tile, tile_list, cnt, rt, cap_items);
-    else if (mode == MODE_EXACT)
-        hipLaunchKernelGGL(k_raster<MODE_EXACT>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
-                           item_tile, tile_list, cnt, rt, cap_items);
-    else
-        hipLaunchKernelGGL(k_raster<MODE_VIS>, dim3(n_workgroups), dim3(256), 0, st, prims, tile_off, chunk_off,
-                           item_tile, tile_list, cnt, rt, cap_items);
+    const k_raster_args a{prims, tile_off, chunk_off, item_tile, tile_list, cnt, rt, cap_items, 0, 0};
+    if (mode == MODE_SHADOW) go(k_raster<MODE_SHADOW>, k_raster_wide<MODE_SHADOW>, dim3(n_workgroups), 256, st, a);
+    else if (mode == MODE_EXACT) go(k_raster<MODE_EXACT>, k_raster_wide<MODE_EXACT>, dim3(n_workgroups), 256, st, a);
+    else go(k_raster<MODE_VIS>, k_raster_wide<MODE_VIS>, dim3(n_workgroups), 256, st, a);
 }
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
                     uint8_t* ao_need) {
     const size_t P = (size_t)rt.W * rt.H;
-    if (exact) hipLaunchKernelGGL(k_resolve<true>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth, s, ao_need);
-    else hipLaunchKernelGGL(k_resolve<false>, dim3(blocks_for(P, 256)), dim3(256), 0, st, rt, cnt, depth, s, ao_need);
+    const k_resolve_args a{rt, cnt, depth, s, ao_need, 0, 0};
+    if (exact) go(k_resolve<true>, k_resolve_wide<true>, dim3(blocks_for(P, 256)), 256, st, a);
+    else go(k_resolve<false>, k_resolve_wide<false>, dim3(blocks_for(P, 256)), 256, st, a);
 }
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const int nb = ((c.v.W + 63) / 64) * c.v.H;
-    hipLaunchKernelGGL(k_ao, dim3(((nb + 7) / 8) * 8), dim3(256), 0, st, c, b.ao_need, b.depth, b.ao, b.ao_tab,
-                       b.ao_fast, b.cnt);
+    go(k_ao, k_ao_wide, dim3(((nb + 7) / 8) * 8), 256, st,
+       k_ao_args{c, b.ao_need, b.depth, b.ao, b.ao_tab, b.ao_fast, b.cnt, 0, 0});
 }
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
-    hipLaunchKernelGGL(k_shade_bg, dim3(1), dim3(64), 0, st, s, c, b);
+    go(k_shade_bg, k_shade_bg_wide, dim3(1), 64, st, k_shade_bg_args{s, c, b, 0, 0});
 }
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     int n_tiles = c.v.tiles_x * c.v.tiles_y;
     int per_xcd = (n_tiles + 7) / 8;
-    hipLaunchKernelGGL(k_shade, dim3(per_xcd * 8), dim3(256), 0, st, s, c, b);
+    go(k_shade, k_shade_wide, dim3(per_xcd * 8), 256, st, k_shade_args{s, c, b, 0, 0});
 }
 
 void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
@@ -1691,8 +2203,8 @@ void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c
 void launch_resize(hipStream_t st, const uint32_t* in, int32_t in_w, uint32_t* out, int32_t out_w, int32_t out_rows,
                    const int16_t* coeffs, const int32_t* start, int32_t fl) {
     if (out_w <= 0 || out_rows <= 0) return;
-    hipLaunchKernelGGL(k_resize, dim3(blocks_for(out_w, 256), out_rows), dim3(256), 0, st, in, in_w, out, out_w,
-                       out_rows, coeffs, start, fl);
+    go(k_resize, k_resize_wide, dim3(blocks_for(out_w, 256), out_rows), 256, st,
+       k_resize_args{in, in_w, out, out_w, out_rows, coeffs, start, fl, 0, 0});
 }
 
 }  // namespace prh

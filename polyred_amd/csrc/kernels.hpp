@@ -145,6 +145,16 @@ void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, co
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
                          uint8_t* out);
+
+// Wide launches: many views' frames as one launch per stage (kernels.hip).
+struct WideBatch;
+WideBatch* wide_new();
+void wide_free(WideBatch* w);
+void wide_record(WideBatch* w);  // launch_* calls record into w (nullptr: launch again)
+void wide_stage(int stage);      // the stage of the following launch_* calls
+size_t wide_layout(WideBatch* w);  // device bytes of the argument arrays (0: unusable)
+int wide_upload(const WideBatch* w, uint8_t* dev);
+void wide_launch(const WideBatch* w, hipStream_t st, const uint8_t* dev);
 // out[y][x] (out_rows x out_w) = 8-bit bilinear filter of row x of `in` at y
 void launch_resize(hipStream_t st, const uint32_t* in, int32_t in_w, uint32_t* out, int32_t out_w, int32_t out_rows,
                    const int16_t* coeffs, const int32_t* start, int32_t fl);

@@ -339,6 +339,12 @@ struct prh_ctx {
     hipGraph_t graph;
     hipGraphExec_t graph_exec;
     std::vector<std::pair<prh_renderer*, int>> graph_key;
+    bool graph_wide;  // the graph replays wide launches
+    // the last batch recorded for wide launches: its argument tables on the
+    // device, keyed as the graph
+    WideBatch* wide;
+    uint8_t* wide_dev;
+    std::vector<std::pair<prh_renderer*, int>> wide_key;
 };
 
 struct prh_scene {
@@ -977,11 +983,10 @@ int prh_renderer_free(prh_renderer* r) {
         return fail(PRH_E_INVALID, "prh_renderer_free: " + std::to_string(r->live_fits) + " fit(s) still use it");
     (void)hipSetDevice(r->ctx->device);
     (void)hipStreamSynchronize(r->ctx->stream);
-    for (const auto& kv : r->ctx->graph_key)
-        if (kv.first == r) {
-            drop_graph(r->ctx);
-            break;
-        }
+    bool held = false;
+    for (const auto& kv : r->ctx->graph_key) held |= kv.first == r;
+    for (const auto& kv : r->ctx->wide_key) held |= kv.first == r;
+    if (held) drop_graph(r->ctx);
     free_list(r->allocs);
     if (r->h_cnt) (void)hipHostFree(r->h_cnt);
     for (int i = 0; i < 8; i++)
@@ -1016,37 +1021,49 @@ ShadeBuffers shade_buffers(const prh_renderer* r) {
 
 // Enqueues one frame; nan_mode re-runs the visibility raster in exact order.
 // `timed` records the per-pass events (not while a batch is captured into a
-// graph).
-void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
+// graph).  `wide`: the launches are being recorded into a wide batch (the
+// stage numbers order its launches; the persistent rasters take fewer
+// workgroups per view, since every view's run together).
+void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bool wide = false) {
     const DevScene& d = r->sc->d;
     Counters* cm = r->cnt;
     Counters* cs = r->cnt + 1;
-    const int persistent = r->n_cu * 4;
+    const int persistent = wide ? std::max(32, r->n_cu / 2) : r->n_cu * 4;
     if (timed) (void)hipEventRecord(r->ev[0], st);
     const size_t P = (size_t)r->W * r->H;
     // counters zeroed, visibility emptied (before the setup, which draws
     // small triangles into it, outside the exact-order pass, whose NaN rule
     // needs every fragment through the ordered raster)
+    wide_stage(1);
     launch_frame_clear(st, (uint32_t*)r->cnt, (int32_t)(2 * sizeof(Counters) / 4), (uint64_t*)r->vis, P);
     if (r->n_slots > 0 && !nan_mode) {
         const int nt = r->n_slots * r->stiles_per_map;
+        wide_stage(2);
         launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
+        wide_stage(3);
         launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->pbox, r->tile_count, cs, r->cap_shadow, r->smaps);
+        wide_stage(4);
         launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
                     r->cap_items, nullptr, 0, 0, cs);
+        wide_stage(5);
         launch_bin(st, r->pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
                    r->n_slots * d.n_tris, d.n_tris);
         RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
+        wide_stage(6);
         launch_raster(st, 2, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                       r->tile_list, cs, rt);
     }
     if (timed) (void)hipEventRecord(r->ev[1], st);
+    wide_stage(7);
     launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
     if (timed) (void)hipEventRecord(r->ev[2], st);
+    wide_stage(8);
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
                       r->tile_count, r->blk_prims, cm, r->cap_main, (unsigned long long*)r->vis, !nan_mode);
+    wide_stage(9);
     launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
                 r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
+    wide_stage(10);
     launch_bin(st, r->pbox, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
                d.n_tris);
     if (timed) (void)hipEventRecord(r->ev[3], st);
@@ -1056,19 +1073,26 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed) {
     }
     RasterTarget rt{r->W, r->H, r->tiles_x, r->n_tiles, r->n_tiles, (unsigned long long*)r->vis, r->first_all,
                     r->first_nan, nullptr};
+    wide_stage(11);
     launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                   r->tile_list, cm, rt);
+    wide_stage(12);
     launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr);
     if (timed) (void)hipEventRecord(r->ev[4], st);
     const ShadeBuffers b = shade_buffers(r);
+    wide_stage(13);
     if (r->sc->any_ao) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
     if (timed) (void)hipEventRecord(r->ev[5], st);
+    wide_stage(14);
     launch_shade_bg(st, d, r->sh, b);
     if (timed) (void)hipEventRecord(r->ev[7], st);
+    wide_stage(15);
     launch_shade(st, d, r->sh, b);
     if (r->msaa > 1) {
         // passAntialiasing (render/raster.go:361-378): Resize after gamma
+        wide_stage(16);
         launch_resize(st, r->image_big, r->W, r->resize_tmp, r->H, r->OW, r->rz_cx, r->rz_ox, r->rz_flx);
+        wide_stage(17);
         launch_resize(st, r->resize_tmp, r->H, r->image, r->OW, r->OH, r->rz_cy, r->rz_oy, r->rz_fly);
     }
     if (timed) (void)hipEventRecord(r->ev[6], st);
@@ -1131,11 +1155,24 @@ int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
     return PRH_OK;
 }
 
-void drop_graph(prh_ctx* ctx) {
+void drop_graph_exec(prh_ctx* ctx) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->graph_exec = nullptr, ctx->graph = nullptr;
     ctx->graph_key.clear();
+}
+
+// The cached graph and the wide tables (a wide graph reads them).
+void drop_graph(prh_ctx* ctx) {
+    drop_graph_exec(ctx);
+    if (ctx->wide) wide_free(ctx->wide);
+    if (ctx->wide_dev) {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (int k = 0; k < ctx->nbs; k++) (void)hipStreamSynchronize(ctx->bst[k]);
+        (void)hipFree(ctx->wide_dev);
+    }
+    ctx->wide = nullptr, ctx->wide_dev = nullptr;
+    ctx->wide_key.clear();
 }
 
 // Stream of each view of a batch: largest estimated cost first, each to the
@@ -1172,23 +1209,64 @@ int batch_streams(prh_ctx* ctx) {
     return PRH_OK;
 }
 
-// Captures one frame of every renderer, spread over the batch streams, into
-// a graph (kept until the batch, or a renderer's work buffers, change).
-int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n) {
+// Records one frame of every renderer as wide launches and uploads their
+// argument tables (kept until the batch, or a renderer's work buffers,
+// change).
+int batch_wide(prh_ctx* ctx, prh_renderer* const* rs, int n) {
     std::vector<std::pair<prh_renderer*, int>> key(n);
     for (int i = 0; i < n; i++) key[i] = {rs[i], rs[i]->work_gen};
-    if (ctx->graph_exec && key == ctx->graph_key) return PRH_OK;
+    if (ctx->wide && key == ctx->wide_key) return PRH_OK;
     drop_graph(ctx);
+    WideBatch* w = wide_new();
+    wide_record(w);
+    for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[0], false, true);
+    wide_record(nullptr);
+    const size_t bytes = wide_layout(w);
+    if (!bytes) {
+        wide_free(w);
+        return fail(PRH_E_HIP, "wide batch: a stage mixes kernels");
+    }
+    uint8_t* dev = nullptr;
+    if (hipMalloc((void**)&dev, bytes) != hipSuccess) {
+        wide_free(w);
+        return fail(PRH_E_NOMEM, "wide batch: hipMalloc failed");
+    }
+    if (wide_upload(w, dev)) {
+        (void)hipFree(dev);
+        wide_free(w);
+        return fail(PRH_E_HIP, "wide batch: upload failed");
+    }
+    ctx->wide = w, ctx->wide_dev = dev, ctx->wide_key = key;
+    return PRH_OK;
+}
+
+// Captures one frame of every renderer into a graph (kept until the batch,
+// a renderer's work buffers or the mode change): the wide launches, or each
+// view's own launches spread over the batch streams.
+int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n, bool wide) {
+    std::vector<std::pair<prh_renderer*, int>> key(n);
+    for (int i = 0; i < n; i++) key[i] = {rs[i], rs[i]->work_gen};
+    if (ctx->graph_exec && key == ctx->graph_key && ctx->graph_wide == wide) return PRH_OK;
+    drop_graph_exec(ctx);
+    if (wide) {
+        int rc = batch_wide(ctx, rs, n);
+        if (rc) return rc;
+    }
     hipStream_t s0 = ctx->bst[0];
     HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
-    hipError_t e = hipEventRecord(ctx->bfork, s0);
-    for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) e = hipStreamWaitEvent(ctx->bst[k], ctx->bfork, 0);
-    if (e == hipSuccess) {
-        const std::vector<int> lane = batch_lanes(rs, n);
-        for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[lane[i]], false);
-        for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) {
-            e = hipEventRecord(ctx->bjoin[k], ctx->bst[k]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(s0, ctx->bjoin[k], 0);
+    hipError_t e = hipSuccess;
+    if (wide) {
+        wide_launch(ctx->wide, s0, ctx->wide_dev);
+    } else {
+        e = hipEventRecord(ctx->bfork, s0);
+        for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) e = hipStreamWaitEvent(ctx->bst[k], ctx->bfork, 0);
+        if (e == hipSuccess) {
+            const std::vector<int> lane = batch_lanes(rs, n);
+            for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[lane[i]], false);
+            for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) {
+                e = hipEventRecord(ctx->bjoin[k], ctx->bst[k]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(s0, ctx->bjoin[k], 0);
+            }
         }
     }
     hipGraph_t g = nullptr;
@@ -1205,6 +1283,7 @@ int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n) {
     }
     ctx->graph = g;
     ctx->graph_key = key;
+    ctx->graph_wide = wide;
     return PRH_OK;
 }
 
@@ -1456,15 +1535,19 @@ int prh_render_batch(prh_renderer* const* rs, int32_t n, int32_t flags, float* m
     HIP_TRY(hipSetDevice(ctx->device));
     int rc = batch_streams(ctx);
     if (rc) return rc;
+    if (flags & ~(PRH_BATCH_GRAPH | PRH_BATCH_STREAMS)) return fail(PRH_E_INVALID, "prh_render_batch: unknown flags");
     const bool graph = (flags & PRH_BATCH_GRAPH) != 0;
+    const bool wide = (flags & PRH_BATCH_STREAMS) == 0;
     hipStream_t s0 = ctx->bst[0];
-    if (graph) {
-        rc = batch_graph(ctx, rs, n);
-        if (rc) return rc;
-    }
+    if (graph) rc = batch_graph(ctx, rs, n, wide);
+    else if (wide) rc = batch_wide(ctx, rs, n);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->bev[0], s0));
     if (graph) {
         HIP_TRY(hipGraphLaunch(ctx->graph_exec, s0));
+    } else if (wide) {
+        wide_launch(ctx->wide, s0, ctx->wide_dev);
+        HIP_TRY(hipGetLastError());
     } else {
         // fork: every batch stream starts after the work already on s0
         for (int k = 1; k < ctx->nbs; k++) HIP_TRY(hipStreamWaitEvent(ctx->bst[k], ctx->bev[0], 0));
@@ -1486,9 +1569,9 @@ int prh_render_batch(prh_renderer* const* rs, int32_t n, int32_t flags, float* m
     // work buffers (or met a NaN depth) is rendered again on its own
     for (int i = 0; i < n; i++) {
         prh_renderer* r = rs[i];
-        if (graph) r->has_frame = false;
+        if (graph || wide) r->has_frame = false;
         bool again = false;
-        rc = finish_frame(r, 0, again, !graph);
+        rc = finish_frame(r, 0, again, !graph && !wide);
         if (rc) return rc;
         if (again && (rc = prh_render(r))) return rc;
     }

@@ -309,19 +309,21 @@ def _close_all() -> None:
             pass
 
 
-def render_batch(renderers, graph: bool = False) -> float:
+def render_batch(renderers, graph: bool = False, streams: bool = False) -> float:
     """One frame of every renderer (all on one Device) in one batched
-    submission (prh_render_batch); with graph=True the batch is captured
-    into a HIP graph once and replayed.  Returns the batch's device time in
-    ms.  Each renderer then holds the frame a Render() would have drawn."""
+    submission (prh_render_batch): one launch per pass over every view, or
+    with streams=True each view's own launches over the batch streams; with
+    graph=True the batch is captured into a HIP graph once and replayed.
+    Returns the batch's device time in ms.  Each renderer then holds the
+    frame a Render() would have drawn."""
     rs = list(renderers)
     if not rs:
         return 0.0
     lib = rs[0]._lib
     arr = (C.c_void_p * len(rs))(*[r.handle for r in rs])
     ms = C.c_float()
-    abi.check(lib.prh_render_batch(arr, len(rs), abi.PRH_BATCH_GRAPH if graph else 0, C.byref(ms)),
-              "prh_render_batch")
+    flags = (abi.PRH_BATCH_GRAPH if graph else 0) | (abi.PRH_BATCH_STREAMS if streams else 0)
+    abi.check(lib.prh_render_batch(arr, len(rs), flags, C.byref(ms)), "prh_render_batch")
     for r in rs:
         for name in ("shadow", "forward", "deferred", "gamma"):
             r.passGPU[name] = True
