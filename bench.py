@@ -252,6 +252,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batched measurement")
     ap.add_argument("--no-fit", action="store_true", help="skip the batched inverse-rendering measurement")
     ap.add_argument("--small", action="store_true", help="reduced scene for a quick check")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default knobs instead of prh_autotune's choice (untimed, before the warm-up)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run 3 elementwise copies of MIB MiB (torch.neg), the known-size kernel that "
                          "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -280,6 +282,11 @@ def main() -> int:
         scene = scenes.c2_scale()
     dev = render.Device(0)
     r = render.renderer_for(scene, dev)
+    tuned = None
+    if not args.no_autotune:
+        t = r.Autotune(frames=5)  # occupancy, bin capacity, small-triangle thresholds; results unchanged
+        tuned = {"raster_wg_per_cu": t.raster_wg_per_cu, "item_prims": t.item_prims,
+                 "shadow_direct_max": t.shadow_direct_max, "main_direct_max": t.main_direct_max}
     for _ in range(args.warmup):
         r.render_device()
     acc = dict(total=0.0, shadow=0.0, vertex=0.0, setup=0.0, raster=0.0, ao=0.0, shade=0.0)
@@ -341,6 +348,7 @@ def main() -> int:
         "roofline_shade": dict(roof("k_shade", sb["bytes"], ms["shade"]), covered_px=sb["covered"],
                                visible_tris=sb["visible_tris"], visible_verts=sb["visible_verts"]),
         "frame": {"prims": st.n_prims, "clipped": st.n_clipped, "bin_pairs": st.n_bin_pairs},
+        "tuning": tuned,
     }
     r.free()
     if not args.no_adjoint:

@@ -318,6 +318,55 @@ int prh_fit_run(prh_fit* f, const prh_diff_params* start, const prh_fit_options*
                 double* loss_history);
 int prh_fit_last_stats(prh_fit* f, prh_fit_stats* out);
 
+/* ---- headless present (gpu/surface.go:1-183) --------------------------
+ * gpu.Surface's headless swapchain: a ring of `frames` RGBA8 textures in HBM
+ * (top row first, as prh_read_image), one per frame, that a renderer draws
+ * into; the seam an on-screen present builds on. */
+typedef struct prh_surface prh_surface;
+/* Device.CreateSurface (surface.go:43-57): frames <= 0 means 2. */
+int prh_surface_new(prh_ctx* ctx, int32_t width, int32_t height, int32_t frames, prh_surface** out);
+int prh_surface_free(prh_surface* s); /* Release */
+/* AcquireNextTexture (surface.go:102-110): *index (may be NULL) receives the
+ * ring slot the next frame is drawn into. */
+int prh_surface_acquire(prh_surface* s, int32_t* index);
+/* One prh_render of r whose final image is written into the acquired
+ * texture instead of r's own image (r's output size must be the surface's). */
+int prh_surface_render(prh_surface* s, prh_renderer* r);
+/* PresentImage's upload (surface.go:133-137): rgba into the acquired texture. */
+int prh_surface_write(prh_surface* s, const uint8_t* rgba, size_t bytes);
+/* Present (surface.go:115-129): completes the acquired frame; an error
+ * without an acquire. */
+int prh_surface_present(prh_surface* s);
+/* Texture().ReadPixels(): the most recently presented frame's pixels. */
+int prh_surface_read(prh_surface* s, uint8_t* rgba, size_t bytes);
+/* Resize (surface.go:163-177): new textures, the ring restarts. */
+int prh_surface_resize(prh_surface* s, int32_t width, int32_t height);
+int prh_surface_size(prh_surface* s, int32_t* width, int32_t* height);
+
+/* ---- autotuning (no counterpart in polyred) ------------------------------
+ * Knobs that change how fast a frame renders but not what it renders: every
+ * merge they touch is order-independent, so frames are bit-identical under
+ * any setting.  The tile is fixed at 16x16: one 256-thread workgroup holds a
+ * tile with one pixel per thread, and the shading's 8x8 wave blocks and the
+ * AO's row segments are laid out on it. */
+typedef struct prh_tuning {
+    int32_t raster_wg_per_cu;  /* persistent raster workgroups per CU (occupancy), 1..16; default 4        */
+    int32_t item_prims;        /* primitives per raster work item (bin capacity), a power of 2 in
+                                  64..8192; default 1024                                                   */
+    int32_t shadow_direct_max; /* shadow-map triangles of at most this many texel centres are drawn by
+                                  the setup, 0..256; default 64                                            */
+    int32_t main_direct_max;   /* main-view triangles of at most this many pixel centres, likewise; 48  */
+    float ms;                  /* prh_autotune: the chosen configuration's median frame time             */
+} prh_tuning;
+int prh_get_tuning(prh_renderer* r, prh_tuning* out);
+int prh_set_tuning(prh_renderer* r, const prh_tuning* t);
+/* Coordinate descent over item_prims {256..4096}, raster_wg_per_cu {1, 2, 4,
+ * 8}, shadow_direct_max {0, 16, 32, 64, 128} and main_direct_max {0, 24, 48,
+ * 96}: each configuration timed as the median of `frames` frames (HIP
+ * events, after one warm-up frame); the fastest stays on r and goes to *out
+ * (may be NULL).  r holds a frame of it on return. */
+int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out);
+
 /* Library build identification ("libprh gfx950 ..."). */
 const char* prh_version(void);
 

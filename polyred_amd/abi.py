@@ -160,6 +160,11 @@ PRH_FIT_ALBEDO, PRH_FIT_KD, PRH_FIT_KS, PRH_FIT_SHININESS, PRH_FIT_LIGHT_POS, PR
 PRH_FIT_STORED_GBUFFER = 256
 
 
+class prh_tuning(C.Structure):
+    _fields_ = [("raster_wg_per_cu", C.c_int32), ("item_prims", C.c_int32), ("shadow_direct_max", C.c_int32),
+                ("main_direct_max", C.c_int32), ("ms", C.c_float)]
+
+
 class prh_fit_options(C.Structure):
     _fields_ = [("steps", C.c_int32), ("groups", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("eps", C.c_float)]
@@ -197,6 +202,18 @@ PROTOTYPES = {
     "prh_fit_run": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.POINTER(prh_fit_options),
                               C.POINTER(prh_diff_grads), C.c_void_p]),
     "prh_fit_last_stats": (C.c_int, [C.c_void_p, C.POINTER(prh_fit_stats)]),
+    "prh_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(prh_tuning)]),
+    "prh_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(prh_tuning)]),
+    "prh_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(prh_tuning)]),
+    "prh_surface_new": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "prh_surface_free": (C.c_int, [C.c_void_p]),
+    "prh_surface_acquire": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "prh_surface_render": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "prh_surface_write": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "prh_surface_present": (C.c_int, [C.c_void_p]),
+    "prh_surface_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "prh_surface_resize": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "prh_surface_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
 }
 
 _lib = None

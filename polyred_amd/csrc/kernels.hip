@@ -406,7 +406,8 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
         }
         const DPrim& d = sp[lo];
         const int l = idx - start[lo], w = d.w;
-        // l / w by multiply-shift, exact for l, w <= 64
+        // l / w by multiply-shift: exact while l * w < 65536, which l < area <= 256
+        // and w <= area keep (the thresholds are at most 256)
         const int q = (int)(((uint32_t)l * ((65536u + (uint32_t)w - 1u) / (uint32_t)w)) >> 16);
         const int px = (d.xy0 & 0xffff) + (l - q * w), py = (d.xy0 >> 16) + q;
         TriSetup t;
@@ -418,12 +419,12 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
     }
 }
 
-// An unclipped triangle whose pixel box holds at most this many pixel
-// centres is drawn by the setup itself outside the exact-order pass: each
+// An unclipped triangle whose pixel box holds at most `draw_small` pixel
+// centres (0: none; prh_tuning.main_direct_max) is drawn by the setup itself
+// outside the exact-order pass: each
 // centre is tested as k_raster<VIS> tests it and merged into the visibility
 // buffer with a global 64-bit atomic min of the same key (order-independent),
 // and the triangle is neither stored nor binned.
-constexpr int MAIN_DIRECT_MAX = 48;
 
 struct k_setup_main_args {
     DevScene s;
@@ -480,7 +481,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
             }
         }
         const uint32_t key = (uint32_t)g * 8u + SUB_DIRECT;
-        if (direct && draw_small && (x1 - x0 + 1) * (y1 - y0 + 1) <= MAIN_DIRECT_MAX) {
+        if (direct && draw_small > 0 && (x1 - x0 + 1) * (y1 - y0 + 1) <= draw_small) {
             // (a thread draws its own: the main view's boxes vary little)
             drawn = true;
             const TriSetup ts = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
@@ -646,13 +647,13 @@ __global__ __launch_bounds__(64) void k_clip_wide(const k_clip_args* __restrict_
 
 // render/shadow.go:152-176 (drawDepth up to the pixel loop) for every
 // (triangle, shadow slot): back-face cull, cullViewFrustum, pixel box.
-// A shadow-map triangle whose pixel box holds at most this many texel
-// centres is drawn here: each centre is tested and merged into the map with a
+// A shadow-map triangle whose pixel box holds at most sc.direct_max texel
+// centres (prh_tuning.shadow_direct_max) is drawn here: each centre is
+// tested and merged into the map with a
 // global atomic max, exactly as k_raster<SHADOW> would (the same Barycoord,
 // depth and max over ordered bits; order-independent), and it is neither
 // stored nor binned.  Most triangles of a map fitted to the whole frustum
 // cover a few texels; past a few dozen the binned raster is no slower.
-constexpr int SHADOW_DIRECT_MAX = 64;
 
 struct k_setup_shadow_args {
     DevScene s;
@@ -698,7 +699,7 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
             }
         }
     }
-    const bool small = emit && (x1 - x0 + 1) * (y1 - y0 + 1) <= SHADOW_DIRECT_MAX;
+    const bool small = emit && (x1 - x0 + 1) * (y1 - y0 + 1) <= sc.direct_max;
     if (small) emit = false;
     {
         __shared__ DPrim sp[256];
@@ -749,15 +750,17 @@ __device__ __forceinline__ int wave_sum(int v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
-__device__ __forceinline__ int item_count(int v) { return (v + ITEM_PRIMS - 1) / ITEM_PRIMS; }
+__device__ __forceinline__ int item_count(int v, int ip) { return (v + ip - 1) / ip; }
 
 struct k_scan_sums_args {
     const int32_t* counts;
     int32_t n;
     int32_t* bsum;
+    int32_t item_prims;
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_scan_sums_body(const k_scan_sums_args& a_, const VB vb) {
+    const int32_t item_prims = a_.item_prims;
     const int32_t* __restrict__ counts = a_.counts;
     const int32_t n = a_.n;
     int32_t* __restrict__ bsum = a_.bsum;
@@ -769,7 +772,7 @@ __device__ __forceinline__ void k_scan_sums_body(const k_scan_sums_args& a_, con
     for (int k = 0; k < 4; k++) {
         const int v = i0 + k < n ? counts[i0 + k] : 0;
         sv += v;
-        sc += item_count(v);
+        sc += item_count(v, item_prims);
     }
     sv = wave_sum(sv), sc = wave_sum(sc);
     if (lane == 0) ws[wv] = sv, wc[wv] = sc;
@@ -808,9 +811,11 @@ struct k_scan_apply_args {
     int32_t n_blk;
     int32_t max_clip_subs;
     Counters* cnt;
+    int32_t item_prims;
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, const VB vb) {
+    const int32_t item_prims = a_.item_prims;
     int32_t* __restrict__ counts = a_.counts;
     const int32_t n = a_.n;
     const int32_t* __restrict__ bsum = a_.bsum;
@@ -844,7 +849,7 @@ __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, c
     int v[4], ch[4];
     for (int k = 0; k < 4; k++) {
         v[k] = i0 + k < n ? counts[i0 + k] : 0;
-        ch[k] = item_count(v[k]);
+        ch[k] = item_count(v[k], item_prims);
     }
     const int local = v[0] + v[1] + v[2] + v[3], localc = ch[0] + ch[1] + ch[2] + ch[3];
     const int incl = wave_incl_scan(local, lane), inclc = wave_incl_scan(localc, lane);
@@ -1049,10 +1054,12 @@ struct k_raster_args {
     Counters* cnt;
     RasterTarget rt;
     int32_t cap_items;
+    int32_t item_prims;
     int32_t gx, gy;  // the grid of this launch
 };
 template <int MODE>
 __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB vb) {
+    const int32_t item_prims = a_.item_prims;
     const Prim* __restrict__ prims = a_.prims;
     const int32_t* __restrict__ tile_off = a_.tile_off;
     const int32_t* __restrict__ chunk_off = a_.chunk_off;
@@ -1081,8 +1088,8 @@ __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB 
         const int px0 = (t % rt.tiles_x) * TILE, py0 = (t / rt.tiles_x) * TILE;
         const int first_item = chunk_off[tile];
         const bool shared_tile = chunk_off[tile + 1] - first_item > 1;
-        const int beg = tile_off[tile] + (item - first_item) * ITEM_PRIMS;
-        const int n = min(ITEM_PRIMS, tile_off[tile + 1] - beg);
+        const int beg = tile_off[tile] + (item - first_item) * item_prims;
+        const int n = min(item_prims, tile_off[tile + 1] - beg);
         if (SHADOW) smax[tid] = depth_ord(0.0f);
         else svis[tid] = VIS_EMPTY;
         if (EXACT) sfirst[tid] = 0xffffffffu, snan[tid] = 0xffffffffu;
@@ -2132,11 +2139,11 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, c
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
-                       unsigned long long* vis, bool draw_small) {
+                       unsigned long long* vis, int32_t draw_small) {
     if (!s.n_tris) return;
     go(k_setup_main, k_setup_main_wide, dim3(blocks_for(s.n_tris, 256)), 256, st,
-       k_setup_main_args{s, vc, tv_scr, prims, pbox, clip_list, tile_count, blk_prims, cnt, cap, vis,
-                         (int32_t)draw_small, 0, 0});
+       k_setup_main_args{s, vc, tv_scr, prims, pbox, clip_list, tile_count, blk_prims, cnt, cap, vis, draw_small, 0,
+                         0});
     go(k_clip, k_clip_wide, dim3(std::min<size_t>(blocks_for(cap.clip_tris, 64), CLIP_GRID)), 64, st,
        k_clip_args{s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap, 0, 0});
 }
@@ -2149,15 +2156,16 @@ void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& 
 int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK); }
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
                  int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
-                 int32_t max_clip_subs, Counters* cnt) {
+                 int32_t max_clip_subs, Counters* cnt, int32_t item_prims) {
     const int nb = scan_blocks(n);
     // (the sums stage always records, so the apply stage keeps its place)
     if (g_wide) g_wide->sub = 0;
-    if (nb > 1) go(k_scan_sums, k_scan_sums_wide, dim3(nb), 1024, st, k_scan_sums_args{counts, n, bsum, 0, 0});
+    if (nb > 1)
+        go(k_scan_sums, k_scan_sums_wide, dim3(nb), 1024, st, k_scan_sums_args{counts, n, bsum, item_prims, 0, 0});
     if (g_wide) g_wide->sub = 1;
     go(k_scan_apply, k_scan_apply_wide, dim3(nb), 1024, st,
        k_scan_apply_args{counts, n, bsum, offsets, chunk_off, cursor, item_tile, cap_items, blk_prims, n_blk,
-                         max_clip_subs, cnt, 0, 0});
+                         max_clip_subs, cnt, item_prims, 0, 0});
 }
 void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris) {
@@ -2168,8 +2176,8 @@ void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_
 }
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
-                   Counters* cnt, const RasterTarget& rt) {
-    const k_raster_args a{prims, tile_off, chunk_off, item_tile, tile_list, cnt, rt, cap_items, 0, 0};
+                   Counters* cnt, const RasterTarget& rt, int32_t item_prims) {
+    const k_raster_args a{prims, tile_off, chunk_off, item_tile, tile_list, cnt, rt, cap_items, item_prims, 0, 0};
     if (mode == MODE_SHADOW) go(k_raster<MODE_SHADOW>, k_raster_wide<MODE_SHADOW>, dim3(n_workgroups), 256, st, a);
     else if (mode == MODE_EXACT) go(k_raster<MODE_EXACT>, k_raster_wide<MODE_EXACT>, dim3(n_workgroups), 256, st, a);
     else go(k_raster<MODE_VIS>, k_raster_wide<MODE_VIS>, dim3(n_workgroups), 256, st, a);

@@ -120,7 +120,7 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* stran
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
-                       unsigned long long* vis, bool draw_small);
+                       unsigned long long* vis, int32_t draw_small);
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
                          Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
                          uint32_t* smaps);
@@ -131,13 +131,13 @@ int scan_blocks(int32_t n);
 // Also lists each work item's tile in item_tile and zeroes the counts.
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
                  int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
-                 int32_t max_clip_subs, Counters* cnt);
+                 int32_t max_clip_subs, Counters* cnt, int32_t item_prims);
 void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris);
 // mode: 0 visibility, 1 visibility in exact order (NaN rule), 2 shadow maps
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
-                   Counters* cnt, const RasterTarget& rt);
+                   Counters* cnt, const RasterTarget& rt, int32_t item_prims);
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
                     uint8_t* ao_need);
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);

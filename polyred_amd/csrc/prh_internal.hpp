@@ -92,6 +92,7 @@ struct ShadowConsts {
     int32_t SW, SH, tiles_x, tiles_y;
     int32_t n_slots, tiles_per_map;
     float vpbox_max_x, vpbox_max_y;  // cullViewFrustum's box: (SW*MSAA, SH*MSAA) (render/cull.go:16-22)
+    int32_t direct_max;              // triangles of at most this many texel centres are drawn by the setup
 };
 
 // A rasterisable primitive: a whole triangle (sub = 7) or one clipped fan
@@ -132,7 +133,7 @@ struct Capacities {
 };
 
 constexpr int CHUNK = 256;          // primitives staged in LDS at once
-constexpr int ITEM_PRIMS = 1024;    // primitives per raster work item
+constexpr int ITEM_PRIMS = 1024;    // primitives per raster work item (the default; prh_tuning.item_prims)
 
 // Where a raster pass writes.
 struct RasterTarget {

@@ -328,7 +328,8 @@ constexpr int PRH_BATCH_MAX = 16;
 struct prh_ctx {
     int device;
     hipStream_t stream;
-    int live_scenes;  // a context outlives every scene uploaded to it
+    int live_scenes;    // a context outlives every scene uploaded to it
+    int live_surfaces;  // and every surface
     // prh_render_batch: its streams and events (created on first use) and
     // the graph of the last captured batch, keyed by its renderers and their
     // work-buffer generations
@@ -416,6 +417,8 @@ struct prh_renderer {
     uint32_t frame_serial;  // bumped by every completed frame
     uint32_t count_serial;
     int live_fits;  // a renderer outlives every prh_fit holding it as a view
+    uint32_t* present_to;  // during prh_surface_render: the acquired surface texture
+    prh_tuning tune;       // speed knobs (results do not depend on them)
 };
 
 namespace {
@@ -442,7 +445,7 @@ int alloc_work(prh_renderer* r) {
     if ((rc = ralloc(r, &r->tile_list, (size_t)std::max(r->cap_main.pairs, r->cap_shadow.pairs)))) return rc;
     if ((rc = ralloc(r, &r->clip_list, (size_t)r->cap_main.clip_tris))) return rc;
     // every item holds >= 1 primitive of one tile: items <= tiles + pairs / ITEM_PRIMS
-    r->cap_items = r->tile_slots + std::max(r->cap_main.pairs, r->cap_shadow.pairs) / ITEM_PRIMS + 1;
+    r->cap_items = r->tile_slots + std::max(r->cap_main.pairs, r->cap_shadow.pairs) / r->tune.item_prims + 1;
     if ((rc = ralloc(r, &r->item_tile, (size_t)r->cap_items))) return rc;
     return PRH_OK;
 }
@@ -487,6 +490,8 @@ int prh_close(prh_ctx* ctx) {
     if (!ctx) return fail(PRH_E_INVALID, "prh_close: null context");
     if (ctx->live_scenes > 0)
         return fail(PRH_E_INVALID, "prh_close: " + std::to_string(ctx->live_scenes) + " scene(s) still alive");
+    if (ctx->live_surfaces > 0)
+        return fail(PRH_E_INVALID, "prh_close: " + std::to_string(ctx->live_surfaces) + " surface(s) still alive");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     drop_graph(ctx);
@@ -795,6 +800,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     HIP_TRY(hipSetDevice(ctx->device));
     prh_renderer* r = new prh_renderer();
     r->ctx = ctx, r->sc = s, r->cam = *cam, r->opt = *o;
+    r->tune = prh_tuning{4, ITEM_PRIMS, 64, 48, 0.0f};
     // every pass renders the MSAA-sized buffer (render/raster.go:149-150)
     r->msaa = msaa, r->OW = o->width, r->OH = o->height;
     r->W = o->width * msaa, r->H = o->height * msaa;
@@ -860,6 +866,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     shc.SW = r->SW, shc.SH = r->SH, shc.tiles_x = r->stiles_x, shc.tiles_y = r->stiles_y;
     shc.n_slots = r->n_slots, shc.tiles_per_map = r->stiles_per_map;
     shc.vpbox_max_x = (float)(msaa * r->SW), shc.vpbox_max_y = (float)(msaa * r->SH);
+    shc.direct_max = r->tune.shadow_direct_max;
 
     ShadeConsts& sh = r->sh;
     memset(&sh, 0, sizeof sh);
@@ -1004,12 +1011,16 @@ namespace {
 
 int d_n_tris(const prh_renderer* r) { return r->sc->d.n_tris; }
 
+// Where the frame's final image goes: the renderer's own, or the surface
+// texture a prh_surface_render is drawing into.
+uint32_t* final_image(const prh_renderer* r) { return r->present_to ? r->present_to : r->image; }
+
 ShadeBuffers shade_buffers(const prh_renderer* r) {
     ShadeBuffers b;
     b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
     b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
     b.light_view = r->d_lview, b.light_proj = r->d_lproj;
-    b.image = r->msaa > 1 ? r->image_big : r->image, b.pregamma = r->pregamma, b.linear = r->linear;
+    b.image = r->msaa > 1 ? r->image_big : final_image(r), b.pregamma = r->pregamma, b.linear = r->linear;
     b.bg = r->bg, b.bg_lin = r->bg_lin;
     b.cnt = r->cnt;
     b.cnt_host = r->d_h_cnt;
@@ -1028,7 +1039,8 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     const DevScene& d = r->sc->d;
     Counters* cm = r->cnt;
     Counters* cs = r->cnt + 1;
-    const int persistent = wide ? std::max(32, r->n_cu / 2) : r->n_cu * 4;
+    const int persistent = wide ? std::max(32, r->n_cu / 2) : r->n_cu * r->tune.raster_wg_per_cu;
+    const int ip = r->tune.item_prims;
     if (timed) (void)hipEventRecord(r->ev[0], st);
     const size_t P = (size_t)r->W * r->H;
     // counters zeroed, visibility emptied (before the setup, which draws
@@ -1044,14 +1056,14 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
         launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->pbox, r->tile_count, cs, r->cap_shadow, r->smaps);
         wide_stage(4);
         launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
-                    r->cap_items, nullptr, 0, 0, cs);
+                    r->cap_items, nullptr, 0, 0, cs, ip);
         wide_stage(5);
         launch_bin(st, r->pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
                    r->n_slots * d.n_tris, d.n_tris);
         RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
         wide_stage(6);
         launch_raster(st, 2, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
-                      r->tile_list, cs, rt);
+                      r->tile_list, cs, rt, ip);
     }
     if (timed) (void)hipEventRecord(r->ev[1], st);
     wide_stage(7);
@@ -1059,10 +1071,11 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     if (timed) (void)hipEventRecord(r->ev[2], st);
     wide_stage(8);
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
-                      r->tile_count, r->blk_prims, cm, r->cap_main, (unsigned long long*)r->vis, !nan_mode);
+                      r->tile_count, r->blk_prims, cm, r->cap_main, (unsigned long long*)r->vis,
+                      nan_mode ? 0 : r->tune.main_direct_max);
     wide_stage(9);
     launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
-                r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm);
+                r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm, ip);
     wide_stage(10);
     launch_bin(st, r->pbox, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
                d.n_tris);
@@ -1075,7 +1088,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
                     r->first_nan, nullptr};
     wide_stage(11);
     launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
-                  r->tile_list, cm, rt);
+                  r->tile_list, cm, rt, ip);
     wide_stage(12);
     launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr);
     if (timed) (void)hipEventRecord(r->ev[4], st);
@@ -1093,7 +1106,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
         wide_stage(16);
         launch_resize(st, r->image_big, r->W, r->resize_tmp, r->H, r->OW, r->rz_cx, r->rz_ox, r->rz_flx);
         wide_stage(17);
-        launch_resize(st, r->resize_tmp, r->H, r->image, r->OW, r->OH, r->rz_cy, r->rz_oy, r->rz_fly);
+        launch_resize(st, r->resize_tmp, r->H, final_image(r), r->OW, r->OH, r->rz_cy, r->rz_oy, r->rz_fly);
     }
     if (timed) (void)hipEventRecord(r->ev[6], st);
     // (k_shade_bg writes the counters into r->h_cnt)
@@ -1422,6 +1435,36 @@ int diff_run(prh_renderer* r, const prh_diff_params* p, const float* target, dou
     }
     HIP_TRY(hipStreamSynchronize(st));
     (void)hipEventElapsedTime(&r->stats.ms_diff, r->dev[0], r->dev[1]);
+    return PRH_OK;
+}
+
+}  // namespace
+
+// ---- headless present: a ring of RGBA8 textures in HBM
+struct prh_surface {
+    prh_ctx* ctx;
+    int32_t w, h, frames;
+    uint32_t* tex[16];
+    int64_t frame;  // frames presented
+    bool acquired;
+};
+
+namespace {
+
+void surface_release(prh_surface* s) {
+    for (int k = 0; k < s->frames; k++)
+        if (s->tex[k]) (void)hipFree(s->tex[k]), s->tex[k] = nullptr;
+}
+
+int surface_alloc(prh_surface* s, int32_t w, int32_t h) {
+    s->w = w, s->h = h;
+    for (int k = 0; k < s->frames; k++) {
+        int rc = dalloc(&s->tex[k], (size_t)w * h);
+        if (rc) {
+            surface_release(s);
+            return rc;
+        }
+    }
     return PRH_OK;
 }
 
@@ -1886,6 +1929,180 @@ int prh_fit_run(prh_fit* f, const prh_diff_params* start, const prh_fit_options*
 int prh_fit_last_stats(prh_fit* f, prh_fit_stats* out) {
     if (!f || !out) return fail(PRH_E_INVALID, "prh_fit_last_stats: null argument");
     *out = f->stats;
+    return PRH_OK;
+}
+
+
+// ---- headless present (gpu/surface.go:1-183)
+int prh_surface_new(prh_ctx* ctx, int32_t width, int32_t height, int32_t frames, prh_surface** out) {
+    if (!ctx || !out) return fail(PRH_E_INVALID, "prh_surface_new: null argument");
+    *out = nullptr;
+    if (width <= 0 || height <= 0) return fail(PRH_E_INVALID, "prh_surface_new: surface size must be > 0");
+    if (frames <= 0) frames = 2;  // SurfaceDescriptor.Frames: double buffering by default
+    if (frames > 16) return fail(PRH_E_INVALID, "prh_surface_new: at most 16 frames");
+    HIP_TRY(hipSetDevice(ctx->device));
+    prh_surface* s = new prh_surface();
+    s->ctx = ctx, s->frames = frames;
+    const int rc = surface_alloc(s, width, height);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    ctx->live_surfaces++;
+    *out = s;
+    return PRH_OK;
+}
+
+int prh_surface_free(prh_surface* s) {
+    if (!s) return PRH_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    surface_release(s);
+    s->ctx->live_surfaces--;
+    delete s;
+    return PRH_OK;
+}
+
+int prh_surface_acquire(prh_surface* s, int32_t* index) {
+    if (!s) return fail(PRH_E_INVALID, "prh_surface_acquire: null surface");
+    s->acquired = true;
+    if (index) *index = s->frame % s->frames;
+    return PRH_OK;
+}
+
+int prh_surface_render(prh_surface* s, prh_renderer* r) {
+    if (!s || !r) return fail(PRH_E_INVALID, "prh_surface_render: null argument");
+    if (!s->acquired) return fail(PRH_E_INVALID, "prh_surface_render: no acquired texture");
+    if (r->ctx != s->ctx) return fail(PRH_E_INVALID, "prh_surface_render: renderer of another context");
+    if (r->OW != s->w || r->OH != s->h) return fail(PRH_E_INVALID, "prh_surface_render: renderer and surface sizes differ");
+    r->present_to = s->tex[s->frame % s->frames];
+    const int rc = prh_render(r);
+    r->present_to = nullptr;
+    return rc;
+}
+
+int prh_surface_write(prh_surface* s, const uint8_t* rgba, size_t bytes) {
+    if (!s || !rgba) return fail(PRH_E_INVALID, "prh_surface_write: null argument");
+    if (!s->acquired) return fail(PRH_E_INVALID, "prh_surface_write: no acquired texture");
+    const size_t need = (size_t)4 * s->w * s->h;
+    if (bytes < need) return fail(PRH_E_INVALID, "prh_surface_write: image too small");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipMemcpyAsync(s->tex[s->frame % s->frames], rgba, need, hipMemcpyHostToDevice, s->ctx->stream));
+    return PRH_OK;
+}
+
+int prh_surface_present(prh_surface* s) {
+    if (!s) return fail(PRH_E_INVALID, "prh_surface_present: null surface");
+    if (!s->acquired) return fail(PRH_E_INVALID, "prh_surface_present: Present without AcquireNextTexture");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));  // headless: the frame is complete
+    s->acquired = false;
+    s->frame++;
+    return PRH_OK;
+}
+
+int prh_surface_read(prh_surface* s, uint8_t* rgba, size_t bytes) {
+    if (!s || !rgba) return fail(PRH_E_INVALID, "prh_surface_read: null argument");
+    if (s->frame == 0) return fail(PRH_E_INVALID, "prh_surface_read: nothing presented yet");
+    const size_t need = (size_t)4 * s->w * s->h;
+    if (bytes < need) return fail(PRH_E_INVALID, "prh_surface_read: buffer too small");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipMemcpyAsync(rgba, s->tex[(s->frame - 1) % s->frames], need, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    return PRH_OK;
+}
+
+int prh_surface_resize(prh_surface* s, int32_t width, int32_t height) {
+    if (!s) return fail(PRH_E_INVALID, "prh_surface_resize: null surface");
+    if (width <= 0 || height <= 0) return fail(PRH_E_INVALID, "prh_surface_resize: surface size must be > 0");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    surface_release(s);
+    s->frame = 0, s->acquired = false;
+    return surface_alloc(s, width, height);
+}
+
+int prh_surface_size(prh_surface* s, int32_t* width, int32_t* height) {
+    if (!s || !width || !height) return fail(PRH_E_INVALID, "prh_surface_size: null argument");
+    *width = s->w, *height = s->h;
+    return PRH_OK;
+}
+
+
+// ---- autotuning
+int prh_get_tuning(prh_renderer* r, prh_tuning* out) {
+    if (!r || !out) return fail(PRH_E_INVALID, "prh_get_tuning: null argument");
+    *out = r->tune;
+    return PRH_OK;
+}
+
+int prh_set_tuning(prh_renderer* r, const prh_tuning* t) {
+    if (!r || !t) return fail(PRH_E_INVALID, "prh_set_tuning: null argument");
+    const bool pow2 = t->item_prims > 0 && (t->item_prims & (t->item_prims - 1)) == 0;
+    if (t->raster_wg_per_cu < 1 || t->raster_wg_per_cu > 16 || !pow2 || t->item_prims < 64 ||
+        t->item_prims > 8192 || t->shadow_direct_max < 0 || t->shadow_direct_max > 256 || t->main_direct_max < 0 ||
+        t->main_direct_max > 256)
+        return fail(PRH_E_INVALID, "prh_set_tuning: a knob out of range");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(hipStreamSynchronize(r->ctx->stream));
+    const bool realloc = t->item_prims != r->tune.item_prims;
+    r->tune.raster_wg_per_cu = t->raster_wg_per_cu, r->tune.item_prims = t->item_prims;
+    r->tune.shadow_direct_max = t->shadow_direct_max, r->tune.main_direct_max = t->main_direct_max;
+    r->shc.direct_max = t->shadow_direct_max;
+    if (realloc) {  // the work-item table is sized by item_prims
+        free_work(r);
+        int rc = alloc_work(r);
+        if (rc) return rc;
+    }
+    r->work_gen++;  // a cached batch recorded the old values
+    return PRH_OK;
+}
+
+int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
+    if (!r) return fail(PRH_E_INVALID, "prh_autotune: null renderer");
+    if (frames < 1) frames = 5;
+    auto time = [&](const prh_tuning& t, float* ms) -> int {
+        int rc = prh_set_tuning(r, &t);
+        if (rc) return rc;
+        if ((rc = prh_render(r))) return rc;  // warm-up (and any regrowth)
+        std::vector<float> v;
+        for (int k = 0; k < frames; k++) {
+            if ((rc = prh_render(r))) return rc;
+            v.push_back(r->stats.ms_total);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        return PRH_OK;
+    };
+    static const int32_t c_ip[] = {256, 512, 1024, 2048, 4096};
+    static const int32_t c_wg[] = {1, 2, 4, 8};
+    static const int32_t c_sd[] = {0, 16, 32, 64, 128};
+    static const int32_t c_md[] = {0, 24, 48, 96};
+    prh_tuning best = r->tune;
+    float best_ms = 0.0f;
+    int rc = time(best, &best_ms);
+    if (rc) return rc;
+    // coordinate descent: each knob over its candidates, the others at their best
+    for (int knob = 0; knob < 4; knob++) {
+        const int32_t* c = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : c_md;
+        const int n = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : 4;
+        for (int k = 0; k < n; k++) {
+            prh_tuning t = best;
+            int32_t* f = knob == 0 ? &t.item_prims
+                       : knob == 1 ? &t.raster_wg_per_cu
+                       : knob == 2 ? &t.shadow_direct_max : &t.main_direct_max;
+            if (*f == c[k]) continue;
+            *f = c[k];
+            float ms = 0.0f;
+            if ((rc = time(t, &ms))) return rc;
+            if (ms < best_ms) best = t, best_ms = ms;
+        }
+    }
+    best.ms = best_ms;
+    if ((rc = prh_set_tuning(r, &best))) return rc;
+    r->tune.ms = best_ms;
+    if ((rc = prh_render(r))) return rc;  // leave a frame of the chosen configuration
+    if (out) *out = r->tune;
     return PRH_OK;
 }
 

@@ -41,10 +41,13 @@ class Device:
         self.handle = h
         self.ordinal = ordinal
         self._scenes: "weakref.WeakSet" = weakref.WeakSet()
+        self._surfaces: "weakref.WeakSet" = weakref.WeakSet()
         _live_devices.add(self)
 
     def close(self) -> None:
         if self.handle:
+            for sf in list(self._surfaces):
+                sf.Release()
             for sc in list(self._scenes):
                 sc.free()
             abi.check(self._lib.prh_close(self.handle), "prh_close")
@@ -274,6 +277,27 @@ class Renderer:
         from . import diff
         return diff.backward(self._lib, self.handle, params, target, self.H * self.msaa, self.W * self.msaa)
 
+    # -------- autotuning (an extension: speed knobs, results unchanged)
+    def tuning(self) -> abi.prh_tuning:
+        t = abi.prh_tuning()
+        abi.check(self._lib.prh_get_tuning(self.handle, C.byref(t)), "prh_get_tuning")
+        return t
+
+    def set_tuning(self, **knobs) -> None:
+        """raster_wg_per_cu, item_prims, shadow_direct_max, main_direct_max."""
+        t = self.tuning()
+        for k, v in knobs.items():
+            setattr(t, k, v)
+        abi.check(self._lib.prh_set_tuning(self.handle, C.byref(t)), "prh_set_tuning")
+
+    def Autotune(self, frames: int = 5) -> abi.prh_tuning:
+        """Time the knobs' candidates (coordinate descent) and keep the fastest."""
+        t = abi.prh_tuning()
+        abi.check(self._lib.prh_autotune(self.handle, frames, C.byref(t)), "prh_autotune")
+        for name in ("shadow", "forward", "deferred", "gamma"):
+            self.passGPU[name] = True
+        return t
+
     def stats(self) -> abi.prh_frame_stats:
         st = abi.prh_frame_stats()
         abi.check(self._lib.prh_last_stats(self.handle, C.byref(st)), "prh_last_stats")
@@ -335,6 +359,79 @@ def NewRenderer(*opts) -> Renderer:  # render/raster.go:84-143
     for opt in opts:
         opt(cfg)
     return Renderer(cfg)
+
+
+class Surface:
+    """gpu.Surface's headless swapchain (gpu/surface.go:20-183) on the
+    device: a ring of RGBA8 textures in HBM that renderers draw into, one per
+    frame (prh_surface_*).
+
+        surf = render.CreateSurface(dev, 1920, 1080)      # Frames 2
+        i = surf.AcquireNextTexture()
+        surf.Render(r)            # r's frame drawn into texture i
+        surf.Present()
+        pix = surf.Texture()      # the presented frame's pixels
+    """
+
+    def __init__(self, dev: Device, width: int, height: int, frames: int = 2):
+        self._lib = abi.load()
+        self.dev = dev
+        self.handle = None
+        h = C.c_void_p()
+        abi.check(self._lib.prh_surface_new(dev.handle, width, height, frames, C.byref(h)), "prh_surface_new")
+        self.handle = h
+        dev._surfaces.add(self)
+
+    def AcquireNextTexture(self) -> int:
+        i = C.c_int32()
+        abi.check(self._lib.prh_surface_acquire(self.handle, C.byref(i)), "prh_surface_acquire")
+        return i.value
+
+    def Render(self, r: "Renderer") -> None:
+        abi.check(self._lib.prh_surface_render(self.handle, r.handle), "prh_surface_render")
+        for name in ("shadow", "forward", "deferred", "gamma"):
+            r.passGPU[name] = True
+
+    def Present(self) -> None:
+        abi.check(self._lib.prh_surface_present(self.handle), "prh_surface_present")
+
+    def PresentImage(self, img: np.ndarray) -> None:
+        """Acquire, upload a (h, w, 4) uint8 image, present (surface.go:133-137)."""
+        self.AcquireNextTexture()
+        a = np.ascontiguousarray(img, np.uint8)
+        abi.check(self._lib.prh_surface_write(self.handle, a.ctypes.data, a.nbytes), "prh_surface_write")
+        self.Present()
+
+    def Texture(self) -> np.ndarray:
+        """The most recently presented frame, (h, w, 4) uint8, top row first."""
+        w, h = self.Size()
+        out = np.empty((h, w, 4), np.uint8)
+        abi.check(self._lib.prh_surface_read(self.handle, out.ctypes.data, out.nbytes), "prh_surface_read")
+        return out
+
+    def Resize(self, width: int, height: int) -> None:
+        abi.check(self._lib.prh_surface_resize(self.handle, width, height), "prh_surface_resize")
+
+    def Size(self) -> tuple:
+        w, h = C.c_int32(), C.c_int32()
+        abi.check(self._lib.prh_surface_size(self.handle, C.byref(w), C.byref(h)), "prh_surface_size")
+        return w.value, h.value
+
+    def Release(self) -> None:
+        if self.handle:
+            abi.check(self._lib.prh_surface_free(self.handle), "prh_surface_free")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.Release()
+        except Exception:
+            pass
+
+
+def CreateSurface(dev: Device, width: int, height: int, frames: int = 2) -> Surface:
+    """Device.CreateSurface (gpu/surface.go:43-57)."""
+    return Surface(dev, width, height, frames)
 
 
 def views_of(scene, views, dev: Device):
