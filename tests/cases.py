@@ -160,3 +160,26 @@ def reference_directional():
                          color=(255, 240, 220, 255))
     cam = dict(position=(0.0, 1.5, 1.0), target=(0.0, 0.0, -0.5), fov=45.0, aspect=1.0, near=0.1, far=3.0)
     return _ref_scene([g], [light], [0.3], cam, 150, 150, name="reference_directional")
+
+
+def stacked_layers(n_layers=100, width=160, height=120, shadows=True):
+    """n_layers large screen-facing quads, one behind the other, each
+    covering most of the viewport: every triangle touches half the tiles, so
+    the (primitive, tile) pairs outgrow the first work-buffer estimate (4 per
+    triangle plus 4 per tile) and the frame is drawn again after the buffers
+    grow.  With a shadow-casting light the shadow pass overflows as well."""
+    P, tri = [], []
+    for k in range(n_layers):
+        z = -0.02 * k
+        b = len(P)
+        P += [[-0.9, -0.85, z, 1], [0.92, -0.8, z, 1], [0.9, 0.88, z, 1], [-0.88, 0.9, z, 1]]
+        tri += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+    P = np.asarray(P, np.float32)
+    tri = scenes._orient(P, np.asarray(tri, np.int32), lambda a, b, c: np.tile([0.0, 0.0, 1.0], (len(a), 1)))
+    mats = [scenes._mat_from((0.5, 0.6, 0.7), 0.5, 8, shadow=shadows)]
+    lights = [scenes.Light(position=(0.5, 1.0, 3.0), cast_shadow=shadows)]
+    s = _scene(P, tri, [0] * len(tri), width, height, dict(position=(0.0, 0.0, 1.8), fov=60.0, near=0.1, far=100.0),
+               materials=mats, lights=lights)
+    s.shadow_map = shadows
+    s.name = f"stacked_{n_layers}"
+    return s
