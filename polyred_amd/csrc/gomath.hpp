@@ -60,6 +60,92 @@ GM_HD double pow64(double x, double y) {
     return std::pow(x, y);
 #endif
 }
+// float32(pow(x, y)) for the shading's x in (0, 1] and y in [2^-8, 2048]
+// (the Blinn-Phong specular term), without the library's double-double pow:
+// t = y log2(x) split as k + f (k an integer, |f| <= 1/2), log2 of the
+// mantissa from the atanh series and 2^f from the exponential series, both
+// in double to a relative error below 2^-40 of the result (the series are
+// truncated below 2^-52; y e is exact; the other roundings are 2^-53 each).
+// That decides the float32 rounding unless the value lies within 2^-40 of a
+// float32 rounding midpoint; then -- and outside the ranges above -- the
+// library pow decides, so the result is always (float)pow64(x, y).
+GM_HD float pow32_fast(float xf, float yf) {
+    const double x = xf, y = yf;
+    if (!(xf > 0.0f && xf <= 1.0f && yf >= 0.00390625f && yf <= 2048.0f)) return (float)pow64(x, y);
+    // x = m 2^e, m in [sqrt(1/2), sqrt(2))
+    int e;
+#ifdef __HIP_DEVICE_COMPILE__
+    double m = __builtin_frexp(x, &e);  // [1/2, 1)
+#else
+    double m = std::frexp(x, &e);
+#endif
+    if (m < 0.70710678118654752) m *= 2.0, e -= 1;
+    // log2(m) = (2 / ln 2) atanh(s), s = (m - 1) / (m + 1), |s| < 0.1716
+    const double d = m + 1.0;
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(d);  // refined twice: relative error < 2^-56
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+#else
+    const double r = 1.0 / d;
+#endif
+    const double sv = (m - 1.0) * r, s2 = sv * sv;
+    // 2/ln2 * (1 + s^2/3 + s^4/5 + ... + s^18/19)
+    double q = 2.0 / (19.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (17.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (15.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (13.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (11.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (9.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (7.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (5.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / (3.0 * 0.69314718055994531);
+    q = q * s2 + 2.0 / 0.69314718055994531;
+    const double l = sv * q;          // log2(m), |l| <= 1/2
+    const double a = y * (double)e;   // exact: 24 x 9 bits
+    const double b = y * l;
+    const double t = a + b;
+    if (t < -152.0) return 0.0f;       // below 2^-152: rounds to +0
+#ifdef __HIP_DEVICE_COMPILE__
+    const double kd = __builtin_rint(t);
+#else
+    const double kd = std::nearbyint(t);
+#endif
+    const double f = (a - kd) + b;    // a - kd exact; |f| <= 1/2 + 2^-40
+    // 2^f = sum (f ln2)^n / n!, n <= 12
+    const double z = f * 0.69314718055994531;
+    double p = 1.0 / 479001600.0;
+    p = p * z + 1.0 / 39916800.0;
+    p = p * z + 1.0 / 3628800.0;
+    p = p * z + 1.0 / 362880.0;
+    p = p * z + 1.0 / 40320.0;
+    p = p * z + 1.0 / 5040.0;
+    p = p * z + 1.0 / 720.0;
+    p = p * z + 1.0 / 120.0;
+    p = p * z + 1.0 / 24.0;
+    p = p * z + 1.0 / 6.0;
+    p = p * z + 0.5;
+    p = p * z + 1.0;
+    p = p * z + 1.0;
+#ifdef __HIP_DEVICE_COMPILE__
+    const double v = __builtin_ldexp(p, (int)kd);
+#else
+    const double v = std::ldexp(p, (int)kd);
+#endif
+    // the float32 rounding of v holds for the exact value when v is farther
+    // than the error bound from both neighbouring rounding midpoints
+    const float fv = (float)v;
+    uint32_t bits;
+    __builtin_memcpy(&bits, &fv, 4);
+    float up, dn;
+    const uint32_t ub = bits + 1u, db = bits == 0u ? 0x80000001u : bits - 1u;
+    __builtin_memcpy(&up, &ub, 4);
+    __builtin_memcpy(&dn, &db, 4);
+    const double mid_up = 0.5 * ((double)fv + (double)up), mid_dn = 0.5 * ((double)fv + (double)dn);
+    const double margin = v * 0x1p-40;
+    if (mid_up - v > margin && v - mid_dn > margin) return fv;
+    return (float)pow64(x, y);
+}
 GM_HD double atan64(double x) {
 #ifdef __HIP_DEVICE_COMPILE__
     return atan(x);

@@ -675,7 +675,6 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
     uint2* __restrict__ pbox = a_.pbox;
     int32_t* __restrict__ tile_count = a_.tile_count;
     Counters* cnt = a_.cnt;
-    const Capacities& cap = a_.cap;
     uint32_t* __restrict__ smaps = a_.smaps;
 
     const int g = vb.bx * blockDim.x + threadIdx.x;
@@ -1429,7 +1428,7 @@ __device__ bool fragment_shader(const DevScene& s, const ShadeConsts& c, const D
         }
         V4 H = unit(add(L, V));
         float Ld = clamp(dot(n, L), 0, 1);
-        float Ls = (float)pow64((double)clamp(dot(n, H), 0, 1), (double)m.shininess);
+        float Ls = pow32_fast(clamp(dot(n, H), 0, 1), m.shininess);  // = (float)pow64(double, double)
         LdR += Ld * tex[0] * I;
         LdG += Ld * tex[1] * I;
         LdB += Ld * tex[2] * I;
