@@ -350,9 +350,9 @@ int prh_surface_size(prh_surface* s, int32_t* width, int32_t* height);
  * tile with one pixel per thread, and the shading's 8x8 wave blocks and the
  * AO's row segments are laid out on it. */
 typedef struct prh_tuning {
-    int32_t raster_wg_per_cu;  /* persistent raster workgroups per CU (occupancy), 1..16; default 4        */
+    int32_t raster_wg_per_cu;  /* persistent raster workgroups per CU (occupancy), 1..16; default 8        */
     int32_t item_prims;        /* primitives per raster work item (bin capacity), a power of 2 in
-                                  64..8192; default 1024                                                   */
+                                  64..8192; default 2048                                                   */
     int32_t shadow_direct_max; /* shadow-map triangles of at most this many texel centres are drawn by
                                   the setup, 0..256; default 64                                            */
     int32_t main_direct_max;   /* main-view triangles of at most this many pixel centres, likewise; 48  */

@@ -1450,15 +1450,15 @@ __device__ bool fragment_shader(const DevScene& s, const ShadeConsts& c, const D
 __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int fx, int fy, float depth) {
     int occ = 0;
     const int64_t len = (int64_t)c.SW * c.SH;
+    // the screen-to-world step is the same for every light
+    const V4 qw = apply(v4((float)fx, (float)fy, depth, 1), c.v.s2w);
     for (int i = 0; i < s.n_lights; i++) {
         const DevLight& l = s.lights[i];
         if (!l.cast_shadow) {
             occ++;
             continue;
         }
-        V4 q = v4((float)fx, (float)fy, depth, 1);
-        q = apply(q, c.v.s2w);
-        q = apply(q, b.light_view[i]);
+        V4 q = apply(qw, b.light_view[i]);
         q = apply(q, b.light_proj[i]);
         q = pos(apply(q, c.svp));
         int64_t lx = f2i(q.x), ly = f2i(q.y);

@@ -133,7 +133,7 @@ struct Capacities {
 };
 
 constexpr int CHUNK = 256;          // primitives staged in LDS at once
-constexpr int ITEM_PRIMS = 1024;    // primitives per raster work item (the default; prh_tuning.item_prims)
+constexpr int ITEM_PRIMS = 2048;    // primitives per raster work item (the default; prh_tuning.item_prims)
 
 // Where a raster pass writes.
 struct RasterTarget {

@@ -800,7 +800,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     HIP_TRY(hipSetDevice(ctx->device));
     prh_renderer* r = new prh_renderer();
     r->ctx = ctx, r->sc = s, r->cam = *cam, r->opt = *o;
-    r->tune = prh_tuning{4, ITEM_PRIMS, 64, 48, 0.0f};
+    r->tune = prh_tuning{8, ITEM_PRIMS, 64, 48, 0.0f};
     // every pass renders the MSAA-sized buffer (render/raster.go:149-150)
     r->msaa = msaa, r->OW = o->width, r->OH = o->height;
     r->W = o->width * msaa, r->H = o->height * msaa;

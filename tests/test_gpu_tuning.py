@@ -30,7 +30,7 @@ def test_knobs_do_not_change_the_frame(device):
     s = scene()
     r = render.renderer_for(s, device)
     t = r.tuning()
-    assert (t.raster_wg_per_cu, t.item_prims, t.shadow_direct_max, t.main_direct_max) == (4, 1024, 64, 48)
+    assert (t.raster_wg_per_cu, t.item_prims, t.shadow_direct_max, t.main_direct_max) == (8, 2048, 64, 48)
     want = frame(r)
     for knobs in (dict(item_prims=64), dict(item_prims=8192, raster_wg_per_cu=1),
                   dict(shadow_direct_max=0, main_direct_max=0), dict(shadow_direct_max=256, main_direct_max=256),
