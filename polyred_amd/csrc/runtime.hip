@@ -328,6 +328,7 @@ constexpr int PRH_BATCH_MAX = 16;
 struct prh_ctx {
     int device;
     hipStream_t stream;
+    hipStream_t stream2;  // a frame's shadow passes, beside the main view's passes on `stream`
     int live_scenes;    // a context outlives every scene uploaded to it
     int live_surfaces;  // and every surface
     // prh_render_batch: its streams and events (created on first use) and
@@ -384,6 +385,13 @@ struct prh_renderer {
     int32_t* scan_sums;  // per scan block: primitive and work-item sums
     int32_t *tile_count, *tile_off, *chunk_off, *tile_cursor, *tile_list, *clip_list, *item_tile;
     int32_t cap_items;
+    // the shadow passes' own work set (they run beside the main view's)
+    Prim* s_prims;
+    uint2* s_pbox;
+    int32_t *s_scan_sums, *s_tile_count, *s_tile_off, *s_chunk_off, *s_tile_cursor, *s_tile_list, *s_item_tile;
+    int32_t s_cap_items;
+    int32_t s_tile_slots;  // shadow slots x tiles per map
+    hipEvent_t ev_fork, ev_join, ev_s[2];
     int32_t tile_slots;
     int32_t n_cu;
     uint32_t *first_all, *first_nan;
@@ -439,18 +447,28 @@ void free_list(std::vector<void*>& v) {
 
 int alloc_work(prh_renderer* r) {
     int rc;
-    if ((rc = ralloc(r, &r->prims, (size_t)std::max(r->cap_main.prims, r->cap_shadow.prims)))) return rc;
-    if ((rc = ralloc(r, &r->pbox, (size_t)std::max(r->cap_main.prims, r->cap_shadow.prims)))) return rc;
+    if ((rc = ralloc(r, &r->prims, (size_t)r->cap_main.prims))) return rc;
+    if ((rc = ralloc(r, &r->pbox, (size_t)r->cap_main.prims))) return rc;
     if ((rc = ralloc(r, &r->clips, (size_t)r->cap_main.clip_subs))) return rc;
-    if ((rc = ralloc(r, &r->tile_list, (size_t)std::max(r->cap_main.pairs, r->cap_shadow.pairs)))) return rc;
+    if ((rc = ralloc(r, &r->tile_list, (size_t)r->cap_main.pairs))) return rc;
     if ((rc = ralloc(r, &r->clip_list, (size_t)r->cap_main.clip_tris))) return rc;
-    // every item holds >= 1 primitive of one tile: items <= tiles + pairs / ITEM_PRIMS
-    r->cap_items = r->tile_slots + std::max(r->cap_main.pairs, r->cap_shadow.pairs) / r->tune.item_prims + 1;
+    // every item holds >= 1 primitive of one tile: items <= tiles + pairs / item_prims
+    r->cap_items = r->n_tiles + r->cap_main.pairs / r->tune.item_prims + 1;
     if ((rc = ralloc(r, &r->item_tile, (size_t)r->cap_items))) return rc;
+    if (r->n_slots > 0) {
+        if ((rc = ralloc(r, &r->s_prims, (size_t)r->cap_shadow.prims))) return rc;
+        if ((rc = ralloc(r, &r->s_pbox, (size_t)r->cap_shadow.prims))) return rc;
+        if ((rc = ralloc(r, &r->s_tile_list, (size_t)r->cap_shadow.pairs))) return rc;
+        r->s_cap_items = r->s_tile_slots + r->cap_shadow.pairs / r->tune.item_prims + 1;
+        if ((rc = ralloc(r, &r->s_item_tile, (size_t)r->s_cap_items))) return rc;
+    }
     return PRH_OK;
 }
 void free_work(prh_renderer* r) {
-    for (void* p : {(void*)r->prims, (void*)r->pbox, (void*)r->clips, (void*)r->tile_list, (void*)r->clip_list, (void*)r->item_tile}) {
+    for (void* p : {(void*)r->prims, (void*)r->pbox, (void*)r->clips, (void*)r->tile_list, (void*)r->clip_list,
+                    (void*)r->item_tile, (void*)r->s_prims, (void*)r->s_pbox, (void*)r->s_tile_list,
+                    (void*)r->s_item_tile}) {
+        if (!p) continue;
         auto it = std::find(r->allocs.begin(), r->allocs.end(), p);
         if (it != r->allocs.end()) {
             (void)hipFree(p);
@@ -478,10 +496,11 @@ int prh_open(int32_t ordinal, prh_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(PRH_E_NODEVICE, "no HIP device");
     if (ordinal < 0 || ordinal >= n) return fail(PRH_E_NODEVICE, "device ordinal out of range");
     HIP_TRY(hipSetDevice(ordinal));
-    hipStream_t st;
+    hipStream_t st, st2;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     prh_ctx* c = new prh_ctx();
-    c->device = ordinal, c->stream = st, c->live_scenes = 0;
+    c->device = ordinal, c->stream = st, c->stream2 = st2, c->live_scenes = 0;
     *out = c;
     return PRH_OK;
 }
@@ -504,6 +523,7 @@ int prh_close(prh_ctx* ctx) {
         (void)hipEventDestroy(ctx->bev[0]), (void)hipEventDestroy(ctx->bev[1]), (void)hipEventDestroy(ctx->bfork);
     }
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return PRH_OK;
 }
@@ -891,7 +911,8 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     r->cap_main = Capacities{Tg + clip0, clip0, 4 * Tg + 4 * r->n_tiles + 1024, clip_tris};
     r->cap_shadow = Capacities{std::max(1, r->n_slots) * Tg, 0,
                                std::max(1, r->n_slots) * (2 * Tg + 4 * r->stiles_per_map) + 1024, 0};
-    r->tile_slots = std::max(r->n_tiles, r->n_slots * r->stiles_per_map) + 1;
+    r->tile_slots = r->n_tiles + 1;
+    r->s_tile_slots = r->n_slots * r->stiles_per_map + 1;
     int rc = 0;
     auto chk = [&](int v) {
         if (!rc) rc = v;
@@ -922,6 +943,13 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->first_all, P));
     if (!rc) chk(ralloc(r, &r->first_nan, P));
     if (!rc) chk(ralloc(r, &r->tile_cursor, r->tile_slots));
+    if (!rc && r->n_slots > 0) {
+        chk(ralloc(r, &r->s_scan_sums, 2 * (size_t)scan_blocks(r->s_tile_slots)));
+        if (!rc) chk(ralloc(r, &r->s_tile_count, r->s_tile_slots));
+        if (!rc) chk(ralloc(r, &r->s_tile_off, r->s_tile_slots + 1));
+        if (!rc) chk(ralloc(r, &r->s_chunk_off, r->s_tile_slots + 1));
+        if (!rc) chk(ralloc(r, &r->s_tile_cursor, r->s_tile_slots));
+    }
     if (!rc) chk(ralloc(r, &r->vis, P));
     if (!rc) chk(ralloc(r, &r->depth, P));
     if (!rc) chk(ralloc(r, &r->ao, P));
@@ -961,6 +989,11 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     }
     for (int i = 0; i < 8 && !rc; i++)
         if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
+    for (int i = 0; i < 2 && !rc; i++)
+        if (hipEventCreate(&r->ev_s[i]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
+    if (!rc && (hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess))
+        rc = fail(PRH_E_HIP, "hipEventCreate failed");
     if (rc) {
         free_list(r->allocs);
         delete r;
@@ -971,6 +1004,8 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     launch_fill_u32(ctx->stream, r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH, depth_ord(0.0f));
     // tile counts start at zero; every scan leaves them at zero
     launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(r->tile_count), r->tile_slots, 0u);
+    if (r->n_slots > 0)
+        launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(r->s_tile_count), r->s_tile_slots, 0u);
     {
         int n_cu = 0;
         if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || n_cu <= 0)
@@ -998,8 +1033,12 @@ int prh_renderer_free(prh_renderer* r) {
     if (r->h_cnt) (void)hipHostFree(r->h_cnt);
     for (int i = 0; i < 8; i++)
         if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
         if (r->dev[i]) (void)hipEventDestroy(r->dev[i]);
+        if (r->ev_s[i]) (void)hipEventDestroy(r->ev_s[i]);
+    }
+    if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
+    if (r->ev_join) (void)hipEventDestroy(r->ev_join);
     r->sc->live_renderers--;
     delete r;
     return PRH_OK;
@@ -1048,22 +1087,36 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     // needs every fragment through the ordered raster)
     wide_stage(1);
     launch_frame_clear(st, (uint32_t*)r->cnt, (int32_t)(2 * sizeof(Counters) / 4), (uint64_t*)r->vis, P);
-    if (r->n_slots > 0 && !nan_mode) {
+    const bool shadows = r->n_slots > 0 && !nan_mode;
+    // a plain frame runs its shadow passes on the context's second stream,
+    // beside the main view's passes (they meet at the shading); a batch
+    // keeps them on its own stream
+    const bool split = shadows && !wide && st == r->ctx->stream;
+    const hipStream_t ss = split ? r->ctx->stream2 : st;
+    if (shadows) {
         const int nt = r->n_slots * r->stiles_per_map;
+        if (split) {
+            (void)hipEventRecord(r->ev_fork, st);
+            (void)hipStreamWaitEvent(ss, r->ev_fork, 0);
+        }
+        if (timed) (void)hipEventRecord(r->ev_s[0], ss);
         wide_stage(2);
-        launch_vertex_shadow(st, d, r->d_strans, r->shc, r->ts);
+        launch_vertex_shadow(ss, d, r->d_strans, r->shc, r->ts);
         wide_stage(3);
-        launch_setup_shadow(st, d, r->shc, r->ts, r->prims, r->pbox, r->tile_count, cs, r->cap_shadow, r->smaps);
+        launch_setup_shadow(ss, d, r->shc, r->ts, r->s_prims, r->s_pbox, r->s_tile_count, cs, r->cap_shadow,
+                            r->smaps);
         wide_stage(4);
-        launch_scan(st, r->tile_count, nt, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
-                    r->cap_items, nullptr, 0, 0, cs, ip);
+        launch_scan(ss, r->s_tile_count, nt, r->s_tile_off, r->s_chunk_off, r->s_tile_cursor, r->s_scan_sums,
+                    r->s_item_tile, r->s_cap_items, nullptr, 0, 0, cs, ip);
         wide_stage(5);
-        launch_bin(st, r->pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->tile_cursor, r->tile_list, r->cap_shadow,
-                   r->n_slots * d.n_tris, d.n_tris);
+        launch_bin(ss, r->s_pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->s_tile_cursor, r->s_tile_list,
+                   r->cap_shadow, r->n_slots * d.n_tris, d.n_tris);
         RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
         wide_stage(6);
-        launch_raster(st, 2, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
-                      r->tile_list, cs, rt, ip);
+        launch_raster(ss, 2, persistent, r->s_prims, r->s_tile_off, r->s_chunk_off, r->s_item_tile, r->s_cap_items,
+                      r->s_tile_list, cs, rt, ip);
+        if (timed) (void)hipEventRecord(r->ev_s[1], ss);
+        if (split) (void)hipEventRecord(r->ev_join, ss);
     }
     if (timed) (void)hipEventRecord(r->ev[1], st);
     wide_stage(7);
@@ -1096,6 +1149,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     wide_stage(13);
     if (r->sc->any_ao) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
     if (timed) (void)hipEventRecord(r->ev[5], st);
+    if (split) (void)hipStreamWaitEvent(st, r->ev_join, 0);  // the shading reads the shadow maps
     wide_stage(14);
     launch_shade_bg(st, d, r->sh, b);
     if (timed) (void)hipEventRecord(r->ev[7], st);
@@ -1156,7 +1210,10 @@ int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
         float ms[6] = {0};
         for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
         (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
-        S.ms_shadow = ms[0], S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
+        // (the shadow passes ran beside the main view's: their own span)
+        S.ms_shadow = 0.0f;
+        if (r->n_slots > 0 && !nan_rerun) (void)hipEventElapsedTime(&S.ms_shadow, r->ev_s[0], r->ev_s[1]);
+        S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
         (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
     } else {
         S.ms_total = S.ms_shadow = S.ms_vertex = S.ms_setup = S.ms_raster = S.ms_ao = S.ms_shade = 0.0f;
