@@ -286,7 +286,8 @@ def main() -> int:
     if not args.no_autotune:
         t = r.Autotune(frames=5)  # occupancy, bin capacity, small-triangle thresholds; results unchanged
         tuned = {"raster_wg_per_cu": t.raster_wg_per_cu, "item_prims": t.item_prims,
-                 "shadow_direct_max": t.shadow_direct_max, "main_direct_max": t.main_direct_max}
+                 "shadow_direct_max": t.shadow_direct_max, "main_direct_max": t.main_direct_max,
+                 "overlap_shadows": t.overlap_shadows}
     for _ in range(args.warmup):
         r.render_device()
     acc = dict(total=0.0, shadow=0.0, vertex=0.0, setup=0.0, raster=0.0, ao=0.0, shade=0.0)

@@ -356,13 +356,16 @@ typedef struct prh_tuning {
     int32_t shadow_direct_max; /* shadow-map triangles of at most this many texel centres are drawn by
                                   the setup, 0..256; default 64                                            */
     int32_t main_direct_max;   /* main-view triangles of at most this many pixel centres, likewise; 48  */
+    int32_t overlap_shadows;   /* 1: a frame's shadow passes run on a second stream beside the main
+                                  view's passes; default 1 from 2^19 framebuffer pixels, else 0          */
     float ms;                  /* prh_autotune: the chosen configuration's median frame time             */
 } prh_tuning;
 int prh_get_tuning(prh_renderer* r, prh_tuning* out);
 int prh_set_tuning(prh_renderer* r, const prh_tuning* t);
 /* Coordinate descent over item_prims {256..4096}, raster_wg_per_cu {1, 2, 4,
- * 8}, shadow_direct_max {0, 16, 32, 64, 128} and main_direct_max {0, 24, 48,
- * 96}: each configuration timed as the median of `frames` frames (HIP
+ * 8}, shadow_direct_max {0, 16, 32, 64, 128}, main_direct_max {0, 24, 48,
+ * 96} and overlap_shadows {0, 1}: each configuration timed as the median of
+ * `frames` frames (HIP
  * events, after one warm-up frame); the fastest stays on r and goes to *out
  * (may be NULL).  r holds a frame of it on return. */
 int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out);

@@ -162,7 +162,7 @@ PRH_FIT_STORED_GBUFFER = 256
 
 class prh_tuning(C.Structure):
     _fields_ = [("raster_wg_per_cu", C.c_int32), ("item_prims", C.c_int32), ("shadow_direct_max", C.c_int32),
-                ("main_direct_max", C.c_int32), ("ms", C.c_float)]
+                ("main_direct_max", C.c_int32), ("overlap_shadows", C.c_int32), ("ms", C.c_float)]
 
 
 class prh_fit_options(C.Structure):

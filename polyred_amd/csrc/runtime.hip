@@ -820,7 +820,8 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     HIP_TRY(hipSetDevice(ctx->device));
     prh_renderer* r = new prh_renderer();
     r->ctx = ctx, r->sc = s, r->cam = *cam, r->opt = *o;
-    r->tune = prh_tuning{8, ITEM_PRIMS, 64, 48, 0.0f};
+    r->tune = prh_tuning{8, ITEM_PRIMS, 64, 48, (size_t)o->width * o->height * msaa * msaa >= (1u << 19) ? 1 : 0,
+                         0.0f};
     // every pass renders the MSAA-sized buffer (render/raster.go:149-150)
     r->msaa = msaa, r->OW = o->width, r->OH = o->height;
     r->W = o->width * msaa, r->H = o->height * msaa;
@@ -1091,7 +1092,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     // a plain frame runs its shadow passes on the context's second stream,
     // beside the main view's passes (they meet at the shading); a batch
     // keeps them on its own stream
-    const bool split = shadows && !wide && st == r->ctx->stream;
+    const bool split = shadows && !wide && st == r->ctx->stream && r->tune.overlap_shadows;
     const hipStream_t ss = split ? r->ctx->stream2 : st;
     if (shadows) {
         const int nt = r->n_slots * r->stiles_per_map;
@@ -2098,13 +2099,14 @@ int prh_set_tuning(prh_renderer* r, const prh_tuning* t) {
     const bool pow2 = t->item_prims > 0 && (t->item_prims & (t->item_prims - 1)) == 0;
     if (t->raster_wg_per_cu < 1 || t->raster_wg_per_cu > 16 || !pow2 || t->item_prims < 64 ||
         t->item_prims > 8192 || t->shadow_direct_max < 0 || t->shadow_direct_max > 256 || t->main_direct_max < 0 ||
-        t->main_direct_max > 256)
+        t->main_direct_max > 256 || (t->overlap_shadows != 0 && t->overlap_shadows != 1))
         return fail(PRH_E_INVALID, "prh_set_tuning: a knob out of range");
     HIP_TRY(hipSetDevice(r->ctx->device));
     HIP_TRY(hipStreamSynchronize(r->ctx->stream));
     const bool realloc = t->item_prims != r->tune.item_prims;
     r->tune.raster_wg_per_cu = t->raster_wg_per_cu, r->tune.item_prims = t->item_prims;
     r->tune.shadow_direct_max = t->shadow_direct_max, r->tune.main_direct_max = t->main_direct_max;
+    r->tune.overlap_shadows = t->overlap_shadows;
     r->shc.direct_max = t->shadow_direct_max;
     if (realloc) {  // the work-item table is sized by item_prims
         free_work(r);
@@ -2135,19 +2137,21 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
     static const int32_t c_wg[] = {1, 2, 4, 8};
     static const int32_t c_sd[] = {0, 16, 32, 64, 128};
     static const int32_t c_md[] = {0, 24, 48, 96};
+    static const int32_t c_ov[] = {0, 1};
     prh_tuning best = r->tune;
     float best_ms = 0.0f;
     int rc = time(best, &best_ms);
     if (rc) return rc;
     // coordinate descent: each knob over its candidates, the others at their best
-    for (int knob = 0; knob < 4; knob++) {
-        const int32_t* c = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : c_md;
-        const int n = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : 4;
+    for (int knob = 0; knob < 5; knob++) {
+        const int32_t* c = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : knob == 3 ? c_md : c_ov;
+        const int n = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : knob == 3 ? 4 : 2;
         for (int k = 0; k < n; k++) {
             prh_tuning t = best;
             int32_t* f = knob == 0 ? &t.item_prims
                        : knob == 1 ? &t.raster_wg_per_cu
-                       : knob == 2 ? &t.shadow_direct_max : &t.main_direct_max;
+                       : knob == 2 ? &t.shadow_direct_max
+                       : knob == 3 ? &t.main_direct_max : &t.overlap_shadows;
             if (*f == c[k]) continue;
             *f = c[k];
             float ms = 0.0f;

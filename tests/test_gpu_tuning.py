@@ -31,13 +31,16 @@ def test_knobs_do_not_change_the_frame(device):
     r = render.renderer_for(s, device)
     t = r.tuning()
     assert (t.raster_wg_per_cu, t.item_prims, t.shadow_direct_max, t.main_direct_max) == (8, 2048, 64, 48)
+    assert t.overlap_shadows == 0  # 320x180: below 2^19 pixels
     want = frame(r)
     for knobs in (dict(item_prims=64), dict(item_prims=8192, raster_wg_per_cu=1),
                   dict(shadow_direct_max=0, main_direct_max=0), dict(shadow_direct_max=256, main_direct_max=256),
-                  dict(raster_wg_per_cu=16, item_prims=256)):
+                  dict(raster_wg_per_cu=16, item_prims=256), dict(overlap_shadows=1),
+                  dict(overlap_shadows=0, item_prims=512)):
         r.set_tuning(**knobs)
         same(frame(r), want)
-    for bad in (dict(item_prims=1000), dict(item_prims=32), dict(raster_wg_per_cu=0), dict(main_direct_max=300)):
+    for bad in (dict(item_prims=1000), dict(item_prims=32), dict(raster_wg_per_cu=0), dict(main_direct_max=300),
+                dict(overlap_shadows=2)):
         with pytest.raises(abi.PrhError) as e:
             r.set_tuning(**bad)
         assert e.value.code == abi.PRH_E_INVALID
