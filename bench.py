@@ -194,10 +194,8 @@ def fit_bench(dev, steps: int = 200) -> dict:
 
     s, cams = scenes.c5_inverse()
     gs, rs = render.views_of(s, cams, dev)
-    t0 = time.perf_counter()
-    for r in rs:
-        r.render_device()
-    ms_render = 1000.0 * (time.perf_counter() - t0)
+    render.render_batch(rs)  # every view's frame (visibility and shadow maps): one wide batch
+    ms_render = render.render_batch(rs)
     truth = diff.scene_params(s)
     rng = np.random.default_rng(5)
     start = truth.copy()
@@ -207,7 +205,7 @@ def fit_bench(dev, steps: int = 200) -> dict:
         a *= (1 + 0.3 * rng.standard_normal(a.shape)).astype(a.dtype)
     out = {"workload": s.name, "views": len(rs), "triangles": s.n_triangles,
            "pixels": int(sum(w * h for _, w, h in cams)), "steps": steps, "lr": 1e-2,
-           "ms_render_views": round(ms_render, 2), "rmse_start": diff.rmse(start, truth),
+           "ms_render_views_batch": round(ms_render, 3), "rmse_start": diff.rmse(start, truth),
            "gauge_free_start": diff.gauge_free_error(start, truth)}
     for mode, stored in (("rebuilt", False), ("stored", True)):
         fit = diff.Fit(rs, stored_gbuffer=stored)
@@ -252,6 +250,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batched measurement")
     ap.add_argument("--no-fit", action="store_true", help="skip the batched inverse-rendering measurement")
     ap.add_argument("--small", action="store_true", help="reduced scene for a quick check")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run the 64-view batched measurement alone (for a profile of the batch)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default knobs instead of prh_autotune's choice (untimed, before the warm-up)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -275,6 +275,12 @@ def main() -> int:
             torch.neg(src, out=dst)  # an elementwise kernel: reads and writes MIB MiB
         torch.cuda.synchronize()
         del src, dst
+
+    if args.only_batch:
+        dev = render.Device(0)
+        print(json.dumps({"batched": batch_bench(dev, 3)}))
+        dev.close()
+        return 0
 
     if args.small:
         scene = scenes.c2_scale(width=480, height=270, n_spheres=200, grid=50, shadow_size=512)

@@ -365,9 +365,10 @@ int prh_set_tuning(prh_renderer* r, const prh_tuning* t);
 /* Coordinate descent over item_prims {256..4096}, raster_wg_per_cu {1, 2, 4,
  * 8}, shadow_direct_max {0, 16, 32, 64, 128}, main_direct_max {0, 24, 48,
  * 96} and overlap_shadows {0, 1}: each configuration timed as the median of
- * `frames` frames (HIP
- * events, after one warm-up frame); the fastest stays on r and goes to *out
- * (may be NULL).  r holds a frame of it on return. */
+ * `frames` frames (HIP events, after one warm-up frame); the winner is timed
+ * again against the starting knobs over twice the frames and the faster of
+ * the two stays on r and goes to *out (may be NULL).  r holds a frame of it
+ * on return. */
 int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out);
 
 /* Library build identification ("libprh gfx950 ..."). */

@@ -2138,7 +2138,8 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
     static const int32_t c_sd[] = {0, 16, 32, 64, 128};
     static const int32_t c_md[] = {0, 24, 48, 96};
     static const int32_t c_ov[] = {0, 1};
-    prh_tuning best = r->tune;
+    const prh_tuning start = r->tune;
+    prh_tuning best = start;
     float best_ms = 0.0f;
     int rc = time(best, &best_ms);
     if (rc) return rc;
@@ -2158,6 +2159,15 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
             if ((rc = time(t, &ms))) return rc;
             if (ms < best_ms) best = t, best_ms = ms;
         }
+    }
+    // a second look at the winner against the starting point, over twice the
+    // frames, so that one noisy median does not move the knobs
+    if (std::memcmp(&best, &start, offsetof(prh_tuning, ms)) != 0) {
+        frames *= 2;
+        float ms_best = 0.0f, ms_start = 0.0f;
+        if ((rc = time(start, &ms_start)) || (rc = time(best, &ms_best))) return rc;
+        if (ms_start <= ms_best) best = start, best_ms = ms_start;
+        else best_ms = ms_best;
     }
     best.ms = best_ms;
     if ((rc = prh_set_tuning(r, &best))) return rc;
