@@ -309,6 +309,14 @@ def main() -> int:
         acc["shade"] += st.ms_shade
         acc["ao"] += st.ms_ao
     dt = time.perf_counter() - t0
+    # (outside the timed region) the frame plus its image copied to the host
+    # over PCIe, as a caller that reads every frame back sees it
+    n_rb = max(3, args.steps // 4)
+    t_rb = time.perf_counter()
+    for _ in range(n_rb):
+        r.render_device()
+        r.image()
+    ms_readback = 1000.0 * (time.perf_counter() - t_rb) / n_rb
     ms = {k: v / args.steps for k, v in acc.items()}
     vis = r.read(abi.PRH_BUF_VIS)
     img = r.image()
@@ -356,6 +364,7 @@ def main() -> int:
                                visible_tris=sb["visible_tris"], visible_verts=sb["visible_verts"]),
         "frame": {"prims": st.n_prims, "clipped": st.n_clipped, "bin_pairs": st.n_bin_pairs},
         "tuning": tuned,
+        "ms_per_frame_with_readback": round(ms_readback, 4),
     }
     r.free()
     if not args.no_adjoint:
