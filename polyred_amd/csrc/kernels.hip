@@ -275,6 +275,8 @@ struct k_vertex_main_args {
     float4* tv_scr;
     float4* tv_nor;
     float4* tv_wp;
+    uint32_t* tv_col;
+    float2* tv_uv;
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_vertex_main_body(const k_vertex_main_args& a_, const VB vb) {
@@ -285,6 +287,8 @@ __device__ __forceinline__ void k_vertex_main_body(const k_vertex_main_args& a_,
     float4* __restrict__ tv_scr = a_.tv_scr;
     float4* __restrict__ tv_nor = a_.tv_nor;
     float4* __restrict__ tv_wp = a_.tv_wp;
+    uint32_t* __restrict__ tv_col = a_.tv_col;
+    float2* __restrict__ tv_uv = a_.tv_uv;
 
     int i = vb.bx * blockDim.x + threadIdx.x;
     if (i >= s.n_tverts) return;
@@ -299,6 +303,9 @@ __device__ __forceinline__ void k_vertex_main_body(const k_vertex_main_args& a_,
     tv_scr[i] = make_float4(sp.x, sp.y, sp.z, rw);
     tv_nor[i] = make_float4(nn.x, nn.y, nn.z, 0.0f);
     tv_wp[i] = make_float4(m.x, m.y, m.z, 0.0f);
+    // the vertex colour and UV by transformed vertex, for the shading
+    tv_col[i] = s.vcol[mv];
+    if (tv_uv) tv_uv[i] = s.vuv[mv];
 }
 __global__ __launch_bounds__(256) void k_vertex_main(k_vertex_main_args a) {
     k_vertex_main_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
@@ -439,6 +446,7 @@ struct k_setup_main_args {
     Capacities cap;
     unsigned long long* vis;
     int32_t draw_small;
+    int4* tri_rec;
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, const VB vb) {
@@ -454,6 +462,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
     const Capacities& cap = a_.cap;
     unsigned long long* __restrict__ vis = a_.vis;
     const int32_t draw_small = a_.draw_small;
+    int4* __restrict__ tri_rec = a_.tri_rec;
 
     const int g = vb.bx * blockDim.x + threadIdx.x;
     bool direct = false, doclip = false, drawn = false;
@@ -466,8 +475,13 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
         const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
         if (s.tri_valid[t]) {
             const int vb = s.obj_vtx_start[o];
-            const float4 A = tv_scr[vb + s.tri_v[3 * t]], B = tv_scr[vb + s.tri_v[3 * t + 1]],
-                         Cc = tv_scr[vb + s.tri_v[3 * t + 2]];
+            const int i0 = vb + s.tri_v[3 * t], i1 = vb + s.tri_v[3 * t + 1], i2 = vb + s.tri_v[3 * t + 2];
+            // the shading's record of the triangle: its transformed vertices and
+            // its material in the flat table (-1: vertex colour), matAt's rule
+            int mat = s.tri_mat[t];
+            if (mat >= 0) mat += s.obj_mat_base[o];
+            tri_rec[g] = make_int4(i0, i1, i2, (mat >= 0 && mat < s.n_materials) ? mat : -1);
+            const float4 A = tv_scr[i0], B = tv_scr[i1], Cc = tv_scr[i2];
             p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(Cc.x, Cc.y, Cc.z, 1);
             if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {  // cull.go:26-28
                 Box3 tb = box_of(p1, p2, p3);
@@ -1164,6 +1178,7 @@ struct k_resolve_args {
     float* depth;
     DevScene s;
     uint8_t* ao_need;
+    const int4* tri_rec;
     int32_t gx, gy;  // the grid of this launch
 };
 template <bool EXACT>
@@ -1173,6 +1188,7 @@ __device__ __forceinline__ void k_resolve_body(const k_resolve_args& a_, const V
     float* __restrict__ depth = a_.depth;
     const DevScene& s = a_.s;
     uint8_t* __restrict__ ao_need = a_.ao_need;
+    const int4* __restrict__ tri_rec = a_.tri_rec;
 
     const size_t o = vb.bx * (size_t)blockDim.x + threadIdx.x;
     if (o >= (size_t)rt.W * rt.H || cnt->overflow) return;
@@ -1196,11 +1212,8 @@ __device__ __forceinline__ void k_resolve_body(const k_resolve_args& a_, const V
             if (o == 0) {
                 need = true;
             } else {
-                const int g = (int)((uint32_t)v >> 3);
-                const int ob = s.tri_obj[g];
-                int mat = s.tri_mat[s.mesh_tri_off[s.obj_mesh[ob]] + (g - s.obj_tri_start[ob])];
-                if (mat >= 0) mat += s.obj_mat_base[ob];
-                need = mat >= 0 && mat < s.n_materials && s.mats[mat].ambient_occlusion;
+                const int mat = tri_rec[(uint32_t)v >> 3].w;
+                need = mat >= 0 && s.mats[mat].ambient_occlusion;
             }
         }
         ao_need[o] = need;
@@ -1242,25 +1255,22 @@ __device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const S
                                 uint64_t key) {
     const uint32_t ord = (uint32_t)key;
     const int g = (int)(ord >> 3), piece = (int)(ord & 7);
-    const int o = s.tri_obj[g];
-    const int mesh = s.obj_mesh[o];
-    const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
+    const int4 rec = b.tri_rec[g];  // the setup's record: vertices and material
     V4 P[3], N[3], M[3];
     float rw[3];
     uint32_t C3[3];
     float2 UV[3];
     const bool tex = s.n_textures > 0;
     if (piece == (int)SUB_DIRECT) {
-        const int vb = s.obj_vtx_start[o], mvb = s.mesh_vtx_off[mesh];
+        const int vid[3] = {rec.x, rec.y, rec.z};
         for (int k = 0; k < 3; k++) {
-            int lv = s.tri_v[3 * t + k];
-            float4 a = b.tv_scr[vb + lv];
+            const float4 a = b.tv_scr[vid[k]];
             P[k] = v4(a.x, a.y, a.z, 1);
             rw[k] = a.w;
-            N[k] = ld(b.tv_nor[vb + lv]);
-            M[k] = ld(b.tv_wp[vb + lv]);
-            C3[k] = s.vcol[mvb + lv];
-            UV[k] = tex ? s.vuv[mvb + lv] : make_float2(0.0f, 0.0f);
+            N[k] = ld(b.tv_nor[vid[k]]);
+            M[k] = ld(b.tv_wp[vid[k]]);
+            C3[k] = b.tv_col[vid[k]];
+            UV[k] = tex ? b.tv_uv[vid[k]] : make_float2(0.0f, 0.0f);
         }
     } else {
         const ClipRec& r = b.clips[b.tri_clip_slot[g] + piece];
@@ -1291,9 +1301,8 @@ __device__ GFrag fetch_fragment(const DevScene& s, const ShadeConsts& c, const S
     f.facenor = unit(cross(sub(m2, m1), sub(m3, m1)));
     f.wpos = v4(bc[0] * m1.x + bc[1] * m2.x + bc[2] * m3.x, bc[0] * m1.y + bc[1] * m2.y + bc[2] * m3.y,
                 bc[0] * m1.z + bc[1] * m2.z + bc[2] * m3.z, 1);
-    int mat = s.tri_mat[t];
-    if (mat >= 0) mat += s.obj_mat_base[o];
-    f.mat = (mat >= 0 && mat < s.n_materials) ? mat : -1;
+    const int mat = rec.w;
+    f.mat = mat;
     f.u = f.v = f.du = f.dv = 0.0f;
     if (tex) {
         f.u = (wc1 * UV[0].x + wc2 * UV[1].x + wc3 * UV[2].x) * norm;
@@ -2125,10 +2134,10 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
     hipLaunchKernelGGL(k_fill_u64, dim3(std::min<size_t>(blocks_for(n, 256), 4096)), dim3(256), 0, st, p, n, v);
 }
 void launch_vertex_main(hipStream_t st, const DevScene& s, const M4* trans, const M4* normal, const ViewConsts& vc,
-                        float4* tv_scr, float4* tv_nor, float4* tv_wp) {
+                        float4* tv_scr, float4* tv_nor, float4* tv_wp, uint32_t* tv_col, float2* tv_uv) {
     if (!s.n_tverts) return;
     go(k_vertex_main, k_vertex_main_wide, dim3(blocks_for(s.n_tverts, 256)), 256, st,
-       k_vertex_main_args{s, trans, normal, vc, tv_scr, tv_nor, tv_wp, 0, 0});
+       k_vertex_main_args{s, trans, normal, vc, tv_scr, tv_nor, tv_wp, tv_col, tv_uv, 0, 0});
 }
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float4* ts) {
     if (!s.n_tverts || !sc.n_slots) return;
@@ -2138,11 +2147,11 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, c
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
-                       unsigned long long* vis, int32_t draw_small) {
+                       unsigned long long* vis, int32_t draw_small, int4* tri_rec) {
     if (!s.n_tris) return;
     go(k_setup_main, k_setup_main_wide, dim3(blocks_for(s.n_tris, 256)), 256, st,
-       k_setup_main_args{s, vc, tv_scr, prims, pbox, clip_list, tile_count, blk_prims, cnt, cap, vis, draw_small, 0,
-                         0});
+       k_setup_main_args{s, vc, tv_scr, prims, pbox, clip_list, tile_count, blk_prims, cnt, cap, vis, draw_small,
+                         tri_rec, 0, 0});
     go(k_clip, k_clip_wide, dim3(std::min<size_t>(blocks_for(cap.clip_tris, 64), CLIP_GRID)), 64, st,
        k_clip_args{s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap, 0, 0});
 }
@@ -2182,9 +2191,9 @@ void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims
     else go(k_raster<MODE_VIS>, k_raster_wide<MODE_VIS>, dim3(n_workgroups), 256, st, a);
 }
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
-                    uint8_t* ao_need) {
+                    uint8_t* ao_need, const int4* tri_rec) {
     const size_t P = (size_t)rt.W * rt.H;
-    const k_resolve_args a{rt, cnt, depth, s, ao_need, 0, 0};
+    const k_resolve_args a{rt, cnt, depth, s, ao_need, tri_rec, 0, 0};
     if (exact) go(k_resolve<true>, k_resolve_wide<true>, dim3(blocks_for(P, 256)), 256, st, a);
     else go(k_resolve<false>, k_resolve_wide<false>, dim3(blocks_for(P, 256)), 256, st, a);
 }

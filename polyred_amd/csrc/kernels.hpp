@@ -23,6 +23,9 @@ struct ShadeBuffers {
     const float4* tv_scr;
     const float4* tv_nor;
     const float4* tv_wp;
+    const int4* tri_rec;       // per triangle: transformed vertices and flat material (k_setup_main)
+    const uint32_t* tv_col;    // per transformed vertex: colour
+    const float2* tv_uv;       // and UV (scenes with textures)
     const ClipRec* clips;
     const int32_t* tri_clip_slot;
     const uint32_t* smaps;     // [slots][SH][SW] ordered depth bits
@@ -113,14 +116,15 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
 void launch_frame_clear(hipStream_t st, uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_t n_vis);
 void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, const gm::M4* normal,
-                        const ViewConsts& vc, float4* tv_scr, float4* tv_nor, float4* tv_wp);
+                        const ViewConsts& vc, float4* tv_scr, float4* tv_nor, float4* tv_wp, uint32_t* tv_col,
+                        float2* tv_uv);
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
                           float4* ts);
 // pbox: the pixel box of every primitive slot (k_bin's input)
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
-                       unsigned long long* vis, int32_t draw_small);
+                       unsigned long long* vis, int32_t draw_small, int4* tri_rec);
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
                          Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
                          uint32_t* smaps);
@@ -139,7 +143,7 @@ void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
                    Counters* cnt, const RasterTarget& rt, int32_t item_prims);
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
-                    uint8_t* ao_need);
+                    uint8_t* ao_need, const int4* tri_rec);
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);

@@ -377,6 +377,9 @@ struct prh_renderer {
     std::vector<void*> allocs;
     M4 *d_trans, *d_normal, *d_strans, *d_lview, *d_lproj;
     float4 *tv_scr, *tv_nor, *tv_wp, *ts;
+    int4* tri_rec;     // per triangle: transformed vertices and flat material (the setup writes it)
+    uint32_t* tv_col;  // per transformed vertex: colour
+    float2* tv_uv;     // and UV (scenes with textures)
     Prim* prims;
     uint2* pbox;  // pixel box per primitive slot
     ClipRec* clips;
@@ -933,6 +936,9 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->tv_scr, Vt));
     if (!rc) chk(ralloc(r, &r->tv_nor, Vt));
     if (!rc) chk(ralloc(r, &r->tv_wp, Vt));
+    if (!rc) chk(ralloc(r, &r->tri_rec, Tg));
+    if (!rc) chk(ralloc(r, &r->tv_col, Vt));
+    if (!rc && s->d.n_textures > 0) chk(ralloc(r, &r->tv_uv, Vt));
     if (!rc) chk(ralloc(r, &r->ts, (size_t)std::max(1, r->n_slots) * Vt));
     if (!rc) chk(ralloc(r, &r->tri_clip_slot, Tg));
     if (!rc) chk(ralloc(r, &r->blk_prims, (size_t)(Tg + 255) / 256));
@@ -1058,6 +1064,7 @@ uint32_t* final_image(const prh_renderer* r) { return r->present_to ? r->present
 ShadeBuffers shade_buffers(const prh_renderer* r) {
     ShadeBuffers b;
     b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
+    b.tri_rec = r->tri_rec, b.tv_col = r->tv_col, b.tv_uv = r->tv_uv;
     b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
     b.light_view = r->d_lview, b.light_proj = r->d_lproj;
     b.image = r->msaa > 1 ? r->image_big : final_image(r), b.pregamma = r->pregamma, b.linear = r->linear;
@@ -1121,12 +1128,12 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     }
     if (timed) (void)hipEventRecord(r->ev[1], st);
     wide_stage(7);
-    launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp);
+    launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp, r->tv_col, r->tv_uv);
     if (timed) (void)hipEventRecord(r->ev[2], st);
     wide_stage(8);
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
                       r->tile_count, r->blk_prims, cm, r->cap_main, (unsigned long long*)r->vis,
-                      nan_mode ? 0 : r->tune.main_direct_max);
+                      nan_mode ? 0 : r->tune.main_direct_max, r->tri_rec);
     wide_stage(9);
     launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
                 r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm, ip);
@@ -1144,7 +1151,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                   r->tile_list, cm, rt, ip);
     wide_stage(12);
-    launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr);
+    launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr, r->tri_rec);
     if (timed) (void)hipEventRecord(r->ev[4], st);
     const ShadeBuffers b = shade_buffers(r);
     wide_stage(13);
