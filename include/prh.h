@@ -371,6 +371,12 @@ int prh_set_tuning(prh_renderer* r, const prh_tuning* t);
  * on return. */
 int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out);
 
+/* Page-locked host memory for the read calls (prh_read_image,
+ * prh_read_buffer, prh_surface_read): a copy into it runs at the link's
+ * full rate, where a copy into pageable memory is staged by the runtime. */
+int prh_host_alloc(size_t bytes, void** out);
+int prh_host_free(void* p);
+
 /* Library build identification ("libprh gfx950 ..."). */
 const char* prh_version(void);
 

@@ -205,6 +205,8 @@ PROTOTYPES = {
     "prh_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(prh_tuning)]),
     "prh_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(prh_tuning)]),
     "prh_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(prh_tuning)]),
+    "prh_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "prh_host_free": (C.c_int, [C.c_void_p]),
     "prh_surface_new": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "prh_surface_free": (C.c_int, [C.c_void_p]),
     "prh_surface_acquire": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),

@@ -2184,4 +2184,19 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
     return PRH_OK;
 }
 
+int prh_host_alloc(size_t bytes, void** out) {
+    if (!out || bytes == 0) return fail(PRH_E_INVALID, "prh_host_alloc: bad argument");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(PRH_E_NOMEM, "prh_host_alloc: hipHostMalloc failed");
+    }
+    return PRH_OK;
+}
+
+int prh_host_free(void* p) {
+    if (p) HIP_TRY(hipHostFree(p));
+    return PRH_OK;
+}
+
 }  // extern "C"

@@ -236,8 +236,13 @@ class Renderer:
     def passOnGPU(self, name: str) -> bool:  # render/raster.go:79
         return self.passGPU.get(name, False)
 
-    def image(self) -> np.ndarray:
-        out = np.empty((self.H, self.W, 4), np.uint8)
+    def image(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """The frame's RGBA8 image (H, W, 4), top row first; into `out` when
+        given (a pinned_array reads at the link's full rate)."""
+        if out is None:
+            out = np.empty((self.H, self.W, 4), np.uint8)
+        if out.dtype != np.uint8 or out.shape != (self.H, self.W, 4) or not out.flags.c_contiguous:
+            raise ValueError("image(out=...) needs a C-contiguous (H, W, 4) uint8 array")
         abi.check(self._lib.prh_read_image(self.handle, out.ctypes.data, out.nbytes), "prh_read_image")
         return out
 
@@ -432,6 +437,19 @@ class Surface:
 def CreateSurface(dev: Device, width: int, height: int, frames: int = 2) -> Surface:
     """Device.CreateSurface (gpu/surface.go:43-57)."""
     return Surface(dev, width, height, frames)
+
+
+def pinned_array(shape, dtype=np.uint8) -> np.ndarray:
+    """A numpy array in page-locked host memory (prh_host_alloc), freed with
+    the array."""
+    lib = abi.load()
+    n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    p = C.c_void_p()
+    abi.check(lib.prh_host_alloc(n, C.byref(p)), "prh_host_alloc")
+    buf = (C.c_uint8 * n).from_address(p.value)
+    arr = np.frombuffer(buf, np.uint8).view(dtype).reshape(shape)
+    weakref.finalize(arr, lib.prh_host_free, p.value)
+    return arr
 
 
 def views_of(scene, views, dev: Device):

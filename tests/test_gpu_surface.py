@@ -69,3 +69,15 @@ def test_bad_surfaces_are_rejected(device):
         with pytest.raises(abi.PrhError) as e:
             render.CreateSurface(device, w, h)
         assert e.value.code == abi.PRH_E_INVALID
+
+
+def test_pinned_readback(device):
+    """prh_host_alloc memory as the destination of prh_read_image."""
+    s = scenes.c1_plain(width=72, height=40)
+    r = render.renderer_for(s, device)
+    want = r.Render()
+    host = render.pinned_array((40, 72, 4))
+    assert r.image(host) is host and np.array_equal(host, want)
+    with pytest.raises(ValueError):
+        r.image(np.empty((40, 72, 3), np.uint8))
+    r.free()
