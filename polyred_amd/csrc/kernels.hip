@@ -1456,6 +1456,25 @@ __device__ bool fragment_shader(const DevScene& s, const ShadeConsts& c, const D
 
 // render/shadow.go:230-283 summed over all lights (raster.go:339-347): the
 // number of lights for which shadingVisibility reports true.
+// apply() of a matrix whose last row is (0 0 0 1), for a finite v: the zero
+// terms of the last row's fma64 chain add zeros, so its result is v.w (up to
+// the sign of a zero).  apply_diag: also zeros off the diagonal of the upper
+// 3x3 (orthographic projections, viewports), each row one fma64.  Nothing
+// the shadow test computes from these vectors tells the sign of a zero: pos()
+// tests w == 0 before dividing, and int() and the depth comparison treat
+// -0 as 0.
+__device__ __forceinline__ V4 apply_last_row_w(V4 v, const M4& m) {
+    const float* a = m.a;
+    return v4(fma64(a[0], v.x, fma64(a[1], v.y, fma64(a[2], v.z, a[3] * v.w))),
+              fma64(a[4], v.x, fma64(a[5], v.y, fma64(a[6], v.z, a[7] * v.w))),
+              fma64(a[8], v.x, fma64(a[9], v.y, fma64(a[10], v.z, a[11] * v.w))), v.w);
+}
+__device__ __forceinline__ V4 apply_diag(V4 v, const M4& m) {
+    const float* a = m.a;
+    return v4(fma64(a[0], v.x, a[3] * v.w), fma64(a[5], v.y, a[7] * v.w), fma64(a[10], v.z, a[11] * v.w), v.w);
+}
+__device__ __forceinline__ bool finite4(V4 v) { return fin(v.x) && fin(v.y) && fin(v.z) && fin(v.w); }
+
 __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int fx, int fy, float depth) {
     int occ = 0;
     const int64_t len = (int64_t)c.SW * c.SH;
@@ -1467,9 +1486,16 @@ __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuff
             occ++;
             continue;
         }
-        V4 q = apply(qw, b.light_view[i]);
-        q = apply(q, b.light_proj[i]);
-        q = pos(apply(q, c.svp));
+        V4 q;
+        if (c.shadow_sparse && finite4(qw)) {
+            q = apply_last_row_w(qw, b.light_view[i]);
+            q = finite4(q) ? apply_diag(q, b.light_proj[i]) : apply(q, b.light_proj[i]);
+            q = pos(finite4(q) ? apply_diag(q, c.svp) : apply(q, c.svp));
+        } else {
+            q = apply(qw, b.light_view[i]);
+            q = apply(q, b.light_proj[i]);
+            q = pos(apply(q, c.svp));
+        }
         int64_t lx = f2i(q.x), ly = f2i(q.y);
         int64_t idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)c.SW);
         if (idx > 0 && idx < len) {

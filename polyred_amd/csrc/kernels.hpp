@@ -15,6 +15,8 @@ struct ShadeConsts {
     float ao_dir[16];         // material/ao.go:26 directions {cos, sin} x 8
     int32_t SW, SH;           // shadow map size
     gm::M4 svp;               // ViewportMatrix(SW, SH)
+    int32_t shadow_sparse;    // the light views' last row is (0 0 0 1) and the light projections and svp
+                              // have zeros everywhere but the diagonal and the last column
 };
 
 struct ShadeBuffers {

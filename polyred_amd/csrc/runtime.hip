@@ -906,6 +906,21 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     r->ao_fast = ao_tab_ok && r->W <= 4096 && r->H <= 4096;
     sh.SW = r->SW, sh.SH = r->SH;
     sh.svp = shc.vp;
+    {
+        // the zero pattern that lets the shading's shadow test skip zero
+        // terms (checked here, entry by entry, for this renderer's matrices)
+        auto last_row = [](const M4& m) { return m.a[12] == 0 && m.a[13] == 0 && m.a[14] == 0 && m.a[15] == 1; };
+        auto diag_col = [&](const M4& m) {
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    if (i != j && m.a[4 * i + j] != 0) return false;
+            return last_row(m);
+        };
+        bool sparse = diag_col(sh.svp);
+        for (int i = 0; i < L && shadows; i++)
+            if (s->lights[i].cast_shadow) sparse = sparse && last_row(lview[i]) && diag_col(lproj[i]);
+        sh.shadow_sparse = sparse ? 1 : 0;
+    }
 
     const size_t P = (size_t)r->W * r->H;
     const int Tg = s->d.n_tris, Vt = s->d.n_tverts;
