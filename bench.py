@@ -38,11 +38,10 @@ def shade_bytes(scene, vis: np.ndarray) -> dict:
     the 8-byte visibility key and the 4-byte RGBA8 output.  Per covered pixel:
     the 4-byte AO factor if its material has AO, and one 4-byte shadow texel
     per shadow map if it receives shadows.  Per distinct visible triangle: its
-    4-byte object index (an unclipped one) or its 144-byte clip record (a clip
-    piece).  Per distinct vertex of the visible unclipped triangles: the
-    3 x 16-byte transformed vertex (screen xyz + 1/w, normal, back-transformed
-    position).  Per distinct visible mesh triangle: its 12-byte index triple
-    and 4-byte material; per distinct mesh vertex, its 4-byte colour."""
+    16-byte shading record (vertex indices, material), and per visible clip
+    piece its 144-byte clip record.  Per distinct vertex of the visible
+    unclipped triangles: the 3 x 16-byte transformed vertex (screen xyz + 1/w,
+    normal, back-transformed position) and its 4-byte colour."""
     P = vis.size
     cov = vis != EMPTY
     order = (vis[cov] & np.uint64(0xFFFFFFFF)).astype(np.int64)
@@ -64,8 +63,6 @@ def shade_bytes(scene, vis: np.ndarray) -> dict:
     od = np.searchsorted(obj_tri_start, ug, side="right") - 1
     td = mesh_tri_off[np.asarray(scene.obj_mesh)[od]] + (ug - obj_tri_start[od])
     verts = np.unique((obj_vtx_start[od][:, None] + tri_v_all[td]).ravel())
-    mesh_tris = np.unique(t)
-    mesh_verts = np.unique((mesh_vtx_off[mesh][:, None] + tri_v_all[t]).ravel())
     mat = tri_mat_all[t] + np.where(tri_mat_all[t] >= 0, np.asarray(scene.obj_mat_base)[o], 0)
     ok = (mat >= 0) & (mat < len(scene.materials))
     ao_m = np.array([m.ambient_occlusion for m in scene.materials] + [False])
@@ -74,8 +71,7 @@ def shade_bytes(scene, vis: np.ndarray) -> dict:
     n_shadow = sum(1 for l in scene.lights if l.cast_shadow) if scene.shadow_map else 0
     n_ao = int(ao_m[mi].sum())
     n_rs = int(rs_m[mi].sum()) if n_shadow else 0
-    b = (P * (8 + 4) + n_ao * 4 + n_rs * 4 * n_shadow + ug.size * 4 + n_clip * 144 + verts.size * 48
-         + mesh_tris.size * 16 + mesh_verts.size * 4)
+    b = (P * (8 + 4) + n_ao * 4 + n_rs * 4 * n_shadow + np.unique(g).size * 16 + n_clip * 144 + verts.size * 52)
     return dict(bytes=int(b), pixels=P, covered=int(cov.sum()), visible_tris=int(ug.size + n_clip),
                 visible_verts=int(verts.size), ao_px=n_ao)
 
