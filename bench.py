@@ -51,7 +51,6 @@ def shade_bytes(scene, vis: np.ndarray) -> dict:
     obj_tri_start = np.concatenate([[0], np.cumsum(tri_counts)])
     obj_vtx_start = np.concatenate([[0], np.cumsum(vtx_counts)])
     mesh_tri_off = np.concatenate([[0], np.cumsum([m.tri.shape[0] for m in scene.meshes])])
-    mesh_vtx_off = np.concatenate([[0], np.cumsum([m.pos.shape[0] for m in scene.meshes])])
     tri_v_all = np.concatenate([m.tri for m in scene.meshes]).astype(np.int64)
     tri_mat_all = np.concatenate([m.mat for m in scene.meshes]).astype(np.int64)
     o = np.searchsorted(obj_tri_start, g, side="right") - 1
