@@ -135,7 +135,8 @@ typedef struct prh_camera {
 /* render.option fields the device path honours (render/options.go:16-33). */
 typedef struct prh_options {
     int32_t width, height;   /* render.Size                                    */
-    int32_t msaa;            /* render.MSAA; only 1 is on the device path      */
+    int32_t msaa;            /* render.MSAA, 1..16: every pass at (w*m, h*m),
+                                then imageutil.Resize                          */
     uint8_t background[4];   /* render.Background                              */
     int32_t shadow_map;      /* render.ShadowMap                               */
     int32_t gamma_correct;   /* render.GammaCorrection                         */
@@ -149,6 +150,12 @@ typedef struct prh_options {
 typedef struct prh_ctx prh_ctx;
 typedef struct prh_scene prh_scene;
 typedef struct prh_renderer prh_renderer;
+
+/* Threads: every call may come from any OS thread (each sets its context's
+ * device first, as goroutines migrate).  Calls on one context -- its
+ * scenes, renderers, fits and surfaces -- must not overlap (the Go side
+ * renders from one goroutine); different contexts may be used from
+ * different threads at once. */
 
 /* Thread-local text of the last error on the calling thread. */
 const char* prh_last_error(void);

@@ -2085,7 +2085,7 @@ struct WideBatch {
     std::map<int, WideGroup> groups;
     int stage = 0, sub = 0;
 };
-static WideBatch* g_wide = nullptr;
+static thread_local WideBatch* g_wide = nullptr;  // the calling thread's recording batch
 
 WideBatch* wide_new() { return new WideBatch(); }
 void wide_free(WideBatch* w) { delete w; }
