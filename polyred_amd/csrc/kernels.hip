@@ -1538,7 +1538,7 @@ __device__ uint32_t shade_fragment(const DevScene& s, const ShadeConsts& c, cons
         float w = __builtin_ldexpf(1.0f, -occ);  // math.Pow(0.5, visibles), exact
         col = u8x4(f2u8(chan(col, 0) * w), f2u8(chan(col, 1) * w), f2u8(chan(col, 2) * w), (uint8_t)(col >> 24));
     }
-    if (m.ambient_occlusion) col = ao_apply(col, b.ao[(size_t)fy * c.v.W + fx]);
+    if (m.ambient_occlusion && !b.col_pre) col = ao_apply(col, b.ao[(size_t)fy * c.v.W + fx]);
     return col;
 }
 
@@ -1955,12 +1955,24 @@ __device__ __forceinline__ void k_shade_body(const k_shade_args& a_, const VB vb
     float lin[3] = {__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf("")};
     bool hl = false;
     if (key == VIS_EMPTY) {
+        if (b.col_pre) return;  // (the second half shades uncovered pixels)
         col = b.bg[0];
         lin[0] = b.bg_lin[0], lin[1] = b.bg_lin[1], lin[2] = b.bg_lin[2];
         hl = true;
     } else {
         GFrag f = fetch_fragment(s, c, b, x, y, key);
         col = shade_fragment(s, c, b, f, x, y, f.mat, f.col, lin, hl);
+    }
+    if (b.col_pre) {
+        // first half of a split shading: the colour before AO, and the
+        // linear sums (which AO does not touch)
+        b.col_pre[o] = col;
+        if (b.linear) {
+            b.linear[3 * o] = hl ? lin[0] : __builtin_nanf("");
+            b.linear[3 * o + 1] = hl ? lin[1] : __builtin_nanf("");
+            b.linear[3 * o + 2] = hl ? lin[2] : __builtin_nanf("");
+        }
+        return;
     }
     if (b.pregamma) b.pregamma[o] = col;
     if (b.linear) {
@@ -1999,6 +2011,29 @@ __global__ __launch_bounds__(256) void k_shadow_count(DevScene s, ShadeConsts c,
         if (f.mat >= 0 && s.mats[f.mat].receive_shadow) occ = occluded(s, c, b, x, y, f.depth);
     }
     out[i] = (uint8_t)occ;
+}
+
+// Second half of a split shading (the first half, k_shade with col_pre set,
+// ran beside k_ao): the AO factor on the covered pixels whose material has
+// AO (AmbientOcclusionShade after the shadow halving, raster.go:339-357),
+// the uncovered pixels' colour, then pre-gamma, gamma and the y-flipped store
+// -- what k_shade does after shade_fragment.
+__global__ __launch_bounds__(256) void k_shade_post(DevScene s, ShadeConsts c, ShadeBuffers b) {
+    const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (o >= (size_t)c.v.W * c.v.H || b.cnt->overflow) return;
+    const int x = (int)(o % (size_t)c.v.W), y = (int)(o / (size_t)c.v.W);
+    const uint64_t key = b.vis[o];
+    uint32_t col;
+    if (key == VIS_EMPTY) {
+        col = b.bg[0];
+        if (b.linear) b.linear[3 * o] = b.bg_lin[0], b.linear[3 * o + 1] = b.bg_lin[1], b.linear[3 * o + 2] = b.bg_lin[2];
+    } else {
+        col = b.col_pre[o];
+        const int mat = b.tri_rec[(uint32_t)key >> 3].w;
+        if (mat >= 0 && s.mats[mat].ambient_occlusion) col = ao_apply(col, b.ao[o]);
+    }
+    if (b.pregamma) b.pregamma[o] = col;
+    b.image[(size_t)(c.v.H - 1 - y) * c.v.W + x] = gamma(c, col);
 }
 
 // ------------------------------------------------------------- launchers
@@ -2237,6 +2272,11 @@ void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const
     go(k_shade, k_shade_wide, dim3(per_xcd * 8), 256, st, k_shade_args{s, c, b, 0, 0});
 }
 
+void launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
+    const size_t P = (size_t)c.v.W * c.v.H;
+    if (!b.col_pre || !b.ao || !b.tri_rec) std::abort();  // the caller's contract: the first half's buffers
+    if (P) hipLaunchKernelGGL(k_shade_post, dim3(blocks_for(P, 256)), dim3(256), 0, st, s, c, b);
+}
 void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
                          uint8_t* out) {
     const size_t P = (size_t)c.v.W * c.v.H;

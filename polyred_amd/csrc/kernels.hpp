@@ -44,6 +44,7 @@ struct ShadeBuffers {
     const uint8_t* ao_need;    // [H][W] 1 where k_ao computes the factor
     const float* ao_tab;       // AO march tables (tap offsets, diagonal distances)
     int32_t ao_fast;           // the tables hold for this frame size
+    uint32_t* col_pre;         // set: k_shade is the first half of a split shading (colour before AO)
 };
 
 // The differentiable shading pass (diff.hip): parameters, fixed inputs and
@@ -149,6 +150,7 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
+void launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
                          uint8_t* out);
 

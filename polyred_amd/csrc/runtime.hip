@@ -395,6 +395,8 @@ struct prh_renderer {
     int32_t s_cap_items;
     int32_t s_tile_slots;  // shadow slots x tiles per map
     hipEvent_t ev_fork, ev_join, ev_s[2];
+    hipEvent_t ev_res, ev_sh[2];  // split shading: resolve done; the first half's span
+    uint32_t* col_pre;            // split shading: the colour before AO
     int32_t tile_slots;
     int32_t n_cu;
     uint32_t *first_all, *first_nan;
@@ -430,6 +432,7 @@ struct prh_renderer {
     int live_fits;  // a renderer outlives every prh_fit holding it as a view
     uint32_t* present_to;  // during prh_surface_render: the acquired surface texture
     prh_tuning tune;       // speed knobs (results do not depend on them)
+    bool split_shade_last;  // the last frame split its shading
 };
 
 namespace {
@@ -976,6 +979,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->depth, P));
     if (!rc) chk(ralloc(r, &r->ao, P));
     if (!rc && s->any_ao) chk(ralloc(r, &r->ao_need, P));
+    if (!rc && s->any_ao && r->n_slots > 0) chk(ralloc(r, &r->col_pre, P));
     if (!rc && msaa == 1) chk(ralloc(r, &r->image, P));
     if (!rc && msaa > 1) {
         // Resize(width, height, img): scale = old / new = MSAA (calcFactors,
@@ -1014,7 +1018,9 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     for (int i = 0; i < 2 && !rc; i++)
         if (hipEventCreate(&r->ev_s[i]) != hipSuccess) rc = fail(PRH_E_HIP, "hipEventCreate failed");
     if (!rc && (hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess))
+                hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&r->ev_res, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreate(&r->ev_sh[0]) != hipSuccess || hipEventCreate(&r->ev_sh[1]) != hipSuccess))
         rc = fail(PRH_E_HIP, "hipEventCreate failed");
     if (rc) {
         free_list(r->allocs);
@@ -1061,6 +1067,9 @@ int prh_renderer_free(prh_renderer* r) {
     }
     if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
     if (r->ev_join) (void)hipEventDestroy(r->ev_join);
+    if (r->ev_res) (void)hipEventDestroy(r->ev_res);
+    for (int i = 0; i < 2; i++)
+        if (r->ev_sh[i]) (void)hipEventDestroy(r->ev_sh[i]);
     r->sc->live_renderers--;
     delete r;
     return PRH_OK;
@@ -1089,6 +1098,7 @@ ShadeBuffers shade_buffers(const prh_renderer* r) {
     b.ao = r->ao;
     b.ao_need = r->ao_need;
     b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
+    b.col_pre = nullptr;
     return b;
 }
 
@@ -1116,6 +1126,9 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     // keeps them on its own stream
     const bool split = shadows && !wide && st == r->ctx->stream && r->tune.overlap_shadows;
     const hipStream_t ss = split ? r->ctx->stream2 : st;
+    // with AO as well, the shading's first half (everything but the AO
+    // factor) runs on the second stream beside k_ao
+    const bool split_shade = split && r->sc->any_ao && r->col_pre;
     if (shadows) {
         const int nt = r->n_slots * r->stiles_per_map;
         if (split) {
@@ -1139,7 +1152,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
         launch_raster(ss, 2, persistent, r->s_prims, r->s_tile_off, r->s_chunk_off, r->s_item_tile, r->s_cap_items,
                       r->s_tile_list, cs, rt, ip);
         if (timed) (void)hipEventRecord(r->ev_s[1], ss);
-        if (split) (void)hipEventRecord(r->ev_join, ss);
+        if (split && !split_shade) (void)hipEventRecord(r->ev_join, ss);
     }
     if (timed) (void)hipEventRecord(r->ev[1], st);
     wide_stage(7);
@@ -1169,6 +1182,18 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr, r->tri_rec);
     if (timed) (void)hipEventRecord(r->ev[4], st);
     const ShadeBuffers b = shade_buffers(r);
+    // the split shading's buffers: both halves read and write col_pre
+    ShadeBuffers bp = b;
+    bp.col_pre = r->col_pre;
+    r->split_shade_last = split_shade;
+    if (split_shade) {
+        (void)hipEventRecord(r->ev_res, st);
+        (void)hipStreamWaitEvent(ss, r->ev_res, 0);
+        if (timed) (void)hipEventRecord(r->ev_sh[0], ss);
+        launch_shade(ss, d, r->sh, bp);
+        if (timed) (void)hipEventRecord(r->ev_sh[1], ss);
+        (void)hipEventRecord(r->ev_join, ss);
+    }
     wide_stage(13);
     if (r->sc->any_ao) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
     if (timed) (void)hipEventRecord(r->ev[5], st);
@@ -1177,7 +1202,8 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     launch_shade_bg(st, d, r->sh, b);
     if (timed) (void)hipEventRecord(r->ev[7], st);
     wide_stage(15);
-    launch_shade(st, d, r->sh, b);
+    if (split_shade) launch_shade_post(st, d, r->sh, bp);
+    else launch_shade(st, d, r->sh, b);
     if (r->msaa > 1) {
         // passAntialiasing (render/raster.go:361-378): Resize after gamma
         wide_stage(16);
@@ -1238,6 +1264,11 @@ int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
         if (r->n_slots > 0 && !nan_rerun) (void)hipEventElapsedTime(&S.ms_shadow, r->ev_s[0], r->ev_s[1]);
         S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
         (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
+        if (r->split_shade_last) {  // plus the first half, beside k_ao
+            float pre = 0.0f;
+            (void)hipEventElapsedTime(&pre, r->ev_sh[0], r->ev_sh[1]);
+            S.ms_shade += pre;
+        }
     } else {
         S.ms_total = S.ms_shadow = S.ms_vertex = S.ms_setup = S.ms_raster = S.ms_ao = S.ms_shade = 0.0f;
     }

@@ -25,8 +25,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EMPTY = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 
-def gpu_frame(device, scene, frames=1):
+def gpu_frame(device, scene, frames=1, **knobs):
     r = render.renderer_for(scene, device, keep_linear=True)
+    if knobs:
+        r.set_tuning(**knobs)
     for _ in range(frames):
         r.render_device()
     out = dict(image=r.image(), vis=r.read(abi.PRH_BUF_VIS), depth=r.read(abi.PRH_BUF_DEPTH),
@@ -202,3 +204,13 @@ def test_lifetimes_nest():
     assert lib.prh_renderer_free(rd) == abi.PRH_OK
     assert lib.prh_scene_free(sc) == abi.PRH_OK
     assert lib.prh_close(ctx) == abi.PRH_OK
+
+
+@pytest.mark.parametrize("overlap", [0, 1])
+def test_shadow_passes_beside_main_passes(device, orc, overlap):
+    """The shadow passes on the second stream and, with AO, the shading split
+    around k_ao (its first half beside it): every output bit-exact, at a size
+    whose default keeps one stream."""
+    s = golden_cases.c1_shadow_ao()
+    g = gpu_frame(device, s, frames=2, overlap_shadows=overlap)
+    assert_frame_equal(g, orc.render(s), scene=s)
