@@ -320,6 +320,8 @@ def main() -> int:
     st = r.stats()
     n_shadow = sum(1 for l in scene.lights if l.cast_shadow)
     sb = shade_bytes(scene, vis)
+    split_shade = bool(r.tuning().overlap_shadows and scene.shadow_map and n_shadow > 0
+                       and any(m.ambient_occlusion for m in scene.materials))
     ab = ao_bytes(scene, vis, sb["ao_px"])
     ms_step = 1000.0 * dt / args.steps
     fps = args.steps / dt
@@ -356,9 +358,13 @@ def main() -> int:
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
         # the dominant kernel (AO, VALU-bound: its HBM fraction is small by
         # construction) and the deferred-shading kernel the HBM target names
-        "roofline": roof("k_ao", ab, ms["ao"]),
+        # (with the shadow passes on the second stream and AO in the scene, the
+        # shading's first half runs beside k_ao: both times are then spans of
+        # kernels sharing the GPU, and "overlapped" says so)
+        "roofline": dict(roof("k_ao", ab, ms["ao"]), overlapped=split_shade),
         "roofline_shade": dict(roof("k_shade", sb["bytes"], ms["shade"]), covered_px=sb["covered"],
-                               visible_tris=sb["visible_tris"], visible_verts=sb["visible_verts"]),
+                               visible_tris=sb["visible_tris"], visible_verts=sb["visible_verts"],
+                               overlapped=split_shade),
         "frame": {"prims": st.n_prims, "clipped": st.n_clipped, "bin_pairs": st.n_bin_pairs},
         "tuning": tuned,
         "ms_per_frame_with_readback": round(ms_readback, 4),
