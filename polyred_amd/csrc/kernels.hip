@@ -1550,6 +1550,20 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 
 }  // namespace
 
+// 24-bit multiplies as the one full-rate instruction.  __umul24/__mul24 in a
+// loop whose other operand is loop-invariant compile to a mask hoisted out of
+// the loop and a quarter-rate v_mul_lo_u32 inside it.
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ int32_t mul_i24(int32_t a, int32_t b) {
+    int32_t r;
+    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // Screen-space AO (material/ao.go:20-73) for every pixel whose material has
 // it, plus pixel (0,0) (the uncovered pixels are shaded from it).  One
 // workgroup = one 64-pixel row segment x the 8 march directions, four waves of
@@ -1711,8 +1725,17 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
             for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
             const int tmin = min(__builtin_amdgcn_readfirstlane(tl), 100);
             int ti = 1;
+            // one loop per sign, so each tap's window entry is a constant
+            // offset from the lane's address
+            if (sgn > 0) {
+                const float* wl = win + lane - 1;
 #pragma unroll 4
-            for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(win[base + sgn * ti] - dp) * rinv_t[ti]);
+                for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(wl[ti] - dp) * rinv_t[ti]);
+            } else {
+                const float* wl = win + lane + 99;
+#pragma unroll 4
+                for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(wl[-ti] - dp) * rinv_t[ti]);
+            }
             for (; ti < 100; ti++) {
                 const double q = (double)(win[base + sgn * ti] - dp) * rinv_t[ti];
                 q_best = (need & (ti < tlim) & (q > q_best)) ? q : q_best;
@@ -1722,10 +1745,14 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
             // buffer for t < tl; the march ends there for the whole wave
             const int sgn = axis[1];
             const int tl = min(sgn > 0 ? H - y : y + 1, 100);
+            // the row's offset is uniform (the load's scalar offset), the
+            // lane's column its vector offset: clamped into the row for the
+            // lanes past the frame's edge, whose values nobody reads
+            const int xo = min(x, W - 1) * 4;
 #pragma unroll 8
             for (int ti = 1; ti < tl; ti++) {
-                const uint32_t off = ((uint32_t)((y + sgn * ti) * W) + (uint32_t)x) * 4u;
-                const float e = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
+                const int row = (y + sgn * ti) * W * 4;
+                const float e = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, xo, row, 0)) - dp;
                 q_best = __builtin_fmax(q_best, (double)e * rinv_t[ti]);
             }
         } else if (ao_fast && (wu & 1) == 0) {
@@ -1740,7 +1767,7 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 const float cx = X + off_x[ti], cy = Y + off_y[ti];
                 const int ix = (int)cx, iy = (int)cy;
                 const bool v = need & ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
-                const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
+                const uint32_t off = (mul_u24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
                 take_q(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, rinv_t[ti]);
             }
         } else if (ao_fast) {
@@ -1766,11 +1793,11 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 // the buffer (checked when the table is built); its offset
                 // then reads 0 or another entry, masked either way
                 const int kx = __builtin_amdgcn_frexp_expf(cx), ky = __builtin_amdgcn_frexp_expf(cy);
-                const int toff = __mul24(kx, ROW_X) + __mul24(ky, ROW_Y) + (ti * 8 + BIAS);
+                const int toff = mul_i24(kx, ROW_X) + mul_i24(ky, ROW_Y) + (ti * 8 + BIAS);
                 Tap r;
                 r.rinv = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(trs, toff, 0, 0));
                 r.in = ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
-                const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
+                const uint32_t off = (mul_u24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
                 r.e = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
                 return r;
             };
@@ -1842,7 +1869,7 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 // the inner FMA(ey,ey,+0) is exactly the float32 product ey*ey
                 const float dist = sqrt32_normal(fma64(ex, ex, ey * ey));
                 const bool v = need & ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H) & (dist >= 1.0f);
-                const uint32_t off = (__umul24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
+                const uint32_t off = (mul_u24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
                 take(v, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp, (double)dist);
             }
         }
