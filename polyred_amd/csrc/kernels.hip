@@ -1475,9 +1475,26 @@ __device__ __forceinline__ V4 apply_diag(V4 v, const M4& m) {
 }
 __device__ __forceinline__ bool finite4(V4 v) { return fin(v.x) && fin(v.y) && fin(v.z) && fin(v.w); }
 
+// 24-bit multiplies as the one full-rate instruction.  __umul24/__mul24 in a
+// loop whose other operand is loop-invariant compile to a mask hoisted out of
+// the loop and a quarter-rate v_mul_lo_u32 inside it.
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ int32_t mul_i24(int32_t a, int32_t b) {
+    int32_t r;
+    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int fx, int fy, float depth) {
     int occ = 0;
     const int64_t len = (int64_t)c.SW * c.SH;
+    // SW and SH exact in float32, y * SW within 32 bits
+    const bool small_map = c.SW > 0 && c.SH > 0 && c.SW < (1 << 24) && c.SH < (1 << 24) && len < ((int64_t)1 << 31);
+    const float fSW = (float)c.SW, fSH = (float)c.SH;
     // the screen-to-world step is the same for every light
     const V4 qw = apply(v4((float)fx, (float)fy, depth, 1), c.v.s2w);
     for (int i = 0; i < s.n_lights; i++) {
@@ -1496,8 +1513,20 @@ __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuff
             q = apply(q, b.light_proj[i]);
             q = pos(apply(q, c.svp));
         }
-        int64_t lx = f2i(q.x), ly = f2i(q.y);
-        int64_t idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)c.SW);
+        // Go's int(x) and int arithmetic (64-bit, wrapping).  Where every
+        // lane's texel is inside the map in both coordinates (-1 < x < SW,
+        // -1 < y < SH: truncation lands in [0, SW) and [0, SH)), the common
+        // case, the wave converts in 32 bits and takes one 24-bit multiply
+        // (a uniform branch: the 64-bit conversions and the quarter-rate
+        // multiplies of the general path are skipped, not just masked)
+        const bool in_map = small_map & (q.x > -1.0f) & (q.x < fSW) & (q.y > -1.0f) & (q.y < fSH);
+        int64_t idx;
+        if (__ballot(!in_map) == 0) {
+            idx = (int64_t)(mul_u24((uint32_t)(int32_t)q.y, (uint32_t)c.SW) + (uint32_t)(int32_t)q.x);
+        } else {
+            const int64_t lx = f2i(q.x), ly = f2i(q.y);
+            idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)c.SW);
+        }
         if (idx > 0 && idx < len) {
             float sz = depth_from_ord(b.smaps[(size_t)l.slot * len + idx]);
             if (q.z < sz - 0.03f) occ++;
@@ -1549,20 +1578,6 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 }
 
 }  // namespace
-
-// 24-bit multiplies as the one full-rate instruction.  __umul24/__mul24 in a
-// loop whose other operand is loop-invariant compile to a mask hoisted out of
-// the loop and a quarter-rate v_mul_lo_u32 inside it.
-__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ int32_t mul_i24(int32_t a, int32_t b) {
-    int32_t r;
-    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 
 // Screen-space AO (material/ao.go:20-73) for every pixel whose material has
 // it, plus pixel (0,0) (the uncovered pixels are shaded from it).  One
