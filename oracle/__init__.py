@@ -30,6 +30,8 @@ class oracle_out(C.Structure):
         ("shade_mask", C.POINTER(C.c_uint8)),
         ("shade_err64", C.POINTER(C.c_double)),
         ("shadow_count", C.POINTER(C.c_uint8)),
+        ("ao_total", C.POINTER(C.c_float)),
+        ("light_matrices", C.POINTER(C.c_float)),
     ]
 
 
@@ -72,6 +74,9 @@ def lib() -> C.CDLL:
         L.oracle_fragment_shader.argtypes = [C.POINTER(abi.prh_material), f32p, f32p, f32p, f32p, C.c_int,
                                              C.POINTER(abi.prh_light), C.c_int, f32p, u8p, f32p]
         L.oracle_ao_dirs.argtypes = [f32p]
+        L.oracle_ao_march.argtypes = [f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, f32p]
+        L.oracle_shadow_lookup.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, f32p, C.c_float, C.c_float,
+                                           C.c_float]
         L.oracle_resize.argtypes = [u8p, C.c_int, C.c_int, u8p, C.c_int, C.c_int]
         L.oracle_texture_query.argtypes = [C.POINTER(abi.prh_texture), C.c_float, C.c_float, C.c_float, u8p]
         L.oracle_texture_level.argtypes = [C.POINTER(abi.prh_texture), C.c_int, C.POINTER(C.c_int),
@@ -110,6 +115,8 @@ class Frame:
         self.err64 = C.c_double(0.0)
         self.mask = None if mask is None else np.ascontiguousarray(mask.astype(np.uint8).reshape(-1))
         self.shadow_count = np.zeros(n, np.uint8)
+        self.ao_total = np.zeros(n, np.float32)
+        self.light_matrices = np.zeros(32 * nl, np.float32)
 
 
 def render(scene, mask=None) -> Frame:
@@ -128,6 +135,8 @@ def render(scene, mask=None) -> Frame:
         o.shade_mask = fr.mask.ctypes.data_as(C.POINTER(C.c_uint8))
     o.shade_err64 = C.pointer(fr.err64)
     o.shadow_count = fr.shadow_count.ctypes.data_as(C.POINTER(C.c_uint8))
+    o.ao_total = fptr(fr.ao_total)
+    o.light_matrices = fptr(fr.light_matrices)
     desc = scene.desc()
     cam = scene.camera_struct()
     opts = scene.options_struct()
@@ -142,7 +151,34 @@ def render(scene, mask=None) -> Frame:
     fr.pregamma = fr.pregamma.reshape(H, W, 4)
     fr.image = fr.image.reshape(scene.height, scene.width, 4)
     fr.shadow_count = fr.shadow_count.reshape(H, W)
+    fr.ao_total = fr.ao_total.reshape(H, W)
+    fr.light_matrices = fr.light_matrices.reshape(-1, 2, 4, 4)
     return fr
+
+
+def ao_march(depth: np.ndarray, ok: np.ndarray, px: int, py: int):
+    """maxElevationAngle of the eight directions and AO's total and factor
+    at raster pixel (px, py) of a fragment buffer given as (H, W) raster rows
+    of depths and Ok flags: returns (angles[8], total, factor), float32."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    okb = np.ascontiguousarray(ok, np.uint8)
+    H, W = depth.shape
+    ang = np.zeros(8, np.float32)
+    tot, fac = C.c_float(), C.c_float()
+    rc = lib().oracle_ao_march(fptr(depth), okb.ctypes.data_as(C.POINTER(C.c_uint8)), W, H, px, py, fptr(ang),
+                               C.byref(tot), C.byref(fac))
+    if rc != 0:
+        raise RuntimeError(f"oracle_ao_march failed: {rc}")
+    return ang, np.float32(tot.value), np.float32(fac.value)
+
+
+def shadow_lookup(s2w, lview, lproj, smap: np.ndarray, x: float, y: float, depth: float) -> bool:
+    """shadingVisibility's lookup of fragment (x, y, depth) in one shadow map
+    ((SH, SW) float32, texel index x + y*SW): True if occluded."""
+    m = [np.ascontiguousarray(np.asarray(a, np.float32).reshape(16)) for a in (s2w, lview, lproj)]
+    smap = np.ascontiguousarray(smap, np.float32)
+    SH, SW = smap.shape
+    return bool(lib().oracle_shadow_lookup(fptr(m[0]), fptr(m[1]), fptr(m[2]), SW, SH, fptr(smap), x, y, depth))
 
 
 class oracle_diff_params(C.Structure):

@@ -621,6 +621,7 @@ typedef struct {
     float ao_dir[16];
     float* linear;
     uint8_t* shadow_count;
+    float* ao_total;
     double shade_err64;
     tex_t* tex; /* the scene's textures with their mip levels */
     int n_tex;
@@ -930,25 +931,35 @@ static void pass_shadows(ctx_t* c, int li) {
         }
     }
 }
+/* render/shadow.go:243-282: the lookup of one fragment (x, y, depth) in one
+ * shadow map: screen -> world -> light view -> light projection -> light
+ * viewport, Pos(); texel int(x) + int(y)*SW with Go's truncating
+ * conversions and wrapping int arithmetic; occluded iff 0 < idx < len and
+ * z < map[idx] - 0.03. */
+static int shadow_lookup(const m4* s2w, const m4* lview, const m4* lproj, int SW, int SH, const float* map, float x,
+                         float y, float depth) {
+    m4 vps = viewport_matrix((float)SW, (float)SH);
+    v4 sc = V4(x, y, depth, 1);
+    sc = apply4(sc, s2w);
+    sc = apply4(sc, lview);
+    sc = apply4(sc, lproj);
+    sc = pos4(apply4(sc, &vps));
+    int64_t lx = gf2i(sc.x), ly = gf2i(sc.y);
+    int64_t idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)SW);
+    int64_t len = (int64_t)SW * SH;
+    int shadow = 0;
+    if (idx > 0 && idx < len) {
+        float sz = map[idx];
+        if (sc.z < sz - 0.03f) shadow++;
+    }
+    return shadow > 0;
+}
 /* render/shadow.go:230-283: true when the fragment is occluded (the name is
  * inverted in the reference; non-casting lights report true). */
 static int shading_visibility(ctx_t* c, int li, const frag_t* info) {
     if (!c->s->lights[li].cast_shadow) return 1;
-    m4 vps = viewport_matrix((float)c->SW, (float)c->SH);
-    v4 sc = V4((float)info->x, (float)info->y, info->depth, 1);
-    sc = apply4(sc, &c->s2w);
-    sc = apply4(sc, &c->lview[li]);
-    sc = apply4(sc, &c->lproj[li]);
-    sc = pos4(apply4(sc, &vps));
-    int64_t lx = gf2i(sc.x), ly = gf2i(sc.y);
-    int64_t idx = (int64_t)((uint64_t)lx + (uint64_t)ly * (uint64_t)(int64_t)c->SW);
-    int64_t len = (int64_t)c->SW * c->SH;
-    int shadow = 0;
-    if (idx > 0 && idx < len) {
-        float sz = c->smaps[(size_t)li * c->SW * c->SH + (size_t)idx];
-        if (sc.z < sz - 0.03f) shadow++;
-    }
-    return shadow > 0;
+    return shadow_lookup(&c->s2w, &c->lview[li], &c->lproj[li], c->SW, c->SH,
+                         c->smaps + (size_t)li * c->SW * c->SH, (float)info->x, (float)info->y, info->depth);
 }
 
 /* -------------------------------------------------------------- shading */
@@ -1069,7 +1080,8 @@ static float max_elevation_angle(ctx_t* c, int px, int py, float dx, float dy) {
     return maxangle;
 }
 /* material/ao.go:20-38 */
-static void ao_shade(ctx_t* c, int fx, int fy, const uint8_t col[4], const prh_material* mat, uint8_t out[4]) {
+static void ao_shade(ctx_t* c, int fx, int fy, const uint8_t col[4], const prh_material* mat, uint8_t out[4],
+                     float* total_out) {
     if (!mat || !mat->ambient_occlusion) {
         memcpy(out, col, 4);
         return;
@@ -1078,6 +1090,7 @@ static void ao_shade(ctx_t* c, int fx, int fy, const uint8_t col[4], const prh_m
     for (int k = 0; k < 8; k++)
         total += (float)(M_PI / 2) - max_elevation_angle(c, fx, fy, c->ao_dir[2 * k], c->ao_dir[2 * k + 1]);
     total /= (float)((M_PI / 2) * 8);
+    if (total_out) *total_out = total;
     total = gpow(total, 10000);
     out[0] = gf2u8(total * (float)col[0]);
     out[1] = gf2u8(total * (float)col[1]);
@@ -1129,7 +1142,7 @@ static void shade(ctx_t* c, int x, int y) {
             }
         }
         uint8_t outc[4];
-        ao_shade(c, frag.x, frag.y, col, mat_at(c->s, frag.matid), outc);
+        ao_shade(c, frag.x, frag.y, col, mat_at(c->s, frag.matid), outc, c->ao_total ? &c->ao_total[lid] : NULL);
         memcpy(col, outc, 4);
     }
     memcpy(fr->col, col, 4);
@@ -1168,6 +1181,9 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
         for (size_t i = 0; i < 3 * np; i++) c.linear[i] = NAN;
     c.shadow_count = out->shadow_count;
     if (c.shadow_count) memset(c.shadow_count, 0, np);
+    c.ao_total = out->ao_total;
+    if (c.ao_total)
+        for (size_t i = 0; i < np; i++) c.ao_total[i] = NAN;
     ao_dirs(c.ao_dir);
     c.view = view_matrix(V3a(cam->position), V3a(cam->target), V3a(cam->up));
     c.proj = cam_proj(cam);
@@ -1239,6 +1255,11 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
     if (out->shadow_maps && c.smaps)
         memcpy(out->shadow_maps, c.smaps, sizeof(float) * (size_t)s->n_lights * c.SW * c.SH);
     if (out->shade_err64) *out->shade_err64 = c.shade_err64;
+    if (out->light_matrices && c.lview)
+        for (int i = 0; i < s->n_lights; i++) {
+            memcpy(out->light_matrices + 32 * (size_t)i, c.lview[i].a, 64);
+            memcpy(out->light_matrices + 32 * (size_t)i + 16, c.lproj[i].a, 64);
+        }
     free(c.f);
     for (int i = 0; i < c.n_tex; i++) tex_free(&c.tex[i]);
     free(c.tex);
@@ -1526,3 +1547,44 @@ void oracle_fragment_shader(const prh_material* m, const float nor[4], const flo
     if (lin) memcpy(lin, l, sizeof l);
 }
 void oracle_ao_dirs(float out[16]) { ao_dirs(out); }
+
+int oracle_ao_march(const float* depth, const uint8_t* ok, int W, int H, int px, int py, float angles[8],
+                    float* total, float* factor) {
+    if (!depth || !ok || W <= 0 || H <= 0 || px < 0 || px >= W || py < 0 || py >= H) return PRH_E_INVALID;
+    ctx_t c;
+    memset(&c, 0, sizeof c);
+    c.W = W;
+    c.H = H;
+    c.f = calloc((size_t)W * H, sizeof(frag_t));
+    if (!c.f) return PRH_E_NOMEM;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            frag_t* f = fat(&c, x, y);
+            f->ok = ok[(size_t)y * W + x] != 0;
+            f->depth = depth[(size_t)y * W + x];
+            f->x = x;
+            f->y = y;
+        }
+    ao_dirs(c.ao_dir);
+    for (int k = 0; k < 8 && angles; k++) angles[k] = max_elevation_angle(&c, px, py, c.ao_dir[2 * k], c.ao_dir[2 * k + 1]);
+    prh_material m;
+    memset(&m, 0, sizeof m);
+    m.ambient_occlusion = 1;
+    const uint8_t white[4] = {255, 255, 255, 255};
+    uint8_t out[4];
+    float t = NAN;
+    ao_shade(&c, px, py, white, &m, out, &t);
+    if (total) *total = t;
+    if (factor) *factor = gpow(t, 10000);
+    free(c.f);
+    return PRH_OK;
+}
+
+int oracle_shadow_lookup(const float s2w[16], const float lview[16], const float lproj[16], int SW, int SH,
+                         const float* map, float x, float y, float depth) {
+    m4 a, b, d;
+    memcpy(a.a, s2w, 64);
+    memcpy(b.a, lview, 64);
+    memcpy(d.a, lproj, 64);
+    return shadow_lookup(&a, &b, &d, SW, SH, map, x, y, depth);
+}

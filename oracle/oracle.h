@@ -26,9 +26,21 @@
  *     Blinn-Phong, and a brute-force FP64 re-evaluation on whole frames;
  *   - the raster: brute force on tiny scenes (every pixel's coverage from an
  *     exact rational edge test), ties and clipping fans by construction;
- *   - AO: range and closed-form cases (flat depth gives a factor of exactly
- *     one) only.  Go's math.Atan/Cos/Sin last bits are not reproducible
- *     without Go: AO is "parity unpinned" beyond those checks.
+ *   - shadows: the light-camera fit by the invariants it must satisfy
+ *     (every main-camera frustum corner inside the light's NDC cube, the
+ *     faces touched, near/far as the fit defines them); the maps texel by
+ *     texel against exact-rational coverage of the float64 light camera;
+ *     the lookup's rules (texel 0 skipped, truncation, row wrap, strict
+ *     bias, transform order) on hand cases; an occluder's footprint, the
+ *     zero-texel quirk and the non-casting-light count in closed form;
+ *   - AO: every direction's largest elevation angle on step, ramp, border,
+ *     truncation and uncovered-fragment depth buffers, and whole frames of
+ *     a two-plane step, against closed forms of the geometry.  Go's
+ *     math.Atan/Cos/Sin last bits are not reproducible without Go; the
+ *     oracle rounds the C library's float64 results to float32.
+ * Each check was also run against planted mistakes in this file (flipped
+ * signs, floor for truncation, missing skips and culls, transposed
+ * indices, reordered transforms), every one of which it rejects.
  */
 #ifndef POLYRED_ORACLE_H
 #define POLYRED_ORACLE_H
@@ -52,6 +64,12 @@ typedef struct oracle_out {
     uint8_t* shadow_count; /* optional [H][W] raster rows: the lights whose shadow test
                               reported true, at covered pixels whose material receives
                               shadows (the colour is scaled by 2^-count); 0 elsewhere  */
+    float* ao_total;       /* optional [H][W] raster rows: AO's total before
+                              pow(total, 10000) (material/ao.go:27-31), NaN where
+                              no AO was applied                                       */
+    float* light_matrices; /* optional [n_lights][2][16]: each shadow-casting
+                              light's camera, view then projection (row-major);
+                              untouched for the others                               */
 } oracle_out;
 
 /* With MSAA m > 1 every buffer is (height*m) x (width*m) except the final
@@ -120,6 +138,17 @@ void oracle_fragment_shader(const prh_material* m, const float nor[4],
                             const float* amb, uint8_t out[4], float lin[3]);
 /* AO direction table (8 x {cos, sin}) as material/ao.go:26 builds it. */
 void oracle_ao_dirs(float out[16]);
+/* maxElevationAngle for the eight directions and AmbientOcclusionShade's
+ * total and factor (material/ao.go:20-73) at raster pixel (px, py) of a
+ * W x H fragment buffer given as raster rows (index y*W + x): its depths and
+ * Ok flags. */
+int oracle_ao_march(const float* depth, const uint8_t* ok, int W, int H, int px, int py, float angles[8],
+                    float* total, float* factor);
+/* The lookup of shadingVisibility (render/shadow.go:243-282) of the fragment
+ * (x, y, depth) in one SW x SH shadow map, with the given screen-to-world,
+ * light view and light projection matrices: 1 if occluded. */
+int oracle_shadow_lookup(const float s2w[16], const float lview[16], const float lproj[16], int SW, int SH,
+                         const float* map, float x, float y, float depth);
 
 #ifdef __cplusplus
 }

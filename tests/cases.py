@@ -183,3 +183,77 @@ def stacked_layers(n_layers=100, width=160, height=120, shadows=True):
     s.shadow_map = shadows
     s.name = f"stacked_{n_layers}"
     return s
+
+
+# ------------------------------------------------------- pinning scenes
+AO_STEP_CAMERA = dict(kind=abi.PRH_CAMERA_ORTHOGRAPHIC, position=(0.0, 0.0, 3.0), near=-1.0, far=-5.0,
+                      frustum=(-4.0, 4.0, -2.0, 2.0))
+
+
+def _quad(x0, x1, y0, y1, z):
+    return [[x0, y0, z, 1], [x1, y0, z, 1], [x1, y1, z, 1], [x0, y1, z, 1]]
+
+
+def ao_step(axis="x", width=640, height=320, h=4e-4):
+    """Two fronto-parallel planes under an orthographic camera looking down
+    -z (80 pixels per unit both ways): a low plane at z = 0 over the whole
+    view and a high plane at z = h over x >= 0 (axis "x": raster columns from
+    width/2 on) or y >= 0 (axis "y": raster rows from height/2 on).  With
+    near -1 and far -5 the low plane's depth is 0 and the high plane's h/2.
+    Both carry one material with AO on and a white uniform texture, and the
+    scene has no lights, so the deferred colour of a pixel is
+    u8(pow(total, 10000) * 255) (shader/blinn_cpu.go's texture colour,
+    material/ao.go:20-38)."""
+    low = _quad(-4.0, 4.0, -2.0, 2.0, 0.0)
+    high = _quad(0.0, 4.0, -2.0, 2.0, h) if axis == "x" else _quad(-4.0, 4.0, 0.0, 2.0, h)
+    P = np.asarray(low + high, np.float32)
+    tri = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]], np.int32)
+    mat = scenes.Material(diffuse=(128, 128, 128, 255), specular=(0, 0, 0, 255), texture=(255, 255, 255, 255),
+                          shininess=1.0, ambient_occlusion=True)
+    s = _scene(P, tri, [0, 0, 0, 0], width, height, dict(AO_STEP_CAMERA), materials=[mat], lights=[])
+    s.gamma = False
+    s.name = f"ao_step_{axis}"
+    return s
+
+
+def shadow_footprint(non_casting=False, back_face=False, width=160, height=120):
+    """A ground square (y = 0, |x|, |z| <= 6) and a floating square occluder
+    (y = 2, |x|, |z| <= 1), both facing up and receiving shadows, under one
+    shadow-casting point light at (2, 9, 1); a perspective camera at (0, 5, 8)
+    looking at the origin (fov 45, near 1, far 14); framebuffer-sized shadow
+    maps as the reference allocates them (render/shadow.go:87-88).  With
+    non_casting a second point light that casts no shadow is added, which
+    shadingVisibility reports as occluding every pixel
+    (render/shadow.go:233-236).  With back_face a third square (y = 3.5,
+    x in [-3.5, -1.5], z in [1, 3]) faces down, away from the light and the
+    camera: the back-face cull drops it from both the shadow map and the
+    frame, so it casts no shadow (render/shadow.go:176-179)."""
+    ground = _quad(-6.0, 6.0, -6.0, 6.0, 0.0)
+    occ = _quad(-1.0, 1.0, -1.0, 1.0, 0.0)
+    quads = ground + occ + (_quad(-3.5, -1.5, -3.0, -1.0, 0.0) if back_face else [])
+    P = np.asarray(quads, np.float32)[:, [0, 2, 1, 3]]  # (x, y, z) -> (x, z, y): squares in y = const planes
+    P[:, 2] *= -1
+    P[4:8, 1] = 2.0
+    P[8:, 1] = 3.5
+    nq = len(P) // 4
+    tri = np.array([[4 * q, 4 * q + 1, 4 * q + 2] for q in range(nq)] + [[4 * q, 4 * q + 2, 4 * q + 3]
+                                                                        for q in range(nq)], np.int32)
+    tri = tri[np.argsort(tri[:, 0], kind="stable")]
+    up = np.tile([0.0, 1.0, 0.0], (1, 1))
+    tri = scenes._orient(P, tri, lambda a, b, c: np.where((a[:, 1] > 3)[:, None], -up, up))
+    n = len(P)
+    nor = np.tile(np.array([0, 1, 0, 0], np.float32), (n, 1))
+    nor[8:, 1] = -1
+    mesh = scenes.Mesh(P, nor, np.tile(np.array([200, 200, 200, 255], np.uint8), (n, 1)),
+                       np.zeros((n, 2), np.float32), tri, np.array([0, 0, 1, 1, 1, 1][: len(tri)], np.int32))
+    mats = [scenes._mat_from((0.6, 0.6, 0.6), 0.2, 8, shadow=True), scenes._mat_from((0.8, 0.3, 0.3), 0.2, 8,
+                                                                                    shadow=True)]
+    lights = [scenes.Light(position=(2.0, 9.0, 1.0), intensity=1.0, cast_shadow=True)]
+    if non_casting:
+        lights.append(scenes.Light(position=(-3.0, 6.0, 4.0), intensity=0.5, cast_shadow=False))
+    s = scenes.Scene(meshes=[mesh], obj_mesh=np.array([0]), obj_model=np.eye(4, dtype=np.float32).reshape(1, 16),
+                     obj_mat_base=np.array([0]), materials=mats, lights=lights, ambient=[0.2],
+                     camera=dict(position=(0.0, 5.0, 8.0), target=(0.0, 0.0, 0.0), fov=45.0, near=1.0, far=14.0),
+                     width=width, height=height, shadow_map=True, gamma=False,
+                     name="shadow_footprint" + ("_nc" if non_casting else "") + ("_bf" if back_face else ""))
+    return s

@@ -35,6 +35,13 @@ def cases_dict():
     d["reference_multimaterial"] = edge.reference_multimaterial()
     d["reference_ao"] = edge.reference_ao()
     d["reference_directional"] = edge.reference_directional()
+    # the scenes whose shadows and AO the oracle tests pin in closed form
+    # (tests/test_oracle_shadow.py, tests/test_oracle_ao.py)
+    d["ao_step_x"] = edge.ao_step("x")
+    d["ao_step_y"] = edge.ao_step("y")
+    d["shadow_footprint"] = edge.shadow_footprint()
+    d["shadow_footprint_nc"] = edge.shadow_footprint(non_casting=True)
+    d["shadow_footprint_bf"] = edge.shadow_footprint(back_face=True)
     return d
 
 
