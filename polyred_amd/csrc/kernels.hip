@@ -2314,10 +2314,11 @@ void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const
     go(k_shade, k_shade_wide, dim3(per_xcd * 8), 256, st, k_shade_args{s, c, b, 0, 0});
 }
 
-void launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
+bool launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const size_t P = (size_t)c.v.W * c.v.H;
-    if (!b.col_pre || !b.ao || !b.tri_rec) std::abort();  // the caller's contract: the first half's buffers
+    if (!b.col_pre || !b.ao || !b.tri_rec) return false;  // it reads the first half's buffers: nothing launched
     if (P) hipLaunchKernelGGL(k_shade_post, dim3(blocks_for(P, 256)), dim3(256), 0, st, s, c, b);
+    return true;
 }
 void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
                          uint8_t* out) {

@@ -150,7 +150,8 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
-void launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
+// false (and nothing launched) without the colour before AO, the AO factors or the triangle records
+bool launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shadow_count(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b,
                          uint8_t* out);
 

@@ -1106,8 +1106,9 @@ ShadeBuffers shade_buffers(const prh_renderer* r) {
 // `timed` records the per-pass events (not while a batch is captured into a
 // graph).  `wide`: the launches are being recorded into a wide batch (the
 // stage numbers order its launches; the persistent rasters take fewer
-// workgroups per view, since every view's run together).
-void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bool wide = false) {
+// workgroups per view, since every view's run together).  A non-zero
+// status means part of the frame may be enqueued and none of it is valid.
+int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bool wide = false) {
     const DevScene& d = r->sc->d;
     Counters* cm = r->cnt;
     Counters* cs = r->cnt + 1;
@@ -1202,8 +1203,9 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     launch_shade_bg(st, d, r->sh, b);
     if (timed) (void)hipEventRecord(r->ev[7], st);
     wide_stage(15);
-    if (split_shade) launch_shade_post(st, d, r->sh, bp);
-    else launch_shade(st, d, r->sh, b);
+    if (!split_shade) launch_shade(st, d, r->sh, b);
+    else if (!launch_shade_post(st, d, r->sh, bp))
+        return fail(PRH_E_INVALID, "split shading without the AO factors or triangle records");
     if (r->msaa > 1) {
         // passAntialiasing (render/raster.go:361-378): Resize after gamma
         wide_stage(16);
@@ -1213,6 +1215,7 @@ void enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, b
     }
     if (timed) (void)hipEventRecord(r->ev[6], st);
     // (k_shade_bg writes the counters into r->h_cnt)
+    return PRH_OK;
 }
 
 // After a frame's work has completed: after an overflow, grows the work
@@ -1250,8 +1253,9 @@ int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
         return fail(PRH_E_UNSUPPORTED, "NaN depth in a shadow pass: its result depends on sequential order");
     bool nan_rerun = false;
     if (cm.nan_flag) {
-        enqueue_frame(r, true, st, timed);
+        const int rc = enqueue_frame(r, true, st, timed);
         HIP_TRY(hipStreamSynchronize(st));
+        if (rc) return rc;
         nan_rerun = true;
     }
     prh_frame_stats& S = r->stats;
@@ -1343,8 +1347,13 @@ int batch_wide(prh_ctx* ctx, prh_renderer* const* rs, int n) {
     drop_graph(ctx);
     WideBatch* w = wide_new();
     wide_record(w);
-    for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[0], false, true);
+    int rc = PRH_OK;
+    for (int i = 0; i < n && !rc; i++) rc = enqueue_frame(rs[i], false, ctx->bst[0], false, true);
     wide_record(nullptr);
+    if (rc) {
+        wide_free(w);
+        return rc;
+    }
     const size_t bytes = wide_layout(w);
     if (!bytes) {
         wide_free(w);
@@ -1379,6 +1388,7 @@ int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n, bool wide) {
     hipStream_t s0 = ctx->bst[0];
     HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
     hipError_t e = hipSuccess;
+    int rc = PRH_OK;
     if (wide) {
         wide_launch(ctx->wide, s0, ctx->wide_dev);
     } else {
@@ -1386,7 +1396,7 @@ int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n, bool wide) {
         for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) e = hipStreamWaitEvent(ctx->bst[k], ctx->bfork, 0);
         if (e == hipSuccess) {
             const std::vector<int> lane = batch_lanes(rs, n);
-            for (int i = 0; i < n; i++) enqueue_frame(rs[i], false, ctx->bst[lane[i]], false);
+            for (int i = 0; i < n && !rc; i++) rc = enqueue_frame(rs[i], false, ctx->bst[lane[i]], false);
             for (int k = 1; k < ctx->nbs && e == hipSuccess; k++) {
                 e = hipEventRecord(ctx->bjoin[k], ctx->bst[k]);
                 if (e == hipSuccess) e = hipStreamWaitEvent(s0, ctx->bjoin[k], 0);
@@ -1395,6 +1405,10 @@ int batch_graph(prh_ctx* ctx, prh_renderer* const* rs, int n, bool wide) {
     }
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(s0, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
     if (e != hipSuccess || e2 != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return fail(PRH_E_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
@@ -1671,8 +1685,9 @@ int prh_render(prh_renderer* r) {
     HIP_TRY(hipSetDevice(r->ctx->device));
     hipStream_t st = r->ctx->stream;
     for (int attempt = 0;; attempt++) {
-        enqueue_frame(r, false, st, true);
+        const int rc0 = enqueue_frame(r, false, st, true);
         HIP_TRY(hipStreamSynchronize(st));
+        if (rc0) return rc0;
         HIP_TRY(hipGetLastError());
         bool again = false;
         int rc = finish_frame(r, attempt, again);
@@ -1706,9 +1721,9 @@ int prh_render_batch(prh_renderer* const* rs, int32_t n, int32_t flags, float* m
         // fork: every batch stream starts after the work already on s0
         for (int k = 1; k < ctx->nbs; k++) HIP_TRY(hipStreamWaitEvent(ctx->bst[k], ctx->bev[0], 0));
         const std::vector<int> lane = batch_lanes(rs, n);
-        for (int i = 0; i < n; i++) {
+        for (int i = 0; i < n && !rc; i++) {
             rs[i]->has_frame = false;
-            enqueue_frame(rs[i], false, ctx->bst[lane[i]], true);
+            rc = enqueue_frame(rs[i], false, ctx->bst[lane[i]], true);
         }
         for (int k = 1; k < ctx->nbs; k++) {
             HIP_TRY(hipEventRecord(ctx->bjoin[k], ctx->bst[k]));
@@ -1717,6 +1732,7 @@ int prh_render_batch(prh_renderer* const* rs, int32_t n, int32_t flags, float* m
     }
     HIP_TRY(hipEventRecord(ctx->bev[1], s0));
     HIP_TRY(hipEventSynchronize(ctx->bev[1]));
+    if (rc) return rc;
     HIP_TRY(hipGetLastError());
     if (ms) (void)hipEventElapsedTime(ms, ctx->bev[0], ctx->bev[1]);
     // per view: the same checks as prh_render; a view that overflowed its
