@@ -269,6 +269,21 @@ int prh_diff_forward(prh_renderer* r, const prh_diff_params* p, float* linear, s
 /* Loss and its gradients; target is float [H][W][3]. */
 int prh_diff_backward(prh_renderer* r, const prh_diff_params* p, const float* target, size_t bytes, double* loss,
                       const prh_diff_grads* grads);
+/* prh_diff_backward without any device allocation.  No counterpart in
+ * polyred: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or optimisation loops, hosts that forbid
+ * hipMalloc).  Same arguments, kernels and results as prh_diff_backward, but
+ * the parameters, gradients, target, loss and shadow counts live in one
+ * static device array of PRH_NOALLOC_SCRATCH_BYTES per GPU (held from the
+ * library's load until the process ends) instead of the renderer's own
+ * buffers; a frame needing more returns PRH_E_INVALID (about 12 bytes per
+ * pixel plus 48 per transformed vertex).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (from several threads or contexts)
+ * race, so serialise them.  The renderer's frame buffers are not touched, and
+ * prh_last_stats' ms_diff is not updated. */
+#define PRH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int prh_diff_backward_noalloc(prh_renderer* r, const prh_diff_params* p, const float* target, size_t bytes,
+                              double* loss, const prh_diff_grads* grads);
 
 /* ---- batched inverse rendering (no counterpart in polyred) -------------
  * Gradient descent on the differentiable shading above over many views of

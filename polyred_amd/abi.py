@@ -20,6 +20,8 @@ PRH_E_HIP = -3
 PRH_E_NOMEM = -4
 PRH_E_NODEVICE = -5
 
+PRH_NOALLOC_SCRATCH_BYTES = 32 << 20  # prh_diff_backward_noalloc's per-GPU scratch array
+
 PRH_LIGHT_POINT = 0
 PRH_LIGHT_DIRECTIONAL = 1
 PRH_CAMERA_PERSPECTIVE = 0
@@ -195,6 +197,8 @@ PROTOTYPES = {
     "prh_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t]),
     "prh_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t,
                                     C.POINTER(C.c_double), C.POINTER(prh_diff_grads)]),
+    "prh_diff_backward_noalloc": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params), C.c_void_p, C.c_size_t,
+                                            C.POINTER(C.c_double), C.POINTER(prh_diff_grads)]),
     "prh_fit_new": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "prh_fit_free": (C.c_int, [C.c_void_p]),
     "prh_fit_targets": (C.c_int, [C.c_void_p, C.POINTER(prh_diff_params)]),

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -23,6 +24,13 @@
 
 using namespace gm;
 using namespace prh;
+
+// prh_diff_backward_noalloc's working memory: one array per GPU, which the
+// code object's loader places when the library is first used on that GPU.
+// Host code never passes `scratch` itself to a kernel or a copy (on the host
+// it names a stand-in, not the device array): prh_open looks up its device
+// address once per GPU and every later use goes through that pointer.
+static __device__ __align__(256) uint8_t scratch[PRH_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -40,6 +48,27 @@ int fail(int code, const std::string& msg) {
             return fail(e_ == hipErrorOutOfMemory ? PRH_E_NOMEM : PRH_E_HIP,                       \
                         std::string(#expr) + ": " + hipGetErrorString(e_));                      \
     } while (0)
+
+std::mutex g_scratch_mu;
+std::vector<uint8_t*> g_scratch_dev;  // per device ordinal: scratch's device address, or null
+
+// With `device` current: the device address of `scratch` on it, looked up on
+// the first call for that device.
+int scratch_setup(int device, uint8_t** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)device >= g_scratch_dev.size()) g_scratch_dev.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return PRH_E_HIP;
+        }
+        g_scratch_dev[device] = static_cast<uint8_t*>(p);
+    }
+    *out = g_scratch_dev[device];
+    return PRH_OK;
+}
 
 // ---------------------------------------------------------- host matrices
 M4 mat(const float* a) {
@@ -331,6 +360,8 @@ struct prh_ctx {
     hipStream_t stream2;  // a frame's shadow passes, beside the main view's passes on `stream`
     int live_scenes;    // a context outlives every scene uploaded to it
     int live_surfaces;  // and every surface
+    uint8_t* scratch;   // this device's `scratch` array (prh_diff_backward_noalloc), or null
+    std::string scratch_err;  // why it is null
     // prh_render_batch: its streams and events (created on first use) and
     // the graph of the last captured batch, keyed by its renderers and their
     // work-buffer generations
@@ -502,11 +533,17 @@ int prh_open(int32_t ordinal, prh_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(PRH_E_NODEVICE, "no HIP device");
     if (ordinal < 0 || ordinal >= n) return fail(PRH_E_NODEVICE, "device ordinal out of range");
     HIP_TRY(hipSetDevice(ordinal));
+    // (a failed lookup leaves the context usable: prh_diff_backward_noalloc
+    // then reports it)
+    uint8_t* scr = nullptr;
+    std::string scr_err;
+    if (scratch_setup(ordinal, &scr) != PRH_OK) scr = nullptr, scr_err = g_err;
     hipStream_t st, st2;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     prh_ctx* c = new prh_ctx();
     c->device = ordinal, c->stream = st, c->stream2 = st2, c->live_scenes = 0;
+    c->scratch = scr, c->scratch_err = scr_err;
     *out = c;
     return PRH_OK;
 }
@@ -1837,6 +1874,67 @@ int prh_diff_backward(prh_renderer* r, const prh_diff_params* p, const float* ta
     if (bytes < (size_t)12 * r->W * r->H) return fail(PRH_E_INVALID, "prh_diff_backward: target too small");
     HIP_TRY(hipSetDevice(r->ctx->device));
     return diff_run(r, p, target, loss, grads);
+}
+
+int prh_diff_backward_noalloc(prh_renderer* r, const prh_diff_params* p, const float* target, size_t bytes,
+                              double* loss, const prh_diff_grads* grads) {
+    if (!r || !p || !target || !loss || !grads) return fail(PRH_E_INVALID, "prh_diff_backward_noalloc: null argument");
+    const size_t P = (size_t)r->W * r->H;
+    if (bytes < 12 * P) return fail(PRH_E_INVALID, "prh_diff_backward_noalloc: target too small");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    if (!r->has_frame) return fail(PRH_E_INVALID, "differentiable shading needs a rendered frame (prh_render)");
+    int rc = check_params(r, p);
+    if (rc) return rc;
+    uint8_t* base = r->ctx->scratch;
+    if (!base) return fail(PRH_E_HIP, "prh_diff_backward_noalloc: no scratch array on this device: " + r->ctx->scratch_err);
+    // the scratch array's layout for this frame, each part 256-byte aligned
+    const DiffLayout l = diff_layout(r);
+    const bool counts = r->sh.shadow_map;
+    size_t need = 0;
+    auto take = [&](size_t n) {
+        const size_t o = need;
+        need += (n + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_par = take(4 * l.total), o_grad = take(4 * l.total), o_tgt = take(12 * P), o_loss = take(8);
+    const size_t o_cnt = counts ? take(P) : 0;
+    if (need > PRH_NOALLOC_SCRATCH_BYTES)
+        return fail(PRH_E_INVALID, "prh_diff_backward_noalloc: the frame needs " + std::to_string(need) +
+                                       " bytes of scratch, more than the " +
+                                       std::to_string(PRH_NOALLOC_SCRATCH_BYTES) + " available");
+    float* par = reinterpret_cast<float*>(base + o_par);
+    float* grad = reinterpret_cast<float*>(base + o_grad);
+    float* tgt = reinterpret_cast<float*>(base + o_tgt);
+    double* lossd = reinterpret_cast<double*>(base + o_loss);
+    uint8_t* cnt = counts ? base + o_cnt : nullptr;
+    const DevScene& d = r->sc->d;
+    const size_t V = d.n_tverts, M = d.n_materials, L = d.n_lights;
+    hipStream_t st = r->ctx->stream;
+    if ((rc = upload_params(r, l, p, par, st))) return rc;
+    HIP_TRY(hipMemcpyAsync(tgt, target, 12 * P, hipMemcpyHostToDevice, st));
+    // the array keeps the previous call's contents: clear what accumulates,
+    // with fill kernels (no memset nodes if a caller captures this)
+    launch_fill_u32(st, reinterpret_cast<uint32_t*>(grad), l.total, 0u);
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(lossd), 1, 0ull);
+    if (counts) launch_shadow_count(st, d, r->sh, shade_buffers(r), cnt);  // every pixel written
+    HIP_TRY(hipGetLastError());
+    DiffBuffers b = diff_buffers(r, l, par, grad, lossd, p->normal != nullptr, cnt);
+    b.target = tgt, b.linear = nullptr;
+    launch_diff(st, true, d, r->vc, b);
+    HIP_TRY(hipGetLastError());
+    auto down = [&](float* dst, size_t off, size_t n) -> hipError_t {
+        return dst && n ? hipMemcpyAsync(dst, grad + off, n * 4, hipMemcpyDeviceToHost, st) : hipSuccess;
+    };
+    HIP_TRY(down(grads->albedo, l.albedo, 3 * V));
+    HIP_TRY(down(grads->normal, l.normal, 3 * V));
+    HIP_TRY(down(grads->kd, l.kd, 3 * M));
+    HIP_TRY(down(grads->ks, l.ks, 3 * M));
+    HIP_TRY(down(grads->shininess, l.shin, M));
+    HIP_TRY(down(grads->light_pos, l.lpos, 3 * L));
+    HIP_TRY(down(grads->light_int, l.lint, L));
+    HIP_TRY(hipMemcpyAsync(loss, lossd, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return PRH_OK;
 }
 
 int prh_fit_new(prh_renderer* const* views, int32_t n, int32_t flags, prh_fit** out) {

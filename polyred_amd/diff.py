@@ -109,7 +109,10 @@ def forward(lib, handle, p: DiffParams, H: int, W: int) -> np.ndarray:
     return out
 
 
-def backward(lib, handle, p: DiffParams, target: np.ndarray, H: int, W: int):
+def backward(lib, handle, p: DiffParams, target: np.ndarray, H: int, W: int, noalloc: bool = False):
+    """(loss, gradients) through prh_diff_backward, or with `noalloc` through
+    prh_diff_backward_noalloc (no device allocation: the working memory is
+    the library's fixed per-GPU scratch array)."""
     nv, nm, nl = sizes(lib, handle)
     check_shapes(p, nv, nm, nl)
     target = np.ascontiguousarray(target, F32)
@@ -121,8 +124,9 @@ def backward(lib, handle, p: DiffParams, target: np.ndarray, H: int, W: int):
                    light_int=np.zeros(nl, F32))
     gs = abi.prh_diff_grads(**{n: _f32p(getattr(g, n)) for n in DiffParams.NAMES})
     loss = C.c_double()
-    abi.check(lib.prh_diff_backward(handle, C.byref(b.s), target.ctypes.data, target.nbytes, C.byref(loss),
-                                    C.byref(gs)), "prh_diff_backward")
+    fn = "prh_diff_backward_noalloc" if noalloc else "prh_diff_backward"
+    abi.check(getattr(lib, fn)(handle, C.byref(b.s), target.ctypes.data, target.nbytes, C.byref(loss), C.byref(gs)),
+              fn)
     return loss.value, g
 
 
