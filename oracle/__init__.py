@@ -49,6 +49,8 @@ def lib() -> C.CDLL:
             build()
         L = C.CDLL(LIB_PATH)
         f32p, u8p = C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+        L.oracle_set_workers.argtypes = [C.c_int]
+        L.oracle_set_workers.restype = None
         L.oracle_render.argtypes = [C.POINTER(abi.prh_scene_desc), C.POINTER(abi.prh_camera),
                                     C.POINTER(abi.prh_options), C.POINTER(oracle_out)]
         L.oracle_barycoord.argtypes = [f32p] * 5
@@ -117,6 +119,14 @@ class Frame:
         self.shadow_count = np.zeros(n, np.uint8)
         self.ao_total = np.zeros(n, np.float32)
         self.light_matrices = np.zeros(32 * nl, np.float32)
+
+
+def set_workers(n: int) -> None:
+    """Threads of later oracle frames: 0 the parity order (the default), 1
+    everything on one thread (Mode A), n > 1 the raster passes and the
+    deferred pass on n threads with per-pixel locks (Mode B: tie order
+    arbitrary, for timing only)."""
+    lib().oracle_set_workers(int(n))
 
 
 def render(scene, mask=None) -> Frame:

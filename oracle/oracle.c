@@ -625,21 +625,47 @@ typedef struct {
     double shade_err64;
     tex_t* tex; /* the scene's textures with their mip levels */
     int n_tex;
+    /* throughput mode (oracle_set_workers > 1, the CPU baseline only):
+     * triangles drawn concurrently, each pixel's test and set under a spin
+     * lock as the reference's FragmentBuffer does (buffer/buffer.go:209-280) */
+    int workers;
+    uint8_t* locks;
 } ctx_t;
+
+/* 0: the parity order (sequential raster, deferred pass on OpenMP's threads);
+ * 1: everything on one thread (the reference's Workers(1), Mode A);
+ * n > 1: n workers for the raster passes and the deferred pass (Mode B; tie
+ * order between concurrent triangles is then arbitrary, as in the
+ * reference with several workers). */
+static int g_workers = 0;
+void oracle_set_workers(int n) { g_workers = n < 0 ? 0 : n; }
 
 static inline frag_t* fat(ctx_t* c, int x, int y) { return &c->f[(size_t)(c->H - 1 - y) * c->W + x]; }
 static inline int in_buf(const ctx_t* c, int64_t x, int64_t y) { return x >= 0 && x < c->W && y >= 0 && y < c->H; }
 
+static inline void px_lock(ctx_t* c, size_t i) {
+    while (__atomic_test_and_set(&c->locks[i], __ATOMIC_ACQUIRE)) {
+    }
+}
+static inline void px_unlock(ctx_t* c, size_t i) { __atomic_clear(&c->locks[i], __ATOMIC_RELEASE); }
+
 /* buffer/buffer.go:269-280 */
 static inline int depth_test(ctx_t* c, int x, int y, float z) {
     frag_t* f = fat(c, x, y);
-    return !f->ok || z > f->depth;
+    if (!c->locks) return !f->ok || z > f->depth;
+    size_t i = (size_t)(c->H - 1 - y) * c->W + x;
+    px_lock(c, i);
+    int r = !f->ok || z > f->depth;
+    px_unlock(c, i);
+    return r;
 }
 /* buffer/buffer.go:221-266 (colour/depth images are produced later) */
 static inline void fset(ctx_t* c, int x, int y, const frag_t* nf) {
     frag_t* f = fat(c, x, y);
-    if (f->ok && nf->depth <= f->depth) return;
-    *f = *nf;
+    size_t i = (size_t)(c->H - 1 - y) * c->W + x;
+    if (c->locks) px_lock(c, i);
+    if (!(f->ok && nf->depth <= f->depth)) *f = *nf;
+    if (c->locks) px_unlock(c, i);
 }
 
 /* render/raster.go:453-460 */
@@ -797,11 +823,53 @@ static m4 obj_model(const prh_scene_desc* s, int o) {
     return m;
 }
 
-/* render/raster.go:226-273 and draw, render/raster.go:380-444 */
+/* The first global triangle of every object (n_objects + 1 entries). */
+static int64_t* obj_tri_starts(const prh_scene_desc* s) {
+    int64_t* st = malloc(sizeof(int64_t) * ((size_t)s->n_objects + 1));
+    if (!st) return NULL;
+    st[0] = 0;
+    for (int o = 0; o < s->n_objects; o++) {
+        int mesh = s->obj_mesh[o];
+        st[o + 1] = st[o] + (s->mesh_tri_off[mesh + 1] - s->mesh_tri_off[mesh]);
+    }
+    return st;
+}
+static int obj_of(const int64_t* st, int n_objects, int64_t g) {
+    int lo = 0, hi = n_objects - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (st[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+static void forward_tri(ctx_t* c, const m4* vp, const m4* vpinv, const m4* pinv, const m4* vinv, const m4* trans,
+                        const m4* normal, int o, int v0, int t, int64_t gtri);
+
+/* render/raster.go:226-273 and draw, render/raster.go:380-444.  In
+ * throughput mode (workers > 1) the triangles are drawn concurrently. */
 static void forward_pass(ctx_t* c) {
     const prh_scene_desc* s = c->s;
     m4 view = c->view, proj = c->proj, vp = c->vp;
     m4 vinv = inv(&view), pinv = inv(&proj), vpinv = inv(&vp);
+    int64_t* st = c->workers > 1 ? obj_tri_starts(s) : NULL;
+    if (st) {
+        m4 pv = mulm(&proj, &view);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(c->workers)
+        for (int64_t g = 0; g < st[s->n_objects]; g++) {
+            int o = obj_of(st, s->n_objects, g);
+            m4 model = obj_model(s, o);
+            m4 mi = inv(&model);
+            m4 normal = transpose(&mi);
+            m4 trans = mulm(&pv, &model);
+            int mesh = s->obj_mesh[o];
+            int t = s->mesh_tri_off[mesh] + (int)(g - st[o]);
+            forward_tri(c, &vp, &vpinv, &pinv, &vinv, &trans, &normal, o, s->mesh_vtx_off[mesh], t, g);
+        }
+        free(st);
+        return;
+    }
     int64_t gtri = 0;
     for (int o = 0; o < s->n_objects; o++) {
         int mesh = s->obj_mesh[o];
@@ -811,39 +879,46 @@ static void forward_pass(ctx_t* c) {
         m4 pv = mulm(&proj, &view);
         m4 trans = mulm(&pv, &model);
         int v0 = s->mesh_vtx_off[mesh];
-        for (int t = s->mesh_tri_off[mesh]; t < s->mesh_tri_off[mesh + 1]; t++, gtri++) {
-            int64_t mat = s->tri_mat[t];
-            if (mat >= 0) mat += s->obj_mat_base[o];
-            vert_t in[3], tv[3];
-            for (int k = 0; k < 3; k++) load_vert(s, v0 + s->tri_v[3 * t + k], &in[k]);
-            if (!tri_is_valid(in[0].pos, in[1].pos, in[2].pos)) continue;
-            for (int k = 0; k < 3; k++) {
-                tv[k] = in[k];
-                tv[k].pos = mulv(&trans, in[k].pos);
-                tv[k].nor = apply4(in[k].nor, &normal);
-            }
-            float recipw[3] = {1, 1, 1};
-            if (c->perspect)
-                for (int k = 0; k < 3; k++) recipw[k] = -1 / tv[k].pos.w;
-            for (int k = 0; k < 3; k++) tv[k].pos = pos4(apply4(tv[k].pos, &vp));
-            if (cull_back_face(tv[0].pos, tv[1].pos, tv[2].pos)) continue;
-            v3 p[3];
-            for (int k = 0; k < 3; k++) p[k] = V3(tv[k].pos.x, tv[k].pos.y, tv[k].pos.z);
-            int msaa = c->o->msaa > 0 ? c->o->msaa : 1;
-            aabb_t vpbox = {V3(0, 0, -1), V3((float)(msaa * c->W), (float)(msaa * c->H), 1)};
-            aabb_t tbox = aabb_new(3, p);
-            if (!aabb_intersect(&vpbox, &tbox)) continue;
-            uint32_t key = (uint32_t)(gtri * 8);
-            if (aabb_contains(&vpbox, 3, p)) {
-                draw_clipped(c, &vpinv, &pinv, &vinv, &tv[0], &tv[1], &tv[2], recipw, mat, key + 7);
-                continue;
-            }
-            vert_t sub[8][3];
-            int ns = clip_triangle(&tv[0], &tv[1], &tv[2], (float)(msaa * c->W), (float)(msaa * c->H), sub);
-            for (int i = 0; i < ns; i++)
-                draw_clipped(c, &vpinv, &pinv, &vinv, &sub[i][0], &sub[i][1], &sub[i][2], recipw, mat, key + i);
-        }
+        for (int t = s->mesh_tri_off[mesh]; t < s->mesh_tri_off[mesh + 1]; t++, gtri++)
+            forward_tri(c, &vp, &vpinv, &pinv, &vinv, &trans, &normal, o, v0, t, gtri);
     }
+}
+
+/* draw (render/raster.go:380-444) of mesh triangle t of object o, global
+ * triangle gtri */
+static void forward_tri(ctx_t* c, const m4* vp, const m4* vpinv, const m4* pinv, const m4* vinv, const m4* trans,
+                        const m4* normal, int o, int v0, int t, int64_t gtri) {
+    const prh_scene_desc* s = c->s;
+    int64_t mat = s->tri_mat[t];
+    if (mat >= 0) mat += s->obj_mat_base[o];
+    vert_t in[3], tv[3];
+    for (int k = 0; k < 3; k++) load_vert(s, v0 + s->tri_v[3 * t + k], &in[k]);
+    if (!tri_is_valid(in[0].pos, in[1].pos, in[2].pos)) return;
+    for (int k = 0; k < 3; k++) {
+        tv[k] = in[k];
+        tv[k].pos = mulv(trans, in[k].pos);
+        tv[k].nor = apply4(in[k].nor, normal);
+    }
+    float recipw[3] = {1, 1, 1};
+    if (c->perspect)
+        for (int k = 0; k < 3; k++) recipw[k] = -1 / tv[k].pos.w;
+    for (int k = 0; k < 3; k++) tv[k].pos = pos4(apply4(tv[k].pos, vp));
+    if (cull_back_face(tv[0].pos, tv[1].pos, tv[2].pos)) return;
+    v3 p[3];
+    for (int k = 0; k < 3; k++) p[k] = V3(tv[k].pos.x, tv[k].pos.y, tv[k].pos.z);
+    int msaa = c->o->msaa > 0 ? c->o->msaa : 1;
+    aabb_t vpbox = {V3(0, 0, -1), V3((float)(msaa * c->W), (float)(msaa * c->H), 1)};
+    aabb_t tbox = aabb_new(3, p);
+    if (!aabb_intersect(&vpbox, &tbox)) return;
+    uint32_t key = (uint32_t)(gtri * 8);
+    if (aabb_contains(&vpbox, 3, p)) {
+        draw_clipped(c, vpinv, pinv, vinv, &tv[0], &tv[1], &tv[2], recipw, mat, key + 7);
+        return;
+    }
+    vert_t sub[8][3];
+    int ns = clip_triangle(&tv[0], &tv[1], &tv[2], (float)(msaa * c->W), (float)(msaa * c->H), sub);
+    for (int i = 0; i < ns; i++)
+        draw_clipped(c, vpinv, pinv, vinv, &sub[i][0], &sub[i][1], &sub[i][2], recipw, mat, key + i);
 }
 
 /* ------------------------------------------------------------------ shadows */
@@ -909,7 +984,20 @@ static void draw_depth(ctx_t* c, int li, const m4* trans, const m4* vp, const ve
             if (bc[0] < -1e-7f || bc[1] < -1e-7f || bc[2] < -1e-7f) continue;
             float z = bc[0] * p[0].z + bc[1] * p[1].z + bc[2] * p[2].z;
             size_t idx = (size_t)(x + y * c->SW);
-            if (!(z <= d[idx])) d[idx] = z;
+            if (!c->locks) {
+                if (!(z <= d[idx])) d[idx] = z;
+                continue;
+            }
+            /* throughput mode: the same keep-the-larger store as one atomic step */
+            uint32_t* cell = (uint32_t*)&d[idx];
+            uint32_t old = __atomic_load_n(cell, __ATOMIC_RELAXED), nz;
+            memcpy(&nz, &z, 4);
+            for (;;) {
+                float of;
+                memcpy(&of, &old, 4);
+                if (z <= of) break;
+                if (__atomic_compare_exchange_n(cell, &old, nz, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) break;
+            }
         }
 }
 /* render/shadow.go:92-150 */
@@ -918,6 +1006,22 @@ static void pass_shadows(ctx_t* c, int li) {
     if (!s->lights[li].cast_shadow) return;
     m4 vp = viewport_matrix((float)c->SW, (float)c->SH);
     m4 pv = mulm(&c->lproj[li], &c->lview[li]);
+    int64_t* st = c->workers > 1 ? obj_tri_starts(s) : NULL;
+    if (st) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(c->workers)
+        for (int64_t g = 0; g < st[s->n_objects]; g++) {
+            int o = obj_of(st, s->n_objects, g);
+            int mesh = s->obj_mesh[o];
+            m4 model = obj_model(s, o);
+            m4 trans = mulm(&pv, &model);
+            int t = s->mesh_tri_off[mesh] + (int)(g - st[o]), v0 = s->mesh_vtx_off[mesh];
+            vert_t in[3];
+            for (int k = 0; k < 3; k++) load_vert(s, v0 + s->tri_v[3 * t + k], &in[k]);
+            if (tri_is_valid(in[0].pos, in[1].pos, in[2].pos)) draw_depth(c, li, &trans, &vp, in);
+        }
+        free(st);
+        return;
+    }
     for (int o = 0; o < s->n_objects; o++) {
         int mesh = s->obj_mesh[o];
         m4 model = obj_model(s, o);
@@ -1165,6 +1269,14 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
     size_t np = (size_t)c.W * c.H;
     c.f = calloc(np, sizeof(frag_t));
     if (!c.f) return PRH_E_NOMEM;
+    c.workers = g_workers;
+    if (c.workers > 1) {
+        c.locks = calloc(np, 1);
+        if (!c.locks) {
+            free(c.f);
+            return PRH_E_NOMEM;
+        }
+    }
     if (s->n_textures > 0) {
         c.tex = calloc((size_t)s->n_textures, sizeof(tex_t));
         int bad = !c.tex;
@@ -1211,7 +1323,10 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
     shade(&c, 0, 0);
     const uint8_t* mask = out->shade_mask;
 #ifdef _OPENMP
-#pragma omp parallel
+    /* (the deferred pass: on OpenMP's threads in the parity order, on one
+     * in Mode A, on `workers` in Mode B) */
+    int nthr = c.workers == 0 ? omp_get_max_threads() : c.workers;
+#pragma omp parallel num_threads(nthr)
     {
         ctx_t cl = c;
         cl.shade_err64 = 0;
@@ -1261,6 +1376,7 @@ int oracle_render(const prh_scene_desc* s, const prh_camera* cam, const prh_opti
             memcpy(out->light_matrices + 32 * (size_t)i + 16, c.lproj[i].a, 64);
         }
     free(c.f);
+    free(c.locks);
     for (int i = 0; i < c.n_tex; i++) tex_free(&c.tex[i]);
     free(c.tex);
     free(c.lview);

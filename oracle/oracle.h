@@ -74,6 +74,16 @@ typedef struct oracle_out {
 
 /* With MSAA m > 1 every buffer is (height*m) x (width*m) except the final
  * image, which is resized to height x width (render/raster.go:361-378). */
+/* Threads of the CPU baseline's timed frames: 0 (the default) renders in
+ * the parity order (sequential raster; the deferred pass, whose result does
+ * not depend on it, on OpenMP's threads); 1 runs everything on one thread
+ * (Workers(1)); n > 1 draws the triangles of every raster pass on n threads
+ * concurrently, each pixel's depth test and set under a spin lock as the
+ * reference's FragmentBuffer does, and shades on n threads (ties between
+ * concurrent triangles then resolve in arbitrary order, as in the reference
+ * with several workers: never used for parity). */
+void oracle_set_workers(int n);
+
 int oracle_render(const prh_scene_desc* scene, const prh_camera* cam,
                   const prh_options* opts, oracle_out* out);
 
