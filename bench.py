@@ -8,9 +8,14 @@ raster, deferred shading with shadows and AO, gamma.  The scene is resident
 in HBM before the timed region; each step is one synchronous prh_render.
 
 Prints one JSON line.  `value` is frames/s on the one GPU; Mtris/s and Mpix/s
-ride along.  `roofline` is for the dominant kernel, timed with HIP events on
-libprh's stream over the timed region; `cpu_baseline` is the C restatement of
-the reference CPU renderer (the parity oracle) timed on this host.
+ride along.  After the timed region the same frames run again with every
+pass on one stream, so each kernel's HIP-event time is its own: `roofline` is
+the longest of them (the dominant kernel), `roofline_shade_pass` the whole
+shading pass (AO + deferred shading) against the survey's byte formula, each
+with its VALU issue rate beside the HBM one.  `cpu_baseline` is the C
+restatement of the reference CPU renderer (the parity oracle) timed on this
+host in the reference's throughput mode, with the one-thread mode beside it;
+`fp64_err` is the shading's largest error against an FP64 re-evaluation.
 """
 from __future__ import annotations
 
@@ -82,11 +87,20 @@ def ao_bytes(scene, vis: np.ndarray, n_ao: int) -> int:
     return int(vis.size * 8 + int((vis != EMPTY).sum()) * 4 + n_ao * 4)
 
 
+def shade_pass_bytes(P: int, ao: bool, n_shadow: int, visible_tris: int) -> int:
+    """Algorithmic DRAM bytes of the whole shading pass by the survey's
+    formula: per pixel the 8-byte visibility key, the 4-byte RGBA8 output,
+    the 4-byte depth if AO is on and one 4-byte shadow texel per shadowed
+    light; per distinct visible triangle 12 bytes of indices and 72 of
+    positions and normals."""
+    return int(P * (8 + 4 + (4 if ao else 0) + 4 * n_shadow) + visible_tris * (12 + 72))
+
+
 def pmc_summary(path: str) -> dict:
     """Per-kernel figures from a committed rocprofv3 --pmc summary: lines
     `kernel.<name>.<field>: <value>` (traffic_bytes, valu_insts per
     dispatch, with FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
-    calibration)."""
+    calibration), and `source_commit: <hash>`, the tree it was recorded on."""
     out = {}
     try:
         with open(path) as f:
@@ -95,9 +109,28 @@ def pmc_summary(path: str) -> dict:
                     k, v = line.split(":", 1)
                     _, name, field_ = k.strip().split(".", 2)
                     out.setdefault(name, {})[field_] = float(v)
+                elif line.startswith("source_commit:"):
+                    out["_commit"] = line.split(":", 1)[1].strip()
     except OSError:
         pass
     return out
+
+
+PASSES = ("total", "shadow", "vertex", "setup", "raster", "ao", "shade")
+
+
+def run_frames(r, steps: int):
+    """`steps` synchronous frames: (wall seconds, mean per-pass ms from the
+    frames' own HIP events)."""
+    acc = dict.fromkeys(PASSES, 0.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render_device()  # synchronous: returns when the frame is complete
+        st = r.stats()
+        for k in PASSES:
+            acc[k] += getattr(st, "ms_" + k)
+    dt = time.perf_counter() - t0
+    return dt, {k: v / steps for k, v in acc.items()}
 
 
 def adjoint_bench(dev, steps: int) -> dict:
@@ -146,6 +179,14 @@ def batch_bench(dev, reps: int) -> dict:
     rs = [render.renderer_for(s, dev) for s in views]
     for r in rs:
         r.render_device()
+    # the wide batch's knobs, per size class, timed before the measurement
+    # (results unchanged: every view stays bit-identical)
+    render.render_batch(rs)
+    ms_default = min(render.render_batch(rs) for _ in range(3))
+    small, large = render.autotune_batch(rs, frames=3)
+    knobs = ("item_prims", "raster_wg_per_cu", "shadow_direct_max", "main_direct_max", "raster_tile")
+    tuned = {"small_views": {k: getattr(small, k) for k in knobs}, "large_views": {k: getattr(large, k) for k in knobs},
+             "ms_batch_default_knobs": round(ms_default, 3), "ms_batch_tuned": round(small.ms, 3)}
     for streams in (False, True):
         render.render_batch(rs, streams=streams)
         render.render_batch(rs, graph=True, streams=streams)
@@ -173,7 +214,7 @@ def batch_bench(dev, reps: int) -> dict:
             "speedup_streams_graph": round(seq / sgr, 3),
             "speedup_batch": round(seq / bat, 3), "speedup_graph": round(seq / gr, 3),
             "views_per_s": round(1000.0 * len(views) / best, 1), "mtris_per_s": round(n_tri / best / 1e3, 1),
-            "mpix_per_s": round(n_px / best / 1e3, 1)}
+            "mpix_per_s": round(n_px / best / 1e3, 1), "tuning": tuned}
 
 
 def fit_bench(dev, steps: int = 200) -> dict:
@@ -219,19 +260,37 @@ def fit_bench(dev, steps: int = 200) -> dict:
 
 
 def cpu_baseline(scene, frames: int) -> dict:
+    """The C restatement of the reference CPU renderer (the parity oracle)
+    timed on this host's cores in the reference's two modes: B, throughput
+    (every raster pass and the deferred pass on all the threads, per-pixel
+    locks, as the default Workers(nproc)), and A, one thread (Workers(1)).
+    Also returns the parity-order frame (for the parity check and the FP64
+    shading error)."""
     import oracle  # the parity oracle, timed as the CPU baseline only
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    oracle.render(scene)  # warm the page cache / lazy init
-    t = time.perf_counter()
-    fr = None
-    for _ in range(frames):
-        fr = oracle.render(scene)
-    dt = time.perf_counter() - t
-    return dict(value=frames / dt, unit="frames/s", cores=threads, kind="port",
-                sample=f"{frames} full frames of the same 1080p workload through the C restatement of the "
-                       f"reference CPU renderer (raster passes sequential, deferred pass on {threads} OpenMP "
-                       f"threads), {dt:.1f} s", _frame=fr)
+    fr = oracle.render(scene)  # the parity order; also warms the page cache
+    try:
+        oracle.set_workers(threads)
+        t = time.perf_counter()
+        for _ in range(frames):
+            oracle.render(scene)
+        dt_b = time.perf_counter() - t
+        oracle.set_workers(1)
+        t = time.perf_counter()
+        oracle.render(scene)
+        dt_a = time.perf_counter() - t
+    finally:
+        oracle.set_workers(0)
+    omit = ("the reference also walks the scene graph for its lights at every pixel (render/raster.go:334), "
+            "which the restatement does once per frame, so both modes are faster than the Go renderer would be")
+    return dict(value=frames / dt_b, unit="frames/s", cores=threads, kind="port",
+                sample=f"Mode B: {frames} full frames of the same 1080p workload through the C restatement of the "
+                       f"reference CPU renderer, every pass on {threads} threads with per-pixel locks, "
+                       f"{dt_b:.1f} s; {omit}",
+                mode_a={"value": round(1.0 / dt_a, 4), "unit": "frames/s", "cores": 1,
+                        "sample": f"Mode A: 1 full frame on one thread (Workers(1)), {dt_a:.1f} s"},
+                _frame=fr)
 
 
 def main() -> int:
@@ -249,6 +308,9 @@ def main() -> int:
                     help="run the 64-view batched measurement alone (for a profile of the batch)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default knobs instead of prh_autotune's choice (untimed, before the warm-up)")
+    ap.add_argument("--overlap", choices=("auto", "0", "1"), default="auto",
+                    help="the timed frames' shadow passes beside the main view's passes on a second stream: "
+                         "the autotuner's choice (auto), off or on")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run 3 elementwise copies of MIB MiB (torch.neg), the known-size kernel that "
                          "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -288,22 +350,20 @@ def main() -> int:
         t = r.Autotune(frames=5)  # occupancy, bin capacity, small-triangle thresholds; results unchanged
         tuned = {"raster_wg_per_cu": t.raster_wg_per_cu, "item_prims": t.item_prims,
                  "shadow_direct_max": t.shadow_direct_max, "main_direct_max": t.main_direct_max,
-                 "overlap_shadows": t.overlap_shadows}
+                 "overlap_shadows": t.overlap_shadows, "raster_tile": t.raster_tile}
+    if args.overlap != "auto":
+        r.set_tuning(overlap_shadows=int(args.overlap))
+    overlap = bool(r.tuning().overlap_shadows)
     for _ in range(args.warmup):
         r.render_device()
-    acc = dict(total=0.0, shadow=0.0, vertex=0.0, setup=0.0, raster=0.0, ao=0.0, shade=0.0)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.render_device()  # synchronous: returns when the frame is complete
-        st = r.stats()
-        acc["total"] += st.ms_total
-        acc["shadow"] += st.ms_shadow
-        acc["vertex"] += st.ms_vertex
-        acc["setup"] += st.ms_setup
-        acc["raster"] += st.ms_raster
-        acc["shade"] += st.ms_shade
-        acc["ao"] += st.ms_ao
-    dt = time.perf_counter() - t0
+    dt, ms = run_frames(r, args.steps)  # the timed region
+    # (outside the timed region) the same frames with every pass on one
+    # stream: each pass's time is its kernels' own, and the passes add up
+    r.set_tuning(overlap_shadows=0)
+    for _ in range(2):
+        r.render_device()
+    _, ms1 = run_frames(r, args.steps)
+    r.set_tuning(overlap_shadows=int(overlap))
     # (outside the timed region) the frame plus its image copied to the host
     # over PCIe, as a caller that reads every frame back sees it
     n_rb = max(3, args.steps // 4)
@@ -314,28 +374,46 @@ def main() -> int:
         r.render_device()
         r.image(host)
     ms_readback = 1000.0 * (time.perf_counter() - t_rb) / n_rb
-    ms = {k: v / args.steps for k, v in acc.items()}
     vis = r.read(abi.PRH_BUF_VIS)
     img = r.image()
     st = r.stats()
-    n_shadow = sum(1 for l in scene.lights if l.cast_shadow)
+    n_shadow = sum(1 for l in scene.lights if l.cast_shadow) if scene.shadow_map else 0
+    any_ao = any(m.ambient_occlusion for m in scene.materials)
     sb = shade_bytes(scene, vis)
-    split_shade = bool(r.tuning().overlap_shadows and scene.shadow_map and n_shadow > 0
-                       and any(m.ambient_occlusion for m in scene.materials))
     ab = ao_bytes(scene, vis, sb["ao_px"])
     ms_step = 1000.0 * dt / args.steps
     fps = args.steps / dt
-    pmc = pmc_summary(os.path.join(ROOT, "profiles", "c2_1080p_pmc.md"))
+    pmc_path = os.path.join("profiles", "c2_1080p_serial_pmc.md")
+    pmc = pmc_summary(os.path.join(ROOT, pmc_path))
 
-    def roof(kernel: str, nbytes: int, ms_kernel: float) -> dict:
+    def roof(kernel: str, nbytes: int, ms_kernel: float, pmc_kernels=()) -> dict:
         achieved = nbytes / (ms_kernel * 1e-3) / 1e9
-        p = pmc.get(kernel, {})
-        r = {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": p.get("traffic_bytes"),
-             "algorithmic_bytes": nbytes, "ms": round(ms_kernel, 4)}
-        if "valu_insts" in p:
-            r["valu_issue_frac"] = round(p["valu_insts"] / (ms_kernel * 1e-3) / VALU_PEAK, 3)
-        return r
+        ks = [pmc.get(k, {}) for k in (pmc_kernels or (kernel,))]
+        traffic = sum(p["traffic_bytes"] for p in ks) if ks and all("traffic_bytes" in p for p in ks) else None
+        insts = sum(p["valu_insts"] for p in ks) if ks and all("valu_insts" in p for p in ks) else None
+        out = {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+               "algorithmic_bytes": nbytes, "ms": round(ms_kernel, 4)}
+        if insts is not None:
+            rate = insts / (ms_kernel * 1e-3)
+            out["valu"] = {"insts_per_launch": int(insts), "achieved": round(rate / 1e9, 1), "peak": VALU_PEAK / 1e9,
+                           "unit": "G wave64 insts/s", "frac": round(rate / VALU_PEAK, 3)}
+        return out
+
+    pmc_note = (f"traffic and VALU instruction counts per launch from {pmc_path} (rocprofv3 --pmc, recorded on "
+                f"commit {pmc.get('_commit', '?')}); times from this run's HIP events with every pass on one stream")
+    # the dominant kernel: the longest of the frame's kernels run alone
+    ao_roof = dict(roof("k_ao", ab, ms1["ao"]), standalone=True, source=pmc_note)
+    shade_roof = dict(roof("k_shade", sb["bytes"], ms1["shade"]), standalone=True, covered_px=sb["covered"],
+                      visible_tris=sb["visible_tris"], visible_verts=sb["visible_verts"], source=pmc_note)
+    dominant = ao_roof if ms1["ao"] >= ms1["shade"] else shade_roof
+    # the whole shading pass (the reference's shade: Blinn-Phong, shadows,
+    # AO, gamma) against the survey's byte formula
+    pass_bytes = shade_pass_bytes(scene.n_pixels, any_ao, n_shadow, sb["visible_tris"])
+    pass_roof = dict(roof("k_ao + k_shade", pass_bytes, ms1["ao"] + ms1["shade"], ("k_ao", "k_shade")),
+                     standalone=True, formula="P*(8 vis + 4 out + 4 depth [AO] + 4 per shadowed light) + "
+                                              "T_vis*(12 + 72)", source=pmc_note)
+    other = ms1["total"] - sum(ms1[k] for k in PASSES[1:])
 
     out = {
         "metric": METRIC,
@@ -352,19 +430,19 @@ def main() -> int:
         "data": "synthetic (seeded scene generator, no assets)",
         "config": {"workload": scene.name, "width": scene.width, "height": scene.height,
                    "triangles": scene.n_triangles, "lights": len(scene.lights), "shadow_maps": n_shadow,
-                   "shadow_size": scene.shadow_size, "ao": True, "srgb": True},
+                   "shadow_size": scene.shadow_size, "ao": any_ao, "srgb": True},
         "mtris_per_s": round(scene.n_triangles * fps / 1e6, 2),
         "mpix_per_s": round(scene.n_pixels * fps / 1e6, 2),
-        "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
-        # the dominant kernel (AO, VALU-bound: its HBM fraction is small by
-        # construction) and the deferred-shading kernel the HBM target names
-        # (with the shadow passes on the second stream and AO in the scene, the
-        # shading's first half runs beside k_ao: both times are then spans of
-        # kernels sharing the GPU, and "overlapped" says so)
-        "roofline": dict(roof("k_ao", ab, ms["ao"]), overlapped=split_shade),
-        "roofline_shade": dict(roof("k_shade", sb["bytes"], ms["shade"]), covered_px=sb["covered"],
-                               visible_tris=sb["visible_tris"], visible_verts=sb["visible_verts"],
-                               overlapped=split_shade),
+        "roofline": dominant,
+        "roofline_shade_pass": pass_roof,
+        "roofline_kernels": {"k_ao": ao_roof, "k_shade": shade_roof},
+        # per-pass device times of the timed frames; with the shadow passes
+        # on the second stream (overlap) the shadow span and the shading's
+        # first half run beside the main view's passes, so they do not add up
+        "kernel_ms": dict({k: round(v, 4) for k, v in ms.items()}, overlap=overlap),
+        # every pass on one stream: the passes and "other" (clears, the
+        # background colour) add up to total
+        "kernel_ms_serial": dict({k: round(v, 4) for k, v in ms1.items()}, other=round(other, 4)),
         "frame": {"prims": st.n_prims, "clipped": st.n_clipped, "bin_pairs": st.n_bin_pairs},
         "tuning": tuned,
         "ms_per_frame_with_readback": round(ms_readback, 4),
@@ -383,6 +461,9 @@ def main() -> int:
         order = (vis & np.uint64(0xFFFFFFFF)).astype(np.uint32)
         order[vis == np.uint64(0xFFFFFFFFFFFFFFFF)] = 0xFFFFFFFF
         out["parity_vs_cpu"] = bool(np.array_equal(order, fr.vis_order) and np.array_equal(img, fr.image))
+        # the shading's largest |float32 - float64| over the frame (0..255
+        # scale of the pre-Round sums; libprh's sums equal the oracle's)
+        out["fp64_err"] = float(fr.err64.value)
         cb["value"] = round(cb["value"], 4)
         out["cpu_baseline"] = cb
     else:

@@ -380,18 +380,32 @@ typedef struct prh_tuning {
     int32_t main_direct_max;   /* main-view triangles of at most this many pixel centres, likewise; 48  */
     int32_t overlap_shadows;   /* 1: a frame's shadow passes run on a second stream beside the main
                                   view's passes; default 1 from 2^19 framebuffer pixels, else 0          */
+    int32_t raster_tile;       /* side of the binning and raster tiles (main view and shadow maps):
+                                  8, 16 or 32 pixels; default 16                                          */
     float ms;                  /* prh_autotune: the chosen configuration's median frame time             */
 } prh_tuning;
 int prh_get_tuning(prh_renderer* r, prh_tuning* out);
 int prh_set_tuning(prh_renderer* r, const prh_tuning* t);
 /* Coordinate descent over item_prims {256..4096}, raster_wg_per_cu {1, 2, 4,
  * 8}, shadow_direct_max {0, 16, 32, 64, 128}, main_direct_max {0, 24, 48,
- * 96} and overlap_shadows {0, 1}: each configuration timed as the median of
+ * 96}, overlap_shadows {0, 1} and raster_tile {8, 16, 32}: each
+ * configuration timed as the median of
  * `frames` frames (HIP events, after one warm-up frame); the winner is timed
  * again against the starting knobs over twice the frames and the faster of
  * the two stays on r and goes to *out (may be NULL).  r holds a frame of it
  * on return. */
 int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out);
+/* The same search for a batch (no counterpart in polyred): the views of a
+ * prh_render_batch of wide launches, split into two size classes (class 0
+ * below 2^19 framebuffer pixels, class 1 the rest), each class with its own
+ * item_prims, raster_wg_per_cu (in a wide batch: n_CU * raster_wg_per_cu /
+ * 16 persistent raster workgroups per view, at least 32), shadow_direct_max,
+ * main_direct_max and raster_tile.  Coordinate descent over the (class, knob) pairs, each
+ * configuration timed as the median of `frames` eager wide batches; the
+ * winner is checked against the starting knobs as prh_autotune does.  Every
+ * view keeps its class's winner; out[0..1] (may be NULL) receive the two,
+ * with ms the batch's median time. */
+int prh_autotune_batch(prh_renderer* const* renderers, int32_t n, int32_t frames, prh_tuning out[2]);
 
 /* Page-locked host memory for the read calls (prh_read_image,
  * prh_read_buffer, prh_surface_read): a copy into it runs at the link's

@@ -164,7 +164,8 @@ PRH_FIT_STORED_GBUFFER = 256
 
 class prh_tuning(C.Structure):
     _fields_ = [("raster_wg_per_cu", C.c_int32), ("item_prims", C.c_int32), ("shadow_direct_max", C.c_int32),
-                ("main_direct_max", C.c_int32), ("overlap_shadows", C.c_int32), ("ms", C.c_float)]
+                ("main_direct_max", C.c_int32), ("overlap_shadows", C.c_int32), ("raster_tile", C.c_int32),
+                ("ms", C.c_float)]
 
 
 class prh_fit_options(C.Structure):
@@ -209,6 +210,7 @@ PROTOTYPES = {
     "prh_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(prh_tuning)]),
     "prh_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(prh_tuning)]),
     "prh_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(prh_tuning)]),
+    "prh_autotune_batch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(prh_tuning)]),
     "prh_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "prh_host_free": (C.c_int, [C.c_void_p]),
     "prh_surface_new": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

@@ -135,10 +135,10 @@ __device__ __forceinline__ int wave_alloc(int32_t* counter, int key, bool active
     return pos;
 }
 // Counts a primitive into every tile its pixel box touches (wave-uniform call).
-__device__ __forceinline__ void count_tiles(int32_t* tile_count, int base, int tiles_x, bool active, int x0, int y0,
-                                            int x1, int y1) {
-    const int tx0 = x0 / TILE, ty0 = y0 / TILE, ntx = active ? x1 / TILE - tx0 + 1 : 0;
-    const int n = active ? ntx * (y1 / TILE - ty0 + 1) : 0;
+__device__ __forceinline__ void count_tiles(int32_t* tile_count, int base, int tiles_x, int shift, bool active, int x0,
+                                            int y0, int x1, int y1) {
+    const int tx0 = x0 >> shift, ty0 = y0 >> shift, ntx = active ? (x1 >> shift) - tx0 + 1 : 0;
+    const int n = active ? ntx * ((y1 >> shift) - ty0 + 1) : 0;
     for (int j = 0; __ballot(j < n) != 0; j++) {
         const bool act = j < n;
         const int key = act ? base + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx : 0;
@@ -512,7 +512,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
         if (direct && !drawn) store_prim(prims, pbox, g, p1, p2, p3, key, x0, y0, x1, y1);
         else store_empty(pbox, g);
     }
-    count_tiles(tile_count, 0, vc.tiles_x, direct && !drawn, x0, y0, x1, y1);
+    count_tiles(tile_count, 0, vc.rtiles_x, vc.rt_shift, direct && !drawn, x0, y0, x1, y1);
     const int pos = wave_alloc(&cnt->n_clipped, 0, doclip);
     if (doclip) {
         if (pos < cap.clip_tris) clip_list[pos] = g;
@@ -605,8 +605,9 @@ __device__ __forceinline__ void clip_one(int qi, const DevScene& s, const ViewCo
         }
         const size_t ps = (size_t)s.n_tris + base + i - 2;
         store_prim(prims, pbox, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)(i - 2), bx0, by0, bx1, by1);
-        for (int ty = by0 / TILE; ty <= by1 / TILE; ty++)
-            for (int tx = bx0 / TILE; tx <= bx1 / TILE; tx++) atomicAdd(&tile_count[ty * vc.tiles_x + tx], 1);
+        for (int ty = by0 >> vc.rt_shift; ty <= by1 >> vc.rt_shift; ty++)
+            for (int tx = bx0 >> vc.rt_shift; tx <= bx1 >> vc.rt_shift; tx++)
+                atomicAdd(&tile_count[ty * vc.rtiles_x + tx], 1);
     }
 }
 
@@ -730,7 +731,7 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
         if (emit) store_prim(prims, pbox, ps, p1, p2, p3, (uint32_t)slot, x0, y0, x1, y1);
         else store_empty(pbox, ps);
     }
-    count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, emit, x0, y0, x1, y1);
+    count_tiles(tile_count, slot * sc.tiles_per_map, sc.tiles_x, sc.rt_shift, emit, x0, y0, x1, y1);
 }
 __global__ __launch_bounds__(256) void k_setup_shadow(k_setup_shadow_args a) {
     k_setup_shadow_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
@@ -921,6 +922,7 @@ struct k_bin_args {
     const uint2* pbox;
     Counters* cnt;
     int32_t tiles_x;
+    int32_t shift;  // log2 of the raster tile side
     int32_t tiles_per_view;
     int32_t shadow;
     int32_t* cursor;
@@ -934,6 +936,7 @@ __device__ __forceinline__ void k_bin_body(const k_bin_args& a_, const VB vb) {
     const uint2* __restrict__ pbox = a_.pbox;
     Counters* cnt = a_.cnt;
     const int32_t tiles_x = a_.tiles_x;
+    const int32_t shift = a_.shift;
     const int32_t tiles_per_view = a_.tiles_per_view;
     const int32_t shadow = a_.shadow;
     int32_t* __restrict__ cursor = a_.cursor;
@@ -959,8 +962,8 @@ __device__ __forceinline__ void k_bin_body(const k_bin_args& a_, const VB vb) {
             base = shadow ? (i / n_tris) * tiles_per_view : 0;  // shadow slot s holds [s*Tg, (s+1)*Tg)
         }
     }
-    const int tx0 = x0 / TILE, ty0 = y0 / TILE, ntx = active ? x1 / TILE - tx0 + 1 : 0;
-    const int n = active ? ntx * (y1 / TILE - ty0 + 1) : 0;
+    const int tx0 = x0 >> shift, ty0 = y0 >> shift, ntx = active ? (x1 >> shift) - tx0 + 1 : 0;
+    const int n = active ? ntx * ((y1 >> shift) - ty0 + 1) : 0;
     for (int j = 0; __ballot(j < n) != 0; j++) {
         const bool act = j < n;
         const int key = act ? base + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx : 0;
@@ -990,15 +993,16 @@ struct alignas(16) LPrim {
     float bcx, bcy, sabc, az;
     float bz, cz;
     uint32_t key;
-    uint32_t rect;  // rx0 | ry0<<4 | (w-1)<<8 | (h-1)<<12 (tile-local)
+    uint32_t rect;  // rx0 | ry0<<5 | (w-1)<<10 | (h-1)<<15 (tile-local)
 };
 
-// Loads one primitive and intersects its pixel box with the tile; returns
-// the candidate count.
+// Loads one primitive and intersects its pixel box with the T x T tile;
+// returns the candidate count.
+template <int T>
 __device__ __forceinline__ int stage_prim(const Prim& p, int px0, int py0, LPrim& lp) {
     uint32_t lo = fbits(p.p2.z), hi = fbits(p.p2.w);
     int x0 = max((int)(lo & 0xffff), px0), y0 = max((int)(lo >> 16), py0);
-    int x1 = min((int)(hi & 0xffff), px0 + TILE - 1), y1 = min((int)(hi >> 16), py0 + TILE - 1);
+    int x1 = min((int)(hi & 0xffff), px0 + T - 1), y1 = min((int)(hi >> 16), py0 + T - 1);
     if (x0 > x1 || y0 > y1) return 0;
     TriSetup t = tri_setup(p.p0.x, p.p0.y, p.p0.w, p.p1.x, p.p1.z, p.p1.w);
     lp.ax = t.ax, lp.ay = t.ay, lp.bx = t.bx, lp.by = t.by;
@@ -1007,12 +1011,13 @@ __device__ __forceinline__ int stage_prim(const Prim& p, int px0, int py0, LPrim
     lp.az = p.p0.z, lp.bz = p.p1.y, lp.cz = p.p2.x;
     lp.key = fbits(p.p2.y);
     int w = x1 - x0 + 1, h = y1 - y0 + 1;
-    lp.rect = (uint32_t)(x0 - px0) | ((uint32_t)(y0 - py0) << 4) | ((uint32_t)(w - 1) << 8) | ((uint32_t)(h - 1) << 12);
+    lp.rect = (uint32_t)(x0 - px0) | ((uint32_t)(y0 - py0) << 5) | ((uint32_t)(w - 1) << 10) | ((uint32_t)(h - 1) << 15);
     return w * h;
 }
 
 // Candidate idx -> (primitive j, tile-local pixel).  l / w by multiply-shift
-// is exact for l < 256, w <= 16.
+// is exact for l < 1024, w <= 32: the product overshoots l / w by less than
+// l / 65536 < 1/64, and l / w's fraction is at most 1 - 1/w.
 __device__ __forceinline__ void candidate(const int32_t* start, const LPrim* sp, int idx, int& j, int& lx, int& ly) {
     int lo = 0, hi = 255;
     while (lo < hi) {
@@ -1023,10 +1028,10 @@ __device__ __forceinline__ void candidate(const int32_t* start, const LPrim* sp,
     j = lo;
     int l = idx - start[j];
     uint32_t r = sp[j].rect;
-    int w = ((r >> 8) & 15) + 1;
+    int w = ((r >> 10) & 31) + 1;
     int q = (int)(((uint32_t)l * ((65536u + (uint32_t)w - 1u) / (uint32_t)w)) >> 16);
-    lx = (int)(r & 15) + (l - q * w);
-    ly = (int)((r >> 4) & 15) + q;
+    lx = (int)(r & 31) + (l - q * w);
+    ly = (int)((r >> 5) & 31) + q;
 }
 
 __device__ __forceinline__ bool eval_candidate(const LPrim& p, float px, float py, float& z) {
@@ -1070,7 +1075,7 @@ struct k_raster_args {
     int32_t item_prims;
     int32_t gx, gy;  // the grid of this launch
 };
-template <int MODE>
+template <int MODE, int T>
 __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB vb) {
     const int32_t item_prims = a_.item_prims;
     const Prim* __restrict__ prims = a_.prims;
@@ -1083,13 +1088,14 @@ __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB 
     const int32_t cap_items = a_.cap_items;
 
     constexpr bool SHADOW = MODE == MODE_SHADOW, EXACT = MODE == MODE_EXACT;
+    constexpr int TP = T * T;  // pixels of a tile: 64, 256 or 1024 (a quarter, one or four per thread)
     __shared__ LPrim sp[CHUNK];
     __shared__ int32_t start[CHUNK + 1];
     __shared__ int32_t wsum[4];
-    __shared__ unsigned long long svis[SHADOW ? 1 : TILE_PX];
-    __shared__ uint32_t smax[SHADOW ? TILE_PX : 1];
-    __shared__ uint32_t sfirst[EXACT ? TILE_PX : 1];
-    __shared__ uint32_t snan[EXACT ? TILE_PX : 1];
+    __shared__ unsigned long long svis[SHADOW ? 1 : TP];
+    __shared__ uint32_t smax[SHADOW ? TP : 1];
+    __shared__ uint32_t sfirst[EXACT ? TP : 1];
+    __shared__ uint32_t snan[EXACT ? TP : 1];
     const int tid = threadIdx.x;
     const int n_items = cnt->n_chunks;
     if (cnt->overflow || n_items > cap_items) return;
@@ -1098,17 +1104,19 @@ __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB 
         const int tile = item_tile[item];
         const int slot = SHADOW ? tile / rt.tiles_per_map : 0;
         const int t = SHADOW ? tile % rt.tiles_per_map : tile;
-        const int px0 = (t % rt.tiles_x) * TILE, py0 = (t / rt.tiles_x) * TILE;
+        const int px0 = (t % rt.tiles_x) * T, py0 = (t / rt.tiles_x) * T;
         const int first_item = chunk_off[tile];
         const bool shared_tile = chunk_off[tile + 1] - first_item > 1;
         const int beg = tile_off[tile] + (item - first_item) * item_prims;
         const int n = min(item_prims, tile_off[tile + 1] - beg);
-        if (SHADOW) smax[tid] = depth_ord(0.0f);
-        else svis[tid] = VIS_EMPTY;
-        if (EXACT) sfirst[tid] = 0xffffffffu, snan[tid] = 0xffffffffu;
+        for (int pp = tid; pp < TP; pp += 256) {
+            if (SHADOW) smax[pp] = depth_ord(0.0f);
+            else svis[pp] = VIS_EMPTY;
+            if (EXACT) sfirst[pp] = 0xffffffffu, snan[pp] = 0xffffffffu;
+        }
         for (int b0 = 0; b0 < n; b0 += CHUNK) {
             int c = 0;
-            if (b0 + tid < n) c = stage_prim(prims[tile_list[beg + b0 + tid]], px0, py0, sp[tid]);
+            if (b0 + tid < n) c = stage_prim<T>(prims[tile_list[beg + b0 + tid]], px0, py0, sp[tid]);
             block_scan_256(c, start, wsum);
             const int total = start[CHUNK];
             for (int idx = tid; idx < total; idx += 256) {
@@ -1116,7 +1124,7 @@ __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB 
                 candidate(start, sp, idx, j, lx, ly);
                 float z;
                 if (!eval_candidate(sp[j], (float)(px0 + lx) + 0.5f, (float)(py0 + ly) + 0.5f, z)) continue;
-                const int pix = ly * TILE + lx;
+                const int pix = ly * T + lx;
                 const uint32_t key = sp[j].key;
                 if (EXACT) atomicMin(&sfirst[pix], key);
                 if (z != z) {
@@ -1129,44 +1137,47 @@ __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB 
             }
             __syncthreads();
         }
-        const int x = px0 + (tid & (TILE - 1)), y = py0 + tid / TILE;
-        if (SHADOW) {
-            if (x < rt.W && y < rt.H && smax[tid] > depth_ord(0.0f)) {
-                uint32_t* m = &rt.smap_ord[(size_t)slot * rt.W * rt.H + (size_t)y * rt.W + x];
-                if (shared_tile) atomicMax(m, smax[tid]);
-                else if (smax[tid] > *m) *m = smax[tid];
-            }
-        } else if (x < rt.W && y < rt.H) {
-            const size_t o = (size_t)y * rt.W + x;
-            // a single-item tile is this item's alone here, but the setup may
-            // have drawn small triangles into it already: keep the smaller key
-            if (!shared_tile) {
-                const unsigned long long cur = rt.vis[o];
-                rt.vis[o] = svis[tid] < cur ? svis[tid] : cur;
-            } else if (svis[tid] != VIS_EMPTY) {
-                atomicMin(&rt.vis[o], svis[tid]);
-            }
-            if (EXACT) {
-                if (sfirst[tid] != 0xffffffffu) atomicMin(&rt.first_all[o], sfirst[tid]);
-                if (snan[tid] != 0xffffffffu) atomicMin(&rt.first_nan[o], snan[tid]);
+        for (int pp = tid; pp < TP; pp += 256) {
+            const int x = px0 + (pp & (T - 1)), y = py0 + pp / T;
+            if (SHADOW) {
+                if (x < rt.W && y < rt.H && smax[pp] > depth_ord(0.0f)) {
+                    uint32_t* m = &rt.smap_ord[(size_t)slot * rt.W * rt.H + (size_t)y * rt.W + x];
+                    if (shared_tile) atomicMax(m, smax[pp]);
+                    else if (smax[pp] > *m) *m = smax[pp];
+                }
+            } else if (x < rt.W && y < rt.H) {
+                const size_t o = (size_t)y * rt.W + x;
+                // a single-item tile is this item's alone here, but the setup
+                // may have drawn small triangles into it already: keep the
+                // smaller key
+                if (!shared_tile) {
+                    const unsigned long long cur = rt.vis[o];
+                    rt.vis[o] = svis[pp] < cur ? svis[pp] : cur;
+                } else if (svis[pp] != VIS_EMPTY) {
+                    atomicMin(&rt.vis[o], svis[pp]);
+                }
+                if (EXACT) {
+                    if (sfirst[pp] != 0xffffffffu) atomicMin(&rt.first_all[o], sfirst[pp]);
+                    if (snan[pp] != 0xffffffffu) atomicMin(&rt.first_nan[o], snan[pp]);
+                }
             }
         }
         __syncthreads();
     }
     if (saw_nan) atomicOr(SHADOW ? &cnt->shadow_nan : &cnt->nan_flag, 1);
 }
-template <int MODE>
+template <int MODE, int T>
 __global__ __launch_bounds__(256) void k_raster(k_raster_args a) {
-    k_raster_body<MODE>(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+    k_raster_body<MODE, T>(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
 }
-template <int MODE>
+template <int MODE, int T>
 __global__ __launch_bounds__(256) void k_raster_wide(const k_raster_args* __restrict__ a, const int32_t* __restrict__ start,
         int32_t n) {
     const int v = wide_view(start, n, (int)blockIdx.x);
     const k_raster_args& x = a[v];
     const int local = (int)blockIdx.x - start[v];
     if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
-    k_raster_body<MODE>(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+    k_raster_body<MODE, T>(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // Fragment depth per pixel from the visibility key (-1 where uncovered, as
@@ -2278,20 +2289,29 @@ void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, i
        k_scan_apply_args{counts, n, bsum, offsets, chunk_off, cursor, item_tile, cap_items, blk_prims, n_blk,
                          max_clip_subs, cnt, item_prims, 0, 0});
 }
-void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
+void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t shift, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris) {
     const int max_prims = n_fixed + (shadow ? 0 : cap.clip_subs);
     if (max_prims <= 0) return;
     go(k_bin, k_bin_wide, dim3(blocks_for(max_prims, 256)), 256, st,
-       k_bin_args{pbox, cnt, tiles_x, tiles_per_view, shadow, cursor, list, cap, n_fixed, n_tris, 0, 0});
+       k_bin_args{pbox, cnt, tiles_x, shift, tiles_per_view, shadow, cursor, list, cap, n_fixed, n_tris, 0, 0});
+}
+template <int MODE>
+static void launch_raster_t(hipStream_t st, int tile, int n_workgroups, const k_raster_args& a) {
+    // (in a recorded batch each tile side is a group of its own in the
+    // stage, so views with different tiles still share wide launches)
+    if (g_wide) g_wide->sub = tile == 8 ? 0 : tile == 16 ? 1 : 2;
+    if (tile == 8) go(k_raster<MODE, 8>, k_raster_wide<MODE, 8>, dim3(n_workgroups), 256, st, a);
+    else if (tile == 32) go(k_raster<MODE, 32>, k_raster_wide<MODE, 32>, dim3(n_workgroups), 256, st, a);
+    else go(k_raster<MODE, 16>, k_raster_wide<MODE, 16>, dim3(n_workgroups), 256, st, a);
 }
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
-                   Counters* cnt, const RasterTarget& rt, int32_t item_prims) {
+                   Counters* cnt, const RasterTarget& rt, int32_t item_prims, int32_t tile) {
     const k_raster_args a{prims, tile_off, chunk_off, item_tile, tile_list, cnt, rt, cap_items, item_prims, 0, 0};
-    if (mode == MODE_SHADOW) go(k_raster<MODE_SHADOW>, k_raster_wide<MODE_SHADOW>, dim3(n_workgroups), 256, st, a);
-    else if (mode == MODE_EXACT) go(k_raster<MODE_EXACT>, k_raster_wide<MODE_EXACT>, dim3(n_workgroups), 256, st, a);
-    else go(k_raster<MODE_VIS>, k_raster_wide<MODE_VIS>, dim3(n_workgroups), 256, st, a);
+    if (mode == MODE_SHADOW) launch_raster_t<MODE_SHADOW>(st, tile, n_workgroups, a);
+    else if (mode == MODE_EXACT) launch_raster_t<MODE_EXACT>(st, tile, n_workgroups, a);
+    else launch_raster_t<MODE_VIS>(st, tile, n_workgroups, a);
 }
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
                     uint8_t* ao_need, const int4* tri_rec) {

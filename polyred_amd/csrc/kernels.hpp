@@ -139,12 +139,14 @@ int scan_blocks(int32_t n);
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
                  int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
                  int32_t max_clip_subs, Counters* cnt, int32_t item_prims);
-void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t tiles_per_view,
+// shift: log2 of the raster tile side
+void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t shift, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris);
-// mode: 0 visibility, 1 visibility in exact order (NaN rule), 2 shadow maps
+// mode: 0 visibility, 1 visibility in exact order (NaN rule), 2 shadow maps;
+// tile: the raster tile side, 8, 16 or 32
 void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims, const int32_t* tile_off,
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
-                   Counters* cnt, const RasterTarget& rt, int32_t item_prims);
+                   Counters* cnt, const RasterTarget& rt, int32_t item_prims, int32_t tile);
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
                     uint8_t* ao_need, const int4* tri_rec);
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);

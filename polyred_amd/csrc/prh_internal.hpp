@@ -10,8 +10,11 @@
 
 namespace prh {
 
-constexpr int TILE = 16;                 // raster/shade tile side (pixels)
+constexpr int TILE = 16;                 // shading tile side (pixels): one 256-thread workgroup
 constexpr int TILE_PX = TILE * TILE;     // 256 pixels = 4 wave64s
+// The raster tile (binning and the binned raster) is a tuning knob of its
+// own: 8, 16 or 32 pixels a side (prh_tuning.raster_tile).
+constexpr int RTILE_MIN = 8, RTILE_MAX = 32;
 constexpr uint32_t SUB_DIRECT = 7;       // key sub-index of an unclipped triangle
 constexpr uint64_t VIS_EMPTY = ~0ull;    // no fragment
 constexpr uint32_t KEY_HI_NAN = 0u;      // NaN-depth winner (exact-order pass only)
@@ -81,7 +84,8 @@ struct DevScene {
 struct ViewConsts {
     gm::M4 vp, vpinv, pinv, vinv, s2w;
     float cam[3];
-    int32_t W, H, tiles_x, tiles_y;
+    int32_t W, H, tiles_x, tiles_y;  // shading tiles (TILE)
+    int32_t rtiles_x, rt_shift;      // raster tiles: per row, log2 of the side
     int32_t perspect;
     float vpbox_max_x, vpbox_max_y;  // viewport AABB max (MSAA*W, MSAA*H)
 };
@@ -89,8 +93,8 @@ struct ViewConsts {
 // Per-frame shadow constants (all shadow-casting lights at once).
 struct ShadowConsts {
     gm::M4 vp;                 // ViewportMatrix(SW, SH)
-    int32_t SW, SH, tiles_x, tiles_y;
-    int32_t n_slots, tiles_per_map;
+    int32_t SW, SH, tiles_x, tiles_y;  // raster tiles of one map
+    int32_t n_slots, tiles_per_map, rt_shift;
     float vpbox_max_x, vpbox_max_y;  // cullViewFrustum's box: (SW*MSAA, SH*MSAA) (render/cull.go:16-22)
     int32_t direct_max;              // triangles of at most this many texel centres are drawn by the setup
 };

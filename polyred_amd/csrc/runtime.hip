@@ -399,9 +399,10 @@ struct prh_renderer {
     prh_scene* sc;
     prh_camera cam;
     prh_options opt;
-    int W, H, tiles_x, tiles_y, n_tiles;  // the MSAA-sized framebuffer (width*MSAA x height*MSAA)
+    int W, H, tiles_x, tiles_y, n_tiles;  // the MSAA-sized framebuffer (width*MSAA x height*MSAA), shading tiles
+    int rtile, rtiles_x, rtiles_y, n_rtiles;  // its raster tiles (prh_tuning.raster_tile)
     int msaa, OW, OH;                       // MSAA factor and the output image size
-    int SW, SH, stiles_x, stiles_y, stiles_per_map, n_slots;
+    int SW, SH, stiles_x, stiles_y, stiles_per_map, n_slots;  // shadow maps and their raster tiles
     ViewConsts vc;
     ShadowConsts shc;
     ShadeConsts sh;
@@ -482,6 +483,21 @@ void free_list(std::vector<void*>& v) {
     v.clear();
 }
 
+// The raster tile grid of the main view and of the shadow maps for the
+// renderer's raster_tile knob (8, 16 or 32), into its constants.
+void raster_grid(prh_renderer* r) {
+    const int T = r->tune.raster_tile, shift = T == 8 ? 3 : T == 32 ? 5 : 4;
+    r->rtile = T;
+    r->rtiles_x = (r->W + T - 1) / T, r->rtiles_y = (r->H + T - 1) / T;
+    r->n_rtiles = r->rtiles_x * r->rtiles_y;
+    r->stiles_x = (r->SW + T - 1) / T, r->stiles_y = (r->SH + T - 1) / T;
+    r->stiles_per_map = r->stiles_x * r->stiles_y;
+    r->vc.rtiles_x = r->rtiles_x, r->vc.rt_shift = shift;
+    r->sh.v.rtiles_x = r->rtiles_x, r->sh.v.rt_shift = shift;
+    r->shc.tiles_x = r->stiles_x, r->shc.tiles_y = r->stiles_y;
+    r->shc.tiles_per_map = r->stiles_per_map, r->shc.rt_shift = shift;
+}
+
 int alloc_work(prh_renderer* r) {
     int rc;
     if ((rc = ralloc(r, &r->prims, (size_t)r->cap_main.prims))) return rc;
@@ -490,7 +506,7 @@ int alloc_work(prh_renderer* r) {
     if ((rc = ralloc(r, &r->tile_list, (size_t)r->cap_main.pairs))) return rc;
     if ((rc = ralloc(r, &r->clip_list, (size_t)r->cap_main.clip_tris))) return rc;
     // every item holds >= 1 primitive of one tile: items <= tiles + pairs / item_prims
-    r->cap_items = r->n_tiles + r->cap_main.pairs / r->tune.item_prims + 1;
+    r->cap_items = r->tile_slots + r->cap_main.pairs / r->tune.item_prims + 1;
     if ((rc = ralloc(r, &r->item_tile, (size_t)r->cap_items))) return rc;
     if (r->n_slots > 0) {
         if ((rc = ralloc(r, &r->s_prims, (size_t)r->cap_shadow.prims))) return rc;
@@ -864,7 +880,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     prh_renderer* r = new prh_renderer();
     r->ctx = ctx, r->sc = s, r->cam = *cam, r->opt = *o;
     r->tune = prh_tuning{8, ITEM_PRIMS, 64, 48, (size_t)o->width * o->height * msaa * msaa >= (1u << 19) ? 1 : 0,
-                         0.0f};
+                         TILE, 0.0f};
     // every pass renders the MSAA-sized buffer (render/raster.go:149-150)
     r->msaa = msaa, r->OW = o->width, r->OH = o->height;
     r->W = o->width * msaa, r->H = o->height * msaa;
@@ -875,9 +891,6 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     r->SH = o->shadow_size > 0 ? o->shadow_size : r->H;
     const bool shadows = o->shadow_map && s->n_slots > 0;
     r->n_slots = shadows ? s->n_slots : 0;
-    r->stiles_x = (r->SW + TILE - 1) / TILE;
-    r->stiles_y = (r->SH + TILE - 1) / TILE;
-    r->stiles_per_map = r->stiles_x * r->stiles_y;
 
     // Main camera (render/raster.go:232-246,281-295).
     M4 view = view_matrix(v3a(cam->position), v3a(cam->target), v3a(cam->up));
@@ -927,11 +940,12 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     }
     ShadowConsts& shc = r->shc;
     shc.vp = viewport_matrix((float)r->SW, (float)r->SH);
-    shc.SW = r->SW, shc.SH = r->SH, shc.tiles_x = r->stiles_x, shc.tiles_y = r->stiles_y;
-    shc.n_slots = r->n_slots, shc.tiles_per_map = r->stiles_per_map;
+    shc.SW = r->SW, shc.SH = r->SH;
+    shc.n_slots = r->n_slots;
     shc.vpbox_max_x = (float)(msaa * r->SW), shc.vpbox_max_y = (float)(msaa * r->SH);
     shc.direct_max = r->tune.shadow_direct_max;
 
+    raster_grid(r);
     ShadeConsts& sh = r->sh;
     memset(&sh, 0, sizeof sh);
     sh.v = vc;
@@ -967,11 +981,15 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     // primitive slots: one per triangle (per shadow slot), then the main
     // view's clipped fan pieces
     const int clip_tris = std::min(Tg, 1 << 16) + 16, clip0 = 7 * clip_tris;
-    r->cap_main = Capacities{Tg + clip0, clip0, 4 * Tg + 4 * r->n_tiles + 1024, clip_tris};
+    // the tile tables hold the smallest raster tile's grid, so the tile
+    // knob never reallocates them
+    const int rt_cap = ((r->W + RTILE_MIN - 1) / RTILE_MIN) * ((r->H + RTILE_MIN - 1) / RTILE_MIN);
+    const int st_cap = ((r->SW + RTILE_MIN - 1) / RTILE_MIN) * ((r->SH + RTILE_MIN - 1) / RTILE_MIN);
+    r->cap_main = Capacities{Tg + clip0, clip0, 4 * Tg + 4 * rt_cap + 1024, clip_tris};
     r->cap_shadow = Capacities{std::max(1, r->n_slots) * Tg, 0,
-                               std::max(1, r->n_slots) * (2 * Tg + 4 * r->stiles_per_map) + 1024, 0};
-    r->tile_slots = r->n_tiles + 1;
-    r->s_tile_slots = r->n_slots * r->stiles_per_map + 1;
+                               std::max(1, r->n_slots) * (2 * Tg + 4 * st_cap) + 1024, 0};
+    r->tile_slots = rt_cap + 1;
+    r->s_tile_slots = r->n_slots * st_cap + 1;
     int rc = 0;
     auto chk = [&](int v) {
         if (!rc) rc = v;
@@ -1149,7 +1167,8 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     const DevScene& d = r->sc->d;
     Counters* cm = r->cnt;
     Counters* cs = r->cnt + 1;
-    const int persistent = wide ? std::max(32, r->n_cu / 2) : r->n_cu * r->tune.raster_wg_per_cu;
+    const int persistent = wide ? std::max(32, r->n_cu * r->tune.raster_wg_per_cu / 16)
+                                : r->n_cu * r->tune.raster_wg_per_cu;
     const int ip = r->tune.item_prims;
     if (timed) (void)hipEventRecord(r->ev[0], st);
     const size_t P = (size_t)r->W * r->H;
@@ -1183,12 +1202,12 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
         launch_scan(ss, r->s_tile_count, nt, r->s_tile_off, r->s_chunk_off, r->s_tile_cursor, r->s_scan_sums,
                     r->s_item_tile, r->s_cap_items, nullptr, 0, 0, cs, ip);
         wide_stage(5);
-        launch_bin(ss, r->s_pbox, cs, r->stiles_x, r->stiles_per_map, 1, r->s_tile_cursor, r->s_tile_list,
+        launch_bin(ss, r->s_pbox, cs, r->stiles_x, r->shc.rt_shift, r->stiles_per_map, 1, r->s_tile_cursor, r->s_tile_list,
                    r->cap_shadow, r->n_slots * d.n_tris, d.n_tris);
         RasterTarget rt{r->SW, r->SH, r->stiles_x, r->stiles_per_map, nt, nullptr, nullptr, nullptr, r->smaps};
         wide_stage(6);
         launch_raster(ss, 2, persistent, r->s_prims, r->s_tile_off, r->s_chunk_off, r->s_item_tile, r->s_cap_items,
-                      r->s_tile_list, cs, rt, ip);
+                      r->s_tile_list, cs, rt, ip, r->rtile);
         if (timed) (void)hipEventRecord(r->ev_s[1], ss);
         if (split && !split_shade) (void)hipEventRecord(r->ev_join, ss);
     }
@@ -1201,21 +1220,21 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
                       r->tile_count, r->blk_prims, cm, r->cap_main, (unsigned long long*)r->vis,
                       nan_mode ? 0 : r->tune.main_direct_max, r->tri_rec);
     wide_stage(9);
-    launch_scan(st, r->tile_count, r->n_tiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
+    launch_scan(st, r->tile_count, r->n_rtiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
                 r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm, ip);
     wide_stage(10);
-    launch_bin(st, r->pbox, cm, r->tiles_x, r->n_tiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
+    launch_bin(st, r->pbox, cm, r->rtiles_x, r->vc.rt_shift, r->n_rtiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
                d.n_tris);
     if (timed) (void)hipEventRecord(r->ev[3], st);
     if (nan_mode) {
         launch_fill_u32(st, r->first_all, P, 0xffffffffu);
         launch_fill_u32(st, r->first_nan, P, 0xffffffffu);
     }
-    RasterTarget rt{r->W, r->H, r->tiles_x, r->n_tiles, r->n_tiles, (unsigned long long*)r->vis, r->first_all,
+    RasterTarget rt{r->W, r->H, r->rtiles_x, r->n_rtiles, r->n_rtiles, (unsigned long long*)r->vis, r->first_all,
                     r->first_nan, nullptr};
     wide_stage(11);
     launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
-                  r->tile_list, cm, rt, ip);
+                  r->tile_list, cm, rt, ip, r->rtile);
     wide_stage(12);
     launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr, r->tri_rec);
     if (timed) (void)hipEventRecord(r->ev[4], st);
@@ -2266,7 +2285,8 @@ int prh_set_tuning(prh_renderer* r, const prh_tuning* t) {
     const bool pow2 = t->item_prims > 0 && (t->item_prims & (t->item_prims - 1)) == 0;
     if (t->raster_wg_per_cu < 1 || t->raster_wg_per_cu > 16 || !pow2 || t->item_prims < 64 ||
         t->item_prims > 8192 || t->shadow_direct_max < 0 || t->shadow_direct_max > 256 || t->main_direct_max < 0 ||
-        t->main_direct_max > 256 || (t->overlap_shadows != 0 && t->overlap_shadows != 1))
+        t->main_direct_max > 256 || (t->overlap_shadows != 0 && t->overlap_shadows != 1) ||
+        (t->raster_tile != 8 && t->raster_tile != 16 && t->raster_tile != 32))
         return fail(PRH_E_INVALID, "prh_set_tuning: a knob out of range");
     HIP_TRY(hipSetDevice(r->ctx->device));
     HIP_TRY(hipStreamSynchronize(r->ctx->stream));
@@ -2274,7 +2294,9 @@ int prh_set_tuning(prh_renderer* r, const prh_tuning* t) {
     r->tune.raster_wg_per_cu = t->raster_wg_per_cu, r->tune.item_prims = t->item_prims;
     r->tune.shadow_direct_max = t->shadow_direct_max, r->tune.main_direct_max = t->main_direct_max;
     r->tune.overlap_shadows = t->overlap_shadows;
+    r->tune.raster_tile = t->raster_tile;
     r->shc.direct_max = t->shadow_direct_max;
+    raster_grid(r);
     if (realloc) {  // the work-item table is sized by item_prims
         free_work(r);
         int rc = alloc_work(r);
@@ -2305,21 +2327,23 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
     static const int32_t c_sd[] = {0, 16, 32, 64, 128};
     static const int32_t c_md[] = {0, 24, 48, 96};
     static const int32_t c_ov[] = {0, 1};
+    static const int32_t c_rt[] = {8, 16, 32};
     const prh_tuning start = r->tune;
     prh_tuning best = start;
     float best_ms = 0.0f;
     int rc = time(best, &best_ms);
     if (rc) return rc;
     // coordinate descent: each knob over its candidates, the others at their best
-    for (int knob = 0; knob < 5; knob++) {
-        const int32_t* c = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : knob == 3 ? c_md : c_ov;
-        const int n = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : knob == 3 ? 4 : 2;
+    for (int knob = 0; knob < 6; knob++) {
+        const int32_t* c = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : knob == 3 ? c_md : knob == 4 ? c_ov : c_rt;
+        const int n = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : knob == 3 ? 4 : knob == 4 ? 2 : 3;
         for (int k = 0; k < n; k++) {
             prh_tuning t = best;
             int32_t* f = knob == 0 ? &t.item_prims
                        : knob == 1 ? &t.raster_wg_per_cu
                        : knob == 2 ? &t.shadow_direct_max
-                       : knob == 3 ? &t.main_direct_max : &t.overlap_shadows;
+                       : knob == 3 ? &t.main_direct_max
+                       : knob == 4 ? &t.overlap_shadows : &t.raster_tile;
             if (*f == c[k]) continue;
             *f = c[k];
             float ms = 0.0f;
@@ -2341,6 +2365,92 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
     r->tune.ms = best_ms;
     if ((rc = prh_render(r))) return rc;  // leave a frame of the chosen configuration
     if (out) *out = r->tune;
+    return PRH_OK;
+}
+
+int prh_autotune_batch(prh_renderer* const* rs, int32_t n, int32_t frames, prh_tuning out[2]) {
+    if (!rs || n <= 0) return fail(PRH_E_INVALID, "prh_autotune_batch: no renderers");
+    for (int i = 0; i < n; i++)
+        if (!rs[i]) return fail(PRH_E_INVALID, "prh_autotune_batch: null renderer");
+    if (frames < 1) frames = 5;
+    std::vector<int> cls(n);
+    for (int i = 0; i < n; i++) cls[i] = (int64_t)rs[i]->W * rs[i]->H >= (1 << 19) ? 1 : 0;
+    std::vector<prh_tuning> start(n);
+    for (int i = 0; i < n; i++) start[i] = rs[i]->tune;
+    // the classes start from the knobs of their first view
+    prh_tuning cur[2];
+    for (int c = 0; c < 2; c++) {
+        cur[c] = rs[0]->tune;
+        for (int i = 0; i < n; i++)
+            if (cls[i] == c) {
+                cur[c] = rs[i]->tune;
+                break;
+            }
+    }
+    auto apply = [&](const prh_tuning* t) -> int {
+        for (int i = 0; i < n; i++) {
+            prh_tuning v = t[cls[i]];
+            v.overlap_shadows = rs[i]->tune.overlap_shadows;  // (not used by a wide batch)
+            int rc = prh_set_tuning(rs[i], &v);
+            if (rc) return rc;
+        }
+        return PRH_OK;
+    };
+    auto time = [&](const prh_tuning* t, int reps, float* ms) -> int {
+        int rc = apply(t);
+        if (rc) return rc;
+        float m = 0.0f;
+        if ((rc = prh_render_batch(rs, n, 0, &m))) return rc;  // warm-up, regrowth, tables
+        std::vector<float> v;
+        for (int k = 0; k < reps; k++) {
+            if ((rc = prh_render_batch(rs, n, 0, &m))) return rc;
+            v.push_back(m);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        return PRH_OK;
+    };
+    static const int32_t c_ip[] = {256, 512, 1024, 2048, 4096};
+    static const int32_t c_wg[] = {2, 4, 8, 16};
+    static const int32_t c_sd[] = {0, 16, 32, 64, 128};
+    static const int32_t c_md[] = {0, 24, 48, 96};
+    static const int32_t c_rt[] = {8, 16, 32};
+    const bool has[2] = {std::count(cls.begin(), cls.end(), 0) > 0, std::count(cls.begin(), cls.end(), 1) > 0};
+    const prh_tuning first[2] = {cur[0], cur[1]};
+    float best_ms = 0.0f;
+    int rc = time(cur, frames, &best_ms);
+    if (rc) return rc;
+    for (int knob = 0; knob < 5; knob++)
+        for (int c = 0; c < 2; c++) {
+            if (!has[c]) continue;
+            const int32_t* cand = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : knob == 3 ? c_md : c_rt;
+            const int nc = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : knob == 3 ? 4 : 3;
+            for (int k = 0; k < nc; k++) {
+                prh_tuning t[2] = {cur[0], cur[1]};
+                int32_t* f = knob == 0 ? &t[c].item_prims
+                           : knob == 1 ? &t[c].raster_wg_per_cu
+                           : knob == 2 ? &t[c].shadow_direct_max
+                           : knob == 3 ? &t[c].main_direct_max : &t[c].raster_tile;
+                if (*f == cand[k]) continue;
+                *f = cand[k];
+                float ms = 0.0f;
+                if ((rc = time(t, frames, &ms))) return rc;
+                if (ms < best_ms) cur[0] = t[0], cur[1] = t[1], best_ms = ms;
+            }
+        }
+    if (std::memcmp(&cur[0], &first[0], offsetof(prh_tuning, ms)) != 0 ||
+        std::memcmp(&cur[1], &first[1], offsetof(prh_tuning, ms)) != 0) {
+        float ms_first = 0.0f, ms_best = 0.0f;
+        if ((rc = time(first, 2 * frames, &ms_first)) || (rc = time(cur, 2 * frames, &ms_best))) return rc;
+        if (ms_first <= ms_best) cur[0] = first[0], cur[1] = first[1], best_ms = ms_first;
+        else best_ms = ms_best;
+    }
+    cur[0].ms = cur[1].ms = best_ms;
+    if ((rc = apply(cur))) return rc;
+    for (int i = 0; i < n; i++) rs[i]->tune.ms = best_ms;
+    float m = 0.0f;
+    if ((rc = prh_render_batch(rs, n, 0, &m))) return rc;  // leave a frame of the chosen knobs in every view
+    if (out) out[0] = cur[0], out[1] = cur[1];
     return PRH_OK;
 }
 

@@ -289,7 +289,8 @@ class Renderer:
         return t
 
     def set_tuning(self, **knobs) -> None:
-        """raster_wg_per_cu, item_prims, shadow_direct_max, main_direct_max."""
+        """raster_wg_per_cu, item_prims, shadow_direct_max, main_direct_max,
+        overlap_shadows, raster_tile."""
         t = self.tuning()
         for k, v in knobs.items():
             setattr(t, k, v)
@@ -357,6 +358,22 @@ def render_batch(renderers, graph: bool = False, streams: bool = False) -> float
         for name in ("shadow", "forward", "deferred", "gamma"):
             r.passGPU[name] = True
     return ms.value
+
+
+def autotune_batch(renderers, frames: int = 3):
+    """prh_autotune_batch: the knobs of a wide batch's views timed by
+    coordinate descent, per size class (views below / from 2^19 framebuffer
+    pixels).  Every view keeps its class's winner; returns the two classes'
+    prh_tuning (ms: the batch's median time)."""
+    rs = list(renderers)
+    lib = rs[0]._lib
+    arr = (C.c_void_p * len(rs))(*[r.handle for r in rs])
+    out = (abi.prh_tuning * 2)()
+    abi.check(lib.prh_autotune_batch(arr, len(rs), frames, out), "prh_autotune_batch")
+    for r in rs:
+        for name in ("shadow", "forward", "deferred", "gamma"):
+            r.passGPU[name] = True
+    return out[0], out[1]
 
 
 def NewRenderer(*opts) -> Renderer:  # render/raster.go:84-143
