@@ -204,12 +204,14 @@ __device__ __forceinline__ void acc_flush(const DevScene& s, const DiffBuffers& 
 
 // One 16x16 tile (`bid` of the view's grid) of the forward or the adjoint;
 // the adjoint adds its light and material gradients and its loss to `acc`.
+// Block i takes tile i: the dispatcher deals consecutive workgroups
+// round-robin to the 8 XCDs, so each XCD gets a share of every part of the
+// view (the covered pixels, which are the work, crowd where the ground is).
 template <bool BACKWARD>
 __device__ __forceinline__ void diff_body(const DevScene& s, const ViewConsts& v, const DiffBuffers& b, int bid,
                                           DiffAcc& acc) {
     const int tiles_x = v.tiles_x, n_tiles = tiles_x * v.tiles_y;
-    const int per_xcd = (n_tiles + 7) / 8;
-    const int tile = (bid % 8) * per_xcd + bid / 8;
+    const int tile = bid;
     if (tile >= n_tiles) return;  // whole workgroup
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int x = (tile % tiles_x) * TILE + (wv & 1) * 8 + (lane & 7);

@@ -1590,11 +1590,56 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 
 }  // namespace
 
+// The pixels k_ao computes, row by row: each row's AO pixels (k_resolve's
+// flags) listed left to right, xs[y * W + r] for r < cnt[y].  One workgroup
+// per row: each thread counts its run of the row, a block scan places the
+// runs.
+struct k_ao_rows_args {
+    const uint8_t* ao_need;
+    int32_t* xs;
+    int32_t* cnt_row;
+    const Counters* cnt;
+    int32_t W, H;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_ao_rows_body(const k_ao_rows_args& a, const VB vb) {
+    __shared__ int32_t start[257];
+    __shared__ int32_t wsum[4];
+    const int y = vb.bx;
+    if (y >= a.H || a.cnt->overflow) return;
+    const int W = a.W, per = (W + 255) / 256, x0 = threadIdx.x * per, x1 = min(x0 + per, W);
+    const uint8_t* need = a.ao_need + (size_t)y * W;
+    int c = 0;
+    for (int x = x0; x < x1; x++) c += need[x] != 0;
+    block_scan_256(c, start, wsum);
+    int32_t* xs = a.xs + (size_t)y * W;
+    int pos = start[threadIdx.x];
+    for (int x = x0; x < x1; x++)
+        if (need[x]) xs[pos++] = x;
+    if (threadIdx.x == 0) a.cnt_row[y] = start[256];
+}
+__global__ __launch_bounds__(256) void k_ao_rows(k_ao_rows_args a) {
+    k_ao_rows_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(256) void k_ao_rows_wide(const k_ao_rows_args* __restrict__ a,
+                                                      const int32_t* __restrict__ start, int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_ao_rows_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_ao_rows_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
+
 // Screen-space AO (material/ao.go:20-73) for every pixel whose material has
 // it, plus pixel (0,0) (the uncovered pixels are shaded from it).  One
-// workgroup = one 64-pixel row segment x the 8 march directions, four waves of
-// two directions each; lanes hold consecutive pixels of the row, so each tap
-// of every direction reads contiguous depths across the wave.
+// workgroup = 64 consecutive AO pixels of one row (k_ao_rows' list) x the 8
+// march directions, four waves of two directions each; the lanes hold the
+// row's AO pixels left to right, so each tap of every direction reads nearly
+// contiguous depths across the wave, and no lane marches for a pixel without
+// AO.  Workgroup i takes chunk i % nbx of row i / nbx, so the dispatcher
+// (workgroup i runs on XCD i % 8) deals every XCD a share of every part of
+// the frame: each XCD gets the same mix of full and partial rows, and of the
+// rows near the borders, whose marches take the per-lane path.
 // maxElevationAngle returns at the first tap outside the buffer; the tap
 // coordinates move monotonically along the ray, so a tap never re-enters once
 // out and masking every later tap gives the same result -- which turns the
@@ -1605,7 +1650,8 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 // k_shade.
 struct k_ao_args {
     ShadeConsts c;
-    const uint8_t* ao_need;
+    const int32_t* ao_xs;    // k_ao_rows' lists
+    const int32_t* ao_cnt;   // and counts
     const float* depth;
     float* ao;
     const float* ao_tab;
@@ -1615,7 +1661,8 @@ struct k_ao_args {
 };
 __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
     const ShadeConsts& c = a_.c;
-    const uint8_t* __restrict__ ao_need = a_.ao_need;
+    const int32_t* __restrict__ ao_xs = a_.ao_xs;
+    const int32_t* __restrict__ ao_cnt = a_.ao_cnt;
     const float* __restrict__ depth = a_.depth;
     float* __restrict__ ao = a_.ao;
     const float* __restrict__ ao_tab = a_.ao_tab;
@@ -1623,42 +1670,50 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
     const Counters* cnt = a_.cnt;
 
     __shared__ float s_m[8][64];
-    __shared__ float s_win[2][192];  // the row around the segment for the two x-axis marches
+    __shared__ float s_win[2][AO_WIN];  // the row around the pixels for the two x-axis marches
     __shared__ int32_t s_dbase[4][100];  // diagonal marches, tap t: byte offset of lane 0's tap
     __shared__ double s_drinv[4][100];   // and fl64(1 / distance), where every lane's tap shares them
     if (cnt->overflow) return;
     const int W = c.v.W, H = c.v.H;
     const int nbx = (W + 63) / 64, nb = nbx * H;
-    const int per_xcd = (nb + 7) / 8;
-    const int blk = (vb.bx % 8) * per_xcd + vb.bx / 8;
+    const int blk = vb.bx;
     if (blk >= nb) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x0 = (blk % nbx) * 64, y = blk / nbx;
-    const int x = x0 + lane;
-    // every wave reads the segment's need flags (k_resolve wrote them): a
-    // segment without AO pixels ends here, for all its waves alike
-    const bool need = x < W && ao_need[(size_t)y * W + x] != 0;
-    if (__ballot(need) == 0) return;
+    // chunk blk % nbx of row y: the row's AO pixels r0 .. r0+63; past the
+    // row's count the workgroup ends here, all its waves alike
+    const int y = blk / nbx, r0 = (blk % nbx) * 64;
+    const int n_row = ao_cnt[y];
+    if (r0 >= n_row) return;
+    const int32_t* xs = ao_xs + (size_t)y * W;
+    const bool need = r0 + lane < n_row;  // lanes past the row's last AO pixel compute nothing anyone reads
+    const int x = xs[min(r0 + lane, n_row - 1)];
+    const int x_first = xs[r0], x_last = xs[min(r0 + 63, n_row - 1)];  // (uniform)
+    const int li = x - x_first;  // the lane's column counted from the chunk's first pixel
+    const int span = x_last - x_first;
+    // the x marches read a window of the row staged in LDS when it spans
+    // every tap of the chunk (else they take the per-lane path)
+    const bool xwin = span + 99 <= AO_WIN;
     const float dp = need ? depth[(size_t)y * W + x] : -1.0f;
     // wave w marches direction 2w (an axis, cheap) then 2w+1 (a diagonal,
     // costlier): four waves of equal work, none idling at the barrier
     for (int half = 0; half < 2; half++) {
     const int wu = __builtin_amdgcn_readfirstlane(2 * wv + half);
     const int* axis = reinterpret_cast<const int*>(ao_tab + AO_TAB_AXIS) + (wu >> 1) * 4;
-    if (ao_fast && (wu & 3) == 0 && axis[0]) {
-        // directions 0 and 4 march along x: depths x0+1 .. x0+162 (or
-        // x0-99 .. x0+62) of this row, 0 outside it
-        const int ws = axis[1] > 0 ? x0 + 1 : x0 - 99;
-        for (int j = lane; j < 192; j += 64) {
+    if (ao_fast && (wu & 3) == 0 && axis[0] && xwin) {
+        // directions 0 and 4 march along x: depths x_first+1 .. x_last+99
+        // (or x_first-99 .. x_last-1) of this row, 0 outside it
+        const int ws = axis[1] > 0 ? x_first + 1 : x_first - 99;
+        for (int j = lane; j < span + 99; j += 64) {
             const int px = ws + j;
             s_win[wu >> 2][j] = (unsigned)px < (unsigned)W ? depth[(size_t)y * W + px] : 0.0f;
         }
     }
-    // Diagonal marches: where lanes 0 and 63 (so every lane) put tap t's x
-    // in one binade, fl(X + fl(dir.x * t)) - X is the same for the whole
-    // segment (the table's invariant), so every lane's tap is lane 0's
-    // shifted by the lane, at one distance: offset and 1/distance per t,
-    // and a mask of those t, for the march below.
+    // Diagonal marches: where the chunk's first and last pixels (so every
+    // lane) put tap t's x in one binade, fl(X + fl(dir.x * t)) - X is the
+    // same for the whole chunk (the table's invariant), so every lane's tap
+    // is the first pixel's shifted by the lane's column, at one distance:
+    // offset and 1/distance per t, and a mask of those t, for the march
+    // below.
     uint64_t uni_lo = 0, uni_hi = 0;
     constexpr int ROW_Y = 100 * 8, ROW_X = AO_EXP_N * ROW_Y, BIAS = -AO_EXP_MIN * (ROW_X + ROW_Y);
     if (ao_fast && (wu & 1)) {
@@ -1666,7 +1721,7 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
         const float* ox_t = ao_tab + AO_TAB_OFF + wu * 200;
         const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(ao_tab + AO_TAB_DIST + 2 * k * AO_DIST_PER_DIR), (short)0, AO_DIST_PER_DIR * 8, BUF_DWORD3);
-        const float X0 = (float)x0, X63 = (float)(x0 + 63), Yf = (float)y;
+        const float X0 = (float)x_first, X63 = (float)x_last, Yf = (float)y;
         bool u0 = false, u1 = false;
         for (int h = 0; h < 2; h++) {
             const int t = lane + 64 * h;
@@ -1740,12 +1795,13 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
         // in the buffer while t < tlim.  Up to the wave's smallest tlim no
         // lane needs the mask (lanes without AO compute values nobody reads):
         // a plain running max of q.
-        const bool row_fast = ao_fast && (wu & 1) == 0 && finite && (axis[0] ? y : x0) >= axis[2];
-        if (row_fast && axis[0] && (wu & 3) == 0) {  // (directions 0 and 4, whose windows were loaded)
-            // along x: tap t of lane i is window entry i - 1 + t (i + 99 - t)
+        const bool row_fast = ao_fast && (wu & 1) == 0 && finite && (axis[0] ? y : x_first) >= axis[2];
+        if (row_fast && axis[0] && (wu & 3) == 0 && xwin) {  // (directions 0 and 4, whose windows were loaded)
+            // along x: tap t of the lane at column li is window entry
+            // li - 1 + t (li + 99 - t)
             const int sgn = axis[1];
             const float* win = s_win[wu >> 2];
-            const int base = sgn > 0 ? lane - 1 : lane + 99;
+            const int base = sgn > 0 ? li - 1 : li + 99;
             const int tlim = need ? (sgn > 0 ? W - x : x + 1) : 100;
             int tl = tlim;  // the wave's smallest
             for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
@@ -1754,11 +1810,11 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
             // one loop per sign, so each tap's window entry is a constant
             // offset from the lane's address
             if (sgn > 0) {
-                const float* wl = win + lane - 1;
+                const float* wl = win + li - 1;
 #pragma unroll 4
                 for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(wl[ti] - dp) * rinv_t[ti]);
             } else {
-                const float* wl = win + lane + 99;
+                const float* wl = win + li + 99;
 #pragma unroll 4
                 for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(wl[-ti] - dp) * rinv_t[ti]);
             }
@@ -1772,9 +1828,8 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
             const int sgn = axis[1];
             const int tl = min(sgn > 0 ? H - y : y + 1, 100);
             // the row's offset is uniform (the load's scalar offset), the
-            // lane's column its vector offset: clamped into the row for the
-            // lanes past the frame's edge, whose values nobody reads
-            const int xo = min(x, W - 1) * 4;
+            // lane's column its vector offset
+            const int xo = x * 4;
 #pragma unroll 8
             for (int ti = 1; ti < tl; ti++) {
                 const int row = (y + sgn * ti) * W * 4;
@@ -1847,7 +1902,7 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
                 const int tmin = __builtin_amdgcn_readfirstlane(tl);
                 const int kd = wu >> 1;
-                const uint32_t lane4 = (uint32_t)lane * 4u;
+                const uint32_t lane4 = (uint32_t)li * 4u;
                 for (; ti + 7 < tmin; ti += 8) {
                     const uint64_t bits = ti < 64 ? (uni_lo >> ti) | (ti > 56 ? uni_hi << (64 - ti) : 0ull)
                                                   : uni_hi >> (ti - 64);
@@ -1979,25 +2034,16 @@ __global__ __launch_bounds__(64) void k_shade_bg_wide(const k_shade_bg_args* __r
 
 // Deferred pass (DrawFragments(shade) then DrawFragments(GammaCorrection),
 // render/raster.go:300-314,369-376): one thread per pixel, each wave an 8x8
-// block; workgroups are spread so that consecutive workgroups (which the
-// dispatcher deals round-robin to the 8 XCDs) land in one XCD's screen band.
+// block of a 16x16 tile.
 struct k_shade_args {
     DevScene s;
     ShadeConsts c;
     ShadeBuffers b;
     int32_t gx, gy;  // the grid of this launch
 };
-__device__ __forceinline__ void k_shade_body(const k_shade_args& a_, const VB vb) {
-    const DevScene& s = a_.s;
-    const ShadeConsts& c = a_.c;
-    const ShadeBuffers& b = a_.b;
-
-    if (b.cnt->overflow) return;
-    const int tiles_x = c.v.tiles_x, n_tiles = tiles_x * c.v.tiles_y;
-    const int per_xcd = (n_tiles + 7) / 8;
-    const int bid = vb.bx;
-    const int tile = (bid % 8) * per_xcd + bid / 8;
-    if (tile >= n_tiles) return;
+// One 16x16 tile of the shading: a pixel per thread, four 8x8 waves.
+__device__ __forceinline__ void shade_tile(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int tile) {
+    const int tiles_x = c.v.tiles_x;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int x = (tile % tiles_x) * TILE + (wv & 1) * 8 + (lane & 7);
     const int y = (tile / tiles_x) * TILE + (wv >> 1) * 8 + (lane >> 3);
@@ -2034,6 +2080,20 @@ __device__ __forceinline__ void k_shade_body(const k_shade_args& a_, const VB vb
         b.linear[3 * o + 2] = hl ? lin[2] : __builtin_nanf("");
     }
     b.image[(size_t)(c.v.H - 1 - y) * c.v.W + x] = gamma(c, col);
+}
+__device__ __forceinline__ void k_shade_body(const k_shade_args& a_, const VB vb) {
+    const DevScene& s = a_.s;
+    const ShadeConsts& c = a_.c;
+    const ShadeBuffers& b = a_.b;
+
+    if (b.cnt->overflow) return;
+    const int tiles_x = c.v.tiles_x, tiles_y = c.v.tiles_y;
+    // workgroup i shades tile i: the dispatcher deals consecutive
+    // workgroups round-robin to the 8 XCDs, so every XCD gets a share of
+    // every part of the frame (the covered pixels, whose shading is the
+    // work, crowd where the ground is)
+    const int tile = vb.bx;
+    if (tile < tiles_x * tiles_y) shade_tile(s, c, b, tile);
 }
 __global__ __launch_bounds__(256) void k_shade(k_shade_args a) {
     k_shade_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
@@ -2322,16 +2382,17 @@ void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters
 }
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const int nb = ((c.v.W + 63) / 64) * c.v.H;
+    go(k_ao_rows, k_ao_rows_wide, dim3(c.v.H), 256, st,
+       k_ao_rows_args{b.ao_need, b.ao_xs, b.ao_cnt, b.cnt, c.v.W, c.v.H, 0, 0});
     go(k_ao, k_ao_wide, dim3(((nb + 7) / 8) * 8), 256, st,
-       k_ao_args{c, b.ao_need, b.depth, b.ao, b.ao_tab, b.ao_fast, b.cnt, 0, 0});
+       k_ao_args{c, b.ao_xs, b.ao_cnt, b.depth, b.ao, b.ao_tab, b.ao_fast, b.cnt, 0, 0});
 }
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     go(k_shade_bg, k_shade_bg_wide, dim3(1), 64, st, k_shade_bg_args{s, c, b, 0, 0});
 }
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     int n_tiles = c.v.tiles_x * c.v.tiles_y;
-    int per_xcd = (n_tiles + 7) / 8;
-    go(k_shade, k_shade_wide, dim3(per_xcd * 8), 256, st, k_shade_args{s, c, b, 0, 0});
+    go(k_shade, k_shade_wide, dim3((n_tiles + 7) / 8 * 8), 256, st, k_shade_args{s, c, b, 0, 0});
 }
 
 bool launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {

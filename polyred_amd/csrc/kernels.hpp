@@ -42,6 +42,8 @@ struct ShadeBuffers {
     Counters* cnt_host;        // pinned host copy of both, written once they are final
     float* ao;                 // [H][W] AO factor of the pixels that need it
     const uint8_t* ao_need;    // [H][W] 1 where k_ao computes the factor
+    int32_t* ao_xs;            // [H][W] per row, its AO pixels' columns left to right (k_ao_rows)
+    int32_t* ao_cnt;           // [H] and their count
     const float* ao_tab;       // AO march tables (tap offsets, diagonal distances)
     int32_t ao_fast;           // the tables hold for this frame size
     uint32_t* col_pre;         // set: k_shade is the first half of a split shading (colour before AO)
@@ -112,6 +114,9 @@ constexpr int AO_TAB_DIST = AO_TAB_T64 + 200;  // double [4 diagonals][AO_EXP_N]
 // minor offset never moves the tap off its row/column (INT32_MAX: none)
 constexpr int AO_TAB_AXIS = AO_TAB_DIST + 2 * 4 * AO_DIST_PER_DIR;
 constexpr int AO_TAB_SIZE = AO_TAB_AXIS + 16;
+// k_ao's LDS window of a row for its x marches: a chunk of AO pixels
+// spanning up to AO_WIN - 99 columns
+constexpr int AO_WIN = 448;
 // Third descriptor word of a raw (unstrided, untyped) buffer on gfx9 parts.
 constexpr uint32_t BUF_DWORD3 = 0x00020000u;
 

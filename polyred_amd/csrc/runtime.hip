@@ -442,6 +442,7 @@ struct prh_renderer {
     int rz_flx, rz_fly;
     float *linear, *bg_lin, *ao;
     uint8_t* ao_need;  // scenes with AO only
+    int32_t *ao_xs, *ao_cnt;  // and the rows' lists of AO pixels
     float* ao_tab;   // AO march tables (ao_tables)
     int32_t ao_fast; // the tables hold for this frame size
     Counters* cnt;      // [2]: main, shadow
@@ -1034,6 +1035,8 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->depth, P));
     if (!rc) chk(ralloc(r, &r->ao, P));
     if (!rc && s->any_ao) chk(ralloc(r, &r->ao_need, P));
+    if (!rc && s->any_ao) chk(ralloc(r, &r->ao_xs, P));
+    if (!rc && s->any_ao) chk(ralloc(r, &r->ao_cnt, (size_t)r->H));
     if (!rc && s->any_ao && r->n_slots > 0) chk(ralloc(r, &r->col_pre, P));
     if (!rc && msaa == 1) chk(ralloc(r, &r->image, P));
     if (!rc && msaa > 1) {
@@ -1152,6 +1155,7 @@ ShadeBuffers shade_buffers(const prh_renderer* r) {
     b.cnt_host = r->d_h_cnt;
     b.ao = r->ao;
     b.ao_need = r->ao_need;
+    b.ao_xs = r->ao_xs, b.ao_cnt = r->ao_cnt;
     b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
     b.col_pre = nullptr;
     return b;
