@@ -747,11 +747,15 @@ __global__ __launch_bounds__(256) void k_setup_shadow_wide(const k_setup_shadow_
 
 // ------------------------------------------------------------ tile scan
 // Exclusive prefix sums of the per-tile primitive counts and of the per-tile
-// work items (ceil(count / ITEM_PRIMS)) in two launches over blocks of
-// SCAN_BLOCK tiles: k_scan_sums reduces each block, k_scan_apply adds the
-// sums of the blocks before it and scans its own block (four consecutive
-// tiles per thread, wave scans, then the 16 wave totals).
+// work items (ceil(count / ITEM_PRIMS)) over blocks of SCAN_BLOCK tiles (four
+// consecutive tiles per thread, wave scans, then the 16 wave totals).  Up to
+// SCAN_WHOLE_MAX tiles one workgroup of k_scan_apply scans the blocks in
+// turn, carrying the sums (one launch: a 1080p view's 8160 tiles); beyond,
+// k_scan_sums reduces each block and every workgroup of k_scan_apply adds
+// the sums of the blocks before its own (a workgroup walking 20 blocks in
+// turn, as five 2048^2 maps would, takes far longer than the two launches).
 constexpr int SCAN_BLOCK = 4096;
+constexpr int SCAN_WHOLE_MAX = 2 * SCAN_BLOCK;
 
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     for (int off = 1; off < 64; off <<= 1) {
@@ -846,43 +850,54 @@ __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, c
     __shared__ int32_t ws[16], wc[16];
     __shared__ int32_t carry_s, carry_c;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int b = vb.bx;
-    // sums of the blocks before this one
-    int ps = 0, pc = 0;
-    for (int k = tid; k < b; k += 1024) ps += bsum[2 * k], pc += bsum[2 * k + 1];
-    ps = wave_sum(ps), pc = wave_sum(pc);
-    if (lane == 0) ws[wv] = ps, wc[wv] = pc;
-    __syncthreads();
-    if (tid == 0) {
-        int a = 0, c = 0;
-        for (int k = 0; k < 16; k++) a += ws[k], c += wc[k];
-        carry_s = a, carry_c = c;
-    }
-    __syncthreads();
-    const int i0 = b * SCAN_BLOCK + 4 * tid;
-    int v[4], ch[4];
-    for (int k = 0; k < 4; k++) {
-        v[k] = i0 + k < n ? counts[i0 + k] : 0;
-        ch[k] = item_count(v[k], item_prims);
-    }
-    const int local = v[0] + v[1] + v[2] + v[3], localc = ch[0] + ch[1] + ch[2] + ch[3];
-    const int incl = wave_incl_scan(local, lane), inclc = wave_incl_scan(localc, lane);
-    if (lane == 63) ws[wv] = incl, wc[wv] = inclc;
-    __syncthreads();
-    int before = carry_s, beforec = carry_c;
-    for (int k = 0; k < wv; k++) before += ws[k], beforec += wc[k];
-    int run = before + incl - local, runc = beforec + inclc - localc;
-    for (int k = 0; k < 4; k++)
-        if (i0 + k < n) {
-            offsets[i0 + k] = run;
-            cursor[i0 + k] = run;
-            chunk_off[i0 + k] = runc;
-            for (int c = runc; c < runc + ch[k] && c < cap_items; c++) item_tile[c] = i0 + k;
-            counts[i0 + k] = 0;
-            run += v[k];
-            runc += ch[k];
+    // one workgroup for every block (the whole scan), or block vb.bx
+    const bool whole = vb.gx == 1;
+    const int b_first = whole ? 0 : vb.bx, b_end = whole ? (n + SCAN_BLOCK - 1) / SCAN_BLOCK : vb.bx + 1;
+    if (tid == 0) carry_s = 0, carry_c = 0;
+    if (!whole) {
+        // sums of the blocks before this one
+        int ps = 0, pc = 0;
+        for (int k = tid; k < vb.bx; k += 1024) ps += bsum[2 * k], pc += bsum[2 * k + 1];
+        ps = wave_sum(ps), pc = wave_sum(pc);
+        if (lane == 0) ws[wv] = ps, wc[wv] = pc;
+        __syncthreads();
+        if (tid == 0) {
+            int a = 0, c = 0;
+            for (int k = 0; k < 16; k++) a += ws[k], c += wc[k];
+            carry_s = a, carry_c = c;
         }
-    if (b != vb.gx - 1) return;
+    }
+    __syncthreads();
+    int run = 0, runc = 0;
+    for (int b = b_first; b < max(b_end, b_first + 1); b++) {
+        const int i0 = b * SCAN_BLOCK + 4 * tid;
+        int v[4], ch[4];
+        for (int k = 0; k < 4; k++) {
+            v[k] = i0 + k < n ? counts[i0 + k] : 0;
+            ch[k] = item_count(v[k], item_prims);
+        }
+        const int local = v[0] + v[1] + v[2] + v[3], localc = ch[0] + ch[1] + ch[2] + ch[3];
+        const int incl = wave_incl_scan(local, lane), inclc = wave_incl_scan(localc, lane);
+        if (lane == 63) ws[wv] = incl, wc[wv] = inclc;
+        __syncthreads();
+        int before = carry_s, beforec = carry_c;
+        for (int k = 0; k < wv; k++) before += ws[k], beforec += wc[k];
+        run = before + incl - local, runc = beforec + inclc - localc;
+        for (int k = 0; k < 4; k++)
+            if (i0 + k < n) {
+                offsets[i0 + k] = run;
+                cursor[i0 + k] = run;
+                chunk_off[i0 + k] = runc;
+                for (int c = runc; c < runc + ch[k] && c < cap_items; c++) item_tile[c] = i0 + k;
+                counts[i0 + k] = 0;
+                run += v[k];
+                runc += ch[k];
+            }
+        __syncthreads();  // every wave has read ws[] and the carry
+        if (tid == 1023) carry_s = run, carry_c = runc;  // (the last thread ends the block)
+        __syncthreads();
+    }
+    if (!whole && vb.bx != vb.gx - 1) return;
     // the last block writes the totals and the frame statistics: unclipped
     // primitives (per-workgroup counts of the setup) plus the clipper's pieces
     if (tid == 1023) {
@@ -1670,7 +1685,7 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
     const Counters* cnt = a_.cnt;
 
     __shared__ float s_m[8][64];
-    __shared__ float s_win[2][AO_WIN];  // the row around the pixels for the two x-axis marches
+    __shared__ float s_win[2][2][AO_WIN];  // the two x-axis marches: each its two rows around the pixels
     __shared__ int32_t s_dbase[4][100];  // diagonal marches, tap t: byte offset of lane 0's tap
     __shared__ double s_drinv[4][100];   // and fl64(1 / distance), where every lane's tap shares them
     if (cnt->overflow) return;
@@ -1699,13 +1714,37 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
     for (int half = 0; half < 2; half++) {
     const int wu = __builtin_amdgcn_readfirstlane(2 * wv + half);
     const int* axis = reinterpret_cast<const int*>(ao_tab + AO_TAB_AXIS) + (wu >> 1) * 4;
-    if (ao_fast && (wu & 3) == 0 && axis[0] && xwin) {
-        // directions 0 and 4 march along x: depths x_first+1 .. x_last+99
-        // (or x_first-99 .. x_last-1) of this row, 0 outside it
-        const int ws = axis[1] > 0 ? x_first + 1 : x_first - 99;
-        for (int j = lane; j < span + 99; j += 64) {
-            const int px = ws + j;
-            s_win[wu >> 2][j] = (unsigned)px < (unsigned)W ? depth[(size_t)y * W + px] : 0.0f;
+    // Directions 0 and 4 march along x, the major offset exactly +-t.  The
+    // minor one, the float32 cos/sin residue rounded onto the row
+    // coordinate, moves the tap to another row from some t on only near the
+    // frame's bottom rows (it is below half an ulp of Y from Y = 257 up):
+    // tap t is on row y for t < x_t1 and on row x_y2 from there, checked
+    // here for this row (x_rows: every tap on one of the two, both in the
+    // frame).  Where the chunk's taps fit it (xwin), both rows' depths
+    // x_first+1 .. x_last+99 (or x_first-99 .. x_last-1) are staged, 0
+    // outside the row; a wider chunk takes the per-lane path.
+    int x_t1 = 100, x_y2 = y;
+    bool x_rows = false;
+    if (ao_fast && (wu & 3) == 0 && axis[0] && axis[2] != 0x7fffffff) {
+        const float* oy_t = ao_tab + AO_TAB_OFF + wu * 200 + 100;  // fl(dir.y * t)
+        const float Yf = (float)y;
+        const int t_a = lane, t_b = lane + 64;
+        const int iy_a = (int)(Yf + oy_t[t_a]), iy_b = t_b < 100 ? (int)(Yf + oy_t[t_b]) : y;
+        const uint64_t off_a = __ballot(t_a >= 1 && iy_a != y), off_b = __ballot(t_b < 100 && iy_b != y);
+        x_t1 = off_a ? __ffsll((unsigned long long)off_a) - 1 : (off_b ? 64 + __ffsll((unsigned long long)off_b) - 1 : 100);
+        x_y2 = x_t1 < 64 ? __shfl(iy_a, x_t1) : (x_t1 < 100 ? __shfl(iy_b, x_t1 - 64) : y);
+        x_t1 = __builtin_amdgcn_readfirstlane(x_t1), x_y2 = __builtin_amdgcn_readfirstlane(x_y2);
+        const bool bad = (t_a >= x_t1 && iy_a != x_y2) | (t_b >= x_t1 && t_b < 100 && iy_b != x_y2);
+        x_rows = __ballot(bad) == 0 && (unsigned)x_y2 < (unsigned)H;
+        if (x_rows && xwin) {
+            const int ws = axis[1] > 0 ? x_first + 1 : x_first - 99;
+            for (int r = 0; r < (x_t1 < 100 ? 2 : 1); r++) {
+                const size_t row = (size_t)(r ? x_y2 : y) * W;
+                for (int j = lane; j < span + 99; j += 64) {
+                    const int px = ws + j;
+                    s_win[wu >> 2][r][j] = (unsigned)px < (unsigned)W ? depth[row + px] : 0.0f;
+                }
+            }
         }
     }
     // Diagonal marches: where the chunk's first and last pixels (so every
@@ -1796,31 +1835,40 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
         // lane needs the mask (lanes without AO compute values nobody reads):
         // a plain running max of q.
         const bool row_fast = ao_fast && (wu & 1) == 0 && finite && (axis[0] ? y : x_first) >= axis[2];
-        if (row_fast && axis[0] && (wu & 3) == 0 && xwin) {  // (directions 0 and 4, whose windows were loaded)
-            // along x: tap t of the lane at column li is window entry
-            // li - 1 + t (li + 99 - t)
+        if (ao_fast && finite && x_rows && xwin) {  // (directions 0 and 4, whose windows were loaded)
+            // along x: the lane at column li reads tap t at window entry
+            // li - 1 + t (li + 99 - t) of its row's window: taps [1, x_t1)
+            // row y's, the rest row x_y2's.  Up to the wave's smallest room
+            // to the border (tmin) no lane needs the mask.
             const int sgn = axis[1];
-            const float* win = s_win[wu >> 2];
             const int base = sgn > 0 ? li - 1 : li + 99;
             const int tlim = need ? (sgn > 0 ? W - x : x + 1) : 100;
             int tl = tlim;  // the wave's smallest
             for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
             const int tmin = min(__builtin_amdgcn_readfirstlane(tl), 100);
-            int ti = 1;
-            // one loop per sign, so each tap's window entry is a constant
-            // offset from the lane's address
-            if (sgn > 0) {
-                const float* wl = win + li - 1;
+            const int n_rows = x_t1 < 100 ? 2 : 1;
+#pragma nounroll
+            for (int r = 0; r < n_rows; r++) {
+                const int t_lo = r ? x_t1 : 1, t_hi = r ? 100 : x_t1;
+                const int t_mid = min(max(tmin, t_lo), t_hi);
+                const float* win = s_win[wu >> 2][r];
+                // one loop per sign, so each tap's window entry is a constant
+                // offset from the lane's address
+                if (sgn > 0) {
+                    const float* wl = win + li - 1;
 #pragma unroll 4
-                for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(wl[ti] - dp) * rinv_t[ti]);
-            } else {
-                const float* wl = win + li + 99;
+                    for (int ti = t_lo; ti < t_mid; ti++)
+                        q_best = __builtin_fmax(q_best, (double)(wl[ti] - dp) * rinv_t[ti]);
+                } else {
+                    const float* wl = win + li + 99;
 #pragma unroll 4
-                for (; ti < tmin; ti++) q_best = __builtin_fmax(q_best, (double)(wl[-ti] - dp) * rinv_t[ti]);
-            }
-            for (; ti < 100; ti++) {
-                const double q = (double)(win[base + sgn * ti] - dp) * rinv_t[ti];
-                q_best = (need & (ti < tlim) & (q > q_best)) ? q : q_best;
+                    for (int ti = t_lo; ti < t_mid; ti++)
+                        q_best = __builtin_fmax(q_best, (double)(wl[-ti] - dp) * rinv_t[ti]);
+                }
+                for (int ti = t_mid; ti < t_hi; ti++) {
+                    const double q = (double)(win[base + sgn * ti] - dp) * rinv_t[ti];
+                    q_best = (need & (ti < tlim) & (q > q_best)) ? q : q_best;
+                }
             }
         } else if (row_fast && !axis[0]) {
             // along y: tap t is row y +- t, the same for every lane, in the
@@ -2339,7 +2387,7 @@ int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOC
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
                  int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
                  int32_t max_clip_subs, Counters* cnt, int32_t item_prims) {
-    const int nb = scan_blocks(n);
+    const int nb = n <= SCAN_WHOLE_MAX ? 1 : scan_blocks(n);
     // (the sums stage always records, so the apply stage keeps its place)
     if (g_wide) g_wide->sub = 0;
     if (nb > 1)
