@@ -439,7 +439,7 @@ struct k_setup_main_args {
     const float4* tv_scr;
     Prim* prims;
     uint2* pbox;
-    int32_t* clip_list;
+    ClipJob* clip_list;
     int32_t* tile_count;
     int32_t* blk_prims;
     Counters* cnt;
@@ -455,7 +455,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
     const float4* __restrict__ tv_scr = a_.tv_scr;
     Prim* __restrict__ prims = a_.prims;
     uint2* __restrict__ pbox = a_.pbox;
-    int32_t* __restrict__ clip_list = a_.clip_list;
+    ClipJob* __restrict__ clip_list = a_.clip_list;
     int32_t* __restrict__ tile_count = a_.tile_count;
     int32_t* __restrict__ blk_prims = a_.blk_prims;
     Counters* cnt = a_.cnt;
@@ -466,6 +466,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
 
     const int g = vb.bx * blockDim.x + threadIdx.x;
     bool direct = false, doclip = false, drawn = false;
+    ClipJob job{g, 0, 0, 0, 0, {0, 0, 0}};
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
@@ -490,7 +491,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
                         direct = pixel_box(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, vc.W, vc.H, x0, y0,
                                            x1, y1);
                     else
-                        doclip = true;
+                        doclip = true, job.i0 = i0, job.i1 = i1, job.i2 = i2, job.dv = s.mesh_vtx_off[mesh] - vb;
                 }
             }
         }
@@ -515,7 +516,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
     count_tiles(tile_count, 0, vc.rtiles_x, vc.rt_shift, direct && !drawn, x0, y0, x1, y1);
     const int pos = wave_alloc(&cnt->n_clipped, 0, doclip);
     if (doclip) {
-        if (pos < cap.clip_tris) clip_list[pos] = g;
+        if (pos < cap.clip_tris) clip_list[pos] = job;
         else atomicOr(&cnt->overflow, 1);
     }
     // the primitive count of the frame statistics, summed by k_scan (one
@@ -533,131 +534,6 @@ __global__ __launch_bounds__(256) void k_setup_main_wide(const k_setup_main_args
     const int local = (int)blockIdx.x - start[v];
     if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
     k_setup_main_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
-}
-
-// The clipper (clipTriangle, clipping.go:67-157) for the queued triangles:
-// Sutherland-Hodgman against the six screen-space planes, a fan of pieces
-// with screen-barycentric attributes that keep the parent's recipw
-// (raster.go:440-443), appended after the per-triangle primitive slots.
-__device__ __forceinline__ void clip_one(int qi, const DevScene& s, const ViewConsts& vc,
-                                         const float4* __restrict__ tv_scr, const float4* __restrict__ tv_nor,
-                                         const int32_t* __restrict__ clip_list, Prim* __restrict__ prims,
-                                         uint2* __restrict__ pbox, ClipRec* __restrict__ clips,
-                                         int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
-                                         Counters* cnt, const Capacities& cap) {
-    const int g = clip_list[qi];
-    const int o = s.tri_obj[g];
-    const int mesh = s.obj_mesh[o];
-    const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
-    const int vb = s.obj_vtx_start[o];
-    const int i0 = vb + s.tri_v[3 * t], i1 = vb + s.tri_v[3 * t + 1], i2 = vb + s.tri_v[3 * t + 2];
-    const float4 A = tv_scr[i0], B = tv_scr[i1], Cc = tv_scr[i2];
-    const V4 p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(Cc.x, Cc.y, Cc.z, 1);
-    const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
-    V4 cl[12];
-    cl[0] = p1, cl[1] = p2, cl[2] = p3;
-    int nc = sutherland_hodgman(cl, w, h);
-    int nsub = nc - 2;
-    if (nsub <= 0) return;
-    int base = atomicAdd(&cnt->n_clip_subs, nsub);
-    if (base + nsub > cap.clip_subs) {
-        atomicOr(&cnt->overflow, 1);
-        return;
-    }
-    tri_clip_slot[g] = base;
-    int mvb = s.mesh_vtx_off[mesh];
-    uint32_t c0 = s.vcol[mvb + s.tri_v[3 * t]], c1 = s.vcol[mvb + s.tri_v[3 * t + 1]],
-             c2 = s.vcol[mvb + s.tri_v[3 * t + 2]];
-    V4 n0 = ld(tv_nor[i0]), n1 = ld(tv_nor[i1]), n2 = ld(tv_nor[i2]);
-    float2 u0 = make_float2(0.0f, 0.0f), u1 = u0, u2 = u0;  // UVs exist in scenes with textures only
-    if (s.n_textures > 0)
-        u0 = s.vuv[mvb + s.tri_v[3 * t]], u1 = s.vuv[mvb + s.tri_v[3 * t + 1]], u2 = s.vuv[mvb + s.tri_v[3 * t + 2]];
-    float rw0 = A.w, rw1 = B.w, rw2 = Cc.w;
-    TriSetup par = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
-    for (int i = 2; i < nc; i++) {
-        ClipRec rec;
-        V4 sv[3];
-        int idx[3] = {0, i - 1, i};
-        for (int k = 0; k < 3; k++) {
-            float b[3];
-            barycoord(par, cl[idx[k]].x, cl[idx[k]].y, b);
-            V4 q = v4(b[0] * p1.x + b[1] * p2.x + b[2] * p3.x, b[0] * p1.y + b[1] * p2.y + b[2] * p3.y,
-                      b[0] * p1.z + b[1] * p2.z + b[2] * p3.z, 1);
-            V4 nq = v4(b[0] * n0.x + b[1] * n1.x + b[2] * n2.x, b[0] * n0.y + b[1] * n1.y + b[2] * n2.y,
-                       b[0] * n0.z + b[1] * n1.z + b[2] * n2.z, 0);
-            uint8_t cc[4];
-            for (int q4 = 0; q4 < 4; q4++)
-                cc[q4] = f2u8(clamp(b[0] * chan(c0, q4) + b[1] * chan(c1, q4) + b[2] * chan(c2, q4), 0, 0xff));
-            V4 m = apply(apply(apply(q, vc.vpinv), vc.pinv), vc.vinv);
-            float rws[3] = {rw0, rw1, rw2};
-            rec.pos[k] = make_float4(q.x, q.y, q.z, rws[k]);
-            rec.nor[k] = make_float4(nq.x, nq.y, nq.z, bitsf(u8x4(cc[0], cc[1], cc[2], cc[3])));
-            rec.wp[k] = make_float4(m.x, m.y, m.z, 0.0f);
-            rec.uv[k] = make_float2(b[0] * u0.x + b[1] * u1.x + b[2] * u2.x, b[0] * u0.y + b[1] * u1.y + b[2] * u2.y);
-            sv[k] = q;
-        }
-        clips[base + i - 2] = rec;
-        int bx0, by0, bx1, by1;
-        if (!pixel_box(sv[0].x, sv[0].y, sv[0].z, sv[1].x, sv[1].y, sv[1].z, sv[2].x, sv[2].y, sv[2].z, vc.W, vc.H,
-                       bx0, by0, bx1, by1)) {
-            store_empty(pbox, (size_t)s.n_tris + base + i - 2);
-            continue;
-        }
-        const size_t ps = (size_t)s.n_tris + base + i - 2;
-        store_prim(prims, pbox, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)(i - 2), bx0, by0, bx1, by1);
-        for (int ty = by0 >> vc.rt_shift; ty <= by1 >> vc.rt_shift; ty++)
-            for (int tx = bx0 >> vc.rt_shift; tx <= bx1 >> vc.rt_shift; tx++)
-                atomicAdd(&tile_count[ty * vc.rtiles_x + tx], 1);
-    }
-}
-
-// One queued triangle per thread, the grid striding over the queue (its
-// length is known on the device only; the grid stays small).
-constexpr int CLIP_GRID = 128;
-struct k_clip_args {
-    DevScene s;
-    ViewConsts vc;
-    const float4* tv_scr;
-    const float4* tv_nor;
-    const int32_t* clip_list;
-    Prim* prims;
-    uint2* pbox;
-    ClipRec* clips;
-    int32_t* tri_clip_slot;
-    int32_t* tile_count;
-    Counters* cnt;
-    Capacities cap;
-    int32_t gx, gy;  // the grid of this launch
-};
-__device__ __forceinline__ void k_clip_body(const k_clip_args& a_, const VB vb) {
-    const DevScene& s = a_.s;
-    const ViewConsts& vc = a_.vc;
-    const float4* __restrict__ tv_scr = a_.tv_scr;
-    const float4* __restrict__ tv_nor = a_.tv_nor;
-    const int32_t* __restrict__ clip_list = a_.clip_list;
-    Prim* __restrict__ prims = a_.prims;
-    uint2* __restrict__ pbox = a_.pbox;
-    ClipRec* __restrict__ clips = a_.clips;
-    int32_t* __restrict__ tri_clip_slot = a_.tri_clip_slot;
-    int32_t* __restrict__ tile_count = a_.tile_count;
-    Counters* cnt = a_.cnt;
-    const Capacities& cap = a_.cap;
-
-    if (cnt->overflow) return;
-    const int n = min(cnt->n_clipped, cap.clip_tris);
-    for (int q = vb.bx * blockDim.x + threadIdx.x; q < n; q += vb.gx * blockDim.x)
-        clip_one(q, s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap);
-}
-__global__ __launch_bounds__(64) void k_clip(k_clip_args a) {
-    k_clip_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
-}
-__global__ __launch_bounds__(64) void k_clip_wide(const k_clip_args* __restrict__ a, const int32_t* __restrict__ start,
-        int32_t n) {
-    const int v = wide_view(start, n, (int)blockIdx.x);
-    const k_clip_args& x = a[v];
-    const int local = (int)blockIdx.x - start[v];
-    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
-    k_clip_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // render/shadow.go:152-176 (drawDepth up to the pixel loop) for every
@@ -747,15 +623,14 @@ __global__ __launch_bounds__(256) void k_setup_shadow_wide(const k_setup_shadow_
 
 // ------------------------------------------------------------ tile scan
 // Exclusive prefix sums of the per-tile primitive counts and of the per-tile
-// work items (ceil(count / ITEM_PRIMS)) over blocks of SCAN_BLOCK tiles (four
-// consecutive tiles per thread, wave scans, then the 16 wave totals).  Up to
-// SCAN_WHOLE_MAX tiles one workgroup of k_scan_apply scans the blocks in
-// turn, carrying the sums (one launch: a 1080p view's 8160 tiles); beyond,
-// k_scan_sums reduces each block and every workgroup of k_scan_apply adds
-// the sums of the blocks before its own (a workgroup walking 20 blocks in
-// turn, as five 2048^2 maps would, takes far longer than the two launches).
+// work items (ceil(count / ITEM_PRIMS)): per-thread runs of consecutive
+// tiles, wave scans, then the 16 wave totals.  Up to SCAN_WHOLE_MAX tiles
+// (a 1080p view's 8160) one workgroup scans them all, SCAN_WHOLE_MAX / 1024
+// a thread, in one launch; beyond, k_scan_sums reduces each block of SCAN_BLOCK tiles
+// and every workgroup of k_scan_apply adds the sums of the blocks before its
+// own and scans its block, 4 tiles a thread.
 constexpr int SCAN_BLOCK = 4096;
-constexpr int SCAN_WHOLE_MAX = 2 * SCAN_BLOCK;
+constexpr int SCAN_WHOLE_MAX = 8192;
 
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     for (int off = 1; off < 64; off <<= 1) {
@@ -827,12 +702,12 @@ struct k_scan_apply_args {
     int32_t cap_items;
     const int32_t* blk_prims;
     int32_t n_blk;
-    int32_t max_clip_subs;
     Counters* cnt;
     int32_t item_prims;
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, const VB vb) {
+    constexpr int NT = 1024, NW = NT / 64, PER = SCAN_WHOLE_MAX / NT;
     const int32_t item_prims = a_.item_prims;
     int32_t* __restrict__ counts = a_.counts;
     const int32_t n = a_.n;
@@ -844,33 +719,91 @@ __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, c
     const int32_t cap_items = a_.cap_items;
     const int32_t* __restrict__ blk_prims = a_.blk_prims;
     const int32_t n_blk = a_.n_blk;
-    const int32_t max_clip_subs = a_.max_clip_subs;
     Counters* cnt = a_.cnt;
 
-    __shared__ int32_t ws[16], wc[16];
+    __shared__ int32_t ws[NW], wc[NW];
     __shared__ int32_t carry_s, carry_c;
+    __shared__ int32_t s_t[SCAN_WHOLE_MAX + SCAN_WHOLE_MAX / 32];  // whole scan: tile i at i + i / 32
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // one workgroup for every block (the whole scan), or block vb.bx
     const bool whole = vb.gx == 1;
-    const int b_first = whole ? 0 : vb.bx, b_end = whole ? (n + SCAN_BLOCK - 1) / SCAN_BLOCK : vb.bx + 1;
-    if (tid == 0) carry_s = 0, carry_c = 0;
-    if (!whole) {
-        // sums of the blocks before this one
+    int run = 0, runc = 0;
+    if (whole) {
+        // One workgroup for every tile: thread t takes tiles PER*t ..
+        // PER*t + PER-1, read and written through LDS so that the global
+        // accesses stay coalesced (the padded slots keep both LDS patterns
+        // free of bank conflicts).
+        {
+            int c[PER];  // (every load issued before the first is waited for)
+#pragma unroll
+            for (int k = 0; k < PER; k++) c[k] = k * NT + tid < n ? counts[k * NT + tid] : 0;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const int i = k * NT + tid;
+                if (i < n) s_t[i + (i >> 5)] = c[k], counts[i] = 0;
+            }
+        }
+        __syncthreads();
+        int v[PER], ch[PER];
+        int local = 0, localc = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = PER * tid + k;
+            v[k] = i < n ? s_t[i + (i >> 5)] : 0;
+            ch[k] = item_count(v[k], item_prims);
+            local += v[k], localc += ch[k];
+        }
+        const int incl = wave_incl_scan(local, lane), inclc = wave_incl_scan(localc, lane);
+        if (lane == 63) ws[wv] = incl, wc[wv] = inclc;
+        __syncthreads();  // (and every thread has read its tiles)
+        run = incl - local, runc = inclc - localc;
+        for (int k = 0; k < wv; k++) run += ws[k], runc += wc[k];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = PER * tid + k;
+            if (i < n) s_t[i + (i >> 5)] = run;
+            run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = k * NT + tid;
+            if (i < n) {
+                const int o = s_t[i + (i >> 5)];
+                offsets[i] = o;
+                cursor[i] = o;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = PER * tid + k;
+            if (i < n) {
+                s_t[i + (i >> 5)] = runc;
+                for (int c = runc; c < runc + ch[k] && c < cap_items; c++) item_tile[c] = i;
+            }
+            runc += ch[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = k * NT + tid;
+            if (i < n) chunk_off[i] = s_t[i + (i >> 5)];
+        }
+    } else {
+        // block vb.bx of SCAN_BLOCK tiles (4 consecutive tiles a thread)
+        // after the sums of the blocks before it
+        const int i0 = vb.bx * SCAN_BLOCK + 4 * tid;
         int ps = 0, pc = 0;
-        for (int k = tid; k < vb.bx; k += 1024) ps += bsum[2 * k], pc += bsum[2 * k + 1];
+        for (int k = tid; k < vb.bx; k += NT) ps += bsum[2 * k], pc += bsum[2 * k + 1];
         ps = wave_sum(ps), pc = wave_sum(pc);
         if (lane == 0) ws[wv] = ps, wc[wv] = pc;
         __syncthreads();
         if (tid == 0) {
             int a = 0, c = 0;
-            for (int k = 0; k < 16; k++) a += ws[k], c += wc[k];
+            for (int k = 0; k < NW; k++) a += ws[k], c += wc[k];
             carry_s = a, carry_c = c;
         }
-    }
-    __syncthreads();
-    int run = 0, runc = 0;
-    for (int b = b_first; b < max(b_end, b_first + 1); b++) {
-        const int i0 = b * SCAN_BLOCK + 4 * tid;
+        __syncthreads();
         int v[4], ch[4];
         for (int k = 0; k < 4; k++) {
             v[k] = i0 + k < n ? counts[i0 + k] : 0;
@@ -893,29 +826,26 @@ __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, c
                 run += v[k];
                 runc += ch[k];
             }
-        __syncthreads();  // every wave has read ws[] and the carry
-        if (tid == 1023) carry_s = run, carry_c = runc;  // (the last thread ends the block)
-        __syncthreads();
     }
     if (!whole && vb.bx != vb.gx - 1) return;
     // the last block writes the totals and the frame statistics: unclipped
     // primitives (per-workgroup counts of the setup) plus the clipper's pieces
-    if (tid == 1023) {
+    if (tid == NT - 1) {
         offsets[n] = run;
         chunk_off[n] = runc;
         cnt->n_pairs = run;
         cnt->n_chunks = runc;
     }
     int np = 0;
-    for (int k = tid; k < n_blk; k += 1024) np += blk_prims[k];
+    for (int k = tid; k < n_blk; k += NT) np += blk_prims[k];
     np = wave_sum(np);
     __syncthreads();  // every wave has read ws[] above
     if (lane == 0) ws[wv] = np;
     __syncthreads();
     if (tid == 0) {
         int a = 0;
-        for (int k = 0; k < 16; k++) a += ws[k];
-        cnt->n_prims = a + min(cnt->n_clip_subs, max_clip_subs);
+        for (int k = 0; k < NW; k++) a += ws[k];
+        cnt->n_prims = a;
     }
 }
 __global__ __launch_bounds__(1024) void k_scan_apply(k_scan_apply_args a) {
@@ -966,7 +896,8 @@ __device__ __forceinline__ void k_bin_body(const k_bin_args& a_, const VB vb) {
         return;
     }
     if (cnt->overflow) return;
-    const int np = n_fixed + (shadow ? 0 : min(cnt->n_clip_subs, cap.clip_subs));
+    // (the clipper's pieces: CLIP_PIECES slots a queued triangle)
+    const int np = n_fixed + (shadow ? 0 : min(CLIP_PIECES * min(cnt->n_clipped, cap.clip_tris), cap.clip_subs));
     bool active = false;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0, base = 0;
     if (i < np) {
@@ -996,6 +927,214 @@ __global__ __launch_bounds__(256) void k_bin_wide(const k_bin_args* __restrict__
     const int local = (int)blockIdx.x - start[v];
     if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
     k_bin_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
+}
+
+// ------------------------------------------------------------ clipper
+// The clipper (clipTriangle, clipping.go:67-157) for the queued triangles:
+// Sutherland-Hodgman against the six screen-space planes, a fan of pieces
+// with screen-barycentric attributes that keep the parent's recipw
+// (raster.go:440-443), appended after the per-triangle primitive slots.
+//
+// One wave per queued triangle (a frame queues a few thousand at most, so
+// the pass is one triangle's latency, not throughput).  Each plane's pass
+// runs over the polygon's edges in parallel: lane i takes edge
+// (vertex i-1, vertex i) and emits what the serial loop emits for it (the
+// intersection and/or vertex i); an exclusive scan of the emit counts
+// places them in the serial order, in the wave's other LDS polygon.  Three
+// lanes then build each fan piece, and the wave spreads each piece's tile
+// counts over its lanes.
+constexpr int CLIP_WAVES = 4;   // waves per workgroup (a lane's fan piece needs up to 256 VGPRs)
+constexpr int CLIP_VERTS = 12;  // polygon capacity (a triangle clipped by six planes keeps at most 9)
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int clip_wave(int qi, const ClipJob& jb, const DevScene& s, const ViewConsts& vc,
+                                          const float4* __restrict__ tv_scr, const float4* __restrict__ tv_nor,
+                                          Prim* __restrict__ prims,
+                                          uint2* __restrict__ pbox, ClipRec* __restrict__ clips,
+                                          int32_t* __restrict__ tri_clip_slot, int32_t* __restrict__ tile_count,
+                                          Counters* cnt, const Capacities& cap, V4 (*poly)[CLIP_VERTS]) {
+    const int lane = threadIdx.x & 63;
+    const int g = jb.g, i0 = jb.i0, i1 = jb.i1, i2 = jb.i2;
+    // Every load first and every store last: a single wave's pass is a chain
+    // of memory latencies, and on this part a wait for a load also waits for
+    // the stores issued before it.
+    const float4 A = tv_scr[i0], B = tv_scr[i1], Cc = tv_scr[i2];
+    // the mesh vertices of the transformed vertices i0, i1, i2
+    const uint32_t c0 = s.vcol[i0 + jb.dv], c1 = s.vcol[i1 + jb.dv], c2 = s.vcol[i2 + jb.dv];
+    const V4 n0 = ld(tv_nor[i0]), n1 = ld(tv_nor[i1]), n2 = ld(tv_nor[i2]);
+    float2 u0 = make_float2(0.0f, 0.0f), u1 = u0, u2 = u0;  // UVs exist in scenes with textures only
+    if (s.n_textures > 0) u0 = s.vuv[i0 + jb.dv], u1 = s.vuv[i1 + jb.dv], u2 = s.vuv[i2 + jb.dv];
+    const V4 p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(Cc.x, Cc.y, Cc.z, 1);
+    const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
+    if (lane == 0) poly[0][0] = p1, poly[0][1] = p2, poly[0][2] = p3;
+    wave_lds_sync();
+    // clipping.go:31-65, the planes in its order
+    const Plane planes[6] = {{v4(w, 0, 0, 1), v4(-1, 0, 0, 1)}, {v4(0, 0, 0, 1), v4(1, 0, 0, 1)},
+                             {v4(0, h, 0, 1), v4(0, -1, 0, 1)}, {v4(0, 0, 0, 1), v4(0, 1, 0, 1)},
+                             {v4(0, 0, 1, 1), v4(0, 0, -1, 1)}, {v4(0, 0, -1, 1), v4(0, 0, 1, 1)}};
+    int na = 3, cur = 0;
+#pragma unroll
+    for (int pi = 0; pi < 6; pi++) {
+        if (na == 0) break;
+        const V4* a = poly[cur];
+        V4* b = poly[cur ^ 1];
+        int emit = 0;
+        V4 o0 = v4(0, 0, 0, 0), o1 = o0;
+        if (lane < na) {
+            const V4 e = a[lane], sv = a[lane == 0 ? na - 1 : lane - 1];
+            const bool fe = in_front(planes[pi], e), fs = in_front(planes[pi], sv);
+            if (fe) {
+                if (!fs) o0 = isect(planes[pi], sv, e), o1 = e, emit = 2;
+                else o0 = e, emit = 1;
+            } else if (fs) {
+                o0 = isect(planes[pi], sv, e), emit = 1;
+            }
+        }
+        int incl = emit;
+        for (int off = 1; off < 16; off <<= 1) {
+            const int u = __shfl_up(incl, off);
+            if (lane >= off) incl += u;
+        }
+        const int pos = incl - emit;
+        const int total = __shfl(incl, na - 1);
+        if (emit > 0 && pos < CLIP_VERTS) b[pos] = o0;
+        if (emit > 1 && pos + 1 < CLIP_VERTS) b[pos + 1] = o1;
+        wave_lds_sync();
+        na = min(total, CLIP_VERTS);
+        cur ^= 1;
+    }
+    // the job's CLIP_PIECES slots: its pieces, the rest empty
+    const int nsub = max(na - 2, 0);
+    const int base = qi * CLIP_PIECES;
+    if (base + CLIP_PIECES > cap.clip_subs) {
+        if (lane == 0) atomicOr(&cnt->overflow, 1);
+        return 0;
+    }
+    // lane 3p + k builds vertex k of fan piece p = (0, p+1, p+2) (one vertex's
+    // code a lane); lane p then boxes piece p
+    const V4* cl = poly[cur];
+    const bool vert = lane < 3 * nsub;
+    const int pc = lane / 3, k = lane - 3 * pc;
+    V4 q = v4(0, 0, 0, 0), nq = q, m = q;
+    float2 uv = make_float2(0.0f, 0.0f);
+    uint32_t col = 0;
+    if (vert) {
+        const V4 cv = cl[k == 0 ? 0 : pc + k];
+        const TriSetup par = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+        float bc[3];
+        barycoord(par, cv.x, cv.y, bc);
+        q = v4(bc[0] * p1.x + bc[1] * p2.x + bc[2] * p3.x, bc[0] * p1.y + bc[1] * p2.y + bc[2] * p3.y,
+               bc[0] * p1.z + bc[1] * p2.z + bc[2] * p3.z, 1);
+        nq = v4(bc[0] * n0.x + bc[1] * n1.x + bc[2] * n2.x, bc[0] * n0.y + bc[1] * n1.y + bc[2] * n2.y,
+                bc[0] * n0.z + bc[1] * n1.z + bc[2] * n2.z, 0);
+        uint8_t cc[4];
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++)
+            cc[q4] = f2u8(clamp(bc[0] * chan(c0, q4) + bc[1] * chan(c1, q4) + bc[2] * chan(c2, q4), 0, 0xff));
+        col = u8x4(cc[0], cc[1], cc[2], cc[3]);
+        m = apply(apply(apply(q, vc.vpinv), vc.pinv), vc.vinv);
+        uv = make_float2(bc[0] * u0.x + bc[1] * u1.x + bc[2] * u2.x, bc[0] * u0.y + bc[1] * u1.y + bc[2] * u2.y);
+    }
+    V4 sv[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int src = min(3 * lane + j, 63);
+        sv[j] = v4(__shfl(q.x, src), __shfl(q.y, src), __shfl(q.z, src), 1);
+    }
+    bool ok = false;
+    int bx0 = 0, by0 = 0, bx1 = 0, by1 = 0;
+    if (lane < nsub)
+        ok = pixel_box(sv[0].x, sv[0].y, sv[0].z, sv[1].x, sv[1].y, sv[1].z, sv[2].x, sv[2].y, sv[2].z, vc.W, vc.H,
+                       bx0, by0, bx1, by1);
+    // each piece's tiles, spread over the wave
+    for (int p = 0; p < nsub; p++) {
+        if (!__shfl((int)ok, p)) continue;
+        const int tx0 = __shfl(bx0, p) >> vc.rt_shift, ty0 = __shfl(by0, p) >> vc.rt_shift;
+        const int ntx = (__shfl(bx1, p) >> vc.rt_shift) - tx0 + 1, nty = (__shfl(by1, p) >> vc.rt_shift) - ty0 + 1;
+        for (int j = lane; j < ntx * nty; j += 64) {
+            const int ty = j / ntx;
+            atomicAdd(&tile_count[(ty0 + ty) * vc.rtiles_x + tx0 + (j - ty * ntx)], 1);
+        }
+    }
+    // the stores
+    if (lane == 0) tri_clip_slot[g] = base;
+    if (vert) {
+        // (vertex k keeps the parent's recipw of its own index, as the fan does)
+        ClipRec* rp = clips + base + pc;
+        rp->pos[k] = make_float4(q.x, q.y, q.z, k == 0 ? A.w : k == 1 ? B.w : Cc.w);
+        rp->nor[k] = make_float4(nq.x, nq.y, nq.z, bitsf(col));
+        rp->wp[k] = make_float4(m.x, m.y, m.z, 0.0f);
+        rp->uv[k] = uv;
+    }
+    if (lane < CLIP_PIECES) {
+        const size_t ps = (size_t)s.n_tris + base + lane;
+        if (ok) store_prim(prims, pbox, ps, sv[0], sv[1], sv[2], (uint32_t)g * 8u + (uint32_t)lane, bx0, by0, bx1, by1);
+        else store_empty(pbox, ps);
+    }
+    return nsub;
+}
+
+// Waves stride over the queue (its length is known on the device only; the
+// grid stays small).  Each workgroup writes its pieces to blk_pieces[its
+// index] for the frame statistics (the tile scan sums them): a counter
+// that every queued triangle hit would serialise them.
+struct k_clip_args {
+    DevScene s;
+    ViewConsts vc;
+    const float4* tv_scr;
+    const float4* tv_nor;
+    const ClipJob* clip_list;
+    Prim* prims;
+    uint2* pbox;
+    ClipRec* clips;
+    int32_t* tri_clip_slot;
+    int32_t* tile_count;
+    int32_t* blk_pieces;
+    Counters* cnt;
+    Capacities cap;
+    int32_t gx, gy;  // the grid of this launch
+};
+__device__ __forceinline__ void k_clip_body(const k_clip_args& a_, const VB vb) {
+    __shared__ V4 s_poly[CLIP_WAVES][2][CLIP_VERTS];
+    __shared__ int32_t s_pieces[CLIP_WAVES];
+    Counters* cnt = a_.cnt;
+    const int wv = threadIdx.x >> 6;
+    // the wave's first job is read before the queue's length is known (a
+    // slot within the queue's capacity either way; used only if queued)
+    const int q0 = __builtin_amdgcn_readfirstlane(vb.bx * CLIP_WAVES + wv);
+    ClipJob jb = a_.clip_list[min(q0, a_.cap.clip_tris - 1)];
+    int pieces = 0;
+    if (!cnt->overflow) {
+        const int n = min(cnt->n_clipped, a_.cap.clip_tris);
+        for (int q = q0; q < n; q += vb.gx * CLIP_WAVES) {
+            if (q != q0) jb = a_.clip_list[q];
+            pieces += clip_wave(q, jb, a_.s, a_.vc, a_.tv_scr, a_.tv_nor, a_.prims, a_.pbox, a_.clips,
+                                a_.tri_clip_slot, a_.tile_count, cnt, a_.cap, s_poly[wv]);
+        }
+    }
+    if ((threadIdx.x & 63) == 0) s_pieces[wv] = pieces;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < CLIP_WAVES; k++) t += s_pieces[k];
+        a_.blk_pieces[vb.bx] = t;
+    }
+}
+__global__ __launch_bounds__(CLIP_WAVES * 64) void k_clip(k_clip_args a) {
+    k_clip_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
+}
+__global__ __launch_bounds__(CLIP_WAVES * 64) void k_clip_wide(const k_clip_args* __restrict__ a, const int32_t* __restrict__ start,
+        int32_t n) {
+    const int v = wide_view(start, n, (int)blockIdx.x);
+    const k_clip_args& x = a[v];
+    const int local = (int)blockIdx.x - start[v];
+    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
+    k_clip_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
 }
 
 // ------------------------------------------------------------- rasters
@@ -2366,16 +2505,19 @@ void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, c
     go(k_vertex_shadow, k_vertex_shadow_wide, dim3(blocks_for(s.n_tverts, 256), sc.n_slots), 256, st,
        k_vertex_shadow_args{s, strans, sc, ts, 0, 0});
 }
+int setup_blocks(int32_t n_tris) { return (n_tris + 255) / 256; }
+int clip_grid(const Capacities& cap) { return (int)std::min<size_t>(blocks_for(cap.clip_tris, CLIP_WAVES), CLIP_GRID); }
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
-                       int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
+                       ClipJob* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
                        unsigned long long* vis, int32_t draw_small, int4* tri_rec) {
     if (!s.n_tris) return;
-    go(k_setup_main, k_setup_main_wide, dim3(blocks_for(s.n_tris, 256)), 256, st,
+    go(k_setup_main, k_setup_main_wide, dim3(setup_blocks(s.n_tris)), 256, st,
        k_setup_main_args{s, vc, tv_scr, prims, pbox, clip_list, tile_count, blk_prims, cnt, cap, vis, draw_small,
                          tri_rec, 0, 0});
-    go(k_clip, k_clip_wide, dim3(std::min<size_t>(blocks_for(cap.clip_tris, 64), CLIP_GRID)), 64, st,
-       k_clip_args{s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count, cnt, cap, 0, 0});
+    go(k_clip, k_clip_wide, dim3(clip_grid(cap)), CLIP_WAVES * 64, st,
+       k_clip_args{s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count,
+                   blk_prims + setup_blocks(s.n_tris), cnt, cap, 0, 0});
 }
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
                          uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap, uint32_t* smaps) {
@@ -2386,7 +2528,7 @@ void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& 
 int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK); }
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
                  int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
-                 int32_t max_clip_subs, Counters* cnt, int32_t item_prims) {
+                 Counters* cnt, int32_t item_prims) {
     const int nb = n <= SCAN_WHOLE_MAX ? 1 : scan_blocks(n);
     // (the sums stage always records, so the apply stage keeps its place)
     if (g_wide) g_wide->sub = 0;
@@ -2395,7 +2537,7 @@ void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, i
     if (g_wide) g_wide->sub = 1;
     go(k_scan_apply, k_scan_apply_wide, dim3(nb), 1024, st,
        k_scan_apply_args{counts, n, bsum, offsets, chunk_off, cursor, item_tile, cap_items, blk_prims, n_blk,
-                         max_clip_subs, cnt, item_prims, 0, 0});
+                         cnt, item_prims, 0, 0});
 }
 void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t shift, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris) {

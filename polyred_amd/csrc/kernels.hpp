@@ -128,22 +128,28 @@ void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, 
                         float2* tv_uv);
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
                           float4* ts);
+// The setup's workgroups (each writes its unclipped primitives to
+// blk_prims[its index]) and then the clipper's (blk_prims[setup_blocks +
+// its index]: its pieces); blk_prims holds setup_blocks(n_tris) + CLIP_GRID.
+constexpr int CLIP_GRID = 512;
+int setup_blocks(int32_t n_tris);
+int clip_grid(const Capacities& cap);
 // pbox: the pixel box of every primitive slot (k_bin's input)
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
-                       int32_t* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
+                       ClipJob* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
                        unsigned long long* vis, int32_t draw_small, int4* tri_rec);
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
                          Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
                          uint32_t* smaps);
-// n_blk > 0: also sum the per-workgroup primitive counts of the setup into
-// the frame statistics (the main view)
+// n_blk > 0: also sum the per-workgroup primitive counts of the setup and
+// the clipper (blk_prims) into the frame statistics (the main view)
 // bsum: 2 * scan_blocks(n) ints of scratch
 int scan_blocks(int32_t n);
 // Also lists each work item's tile in item_tile and zeroes the counts.
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,
                  int32_t* bsum, int32_t* item_tile, int32_t cap_items, const int32_t* blk_prims, int32_t n_blk,
-                 int32_t max_clip_subs, Counters* cnt, int32_t item_prims);
+                 Counters* cnt, int32_t item_prims);
 // shift: log2 of the raster tile side
 void launch_bin(hipStream_t st, const uint2* pbox, Counters* cnt, int32_t tiles_x, int32_t shift, int32_t tiles_per_view,
                 int32_t shadow, int32_t* cursor, int32_t* list, Capacities cap, int32_t n_fixed, int32_t n_tris);

@@ -115,9 +115,18 @@ struct ClipRec {
     float2 uv[3];   // texture coordinates (render/clipping.go:88-91)
 };
 
+// A triangle queued for the clipper by the setup: its global index, its
+// transformed vertices and the offset from those to its mesh vertices
+// (colours, UVs).  Its fan pieces take the primitive slots
+// n_tris + CLIP_PIECES * (queue position) + piece.
+struct ClipJob {
+    int32_t g, i0, i1, i2;
+    int32_t dv, pad[3];
+};
+constexpr int CLIP_PIECES = 7;  // a triangle clipped by six planes keeps at most 9 vertices: 7 pieces
+
 struct Counters {
     int32_t n_prims;
-    int32_t n_clip_subs;
     int32_t n_clipped;      // triangles queued for the clipper
     int32_t n_pairs;        // (primitive, tile) pairs
     int32_t nan_flag;
@@ -126,7 +135,7 @@ struct Counters {
     int32_t n_chunks;       // raster work items: (tile, up to ITEM_PRIMS primitives)
     int32_t next_item;      // persistent raster work queue head
     int32_t depth_nonfinite;  // some pixel's resolved depth is NaN or infinite
-    int32_t pad[6];
+    int32_t pad[7];
 };
 
 struct Capacities {

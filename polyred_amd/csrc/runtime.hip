@@ -416,9 +416,10 @@ struct prh_renderer {
     uint2* pbox;  // pixel box per primitive slot
     ClipRec* clips;
     int32_t* tri_clip_slot;
-    int32_t* blk_prims;  // per setup workgroup: unclipped primitives (statistics)
+    int32_t* blk_prims;  // per setup / clipper workgroup: unclipped primitives, pieces (statistics)
     int32_t* scan_sums;  // per scan block: primitive and work-item sums
-    int32_t *tile_count, *tile_off, *chunk_off, *tile_cursor, *tile_list, *clip_list, *item_tile;
+    int32_t *tile_count, *tile_off, *chunk_off, *tile_cursor, *tile_list, *item_tile;
+    ClipJob* clip_list;
     int32_t cap_items;
     // the shadow passes' own work set (they run beside the main view's)
     Prim* s_prims;
@@ -981,7 +982,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     const int Tg = s->d.n_tris, Vt = s->d.n_tverts;
     // primitive slots: one per triangle (per shadow slot), then the main
     // view's clipped fan pieces
-    const int clip_tris = std::min(Tg, 1 << 16) + 16, clip0 = 7 * clip_tris;
+    const int clip_tris = std::min(Tg, 1 << 16) + 16, clip0 = CLIP_PIECES * clip_tris;
     // the tile tables hold the smallest raster tile's grid, so the tile
     // knob never reallocates them
     const int rt_cap = ((r->W + RTILE_MIN - 1) / RTILE_MIN) * ((r->H + RTILE_MIN - 1) / RTILE_MIN);
@@ -1015,7 +1016,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc && s->d.n_textures > 0) chk(ralloc(r, &r->tv_uv, Vt));
     if (!rc) chk(ralloc(r, &r->ts, (size_t)std::max(1, r->n_slots) * Vt));
     if (!rc) chk(ralloc(r, &r->tri_clip_slot, Tg));
-    if (!rc) chk(ralloc(r, &r->blk_prims, (size_t)(Tg + 255) / 256));
+    if (!rc) chk(ralloc(r, &r->blk_prims, (size_t)setup_blocks(Tg) + CLIP_GRID));
     if (!rc) chk(ralloc(r, &r->scan_sums, 2 * (size_t)scan_blocks(r->tile_slots)));
     if (!rc) chk(ralloc(r, &r->tile_count, r->tile_slots));
     if (!rc) chk(ralloc(r, &r->tile_off, r->tile_slots + 1));
@@ -1204,7 +1205,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
                             r->smaps);
         wide_stage(4);
         launch_scan(ss, r->s_tile_count, nt, r->s_tile_off, r->s_chunk_off, r->s_tile_cursor, r->s_scan_sums,
-                    r->s_item_tile, r->s_cap_items, nullptr, 0, 0, cs, ip);
+                    r->s_item_tile, r->s_cap_items, nullptr, 0, cs, ip);
         wide_stage(5);
         launch_bin(ss, r->s_pbox, cs, r->stiles_x, r->shc.rt_shift, r->stiles_per_map, 1, r->s_tile_cursor, r->s_tile_list,
                    r->cap_shadow, r->n_slots * d.n_tris, d.n_tris);
@@ -1225,7 +1226,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
                       nan_mode ? 0 : r->tune.main_direct_max, r->tri_rec);
     wide_stage(9);
     launch_scan(st, r->tile_count, r->n_rtiles, r->tile_off, r->chunk_off, r->tile_cursor, r->scan_sums, r->item_tile,
-                r->cap_items, r->blk_prims, (d.n_tris + 255) / 256, r->cap_main.clip_subs, cm, ip);
+                r->cap_items, r->blk_prims, d.n_tris ? setup_blocks(d.n_tris) + clip_grid(r->cap_main) : 0, cm, ip);
     wide_stage(10);
     launch_bin(st, r->pbox, cm, r->rtiles_x, r->vc.rt_shift, r->n_rtiles, 0, r->tile_cursor, r->tile_list, r->cap_main, d.n_tris,
                d.n_tris);
@@ -1291,12 +1292,10 @@ int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
         r->cap_shadow.pairs = std::max(r->cap_shadow.pairs, cs.n_pairs + cs.n_pairs / 4 + 16);
         grow = true;
     }
-    if (cm.overflow || cm.n_pairs > r->cap_main.pairs || cm.n_clip_subs > r->cap_main.clip_subs ||
-        cm.n_clipped > r->cap_main.clip_tris) {
+    if (cm.overflow || cm.n_pairs > r->cap_main.pairs || cm.n_clipped > r->cap_main.clip_tris) {
         r->cap_main.pairs = std::max(r->cap_main.pairs, cm.n_pairs + cm.n_pairs / 4 + 16);
-        r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, cm.n_clip_subs + cm.n_clip_subs / 4 + 16);
         r->cap_main.clip_tris = std::max(r->cap_main.clip_tris, cm.n_clipped + cm.n_clipped / 4 + 16);
-        r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, 7 * r->cap_main.clip_tris);
+        r->cap_main.clip_subs = std::max(r->cap_main.clip_subs, CLIP_PIECES * r->cap_main.clip_tris);
         r->cap_main.prims = d_n_tris(r) + r->cap_main.clip_subs;
         grow = true;
     }
