@@ -467,6 +467,9 @@ struct prh_renderer {
     uint32_t* present_to;  // during prh_surface_render: the acquired surface texture
     prh_tuning tune;       // speed knobs (results do not depend on them)
     bool split_shade_last;  // the last frame split its shading
+    // the last frame's per-pass times wait in its events until the stats are
+    // read (prh_last_stats): a frame's own host work stays short
+    bool times_pending, times_nan_rerun;
 };
 
 namespace {
@@ -1318,23 +1321,8 @@ int finish_frame(prh_renderer* r, int attempt, bool& again, bool timed = true) {
         nan_rerun = true;
     }
     prh_frame_stats& S = r->stats;
-    if (timed) {
-        float ms[6] = {0};
-        for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
-        (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
-        // (the shadow passes ran beside the main view's: their own span)
-        S.ms_shadow = 0.0f;
-        if (r->n_slots > 0 && !nan_rerun) (void)hipEventElapsedTime(&S.ms_shadow, r->ev_s[0], r->ev_s[1]);
-        S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
-        (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
-        if (r->split_shade_last) {  // plus the first half, beside k_ao
-            float pre = 0.0f;
-            (void)hipEventElapsedTime(&pre, r->ev_sh[0], r->ev_sh[1]);
-            S.ms_shade += pre;
-        }
-    } else {
-        S.ms_total = S.ms_shadow = S.ms_vertex = S.ms_setup = S.ms_raster = S.ms_ao = S.ms_shade = 0.0f;
-    }
+    S.ms_total = S.ms_shadow = S.ms_vertex = S.ms_setup = S.ms_raster = S.ms_ao = S.ms_shade = 0.0f;
+    r->times_pending = timed, r->times_nan_rerun = nan_rerun;
     S.n_prims = cm.n_prims, S.n_clipped = cm.n_clipped, S.n_bin_pairs = cm.n_pairs;
     S.nan_rerun = nan_rerun ? 1 : 0;
     r->has_frame = true;
@@ -1867,8 +1855,30 @@ int prh_read_buffer(prh_renderer* r, int32_t which, void* dst, size_t bytes) {
     return PRH_OK;
 }
 
+// The last frame's per-pass times from its events (finish_frame left them).
+void fill_times(prh_renderer* r) {
+    if (!r->times_pending) return;
+    r->times_pending = false;
+    prh_frame_stats& S = r->stats;
+    float ms[6] = {0};
+    for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&ms[i], r->ev[i], r->ev[i + 1]);
+    (void)hipEventElapsedTime(&S.ms_total, r->ev[0], r->ev[6]);
+    // (the shadow passes ran beside the main view's: their own span)
+    S.ms_shadow = 0.0f;
+    if (r->n_slots > 0 && !r->times_nan_rerun) (void)hipEventElapsedTime(&S.ms_shadow, r->ev_s[0], r->ev_s[1]);
+    S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
+    (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
+    if (r->split_shade_last) {  // plus the first half, beside k_ao
+        float pre = 0.0f;
+        (void)hipEventElapsedTime(&pre, r->ev_sh[0], r->ev_sh[1]);
+        S.ms_shade += pre;
+    }
+}
+
 int prh_last_stats(prh_renderer* r, prh_frame_stats* out) {
     if (!r || !out) return fail(PRH_E_INVALID, "prh_last_stats: null argument");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    fill_times(r);
     *out = r->stats;
     return PRH_OK;
 }
@@ -2319,6 +2329,7 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
         std::vector<float> v;
         for (int k = 0; k < frames; k++) {
             if ((rc = prh_render(r))) return rc;
+            fill_times(r);
             v.push_back(r->stats.ms_total);
         }
         std::sort(v.begin(), v.end());
