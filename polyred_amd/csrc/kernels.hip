@@ -381,16 +381,15 @@ __device__ __forceinline__ void block_scan_256(int v, int32_t* start /*[257]*/, 
 // the workgroup, each centre tested as the rasters test it (the same
 // Barycoord, inside test and depth, raster.go:487-496) and handed to `put`
 // with its triangle's staged record.
-struct DPrim {
+struct __align__(16) DPrim {
     float ax, ay, bx, by, abx, aby, acx, acy, bcx, bcy, sabc;  // TriSetup
     float az, bz, cz;
     int32_t xy0;  // x0 | y0 << 16
-    int32_t w;    // box width
-    uint32_t key;
+    uint32_t wq;  // box width w | ceil(65536 / w) << 9
 };
 template <class F>
-__device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, uint32_t key, int x0, int y0, int x1,
-                                           int y1, DPrim* sp, int32_t* start, int32_t* wsum, F&& put) {
+__device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, int x0, int y0, int x1, int y1,
+                                                DPrim* sp, int32_t* start, int32_t* wsum, F&& put) {
     const int tid = threadIdx.x;
     int c = 0;
     if (mine) {
@@ -399,30 +398,31 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
         d.ax = t.ax, d.ay = t.ay, d.bx = t.bx, d.by = t.by, d.abx = t.abx, d.aby = t.aby;
         d.acx = t.acx, d.acy = t.acy, d.bcx = t.bcx, d.bcy = t.bcy, d.sabc = t.sabc;
         d.az = p1.z, d.bz = p2.z, d.cz = p3.z;
-        d.xy0 = x0 | (y0 << 16), d.w = x1 - x0 + 1, d.key = key;
+        const uint32_t w = (uint32_t)(x1 - x0 + 1);
+        d.xy0 = x0 | (y0 << 16), d.wq = w | ((65536u + w - 1u) / w) << 9;
         c = (x1 - x0 + 1) * (y1 - y0 + 1);
     }
     block_scan_256(c, start, wsum);
     const int total = start[256];
     for (int idx = tid; idx < total; idx += 256) {
-        int lo = 0, hi = 255;  // the thread whose range holds idx (empty ranges are skipped)
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (start[mid] <= idx) lo = mid;
-            else hi = mid - 1;
-        }
-        const DPrim& d = sp[lo];
-        const int l = idx - start[lo], w = d.w;
+        // the thread whose range holds idx: the last with start <= idx (empty
+        // ranges are skipped), in eight fixed steps
+        int lo = 0;
+#pragma unroll
+        for (int step = 128; step >= 1; step >>= 1)
+            if (start[lo + step] <= idx) lo += step;
+        const DPrim d = sp[lo];
+        const int l = idx - start[lo], w = (int)(d.wq & 511u);
         // l / w by multiply-shift: exact while l * w < 65536, which l < area <= 256
         // and w <= area keep (the thresholds are at most 256)
-        const int q = (int)(((uint32_t)l * ((65536u + (uint32_t)w - 1u) / (uint32_t)w)) >> 16);
+        const int q = (int)(((uint32_t)l * (d.wq >> 9)) >> 16);
         const int px = (d.xy0 & 0xffff) + (l - q * w), py = (d.xy0 >> 16) + q;
         TriSetup t;
         t.ax = d.ax, t.ay = d.ay, t.bx = d.bx, t.by = d.by, t.abx = d.abx, t.aby = d.aby;
         t.acx = d.acx, t.acy = d.acy, t.bcx = d.bcx, t.bcy = d.bcy, t.sabc = d.sabc;
         float z;
         if (!cover_z(t, (float)px + 0.5f, (float)py + 0.5f, d.az, d.bz, d.cz, z)) continue;
-        put(d, px, py, z);
+        put(px, py, z);
     }
 }
 
@@ -596,7 +596,7 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
         __shared__ int32_t start[257], wsum[4];
         uint32_t* map = smaps + (size_t)slot * sc.SW * sc.SH;
         bool nan = false;
-        draw_small_tris(small, p1, p2, p3, 0u, x0, y0, x1, y1, sp, start, wsum, [&](const DPrim&, int px, int py, float z) {
+        draw_small_tris(small, p1, p2, p3, x0, y0, x1, y1, sp, start, wsum, [&](int px, int py, float z) {
             if (z != z) nan = true;
             else if (z > 0.0f) atomicMax(&map[(size_t)py * sc.SW + px], depth_ord(z));  // the map starts at 0
         });
