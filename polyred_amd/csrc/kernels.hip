@@ -1147,7 +1147,7 @@ struct alignas(16) LPrim {
     float bcx, bcy, sabc, az;
     float bz, cz;
     uint32_t key;
-    uint32_t rect;  // rx0 | ry0<<5 | (w-1)<<10 | (h-1)<<15 (tile-local)
+    uint32_t rect;  // rx0 | ry0<<5 | (w-1)<<10 | ceil(65536 / w)<<15 (tile-local)
 };
 
 // Loads one primitive and intersects its pixel box with the T x T tile;
@@ -1164,8 +1164,9 @@ __device__ __forceinline__ int stage_prim(const Prim& p, int px0, int py0, LPrim
     lp.bcx = t.bcx, lp.bcy = t.bcy, lp.sabc = t.sabc;
     lp.az = p.p0.z, lp.bz = p.p1.y, lp.cz = p.p2.x;
     lp.key = fbits(p.p2.y);
-    int w = x1 - x0 + 1, h = y1 - y0 + 1;
-    lp.rect = (uint32_t)(x0 - px0) | ((uint32_t)(y0 - py0) << 5) | ((uint32_t)(w - 1) << 10) | ((uint32_t)(h - 1) << 15);
+    const int w = x1 - x0 + 1, h = y1 - y0 + 1;
+    lp.rect = (uint32_t)(x0 - px0) | ((uint32_t)(y0 - py0) << 5) | ((uint32_t)(w - 1) << 10) |
+              (((65536u + (uint32_t)w - 1u) / (uint32_t)w) << 15);
     return w * h;
 }
 
@@ -1173,17 +1174,17 @@ __device__ __forceinline__ int stage_prim(const Prim& p, int px0, int py0, LPrim
 // is exact for l < 1024, w <= 32: the product overshoots l / w by less than
 // l / 65536 < 1/64, and l / w's fraction is at most 1 - 1/w.
 __device__ __forceinline__ void candidate(const int32_t* start, const LPrim* sp, int idx, int& j, int& lx, int& ly) {
-    int lo = 0, hi = 255;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= idx) lo = mid;
-        else hi = mid - 1;
-    }
+    // the last primitive with start <= idx (empty ranges are skipped), in
+    // eight fixed steps
+    int lo = 0;
+#pragma unroll
+    for (int step = 128; step >= 1; step >>= 1)
+        if (start[lo + step] <= idx) lo += step;
     j = lo;
     int l = idx - start[j];
     uint32_t r = sp[j].rect;
     int w = ((r >> 10) & 31) + 1;
-    int q = (int)(((uint32_t)l * ((65536u + (uint32_t)w - 1u) / (uint32_t)w)) >> 16);
+    int q = (int)(((uint32_t)l * (r >> 15)) >> 16);
     lx = (int)(r & 31) + (l - q * w);
     ly = (int)((r >> 5) & 31) + q;
 }
