@@ -1828,6 +1828,10 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
     __shared__ float s_win[2][2][AO_WIN];  // the two x-axis marches: each its two rows around the pixels
     __shared__ int32_t s_dbase[4][100];  // diagonal marches, tap t: byte offset of lane 0's tap
     __shared__ double s_drinv[4][100];   // and fl64(1 / distance), where every lane's tap shares them
+    // and for every diagonal tap, the row's part (the chunk shares its row):
+    // the row's first pixel (negative outside the buffer) and the distance
+    // table's row term
+    __shared__ int32_t s_drow[4][100], s_dtab[4][100];
     if (cnt->overflow) return;
     const int W = c.v.W, H = c.v.H;
     const int nbx = (W + 63) / 64, nb = nbx * H;
@@ -1911,6 +1915,8 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 uni = (lo > 0.0f) & (kl == kh);
                 const int iy = (int)cy, ky = __builtin_amdgcn_frexp_expf(cy);
                 s_dbase[k][t] = (iy * W + (int)lo) * 4;
+                s_drow[k][t] = (unsigned)iy < (unsigned)H ? iy * W : -(1 << 28);
+                s_dtab[k][t] = ky * ROW_Y + t * 8 + BIAS;
                 const int toff = kl * ROW_X + ky * ROW_Y + t * 8 + BIAS;
                 s_drinv[k][t] =
                     uni ? __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(trs, toff, 0, 0)) : 0.0;
@@ -2055,18 +2061,22 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 float e;
                 bool in;  // in the buffer
             };
+            const int kd = wu >> 1;
             auto tap = [&](int ti) {
-                const float cx = X + off_x[ti], cy = Y + off_y[ti];
-                const int ix = (int)cx, iy = (int)cy;
+                const float cx = X + off_x[ti];
+                const int ix = (int)cx;
                 // an exponent outside the table only occurs for a tap outside
                 // the buffer (checked when the table is built); its offset
-                // then reads 0 or another entry, masked either way
-                const int kx = __builtin_amdgcn_frexp_expf(cx), ky = __builtin_amdgcn_frexp_expf(cy);
-                const int toff = mul_i24(kx, ROW_X) + mul_i24(ky, ROW_Y) + (ti * 8 + BIAS);
+                // then reads 0 or another entry, masked either way.  The row
+                // (iy = (int)(Y + fl(dir.y * t)), its exponent) is the chunk's,
+                // from the tables above.
+                const int kx = __builtin_amdgcn_frexp_expf(cx);
+                const int row = s_drow[kd][ti];
+                const int toff = mul_i24(kx, ROW_X) + s_dtab[kd][ti];
                 Tap r;
                 r.rinv = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(trs, toff, 0, 0));
-                r.in = ((unsigned)ix < (unsigned)W) & ((unsigned)iy < (unsigned)H);
-                const uint32_t off = (mul_u24((unsigned)iy, (unsigned)W) + (unsigned)ix) * 4u;
+                r.in = ((unsigned)ix < (unsigned)W) & (row >= 0);
+                const uint32_t off = ((unsigned)row + (unsigned)ix) * 4u;
                 r.e = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(drs, (int)off, 0, 0)) - dp;
                 return r;
             };
@@ -2089,7 +2099,6 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 }
                 for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
                 const int tmin = __builtin_amdgcn_readfirstlane(tl);
-                const int kd = wu >> 1;
                 const uint32_t lane4 = (uint32_t)li * 4u;
                 for (; ti + 7 < tmin; ti += 8) {
                     const uint64_t bits = ti < 64 ? (uni_lo >> ti) | (ti > 56 ? uni_hi << (64 - ti) : 0ull)
