@@ -446,7 +446,7 @@ struct k_setup_main_args {
     Capacities cap;
     unsigned long long* vis;
     int32_t draw_small;
-    int4* tri_rec;
+    const int4* tri_rec;
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, const VB vb) {
@@ -462,7 +462,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
     const Capacities& cap = a_.cap;
     unsigned long long* __restrict__ vis = a_.vis;
     const int32_t draw_small = a_.draw_small;
-    int4* __restrict__ tri_rec = a_.tri_rec;
+    const int4* __restrict__ tri_rec = a_.tri_rec;
 
     const int g = vb.bx * blockDim.x + threadIdx.x;
     bool direct = false, doclip = false, drawn = false;
@@ -471,17 +471,9 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     const float w = vc.vpbox_max_x, h = vc.vpbox_max_y;
     if (g < s.n_tris) {
-        const int o = s.tri_obj[g];
-        const int mesh = s.obj_mesh[o];
-        const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
-        if (s.tri_valid[t]) {
-            const int vb = s.obj_vtx_start[o];
-            const int i0 = vb + s.tri_v[3 * t], i1 = vb + s.tri_v[3 * t + 1], i2 = vb + s.tri_v[3 * t + 2];
-            // the shading's record of the triangle: its transformed vertices and
-            // its material in the flat table (-1: vertex colour), matAt's rule
-            int mat = s.tri_mat[t];
-            if (mat >= 0) mat += s.obj_mat_base[o];
-            tri_rec[g] = make_int4(i0, i1, i2, (mat >= 0 && mat < s.n_materials) ? mat : -1);
+        const int4 rec = tri_rec[g];  // (x < 0: not Triangle.IsValid)
+        if (rec.x >= 0) {
+            const int i0 = rec.x, i1 = rec.y, i2 = rec.z;
             const float4 A = tv_scr[i0], B = tv_scr[i1], Cc = tv_scr[i2];
             p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(Cc.x, Cc.y, Cc.z, 1);
             if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {  // cull.go:26-28
@@ -490,8 +482,12 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
                     if (vp_contains(p1, w, h) && vp_contains(p2, w, h) && vp_contains(p3, w, h))
                         direct = pixel_box(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, vc.W, vc.H, x0, y0,
                                            x1, y1);
-                    else
-                        doclip = true, job.i0 = i0, job.i1 = i1, job.i2 = i2, job.dv = s.mesh_vtx_off[mesh] - vb;
+                    else {
+                        // (the offset from the transformed vertices to the mesh's)
+                        const int o = s.tri_obj[g];
+                        doclip = true, job.i0 = i0, job.i1 = i1, job.i2 = i2;
+                        job.dv = s.mesh_vtx_off[s.obj_mesh[o]] - s.obj_vtx_start[o];
+                    }
                 }
             }
         }
@@ -549,6 +545,7 @@ __global__ __launch_bounds__(256) void k_setup_main_wide(const k_setup_main_args
 struct k_setup_shadow_args {
     DevScene s;
     ShadowConsts sc;
+    const int4* tri_rec;
     const float4* ts;
     Prim* prims;
     uint2* pbox;
@@ -574,13 +571,10 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
     V4 p1 = v4(0, 0, 0, 1), p2 = p1, p3 = p1;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     if (g < s.n_tris) {
-        int o = s.tri_obj[g];
-        int mesh = s.obj_mesh[o];
-        int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
-        if (s.tri_valid[t]) {
+        const int4 rec = a_.tri_rec[g];  // (x < 0: not Triangle.IsValid)
+        if (rec.x >= 0) {
             const float4* T = ts + (size_t)slot * s.n_tverts;
-            int vb = s.obj_vtx_start[o];
-            float4 A = T[vb + s.tri_v[3 * t]], B = T[vb + s.tri_v[3 * t + 1]], C = T[vb + s.tri_v[3 * t + 2]];
+            float4 A = T[rec.x], B = T[rec.y], C = T[rec.z];
             p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(C.x, C.y, C.z, 1);
             if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {
                 Box3 tb = box_of(p1, p2, p3);
@@ -2520,7 +2514,7 @@ int clip_grid(const Capacities& cap) { return (int)std::min<size_t>(blocks_for(c
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        ClipJob* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
-                       unsigned long long* vis, int32_t draw_small, int4* tri_rec) {
+                       unsigned long long* vis, int32_t draw_small, const int4* tri_rec) {
     if (!s.n_tris) return;
     go(k_setup_main, k_setup_main_wide, dim3(setup_blocks(s.n_tris)), 256, st,
        k_setup_main_args{s, vc, tv_scr, prims, pbox, clip_list, tile_count, blk_prims, cnt, cap, vis, draw_small,
@@ -2529,11 +2523,33 @@ void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, 
        k_clip_args{s, vc, tv_scr, tv_nor, clip_list, prims, pbox, clips, tri_clip_slot, tile_count,
                    blk_prims + setup_blocks(s.n_tris), cnt, cap, 0, 0});
 }
-void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts, Prim* prims,
-                         uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap, uint32_t* smaps) {
+void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const int4* tri_rec,
+                         const float4* ts, Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
+                         uint32_t* smaps) {
     if (!s.n_tris || !sc.n_slots) return;
     go(k_setup_shadow, k_setup_shadow_wide, dim3(blocks_for(s.n_tris, 256), sc.n_slots), 256, st,
-       k_setup_shadow_args{s, sc, ts, prims, pbox, tile_count, cnt, cap, smaps, 0, 0});
+       k_setup_shadow_args{s, sc, tri_rec, ts, prims, pbox, tile_count, cnt, cap, smaps, 0, 0});
+}
+// Each global triangle's record, fixed by the scene (k_tri_rec).
+__global__ __launch_bounds__(256) void k_tri_rec(DevScene s, int4* tri_rec) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= s.n_tris) return;
+    const int o = s.tri_obj[g];
+    const int mesh = s.obj_mesh[o];
+    const int t = s.mesh_tri_off[mesh] + (g - s.obj_tri_start[o]);
+    if (!s.tri_valid[t]) {
+        tri_rec[g] = make_int4(-1, -1, -1, -1);
+        return;
+    }
+    const int vb = s.obj_vtx_start[o];
+    // the material in the flat table (-1: vertex colour), matAt's rule
+    int mat = s.tri_mat[t];
+    if (mat >= 0) mat += s.obj_mat_base[o];
+    tri_rec[g] = make_int4(vb + s.tri_v[3 * t], vb + s.tri_v[3 * t + 1], vb + s.tri_v[3 * t + 2],
+                           (mat >= 0 && mat < s.n_materials) ? mat : -1);
+}
+void launch_tri_rec(hipStream_t st, const DevScene& s, int4* tri_rec) {
+    if (s.n_tris) hipLaunchKernelGGL(k_tri_rec, dim3(blocks_for(s.n_tris, 256)), dim3(256), 0, st, s, tri_rec);
 }
 int scan_blocks(int32_t n) { return std::max(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK); }
 void launch_scan(hipStream_t st, int32_t* counts, int32_t n, int32_t* offsets, int32_t* chunk_off, int32_t* cursor,

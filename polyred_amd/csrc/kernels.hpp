@@ -25,7 +25,7 @@ struct ShadeBuffers {
     const float4* tv_scr;
     const float4* tv_nor;
     const float4* tv_wp;
-    const int4* tri_rec;       // per triangle: transformed vertices and flat material (k_setup_main)
+    const int4* tri_rec;       // per triangle: transformed vertices and flat material (k_tri_rec)
     const uint32_t* tv_col;    // per transformed vertex: colour
     const float2* tv_uv;       // and UV (scenes with textures)
     const ClipRec* clips;
@@ -138,10 +138,14 @@ int clip_grid(const Capacities& cap);
 void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, const float4* tv_scr,
                        const float4* tv_nor, Prim* prims, uint2* pbox, ClipRec* clips, int32_t* tri_clip_slot,
                        ClipJob* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
-                       unsigned long long* vis, int32_t draw_small, int4* tri_rec);
-void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const float4* ts,
-                         Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
+                       unsigned long long* vis, int32_t draw_small, const int4* tri_rec);
+void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const int4* tri_rec,
+                         const float4* ts, Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
                          uint32_t* smaps);
+// Each global triangle's record, fixed by the scene: its transformed
+// vertices and its material in the flat table (-1: vertex colour, matAt's
+// rule), or x = -1 when Triangle.IsValid is false.  Once per renderer.
+void launch_tri_rec(hipStream_t st, const DevScene& s, int4* tri_rec);
 // n_blk > 0: also sum the per-workgroup primitive counts of the setup and
 // the clipper (blk_prims) into the frame statistics (the main view)
 // bsum: 2 * scan_blocks(n) ints of scratch

@@ -409,7 +409,7 @@ struct prh_renderer {
     std::vector<void*> allocs;
     M4 *d_trans, *d_normal, *d_strans, *d_lview, *d_lproj;
     float4 *tv_scr, *tv_nor, *tv_wp, *ts;
-    int4* tri_rec;     // per triangle: transformed vertices and flat material (the setup writes it)
+    int4* tri_rec;     // per triangle: transformed vertices and flat material (once, k_tri_rec)
     uint32_t* tv_col;  // per transformed vertex: colour
     float2* tv_uv;     // and UV (scenes with textures)
     Prim* prims;
@@ -1092,6 +1092,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     // Shadow maps start at zero once, here, as make([]float32) in initShadowMaps
     // (stored as ordered depth bits).
     launch_fill_u32(ctx->stream, r->smaps, (size_t)std::max(1, r->n_slots) * r->SW * r->SH, depth_ord(0.0f));
+    launch_tri_rec(ctx->stream, s->d, r->tri_rec);
     // tile counts start at zero; every scan leaves them at zero
     launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(r->tile_count), r->tile_slots, 0u);
     if (r->n_slots > 0)
@@ -1204,7 +1205,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
         wide_stage(2);
         launch_vertex_shadow(ss, d, r->d_strans, r->shc, r->ts);
         wide_stage(3);
-        launch_setup_shadow(ss, d, r->shc, r->ts, r->s_prims, r->s_pbox, r->s_tile_count, cs, r->cap_shadow,
+        launch_setup_shadow(ss, d, r->shc, r->tri_rec, r->ts, r->s_prims, r->s_pbox, r->s_tile_count, cs, r->cap_shadow,
                             r->smaps);
         wide_stage(4);
         launch_scan(ss, r->s_tile_count, nt, r->s_tile_off, r->s_chunk_off, r->s_tile_cursor, r->s_scan_sums,
