@@ -428,7 +428,8 @@ struct prh_renderer {
     int32_t s_cap_items;
     int32_t s_tile_slots;  // shadow slots x tiles per map
     hipEvent_t ev_fork, ev_join, ev_s[2];
-    hipEvent_t ev_res, ev_sh[2];  // split shading: resolve done; the first half's span
+    // split shading: resolve done, the shadow passes done, the AO's span
+    hipEvent_t ev_res, ev_sdone, ev_sh[2];
     uint32_t* col_pre;            // split shading: the colour before AO
     int32_t tile_slots;
     int32_t n_cu;
@@ -1082,6 +1083,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc && (hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&r->ev_res, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&r->ev_sdone, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreate(&r->ev_sh[0]) != hipSuccess || hipEventCreate(&r->ev_sh[1]) != hipSuccess))
         rc = fail(PRH_E_HIP, "hipEventCreate failed");
     if (rc) {
@@ -1131,6 +1133,7 @@ int prh_renderer_free(prh_renderer* r) {
     if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
     if (r->ev_join) (void)hipEventDestroy(r->ev_join);
     if (r->ev_res) (void)hipEventDestroy(r->ev_res);
+    if (r->ev_sdone) (void)hipEventDestroy(r->ev_sdone);
     for (int i = 0; i < 2; i++)
         if (r->ev_sh[i]) (void)hipEventDestroy(r->ev_sh[i]);
     r->sc->live_renderers--;
@@ -1192,8 +1195,9 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     // keeps them on its own stream
     const bool split = shadows && !wide && st == r->ctx->stream && r->tune.overlap_shadows;
     const hipStream_t ss = split ? r->ctx->stream2 : st;
-    // with AO as well, the shading's first half (everything but the AO
-    // factor) runs on the second stream beside k_ao
+    // with AO as well, the AO runs on the second stream (after the shadow
+    // passes) beside the shading's first half (everything but the AO factor)
+    // on this one
     const bool split_shade = split && r->sc->any_ao && r->col_pre;
     if (shadows) {
         const int nt = r->n_slots * r->stiles_per_map;
@@ -1253,17 +1257,21 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     bp.col_pre = r->col_pre;
     r->split_shade_last = split_shade;
     if (split_shade) {
+        // (k_shade there finishes last when it runs on the second stream)
+        (void)hipEventRecord(r->ev_sdone, ss);
         (void)hipEventRecord(r->ev_res, st);
         (void)hipStreamWaitEvent(ss, r->ev_res, 0);
         if (timed) (void)hipEventRecord(r->ev_sh[0], ss);
-        launch_shade(ss, d, r->sh, bp);
+        launch_ao(ss, d, r->sh, b);
         if (timed) (void)hipEventRecord(r->ev_sh[1], ss);
         (void)hipEventRecord(r->ev_join, ss);
+        (void)hipStreamWaitEvent(st, r->ev_sdone, 0);  // the shading reads the shadow maps
+        launch_shade(st, d, r->sh, bp);
     }
     wide_stage(13);
-    if (r->sc->any_ao) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
+    if (r->sc->any_ao && !split_shade) launch_ao(st, d, r->sh, b);  // else no pixel reads an AO factor
     if (timed) (void)hipEventRecord(r->ev[5], st);
-    if (split) (void)hipStreamWaitEvent(st, r->ev_join, 0);  // the shading reads the shadow maps
+    if (split) (void)hipStreamWaitEvent(st, r->ev_join, 0);  // the shadow maps (and the AO factors)
     wide_stage(14);
     launch_shade_bg(st, d, r->sh, b);
     if (timed) (void)hipEventRecord(r->ev[7], st);
@@ -1869,10 +1877,9 @@ void fill_times(prh_renderer* r) {
     if (r->n_slots > 0 && !r->times_nan_rerun) (void)hipEventElapsedTime(&S.ms_shadow, r->ev_s[0], r->ev_s[1]);
     S.ms_vertex = ms[1], S.ms_setup = ms[2], S.ms_raster = ms[3], S.ms_ao = ms[4];
     (void)hipEventElapsedTime(&S.ms_shade, r->ev[7], r->ev[6]);  // k_shade alone
-    if (r->split_shade_last) {  // plus the first half, beside k_ao
-        float pre = 0.0f;
-        (void)hipEventElapsedTime(&pre, r->ev_sh[0], r->ev_sh[1]);
-        S.ms_shade += pre;
+    if (r->split_shade_last) {  // the AO on the second stream; the shading's first half beside it
+        (void)hipEventElapsedTime(&S.ms_ao, r->ev_sh[0], r->ev_sh[1]);
+        S.ms_shade += ms[4];
     }
 }
 
