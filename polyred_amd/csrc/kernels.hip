@@ -240,15 +240,13 @@ struct k_frame_clear_args {
     size_t n_vis;
     int32_t gx, gy;  // the grid of this launch
 };
+__device__ __forceinline__ void frame_clear_part(const FrameClear& c, size_t i0, size_t stride) {
+    if (i0 < (size_t)c.n_cnt) c.cnt[i0] = 0u;
+    for (size_t i = i0; i < c.n_vis; i += stride) c.vis[i] = VIS_EMPTY;
+}
 __device__ __forceinline__ void k_frame_clear_body(const k_frame_clear_args& a_, const VB vb) {
-    uint32_t* cnt = a_.cnt;
-    const int32_t n_cnt = a_.n_cnt;
-    uint64_t* vis = a_.vis;
-    const size_t n_vis = a_.n_vis;
-
-    const size_t i0 = vb.bx * (size_t)blockDim.x + threadIdx.x;
-    if (i0 < (size_t)n_cnt) cnt[i0] = 0u;
-    for (size_t i = i0; i < n_vis; i += (size_t)vb.gx * blockDim.x) vis[i] = VIS_EMPTY;
+    frame_clear_part(FrameClear{a_.cnt, a_.n_cnt, a_.vis, a_.n_vis}, vb.bx * (size_t)blockDim.x + threadIdx.x,
+                     (size_t)vb.gx * blockDim.x);
 }
 __global__ __launch_bounds__(256) void k_frame_clear(k_frame_clear_args a) {
     k_frame_clear_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
@@ -277,9 +275,11 @@ struct k_vertex_main_args {
     float4* tv_wp;
     uint32_t* tv_col;
     float2* tv_uv;
+    FrameClear clr;  // the frame's start, done by this pass
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_vertex_main_body(const k_vertex_main_args& a_, const VB vb) {
+    frame_clear_part(a_.clr, vb.bx * (size_t)blockDim.x + threadIdx.x, (size_t)vb.gx * blockDim.x);
     const DevScene& s = a_.s;
     const M4* __restrict__ trans = a_.trans;
     const M4* __restrict__ normal = a_.normal;
@@ -326,9 +326,11 @@ struct k_vertex_shadow_args {
     const M4* strans;
     ShadowConsts sc;
     float4* ts;
+    FrameClear clr;  // the shadow passes' counters, cleared by this pass
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_vertex_shadow_body(const k_vertex_shadow_args& a_, const VB vb) {
+    if (vb.by == 0) frame_clear_part(a_.clr, vb.bx * (size_t)blockDim.x + threadIdx.x, (size_t)vb.gx * blockDim.x);
     const DevScene& s = a_.s;
     const M4* __restrict__ strans = a_.strans;
     const ShadowConsts& sc = a_.sc;
@@ -2498,16 +2500,25 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u64, dim3(std::min<size_t>(blocks_for(n, 256), 4096)), dim3(256), 0, st, p, n, v);
 }
-void launch_vertex_main(hipStream_t st, const DevScene& s, const M4* trans, const M4* normal, const ViewConsts& vc,
-                        float4* tv_scr, float4* tv_nor, float4* tv_wp, uint32_t* tv_col, float2* tv_uv) {
-    if (!s.n_tverts) return;
-    go(k_vertex_main, k_vertex_main_wide, dim3(blocks_for(s.n_tverts, 256)), 256, st,
-       k_vertex_main_args{s, trans, normal, vc, tv_scr, tv_nor, tv_wp, tv_col, tv_uv, 0, 0});
+// The clear on its own where the vertex pass has nothing to launch (a group
+// of its own in a recorded batch, beside the views' vertex passes).
+static void clear_alone(hipStream_t st, const FrameClear& clr) {
+    if (!clr.n_cnt && !clr.n_vis) return;
+    if (g_wide) g_wide->sub = 1;
+    launch_frame_clear(st, clr.cnt, clr.n_cnt, clr.vis, clr.n_vis);
 }
-void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float4* ts) {
-    if (!s.n_tverts || !sc.n_slots) return;
+void launch_vertex_main(hipStream_t st, const DevScene& s, const M4* trans, const M4* normal, const ViewConsts& vc,
+                        float4* tv_scr, float4* tv_nor, float4* tv_wp, uint32_t* tv_col, float2* tv_uv,
+                        const FrameClear& clr) {
+    if (!s.n_tverts) return clear_alone(st, clr);
+    go(k_vertex_main, k_vertex_main_wide, dim3(blocks_for(s.n_tverts, 256)), 256, st,
+       k_vertex_main_args{s, trans, normal, vc, tv_scr, tv_nor, tv_wp, tv_col, tv_uv, clr, 0, 0});
+}
+void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float4* ts,
+                          const FrameClear& clr) {
+    if (!s.n_tverts || !sc.n_slots) return clear_alone(st, clr);
     go(k_vertex_shadow, k_vertex_shadow_wide, dim3(blocks_for(s.n_tverts, 256), sc.n_slots), 256, st,
-       k_vertex_shadow_args{s, strans, sc, ts, 0, 0});
+       k_vertex_shadow_args{s, strans, sc, ts, clr, 0, 0});
 }
 int setup_blocks(int32_t n_tris) { return (n_tris + 255) / 256; }
 int clip_grid(const Capacities& cap) { return (int)std::min<size_t>(blocks_for(cap.clip_tris, CLIP_WAVES), CLIP_GRID); }

@@ -123,11 +123,20 @@ constexpr uint32_t BUF_DWORD3 = 0x00020000u;
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
 void launch_frame_clear(hipStream_t st, uint32_t* cnt, int32_t n_cnt, uint64_t* vis, size_t n_vis);
+// A frame's start: counters to zero, the visibility buffer emptied.  Each
+// stream's first pass (the vertex passes) does its part before its own
+// work, with k_frame_clear only where that pass has nothing to launch.
+struct FrameClear {
+    uint32_t* cnt;
+    int32_t n_cnt;
+    uint64_t* vis;
+    size_t n_vis;
+};
 void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, const gm::M4* normal,
                         const ViewConsts& vc, float4* tv_scr, float4* tv_nor, float4* tv_wp, uint32_t* tv_col,
-                        float2* tv_uv);
+                        float2* tv_uv, const FrameClear& clr);
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
-                          float4* ts);
+                          float4* ts, const FrameClear& clr);
 // The setup's workgroups (each writes its unclipped primitives to
 // blk_prims[its index]) and then the clipper's (blk_prims[setup_blocks +
 // its index]: its pieces); blk_prims holds setup_blocks(n_tris) + CLIP_GRID.

@@ -1186,10 +1186,9 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     const size_t P = (size_t)r->W * r->H;
     // counters zeroed, visibility emptied (before the setup, which draws
     // small triangles into it, outside the exact-order pass, whose NaN rule
-    // needs every fragment through the ordered raster)
-    wide_stage(1);
-    launch_frame_clear(st, (uint32_t*)r->cnt, (int32_t)(2 * sizeof(Counters) / 4), (uint64_t*)r->vis, P);
+    // needs every fragment through the ordered raster): by the vertex passes
     const bool shadows = r->n_slots > 0 && !nan_mode;
+    constexpr int32_t CNT_WORDS = (int32_t)(sizeof(Counters) / 4);
     // a plain frame runs its shadow passes on the context's second stream,
     // beside the main view's passes (they meet at the shading); a batch
     // keeps them on its own stream
@@ -1207,7 +1206,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
         }
         if (timed) (void)hipEventRecord(r->ev_s[0], ss);
         wide_stage(2);
-        launch_vertex_shadow(ss, d, r->d_strans, r->shc, r->ts);
+        launch_vertex_shadow(ss, d, r->d_strans, r->shc, r->ts, FrameClear{(uint32_t*)cs, CNT_WORDS, nullptr, 0});
         wide_stage(3);
         launch_setup_shadow(ss, d, r->shc, r->tri_rec, r->ts, r->s_prims, r->s_pbox, r->s_tile_count, cs, r->cap_shadow,
                             r->smaps);
@@ -1226,7 +1225,9 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     }
     if (timed) (void)hipEventRecord(r->ev[1], st);
     wide_stage(7);
-    launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp, r->tv_col, r->tv_uv);
+    // (both counter blocks where no shadow pass runs)
+    launch_vertex_main(st, d, r->d_trans, r->d_normal, r->vc, r->tv_scr, r->tv_nor, r->tv_wp, r->tv_col, r->tv_uv,
+                       FrameClear{(uint32_t*)cm, (shadows ? 1 : 2) * CNT_WORDS, (uint64_t*)r->vis, P});
     if (timed) (void)hipEventRecord(r->ev[2], st);
     wide_stage(8);
     launch_setup_main(st, d, r->vc, r->tv_scr, r->tv_nor, r->prims, r->pbox, r->clips, r->tri_clip_slot, r->clip_list,
