@@ -3,7 +3,8 @@
 and shininess y, it must give the float32 rounding of the float64 pow bit for
 bit -- the series are evaluated to a relative error below 2^-40 and values
 within that of a float32 rounding midpoint go to the library pow.  Checked
-here against the float64 pow of the C library on random and boundary inputs,
+here against the float64 pow of the C library on random and boundary inputs
+(mantissas on and beside binade and interval edges, x near 1 with large y),
 with gomath.hpp built for the host by g++ (no GPU)."""
 import ctypes as C
 import os
@@ -59,6 +60,35 @@ def test_random(lib):
     bad = np.nonzero(got.view(np.uint32) != ref.view(np.uint32))[0]
     assert bad.size == 0, (x[bad[:5]], y[bad[:5]], got[bad[:5]], ref[bad[:5]])
     assert (ref == 0).mean() > 0.01 and ((ref > 0) & (ref < 2.0 ** -126)).any()  # zero and subnormal results
+
+
+def test_large_exponents_and_bases_near_one(lib):
+    # the error budget's worst case: y up to 2048 amplifies log2's error, and
+    # x near 1 puts log2 at its smallest
+    rng = np.random.default_rng(5)
+    n = 4_000_000
+    x = (1.0 - rng.uniform(0, 2.0 ** -6, n)).astype(np.float32)
+    x[: n // 2] = rng.uniform(0.5, 1.0, n // 2).astype(np.float32)
+    y = rng.uniform(256, 2048, n).astype(np.float32)
+    y[: n // 4] = np.round(y[: n // 4])  # integer shininess
+    got, ref = run(lib, x, y), want(x, y)
+    bad = np.nonzero(got.view(np.uint32) != ref.view(np.uint32))[0]
+    assert bad.size == 0, (x[bad[:5]], y[bad[:5]], got[bad[:5]], ref[bad[:5]])
+
+
+def test_mantissa_edges(lib):
+    # bases on and beside 64 evenly spaced mantissas of several binades (the
+    # sqrt(1/2) split of the mantissa included among their neighbours)
+    m = 1.0 + np.arange(65) / 64.0
+    base = np.concatenate([m * 2.0 ** -k for k in (1, 2, 7, 20)]).astype(np.float32)
+    base = base[base <= 1.0]
+    xs = np.concatenate([base, np.nextafter(base, np.float32(0)), np.nextafter(base, np.float32(2))])
+    xs = xs[(xs > 0) & (xs <= 1.0)].astype(np.float32)
+    ys = np.array([0.00390625, 0.3, 1.0, 2.5, 8.0, 33.3, 100.0, 517.0, 2048.0], np.float32)
+    x = np.repeat(xs, ys.size)
+    y = np.tile(ys, xs.size)
+    got, ref = run(lib, x, y), want(x, y)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
 def test_special_and_outside(lib):
