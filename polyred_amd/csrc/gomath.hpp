@@ -71,6 +71,11 @@ GM_HD double pow64(double x, double y) {
 // library pow decides, so the result is always (float)pow64(x, y).
 GM_HD float pow32_fast(float xf, float yf) {
     const double x = xf, y = yf;
+    // pow(+0, y) = +0 for every y > 0 (the specular term of a light behind
+    // the surface's half vector), without the library
+    uint32_t xb;
+    __builtin_memcpy(&xb, &xf, 4);
+    if (xb == 0u && yf > 0.0f) return 0.0f;
     if (!(xf > 0.0f && xf <= 1.0f && yf >= 0.00390625f && yf <= 2048.0f)) return (float)pow64(x, y);
     // x = m 2^e, m in [sqrt(1/2), sqrt(2))
     int e;

@@ -163,7 +163,8 @@ __device__ __forceinline__ void store_prim(Prim* prims, uint2* pbox, size_t slot
     pbox[slot] = make_uint2(pack_bbox(x0, y0), pack_bbox(x1, y1));
 }
 
-// Sutherland-Hodgman against the six screen-space planes, clipping.go:31-65.
+// The six screen-space planes of Sutherland-Hodgman, clipping.go:31-65
+// (the clipper itself is clip_wave below).
 struct Plane {
     V4 pos, nor;
 };
@@ -175,32 +176,6 @@ __device__ __forceinline__ V4 isect(const Plane& p, V4 v0, V4 v1) {
     float s = n / d;
     return add(v0, scale(u, s, s, s, s));
 }
-__device__ int sutherland_hodgman(V4* a, float w, float h) {
-    Plane planes[6] = {{v4(w, 0, 0, 1), v4(-1, 0, 0, 1)}, {v4(0, 0, 0, 1), v4(1, 0, 0, 1)},
-                       {v4(0, h, 0, 1), v4(0, -1, 0, 1)}, {v4(0, 0, 0, 1), v4(0, 1, 0, 1)},
-                       {v4(0, 0, 1, 1), v4(0, 0, -1, 1)}, {v4(0, 0, -1, 1), v4(0, 0, 1, 1)}};
-    V4 b[12];
-    int na = 3;
-    for (int pi = 0; pi < 6; pi++) {
-        if (na == 0) return 0;
-        int nb = 0;
-        V4 s = a[na - 1];
-        for (int i = 0; i < na; i++) {
-            V4 e = a[i];
-            if (in_front(planes[pi], e)) {
-                if (!in_front(planes[pi], s)) b[nb++] = isect(planes[pi], s, e);
-                b[nb++] = e;
-            } else if (in_front(planes[pi], s)) {
-                b[nb++] = isect(planes[pi], s, e);
-            }
-            s = e;
-        }
-        for (int i = 0; i < nb; i++) a[i] = b[i];
-        na = nb;
-    }
-    return na;
-}
-
 __device__ __forceinline__ uint32_t u8x4(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
     return (uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | ((uint32_t)a << 24);
 }
@@ -1620,22 +1595,25 @@ __device__ bool fragment_shader(const DevScene& s, const ShadeConsts& c, const D
 // number of lights for which shadingVisibility reports true.
 // apply() of a matrix whose last row is (0 0 0 1), for a finite v: the zero
 // terms of the last row's fma64 chain add zeros, so its result is v.w (up to
-// the sign of a zero).  apply_diag: also zeros off the diagonal of the upper
-// 3x3 (orthographic projections, viewports), each row one fma64.  Nothing
-// the shadow test computes from these vectors tells the sign of a zero: pos()
-// tests w == 0 before dividing, and int() and the depth comparison treat
-// -0 as 0.
-__device__ __forceinline__ V4 apply_last_row_w(V4 v, const M4& m) {
-    const float* a = m.a;
-    return v4(fma64(a[0], v.x, fma64(a[1], v.y, fma64(a[2], v.z, a[3] * v.w))),
-              fma64(a[4], v.x, fma64(a[5], v.y, fma64(a[6], v.z, a[7] * v.w))),
-              fma64(a[8], v.x, fma64(a[9], v.y, fma64(a[10], v.z, a[11] * v.w))), v.w);
-}
-__device__ __forceinline__ V4 apply_diag(V4 v, const M4& m) {
-    const float* a = m.a;
-    return v4(fma64(a[0], v.x, a[3] * v.w), fma64(a[5], v.y, a[7] * v.w), fma64(a[10], v.z, a[11] * v.w), v.w);
-}
+// the sign of a zero).  With zeros off the diagonal of the upper 3x3 as well
+// (orthographic projections, viewports) each row is one fma64.  Nothing the
+// shadow test computes from these vectors tells the sign of a zero: pos()
+// tests w == 0 before dividing, and int() and the depth comparison treat -0
+// as 0.  The coefficients come from a ShadowXf: fma64(a, v, c) with a already
+// float64 ((double)a is exact, so the result is the same) and the vector's
+// components converted once.
 __device__ __forceinline__ bool finite4(V4 v) { return fin(v.x) && fin(v.y) && fin(v.z) && fin(v.w); }
+__device__ __forceinline__ float fma64d(double a, double v, float c) { return (float)__builtin_fma(a, v, (double)c); }
+__device__ __forceinline__ V4 apply_view_xf(V4 v, const ShadowXf& m) {
+    const double x = v.x, y = v.y, z = v.z;
+    const double* a = m.view;
+    return v4(fma64d(a[0], x, fma64d(a[1], y, fma64d(a[2], z, m.view_c[0] * v.w))),
+              fma64d(a[3], x, fma64d(a[4], y, fma64d(a[5], z, m.view_c[1] * v.w))),
+              fma64d(a[6], x, fma64d(a[7], y, fma64d(a[8], z, m.view_c[2] * v.w))), v.w);
+}
+__device__ __forceinline__ V4 apply_diag_xf(V4 v, const double* d, const float* c) {
+    return v4(fma64d(d[0], v.x, c[0] * v.w), fma64d(d[1], v.y, c[1] * v.w), fma64d(d[2], v.z, c[2] * v.w), v.w);
+}
 
 // 24-bit multiplies as the one full-rate instruction.  __umul24/__mul24 in a
 // loop whose other operand is loop-invariant compile to a mask hoisted out of
@@ -1659,6 +1637,10 @@ __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuff
     const float fSW = (float)c.SW, fSH = (float)c.SH;
     // the screen-to-world step is the same for every light
     const V4 qw = apply(v4((float)fx, (float)fy, depth, 1), c.v.s2w);
+    // pos()'s reciprocal of w, the same for every light whose transforms
+    // keep w (the sparse shape with finite values)
+    const bool qw_unit = qw.w == 1 || qw.w == 0;
+    const float qw_inv = 1.0f / qw.w;
     for (int i = 0; i < s.n_lights; i++) {
         const DevLight& l = s.lights[i];
         if (!l.cast_shadow) {
@@ -1667,9 +1649,17 @@ __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuff
         }
         V4 q;
         if (c.shadow_sparse && finite4(qw)) {
-            q = apply_last_row_w(qw, b.light_view[i]);
-            q = finite4(q) ? apply_diag(q, b.light_proj[i]) : apply(q, b.light_proj[i]);
-            q = pos(finite4(q) ? apply_diag(q, c.svp) : apply(q, c.svp));
+            const ShadowXf& xf = b.light_xf[i];
+            q = apply_view_xf(qw, xf);
+            if (finite4(q)) q = apply_diag_xf(q, xf.proj, xf.proj_c);
+            else q = apply(q, b.light_proj[i]);
+            if (finite4(q)) {
+                // w is still qw's: pos() with the pixel's one reciprocal
+                q = apply_diag_xf(q, xf.svp, xf.svp_c);
+                q = qw_unit ? v4(q.x, q.y, q.z, 1) : v4(q.x * qw_inv, q.y * qw_inv, q.z * qw_inv, 1);
+            } else {
+                q = pos(apply(q, c.svp));
+            }
         } else {
             q = apply(qw, b.light_view[i]);
             q = apply(q, b.light_proj[i]);

@@ -19,6 +19,21 @@ struct ShadeConsts {
                               // have zeros everywhere but the diagonal and the last column
 };
 
+// A shadow-casting light's transforms for the shading's shadow test where
+// they have the sparse shape (light view's last row (0 0 0 1), projection and
+// map viewport diagonal plus last column): the coefficients that multiply a
+// coordinate in an fma64 held as float64 (exact), so the test converts none
+// of them, and the last-column terms it multiplies in float32 as float.
+struct ShadowXf {
+    double view[9];   // light view rows 0-2, columns 0-2
+    double proj[3];   // light projection diagonal
+    double svp[3];    // map viewport diagonal
+    float view_c[3];  // light view column 3, rows 0-2
+    float proj_c[3];
+    float svp_c[3];
+    float pad[3];
+};
+
 struct ShadeBuffers {
     const uint64_t* vis;
     const float* depth;
@@ -33,6 +48,7 @@ struct ShadeBuffers {
     const uint32_t* smaps;     // [slots][SH][SW] ordered depth bits
     const gm::M4* light_view;  // [n_lights]
     const gm::M4* light_proj;  // [n_lights]
+    const ShadowXf* light_xf;  // [n_lights], read where ShadeConsts.shadow_sparse is set
     uint32_t* image;           // [H][W] RGBA8, top row first
     uint32_t* pregamma;        // optional [H][W] raster rows
     float* linear;             // optional [H][W][3]

@@ -408,6 +408,7 @@ struct prh_renderer {
     ShadeConsts sh;
     std::vector<void*> allocs;
     M4 *d_trans, *d_normal, *d_strans, *d_lview, *d_lproj;
+    ShadowXf* d_lxf;
     float4 *tv_scr, *tv_nor, *tv_wp, *ts;
     int4* tri_rec;     // per triangle: transformed vertices and flat material (once, k_tri_rec)
     uint32_t* tv_col;  // per transformed vertex: colour
@@ -1010,6 +1011,21 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) r->allocs.push_back(r->d_lview);
     if (!rc) chk(upload(&r->d_lproj, lproj.data(), lproj.size()));
     if (!rc) r->allocs.push_back(r->d_lproj);
+    {
+        std::vector<ShadowXf> lxf(std::max(1, L));
+        memset(lxf.data(), 0, lxf.size() * sizeof(ShadowXf));
+        for (int i = 0; i < L; i++) {
+            ShadowXf& x = lxf[i];
+            for (int row = 0; row < 3; row++) {
+                for (int col = 0; col < 3; col++) x.view[3 * row + col] = lview[i].a[4 * row + col];
+                x.view_c[row] = lview[i].a[4 * row + 3];
+                x.proj[row] = lproj[i].a[5 * row], x.proj_c[row] = lproj[i].a[4 * row + 3];
+                x.svp[row] = sh.svp.a[5 * row], x.svp_c[row] = sh.svp.a[4 * row + 3];
+            }
+        }
+        if (!rc) chk(upload(&r->d_lxf, lxf.data(), lxf.size()));
+        if (!rc) r->allocs.push_back(r->d_lxf);
+    }
     if (!rc) chk(upload(&r->ao_tab, aotab.data(), aotab.size()));
     if (!rc) r->allocs.push_back(r->ao_tab);
     if (!rc) chk(ralloc(r, &r->tv_scr, Vt));
@@ -1156,7 +1172,7 @@ ShadeBuffers shade_buffers(const prh_renderer* r) {
     b.vis = r->vis, b.depth = r->depth, b.tv_scr = r->tv_scr, b.tv_nor = r->tv_nor, b.tv_wp = r->tv_wp;
     b.tri_rec = r->tri_rec, b.tv_col = r->tv_col, b.tv_uv = r->tv_uv;
     b.clips = r->clips, b.tri_clip_slot = r->tri_clip_slot, b.smaps = r->smaps;
-    b.light_view = r->d_lview, b.light_proj = r->d_lproj;
+    b.light_view = r->d_lview, b.light_proj = r->d_lproj, b.light_xf = r->d_lxf;
     b.image = r->msaa > 1 ? r->image_big : final_image(r), b.pregamma = r->pregamma, b.linear = r->linear;
     b.bg = r->bg, b.bg_lin = r->bg_lin;
     b.cnt = r->cnt;
