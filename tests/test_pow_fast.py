@@ -93,8 +93,9 @@ def test_mantissa_edges(lib):
 
 def test_special_and_outside(lib):
     x = np.array([1.0, 1.0, 0.0, 0.0, 0.5, 0.5, 0.5, 1e-30, np.nextafter(np.float32(1), np.float32(0)), 0.7, 0.7,
-                  0.7, np.nan, 0.5], np.float32)
-    y = np.array([32.0, 0.5, 5.0, 0.0, 0.0, 1.0, 150.0, 5.0, 2048.0, 3000.0, 1e-5, -2.0, 2.0, np.nan], np.float32)
+                  0.7, np.nan, 0.5, 0.0, 0.0, 0.0, -0.0, -0.0, -0.0, 0.0, -0.0], np.float32)
+    y = np.array([32.0, 0.5, 5.0, 0.0, 0.0, 1.0, 150.0, 5.0, 2048.0, 3000.0, 1e-5, -2.0, 2.0, np.nan, 1e-9, 3.0,
+                  np.inf, 3.0, 2.0, 0.5, -1.0, -3.0], np.float32)
     got, ref = run(lib, x, y), want(x, y)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (got, ref)
 
