@@ -292,13 +292,30 @@ GM_HD bool inside(const float bc[3]) {  // render/raster.go:491
 // exactly, the product of M (25 bits) and Sabc (24 bits) being exact in
 // float64.  Other Sabc take the divisions.
 constexpr double BC_EDGE_MID = -0x1.ad7f2bp-24;
+// IEEE minNum / maxNum: a NaN operand gives the other one
+GM_HD float gm_fmin(float a, float b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_fminf(a, b);
+#else
+    return std::fmin(a, b);
+#endif
+}
+GM_HD float gm_fmax(float a, float b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_fmaxf(a, b);
+#else
+    return std::fmax(a, b);
+#endif
+}
 GM_HD bool cover_z(const TriSetup& t, float px, float py, float az, float bz, float cz, float& z) {
     float s[3];
     tri_areas(t, px, py, s);
     if (t.sabc != 0.0f && t.sabc - t.sabc == 0.0f) {  // finite, non-zero
+        // (some S beyond M * Sabc: the least (greatest) of the three beyond
+        // it, NaNs left out by fmin/fmax as by the comparisons)
         const double ms = BC_EDGE_MID * (double)t.sabc;
-        const bool out = t.sabc > 0.0f ? ((double)s[0] < ms) | ((double)s[1] < ms) | ((double)s[2] < ms)
-                                       : ((double)s[0] > ms) | ((double)s[1] > ms) | ((double)s[2] > ms);
+        const bool out = t.sabc > 0.0f ? (double)gm_fmin(gm_fmin(s[0], s[1]), s[2]) < ms
+                                       : (double)gm_fmax(gm_fmax(s[0], s[1]), s[2]) > ms;
         if (out) return false;
     }
     const float bc[3] = {s[0] / t.sabc, s[1] / t.sabc, s[2] / t.sabc};

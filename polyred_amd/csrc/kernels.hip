@@ -16,6 +16,20 @@ namespace prh {
 
 namespace {
 
+// 24-bit multiplies as the one full-rate instruction.  __umul24/__mul24 in a
+// loop whose other operand is loop-invariant compile to a mask hoisted out of
+// the loop and a quarter-rate v_mul_lo_u32 inside it.
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ int32_t mul_i24(int32_t a, int32_t b) {
+    int32_t r;
+    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ V4 ld(float4 a) { return v4(a.x, a.y, a.z, a.w); }
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
@@ -353,11 +367,38 @@ __device__ __forceinline__ void block_scan_256(int v, int32_t* start /*[257]*/, 
     __syncthreads();
 }
 
-// Small triangles drawn by the setup workgroups themselves: each thread's
-// candidate pixel centres (its pixel box) are spread over the 256 threads of
-// the workgroup, each centre tested as the rasters test it (the same
-// Barycoord, inside test and depth, raster.go:487-496) and handed to `put`
-// with its triangle's staged record.
+// Inclusive scan of one int per lane over the wave (sum or max, values >= 0)
+// in DPP steps: Hillis-Steele over each row of 16 lanes (row_shr 1, 2, 4,
+// 8; a lane past its row's start reads 0, the identity of both), then the
+// last lane of row 0 and of row 2 into rows 1 and 3 (row_bcast:15), and the
+// last lane of row 1 into rows 2 and 3 (row_bcast:31).
+template <bool MAX>
+__device__ __forceinline__ int wave_scan_incl(int v) {
+    auto op = [](int a, int b) { return MAX ? max(a, b) : a + b; };
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+// LDS written by some lanes of a wave and read by others: visible past a
+// wave-level fence (no workgroup barrier needed)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Small triangles drawn by the setup workgroups themselves: each wave's
+// candidate pixel centres (its lanes' pixel boxes, a wave scan placing
+// them) are spread over its 64 lanes, 64 at a time, each centre tested as
+// the rasters test it (the same Barycoord, inside test and depth,
+// raster.go:487-496) and handed to `put` with its triangle's staged record.
+// A round's owners: each lane whose range starts in the round marks its
+// start slot, and a max-scan of the marks (and of the owner running into
+// the round) gives every slot the last lane starting at or before it.
 struct __align__(16) DPrim {
     float ax, ay, bx, by, abx, aby, acx, acy, bcx, bcy, sabc;  // TriSetup
     float az, bz, cz;
@@ -366,12 +407,14 @@ struct __align__(16) DPrim {
 };
 template <class F>
 __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, int x0, int y0, int x1, int y1,
-                                                DPrim* sp, int32_t* start, int32_t* wsum, F&& put) {
-    const int tid = threadIdx.x;
+                                                DPrim* sp /*[256]*/, int32_t* mark /*[256]*/, F&& put) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    DPrim* wsp = sp + 64 * wv;
+    int32_t* wmark = mark + 64 * wv;
     int c = 0;
     if (mine) {
         const TriSetup t = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
-        DPrim& d = sp[tid];
+        DPrim& d = wsp[lane];
         d.ax = t.ax, d.ay = t.ay, d.bx = t.bx, d.by = t.by, d.abx = t.abx, d.aby = t.aby;
         d.acx = t.acx, d.acy = t.acy, d.bcx = t.bcx, d.bcy = t.bcy, d.sabc = t.sabc;
         d.az = p1.z, d.bz = p2.z, d.cz = p3.z;
@@ -379,20 +422,29 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
         d.xy0 = x0 | (y0 << 16), d.wq = w | ((65536u + w - 1u) / w) << 9;
         c = (x1 - x0 + 1) * (y1 - y0 + 1);
     }
-    block_scan_256(c, start, wsum);
-    const int total = start[256];
-    for (int idx = tid; idx < total; idx += 256) {
-        // the thread whose range holds idx: the last with start <= idx (empty
-        // ranges are skipped), in eight fixed steps
-        int lo = 0;
-#pragma unroll
-        for (int step = 128; step >= 1; step >>= 1)
-            if (start[lo + step] <= idx) lo += step;
-        const DPrim d = sp[lo];
-        const int l = idx - start[lo], w = (int)(d.wq & 511u);
+    const int incl = wave_scan_incl<false>(c);
+    const int s0 = incl - c;  // this lane's first candidate
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    int carry = 0;  // 1 + the lane owning the slot before the round
+    for (int base = 0; base < total; base += 64) {
+        wave_lds_sync();  // (the previous round's reads, this round's writes)
+        wmark[lane] = 0;
+        wave_lds_sync();
+        if (c > 0 && s0 >= base && s0 < base + 64) wmark[s0 - base] = lane + 1;
+        wave_lds_sync();
+        int m = wmark[lane];
+        if (lane == 0) m = max(m, carry);
+        m = wave_scan_incl<true>(m);
+        carry = __builtin_amdgcn_readlane(m, 63);
+        const int idx = base + lane;
+        const int o = m - 1;
+        const int so = __builtin_amdgcn_ds_bpermute(max(o, 0) * 4, s0);
+        if (idx >= total || o < 0) continue;
+        const DPrim d = wsp[o];
+        const int l = idx - so, w = (int)(d.wq & 511u);
         // l / w by multiply-shift: exact while l * w < 65536, which l < area <= 256
         // and w <= area keep (the thresholds are at most 256)
-        const int q = (int)(((uint32_t)l * (d.wq >> 9)) >> 16);
+        const int q = (int)(mul_u24((uint32_t)l, d.wq >> 9) >> 16);
         const int px = (d.xy0 & 0xffff) + (l - q * w), py = (d.xy0 >> 16) + q;
         TriSetup t;
         t.ax = d.ax, t.ay = d.ay, t.bx = d.bx, t.by = d.by, t.abx = d.abx, t.aby = d.aby;
@@ -564,12 +616,13 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
     if (small) emit = false;
     {
         __shared__ DPrim sp[256];
-        __shared__ int32_t start[257], wsum[4];
+        __shared__ int32_t mark[256];
         uint32_t* map = smaps + (size_t)slot * sc.SW * sc.SH;
         bool nan = false;
-        draw_small_tris(small, p1, p2, p3, x0, y0, x1, y1, sp, start, wsum, [&](int px, int py, float z) {
+        draw_small_tris(small, p1, p2, p3, x0, y0, x1, y1, sp, mark, [&](int px, int py, float z) {
             if (z != z) nan = true;
-            else if (z > 0.0f) atomicMax(&map[(size_t)py * sc.SW + px], depth_ord(z));  // the map starts at 0
+            else if (z > 0.0f && (unsigned)px < (unsigned)sc.SW && (unsigned)py < (unsigned)sc.SH)  // (inside by construction)
+                atomicMax(&map[(size_t)py * sc.SW + px], depth_ord(z));  // the map starts at 0
         });
         if (nan) atomicOr(&cnt->shadow_nan, 1);
     }
@@ -917,12 +970,6 @@ __global__ __launch_bounds__(256) void k_bin_wide(const k_bin_args* __restrict__
 constexpr int CLIP_WAVES = 4;   // waves per workgroup (a lane's fan piece needs up to 256 VGPRs)
 constexpr int CLIP_VERTS = 12;  // polygon capacity (a triangle clipped by six planes keeps at most 9)
 
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ int clip_wave(int qi, const ClipJob& jb, const DevScene& s, const ViewConsts& vc,
                                           const float4* __restrict__ tv_scr, const float4* __restrict__ tv_nor,
                                           Prim* __restrict__ prims,
@@ -1155,7 +1202,7 @@ __device__ __forceinline__ void candidate(const int32_t* start, const LPrim* sp,
     int l = idx - start[j];
     uint32_t r = sp[j].rect;
     int w = ((r >> 10) & 31) + 1;
-    int q = (int)(((uint32_t)l * (r >> 15)) >> 16);
+    int q = (int)(mul_u24((uint32_t)l, r >> 15) >> 16);
     lx = (int)(r & 31) + (l - q * w);
     ly = (int)((r >> 5) & 31) + q;
 }
@@ -1615,19 +1662,6 @@ __device__ __forceinline__ V4 apply_diag_xf(V4 v, const double* d, const float* 
     return v4(fma64d(d[0], v.x, c[0] * v.w), fma64d(d[1], v.y, c[1] * v.w), fma64d(d[2], v.z, c[2] * v.w), v.w);
 }
 
-// 24-bit multiplies as the one full-rate instruction.  __umul24/__mul24 in a
-// loop whose other operand is loop-invariant compile to a mask hoisted out of
-// the loop and a quarter-rate v_mul_lo_u32 inside it.
-__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ int32_t mul_i24(int32_t a, int32_t b) {
-    int32_t r;
-    asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 
 __device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int fx, int fy, float depth) {
     int occ = 0;
