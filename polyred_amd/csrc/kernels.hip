@@ -451,7 +451,7 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
         t.acx = d.acx, t.acy = d.acy, t.bcx = d.bcx, t.bcy = d.bcy, t.sabc = d.sabc;
         float z;
         if (!cover_z(t, (float)px + 0.5f, (float)py + 0.5f, d.az, d.bz, d.cz, z)) continue;
-        put(px, py, z);
+        put(px, py, z, o);
     }
 }
 
@@ -521,22 +521,22 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
             }
         }
         const uint32_t key = (uint32_t)g * 8u + SUB_DIRECT;
-        if (direct && draw_small > 0 && (x1 - x0 + 1) * (y1 - y0 + 1) <= draw_small) {
-            // (a thread draws its own: the main view's boxes vary little)
-            drawn = true;
-            const TriSetup ts = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
-            bool nan = false;
-            for (int py = y0; py <= y1; py++)
-                for (int px = x0; px <= x1; px++) {
-                    float z;
-                    if (!cover_z(ts, (float)px + 0.5f, (float)py + 0.5f, p1.z, p2.z, p3.z, z)) continue;
-                    if (z != z) nan = true;
-                    else atomicMin(&vis[(size_t)py * vc.W + px], ((unsigned long long)depth_key_hi(z) << 32) | key);
-                }
-            if (nan) atomicOr(&cnt->nan_flag, 1);
-        }
+        drawn = direct && draw_small > 0 && (x1 - x0 + 1) * (y1 - y0 + 1) <= draw_small;
         if (direct && !drawn) store_prim(prims, pbox, g, p1, p2, p3, key, x0, y0, x1, y1);
         else store_empty(pbox, g);
+    }
+    {
+        __shared__ DPrim sp[256];
+        __shared__ int32_t mark[256];
+        const uint32_t key0 = (uint32_t)(g - (int)(threadIdx.x & 63)) * 8u + SUB_DIRECT;  // the wave's first triangle
+        bool nan = false;
+        draw_small_tris(drawn, p1, p2, p3, x0, y0, x1, y1, sp, mark, [&](int px, int py, float z, int o) {
+            if (z != z) nan = true;
+            else if ((unsigned)px < (unsigned)vc.W && (unsigned)py < (unsigned)vc.H)  // (inside by construction)
+                atomicMin(&vis[(size_t)py * vc.W + px],
+                          ((unsigned long long)depth_key_hi(z) << 32) | (key0 + 8u * (uint32_t)o));
+        });
+        if (nan) atomicOr(&cnt->nan_flag, 1);
     }
     count_tiles(tile_count, 0, vc.rtiles_x, vc.rt_shift, direct && !drawn, x0, y0, x1, y1);
     const int pos = wave_alloc(&cnt->n_clipped, 0, doclip);
@@ -619,7 +619,7 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
         __shared__ int32_t mark[256];
         uint32_t* map = smaps + (size_t)slot * sc.SW * sc.SH;
         bool nan = false;
-        draw_small_tris(small, p1, p2, p3, x0, y0, x1, y1, sp, mark, [&](int px, int py, float z) {
+        draw_small_tris(small, p1, p2, p3, x0, y0, x1, y1, sp, mark, [&](int px, int py, float z, int) {
             if (z != z) nan = true;
             else if (z > 0.0f && (unsigned)px < (unsigned)sc.SW && (unsigned)py < (unsigned)sc.SH)  // (inside by construction)
                 atomicMax(&map[(size_t)py * sc.SW + px], depth_ord(z));  // the map starts at 0
@@ -2111,14 +2111,21 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                 // rounding of fl(X + fl(dir * t)) (off by far less than the
                 // 0.1 % margin): up to the wave's smallest such bound no lane
                 // needs the mask.
-                int tl = 100;
+                // And from t >= (r + 2) / (|dir| * 0.999) + 2 on, a lane's tap
+                // lies more than a pixel past the border (far beyond the
+                // rounding), where its march has ended: past the wave's
+                // largest such bound no lane has a tap left.
+                int tl = 100, tu = 0;
                 if (need) {
                     const float rx = dx > 0 ? (float)(W - 1 - x) : X, ry = dy > 0 ? (float)(H - 1 - y) : Y;
-                    const float lim = fminf(rx / (__builtin_fabsf(dx) * 1.001f), ry / (__builtin_fabsf(dy) * 1.001f));
+                    const float ax = __builtin_fabsf(dx), ay = __builtin_fabsf(dy);
+                    const float lim = fminf(rx / (ax * 1.001f), ry / (ay * 1.001f));
                     tl = min((int)lim + 1, 100);
+                    const float out = fminf((rx + 2.0f) / (ax * 0.999f), (ry + 2.0f) / (ay * 0.999f));
+                    tu = min((int)out + 2, 100);
                 }
-                for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k));
-                const int tmin = __builtin_amdgcn_readfirstlane(tl);
+                for (int k = 32; k >= 1; k >>= 1) tl = min(tl, __shfl_xor(tl, k)), tu = max(tu, __shfl_xor(tu, k));
+                const int tmin = __builtin_amdgcn_readfirstlane(tl), tmax = __builtin_amdgcn_readfirstlane(tu);
                 const uint32_t lane4 = (uint32_t)li * 4u;
                 for (; ti + 7 < tmin; ti += 8) {
                     const uint64_t bits = ti < 64 ? (uni_lo >> ti) | (ti > 56 ? uni_hi << (64 - ti) : 0ull)
@@ -2144,7 +2151,7 @@ __device__ __forceinline__ void k_ao_body(const k_ao_args& a_, const VB vb) {
                     q_best = __builtin_fmax(q_best, (double)a.e * a.rinv);
                 }
 #pragma unroll 2
-                for (; ti < 100; ti++) {
+                for (; ti < tmax; ti++) {
                     const Tap a = tap(ti);
                     const double q = (double)a.e * a.rinv;
                     q_best = (need & a.in & (q > q_best)) ? q : q_best;
