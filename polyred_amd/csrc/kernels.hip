@@ -391,39 +391,17 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Small triangles drawn by the setup workgroups themselves: each wave's
-// candidate pixel centres (its lanes' pixel boxes, a wave scan placing
-// them) are spread over its 64 lanes, 64 at a time, each centre tested as
-// the rasters test it (the same Barycoord, inside test and depth,
-// raster.go:487-496) and handed to `put` with its triangle's staged record.
-// A round's owners: each lane whose range starts in the round marks its
-// start slot, and a max-scan of the marks (and of the owner running into
-// the round) gives every slot the last lane starting at or before it.
-struct __align__(16) DPrim {
-    float ax, ay, bx, by, abx, aby, acx, acy, bcx, bcy, sabc;  // TriSetup
-    float az, bz, cz;
-    int32_t xy0;  // x0 | y0 << 16
-    uint32_t wq;  // box width w | ceil(65536 / w) << 9
-};
+// Each lane's c items spread over the wave's lanes, 64 a round: a wave
+// scan places them, and per round each lane whose items start in the round
+// marks its start slot (LDS, 64 a wave); a max-scan of the marks (and of
+// the lane running into the round) names each slot's owner.  body(o, l,
+// valid) runs on every lane each round (so that it may gather from other
+// lanes), for item l of lane o where valid.
 template <class F>
-__device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, int x0, int y0, int x1, int y1,
-                                                DPrim* sp /*[256]*/, int32_t* mark /*[256]*/, F&& put) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    DPrim* wsp = sp + 64 * wv;
-    int32_t* wmark = mark + 64 * wv;
-    int c = 0;
-    if (mine) {
-        const TriSetup t = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
-        DPrim& d = wsp[lane];
-        d.ax = t.ax, d.ay = t.ay, d.bx = t.bx, d.by = t.by, d.abx = t.abx, d.aby = t.aby;
-        d.acx = t.acx, d.acy = t.acy, d.bcx = t.bcx, d.bcy = t.bcy, d.sabc = t.sabc;
-        d.az = p1.z, d.bz = p2.z, d.cz = p3.z;
-        const uint32_t w = (uint32_t)(x1 - x0 + 1);
-        d.xy0 = x0 | (y0 << 16), d.wq = w | ((65536u + w - 1u) / w) << 9;
-        c = (x1 - x0 + 1) * (y1 - y0 + 1);
-    }
+__device__ __forceinline__ void wave_spread(int c, int32_t* wmark /*[64]*/, F&& body) {
+    const int lane = threadIdx.x & 63;
     const int incl = wave_scan_incl<false>(c);
-    const int s0 = incl - c;  // this lane's first candidate
+    const int s0 = incl - c;  // this lane's first item
     const int total = __builtin_amdgcn_readlane(incl, 63);
     int carry = 0;  // 1 + the lane owning the slot before the round
     for (int base = 0; base < total; base += 64) {
@@ -436,12 +414,43 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
         if (lane == 0) m = max(m, carry);
         m = wave_scan_incl<true>(m);
         carry = __builtin_amdgcn_readlane(m, 63);
-        const int idx = base + lane;
-        const int o = m - 1;
-        const int so = __builtin_amdgcn_ds_bpermute(max(o, 0) * 4, s0);
-        if (idx >= total || o < 0) continue;
+        const int idx = base + lane, o = max(m - 1, 0);
+        const int so = __builtin_amdgcn_ds_bpermute(o * 4, s0);
+        body(o, idx - so, idx < total && m > 0);
+    }
+}
+
+// Small triangles drawn by the setup workgroups themselves: each wave's
+// candidate pixel centres (its lanes' pixel boxes) are spread over its 64
+// lanes (wave_spread), each centre tested as the rasters test it (the same
+// Barycoord, inside test and depth, raster.go:487-496) and handed to `put`
+// with the lane of its triangle, whose record is staged in LDS.
+struct __align__(16) DPrim {
+    float ax, ay, bx, by, abx, aby, acx, acy, bcx, bcy, sabc;  // TriSetup
+    float az, bz, cz;
+    int32_t xy0;  // x0 | y0 << 16
+    uint32_t wq;  // box width w | ceil(65536 / w) << 9
+};
+template <class F>
+__device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, int x0, int y0, int x1, int y1,
+                                                DPrim* sp /*[256]*/, int32_t* mark /*[256]*/, F&& put) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    DPrim* wsp = sp + 64 * wv;
+    int c = 0;
+    if (mine) {
+        const TriSetup t = tri_setup(p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+        DPrim& d = wsp[lane];
+        d.ax = t.ax, d.ay = t.ay, d.bx = t.bx, d.by = t.by, d.abx = t.abx, d.aby = t.aby;
+        d.acx = t.acx, d.acy = t.acy, d.bcx = t.bcx, d.bcy = t.bcy, d.sabc = t.sabc;
+        d.az = p1.z, d.bz = p2.z, d.cz = p3.z;
+        const uint32_t w = (uint32_t)(x1 - x0 + 1);
+        d.xy0 = x0 | (y0 << 16), d.wq = w | ((65536u + w - 1u) / w) << 9;
+        c = (x1 - x0 + 1) * (y1 - y0 + 1);
+    }
+    wave_spread(c, mark + 64 * wv, [&](int o, int l, bool valid) {
+        if (!valid) return;
         const DPrim d = wsp[o];
-        const int l = idx - so, w = (int)(d.wq & 511u);
+        const int w = (int)(d.wq & 511u);
         // l / w by multiply-shift: exact while l * w < 65536, which l < area <= 256
         // and w <= area keep (the thresholds are at most 256)
         const int q = (int)(mul_u24((uint32_t)l, d.wq >> 9) >> 16);
@@ -450,9 +459,8 @@ __device__ __forceinline__ void draw_small_tris(bool mine, V4 p1, V4 p2, V4 p3, 
         t.ax = d.ax, t.ay = d.ay, t.bx = d.bx, t.by = d.by, t.abx = d.abx, t.aby = d.aby;
         t.acx = d.acx, t.acy = d.acy, t.bcx = d.bcx, t.bcy = d.bcy, t.sabc = d.sabc;
         float z;
-        if (!cover_z(t, (float)px + 0.5f, (float)py + 0.5f, d.az, d.bz, d.cz, z)) continue;
-        put(px, py, z, o);
-    }
+        if (cover_z(t, (float)px + 0.5f, (float)py + 0.5f, d.az, d.bz, d.cz, z)) put(px, py, z, o);
+    });
 }
 
 // An unclipped triangle whose pixel box holds at most `draw_small` pixel
@@ -934,12 +942,16 @@ __device__ __forceinline__ void k_bin_body(const k_bin_args& a_, const VB vb) {
     }
     const int tx0 = x0 >> shift, ty0 = y0 >> shift, ntx = active ? (x1 >> shift) - tx0 + 1 : 0;
     const int n = active ? ntx * ((y1 >> shift) - ty0 + 1) : 0;
-    for (int j = 0; __ballot(j < n) != 0; j++) {
-        const bool act = j < n;
-        const int key = act ? base + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx : 0;
-        const int pos = wave_alloc(cursor, key, act);
-        if (act) list[pos] = i;
-    }
+    // the wave's (primitive, tile) pairs spread over its lanes, each lane's
+    // cursor atomic issued at once (not one returning atomic after another)
+    __shared__ int32_t mark[256];
+    const int lane = threadIdx.x & 63;
+    wave_spread(n, mark + (threadIdx.x & ~63), [&](int o, int j, bool valid) {
+        const int otx0 = __shfl(tx0, o), oty0 = __shfl(ty0, o), ontx = __shfl(ntx, o), obase = __shfl(base, o);
+        if (!valid) return;
+        const int key = obase + (oty0 + j / ontx) * tiles_x + otx0 + j % ontx;
+        list[atomicAdd(&cursor[key], 1)] = i - lane + o;
+    });
 }
 __global__ __launch_bounds__(256) void k_bin(k_bin_args a) {
     k_bin_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
