@@ -309,7 +309,8 @@ __global__ __launch_bounds__(256) void k_vertex_main_wide(const k_vertex_main_ar
 }
 
 // render/shadow.go:155-173: light-space screen position of every vertex for
-// every shadow slot (grid.y = slot).
+// every shadow slot (a thread per vertex, which reads it once for all the
+// slots).
 struct k_vertex_shadow_args {
     DevScene s;
     const M4* strans;
@@ -319,19 +320,21 @@ struct k_vertex_shadow_args {
     int32_t gx, gy;  // the grid of this launch
 };
 __device__ __forceinline__ void k_vertex_shadow_body(const k_vertex_shadow_args& a_, const VB vb) {
-    if (vb.by == 0) frame_clear_part(a_.clr, vb.bx * (size_t)blockDim.x + threadIdx.x, (size_t)vb.gx * blockDim.x);
+    frame_clear_part(a_.clr, vb.bx * (size_t)blockDim.x + threadIdx.x, (size_t)vb.gx * blockDim.x);
     const DevScene& s = a_.s;
     const M4* __restrict__ strans = a_.strans;
     const ShadowConsts& sc = a_.sc;
     float4* __restrict__ ts = a_.ts;
 
-    int i = vb.bx * blockDim.x + threadIdx.x;
-    int slot = vb.by;
+    const int i = vb.bx * blockDim.x + threadIdx.x;
     if (i >= s.n_tverts) return;
-    int o = s.vtx_obj[i];
-    int mv = s.mesh_vtx_off[s.obj_mesh[o]] + (i - s.obj_vtx_start[o]);
-    V4 q = pos(apply(mulv(strans[(size_t)slot * s.n_objects + o], ld(s.vpos[mv])), sc.vp));
-    ts[(size_t)slot * s.n_tverts + i] = make_float4(q.x, q.y, q.z, 0.0f);
+    const int o = s.vtx_obj[i];
+    const int mv = s.mesh_vtx_off[s.obj_mesh[o]] + (i - s.obj_vtx_start[o]);
+    const V4 v = ld(s.vpos[mv]);
+    for (int slot = 0; slot < sc.n_slots; slot++) {
+        const V4 q = pos(apply(mulv(strans[(size_t)slot * s.n_objects + o], v), sc.vp));
+        ts[(size_t)slot * s.n_tverts + i] = make_float4(q.x, q.y, q.z, 0.0f);
+    }
 }
 __global__ __launch_bounds__(256) void k_vertex_shadow(k_vertex_shadow_args a) {
     k_vertex_shadow_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
@@ -1373,47 +1376,92 @@ struct k_resolve_args {
     Counters* cnt;
     float* depth;
     DevScene s;
-    uint8_t* ao_need;
+    int32_t* ao_xs;   // scenes with AO: each row's AO pixels, left to right (for k_ao)
+    int32_t* ao_cnt;  // and their count
     const int4* tri_rec;
     int32_t gx, gy;  // the grid of this launch
 };
+// One workgroup per row, 2048 pixels at a time (eight of 256 a thread, all
+// loads first): the depth, and with AO the row's list of the pixels k_ao
+// computes -- those whose material has AmbientOcclusion, and pixel (0,0)
+// when covered (uncovered pixels are shaded from it) -- placed by wave
+// ballots and a scan of the 32 wave counts.
+constexpr int RESOLVE_PER = 8;
 template <bool EXACT>
 __device__ __forceinline__ void k_resolve_body(const k_resolve_args& a_, const VB vb) {
     const RasterTarget& rt = a_.rt;
     Counters* cnt = a_.cnt;
     float* __restrict__ depth = a_.depth;
     const DevScene& s = a_.s;
-    uint8_t* __restrict__ ao_need = a_.ao_need;
+    int32_t* __restrict__ ao_xs = a_.ao_xs;
     const int4* __restrict__ tri_rec = a_.tri_rec;
+    __shared__ int32_t s_cnt[RESOLVE_PER * 4 + 1];
 
-    const size_t o = vb.bx * (size_t)blockDim.x + threadIdx.x;
-    if (o >= (size_t)rt.W * rt.H || cnt->overflow) return;
-    unsigned long long v = rt.vis[o];
-    if (EXACT) {
-        const uint32_t fn = rt.first_nan[o];
-        if (fn != 0xffffffffu && fn == rt.first_all[o]) {
-            v = ((unsigned long long)KEY_HI_NAN << 32) | fn;
-            rt.vis[o] = v;
+    const int y = vb.bx, W = rt.W;
+    if (y >= rt.H || cnt->overflow) return;
+    const int tid = threadIdx.x, wv = tid >> 6;
+    const size_t row = (size_t)y * W;
+    int running = 0;  // the row's AO pixels so far
+    for (int x0 = 0; x0 < W; x0 += 256 * RESOLVE_PER) {
+        unsigned long long v[RESOLVE_PER];
+#pragma unroll
+        for (int k = 0; k < RESOLVE_PER; k++) {
+            const int x = x0 + 256 * k + tid;
+            v[k] = x < W ? rt.vis[row + x] : VIS_EMPTY;
         }
-    }
-    const uint32_t hi = (uint32_t)(v >> 32);
-    const float d = v == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
-    depth[o] = d;
-    if (!(__builtin_fabsf(d) <= 3.40282347e38f)) atomicOr(&cnt->depth_nonfinite, 1);  // lets k_ao skip NaN checks
-    if (ao_need) {
-        // the pixels k_ao computes: those whose material has AmbientOcclusion,
-        // and pixel (0,0) when covered (uncovered pixels are shaded from it)
-        bool need = false;
-        if (v != VIS_EMPTY) {
-            if (o == 0) {
-                need = true;
-            } else {
-                const int mat = tri_rec[(uint32_t)v >> 3].w;
-                need = mat >= 0 && s.mats[mat].ambient_occlusion;
+        bool need[RESOLVE_PER];
+#pragma unroll
+        for (int k = 0; k < RESOLVE_PER; k++) {
+            const int x = x0 + 256 * k + tid;
+            need[k] = false;
+            if (x >= W) continue;
+            const size_t o = row + x;
+            if (EXACT) {
+                const uint32_t fn = rt.first_nan[o];
+                if (fn != 0xffffffffu && fn == rt.first_all[o]) {
+                    v[k] = ((unsigned long long)KEY_HI_NAN << 32) | fn;
+                    rt.vis[o] = v[k];
+                }
+            }
+            const uint32_t hi = (uint32_t)(v[k] >> 32);
+            const float d = v[k] == VIS_EMPTY ? -1.0f : (hi == KEY_HI_NAN ? __builtin_nanf("") : depth_from_key_hi(hi));
+            depth[o] = d;
+            if (!(__builtin_fabsf(d) <= 3.40282347e38f)) atomicOr(&cnt->depth_nonfinite, 1);  // lets k_ao skip NaN checks
+            if (ao_xs && v[k] != VIS_EMPTY) {
+                if (o == 0) {
+                    need[k] = true;
+                } else {
+                    const int mat = tri_rec[(uint32_t)v[k] >> 3].w;
+                    need[k] = mat >= 0 && s.mats[mat].ambient_occlusion;
+                }
             }
         }
-        ao_need[o] = need;
+        if (!ao_xs) continue;
+        uint64_t bal[RESOLVE_PER];
+#pragma unroll
+        for (int k = 0; k < RESOLVE_PER; k++) {
+            bal[k] = __ballot(need[k]);
+            if ((tid & 63) == 0) s_cnt[4 * k + wv] = __popcll(bal[k]);
+        }
+        __syncthreads();
+        if (tid == 0) {  // exclusive scan, pass-major then wave: the pixels' order
+            int acc = 0;
+            for (int j = 0; j < RESOLVE_PER * 4; j++) {
+                const int c = s_cnt[j];
+                s_cnt[j] = acc;
+                acc += c;
+            }
+            s_cnt[RESOLVE_PER * 4] = acc;
+        }
+        __syncthreads();
+        const uint64_t below = (1ull << (tid & 63)) - 1ull;
+#pragma unroll
+        for (int k = 0; k < RESOLVE_PER; k++)
+            if (need[k]) ao_xs[row + running + s_cnt[4 * k + wv] + __popcll(bal[k] & below)] = x0 + 256 * k + tid;
+        running += s_cnt[RESOLVE_PER * 4];
+        __syncthreads();  // (s_cnt is rewritten by the next 2048 pixels)
     }
+    if (ao_xs && tid == 0) a_.ao_cnt[y] = running;
 }
 template <bool EXACT>
 __global__ __launch_bounds__(256) void k_resolve(k_resolve_args a) {
@@ -1777,49 +1825,9 @@ __device__ __forceinline__ uint32_t gamma(const ShadeConsts& c, uint32_t col) {
 
 }  // namespace
 
-// The pixels k_ao computes, row by row: each row's AO pixels (k_resolve's
-// flags) listed left to right, xs[y * W + r] for r < cnt[y].  One workgroup
-// per row: each thread counts its run of the row, a block scan places the
-// runs.
-struct k_ao_rows_args {
-    const uint8_t* ao_need;
-    int32_t* xs;
-    int32_t* cnt_row;
-    const Counters* cnt;
-    int32_t W, H;
-    int32_t gx, gy;  // the grid of this launch
-};
-__device__ __forceinline__ void k_ao_rows_body(const k_ao_rows_args& a, const VB vb) {
-    __shared__ int32_t start[257];
-    __shared__ int32_t wsum[4];
-    const int y = vb.bx;
-    if (y >= a.H || a.cnt->overflow) return;
-    const int W = a.W, per = (W + 255) / 256, x0 = threadIdx.x * per, x1 = min(x0 + per, W);
-    const uint8_t* need = a.ao_need + (size_t)y * W;
-    int c = 0;
-    for (int x = x0; x < x1; x++) c += need[x] != 0;
-    block_scan_256(c, start, wsum);
-    int32_t* xs = a.xs + (size_t)y * W;
-    int pos = start[threadIdx.x];
-    for (int x = x0; x < x1; x++)
-        if (need[x]) xs[pos++] = x;
-    if (threadIdx.x == 0) a.cnt_row[y] = start[256];
-}
-__global__ __launch_bounds__(256) void k_ao_rows(k_ao_rows_args a) {
-    k_ao_rows_body(a, VB{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y});
-}
-__global__ __launch_bounds__(256) void k_ao_rows_wide(const k_ao_rows_args* __restrict__ a,
-                                                      const int32_t* __restrict__ start, int32_t n) {
-    const int v = wide_view(start, n, (int)blockIdx.x);
-    const k_ao_rows_args& x = a[v];
-    const int local = (int)blockIdx.x - start[v];
-    if (local >= x.gx * x.gy) return;  // padding to a multiple of 8 workgroups
-    k_ao_rows_body(x, VB{local % x.gx, local / x.gx, x.gx, x.gy});
-}
-
 // Screen-space AO (material/ao.go:20-73) for every pixel whose material has
 // it, plus pixel (0,0) (the uncovered pixels are shaded from it).  One
-// workgroup = 64 consecutive AO pixels of one row (k_ao_rows' list) x the 8
+// workgroup = 64 consecutive AO pixels of one row (k_resolve's list) x the 8
 // march directions, four waves of two directions each; the lanes hold the
 // row's AO pixels left to right, so each tap of every direction reads nearly
 // contiguous depths across the wave, and no lane marches for a pixel without
@@ -1837,7 +1845,7 @@ __global__ __launch_bounds__(256) void k_ao_rows_wide(const k_ao_rows_args* __re
 // k_shade.
 struct k_ao_args {
     ShadeConsts c;
-    const int32_t* ao_xs;    // k_ao_rows' lists
+    const int32_t* ao_xs;    // k_resolve's lists
     const int32_t* ao_cnt;   // and counts
     const float* depth;
     float* ao;
@@ -2560,7 +2568,7 @@ void launch_vertex_main(hipStream_t st, const DevScene& s, const M4* trans, cons
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float4* ts,
                           const FrameClear& clr) {
     if (!s.n_tverts || !sc.n_slots) return clear_alone(st, clr);
-    go(k_vertex_shadow, k_vertex_shadow_wide, dim3(blocks_for(s.n_tverts, 256), sc.n_slots), 256, st,
+    go(k_vertex_shadow, k_vertex_shadow_wide, dim3(blocks_for(s.n_tverts, 256)), 256, st,
        k_vertex_shadow_args{s, strans, sc, ts, clr, 0, 0});
 }
 int setup_blocks(int32_t n_tris) { return (n_tris + 255) / 256; }
@@ -2644,16 +2652,14 @@ void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims
     else launch_raster_t<MODE_VIS>(st, tile, n_workgroups, a);
 }
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
-                    uint8_t* ao_need, const int4* tri_rec) {
-    const size_t P = (size_t)rt.W * rt.H;
-    const k_resolve_args a{rt, cnt, depth, s, ao_need, tri_rec, 0, 0};
-    if (exact) go(k_resolve<true>, k_resolve_wide<true>, dim3(blocks_for(P, 256)), 256, st, a);
-    else go(k_resolve<false>, k_resolve_wide<false>, dim3(blocks_for(P, 256)), 256, st, a);
+                    int32_t* ao_xs, int32_t* ao_cnt, const int4* tri_rec) {
+    if (!rt.W || !rt.H) return;
+    const k_resolve_args a{rt, cnt, depth, s, ao_xs, ao_cnt, tri_rec, 0, 0};
+    if (exact) go(k_resolve<true>, k_resolve_wide<true>, dim3(rt.H), 256, st, a);
+    else go(k_resolve<false>, k_resolve_wide<false>, dim3(rt.H), 256, st, a);
 }
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const int nb = ((c.v.W + 63) / 64) * c.v.H;
-    go(k_ao_rows, k_ao_rows_wide, dim3(c.v.H), 256, st,
-       k_ao_rows_args{b.ao_need, b.ao_xs, b.ao_cnt, b.cnt, c.v.W, c.v.H, 0, 0});
     go(k_ao, k_ao_wide, dim3(((nb + 7) / 8) * 8), 256, st,
        k_ao_args{c, b.ao_xs, b.ao_cnt, b.depth, b.ao, b.ao_tab, b.ao_fast, b.cnt, 0, 0});
 }

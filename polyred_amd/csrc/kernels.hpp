@@ -57,8 +57,7 @@ struct ShadeBuffers {
     const Counters* cnt;       // main-pass counters, then the shadow ones: nothing is shaded after an overflow
     Counters* cnt_host;        // pinned host copy of both, written once they are final
     float* ao;                 // [H][W] AO factor of the pixels that need it
-    const uint8_t* ao_need;    // [H][W] 1 where k_ao computes the factor
-    int32_t* ao_xs;            // [H][W] per row, its AO pixels' columns left to right (k_ao_rows)
+    int32_t* ao_xs;            // [H][W] per row, the columns of the pixels k_ao computes, left to right (k_resolve)
     int32_t* ao_cnt;           // [H] and their count
     const float* ao_tab;       // AO march tables (tap offsets, diagonal distances)
     int32_t ao_fast;           // the tables hold for this frame size
@@ -188,7 +187,7 @@ void launch_raster(hipStream_t st, int mode, int n_workgroups, const Prim* prims
                    const int32_t* chunk_off, int32_t* item_tile, int32_t cap_items, const int32_t* tile_list,
                    Counters* cnt, const RasterTarget& rt, int32_t item_prims, int32_t tile);
 void launch_resolve(hipStream_t st, bool exact, const RasterTarget& rt, Counters* cnt, float* depth, const DevScene& s,
-                    uint8_t* ao_need, const int4* tri_rec);
+                    int32_t* ao_xs, int32_t* ao_cnt, const int4* tri_rec);
 void launch_ao(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade_bg(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);
 void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b);

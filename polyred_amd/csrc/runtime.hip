@@ -444,8 +444,7 @@ struct prh_renderer {
     int32_t *rz_ox, *rz_oy;  // and their start offsets
     int rz_flx, rz_fly;
     float *linear, *bg_lin, *ao;
-    uint8_t* ao_need;  // scenes with AO only
-    int32_t *ao_xs, *ao_cnt;  // and the rows' lists of AO pixels
+    int32_t *ao_xs, *ao_cnt;  // scenes with AO only: the rows' lists of AO pixels
     float* ao_tab;   // AO march tables (ao_tables)
     int32_t ao_fast; // the tables hold for this frame size
     Counters* cnt;      // [2]: main, shadow
@@ -1055,7 +1054,6 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc) chk(ralloc(r, &r->vis, P));
     if (!rc) chk(ralloc(r, &r->depth, P));
     if (!rc) chk(ralloc(r, &r->ao, P));
-    if (!rc && s->any_ao) chk(ralloc(r, &r->ao_need, P));
     if (!rc && s->any_ao) chk(ralloc(r, &r->ao_xs, P));
     if (!rc && s->any_ao) chk(ralloc(r, &r->ao_cnt, (size_t)r->H));
     if (!rc && s->any_ao && r->n_slots > 0) chk(ralloc(r, &r->col_pre, P));
@@ -1178,7 +1176,6 @@ ShadeBuffers shade_buffers(const prh_renderer* r) {
     b.cnt = r->cnt;
     b.cnt_host = r->d_h_cnt;
     b.ao = r->ao;
-    b.ao_need = r->ao_need;
     b.ao_xs = r->ao_xs, b.ao_cnt = r->ao_cnt;
     b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
     b.col_pre = nullptr;
@@ -1266,7 +1263,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     launch_raster(st, nan_mode ? 1 : 0, persistent, r->prims, r->tile_off, r->chunk_off, r->item_tile, r->cap_items,
                   r->tile_list, cm, rt, ip, r->rtile);
     wide_stage(12);
-    launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_need : nullptr, r->tri_rec);
+    launch_resolve(st, nan_mode, rt, cm, r->depth, d, r->sc->any_ao ? r->ao_xs : nullptr, r->ao_cnt, r->tri_rec);
     if (timed) (void)hipEventRecord(r->ev[4], st);
     const ShadeBuffers b = shade_buffers(r);
     // the split shading's buffers: both halves read and write col_pre
