@@ -387,8 +387,8 @@ typedef struct prh_tuning {
 int prh_get_tuning(prh_renderer* r, prh_tuning* out);
 int prh_set_tuning(prh_renderer* r, const prh_tuning* t);
 /* Coordinate descent over item_prims {256..4096}, raster_wg_per_cu {1, 2, 4,
- * 8}, shadow_direct_max {0, 16, 32, 64, 128}, main_direct_max {0, 24, 48,
- * 96}, overlap_shadows {0, 1} and raster_tile {8, 16, 32}: each
+ * 8}, shadow_direct_max {0, 16, 32, 64, 128, 256}, main_direct_max {0, 24,
+ * 48, 96, 192}, overlap_shadows {0, 1} and raster_tile {8, 16, 32}: each
  * configuration timed as the median of
  * `frames` frames (HIP events, after one warm-up frame); the winner is timed
  * again against the starting knobs over twice the frames and the faster of

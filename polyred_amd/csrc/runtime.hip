@@ -2360,8 +2360,8 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
     };
     static const int32_t c_ip[] = {256, 512, 1024, 2048, 4096};
     static const int32_t c_wg[] = {1, 2, 4, 8};
-    static const int32_t c_sd[] = {0, 16, 32, 64, 128};
-    static const int32_t c_md[] = {0, 24, 48, 96};
+    static const int32_t c_sd[] = {0, 16, 32, 64, 128, 256};
+    static const int32_t c_md[] = {0, 24, 48, 96, 192};
     static const int32_t c_ov[] = {0, 1};
     static const int32_t c_rt[] = {8, 16, 32};
     const prh_tuning start = r->tune;
@@ -2372,7 +2372,7 @@ int prh_autotune(prh_renderer* r, int32_t frames, prh_tuning* out) {
     // coordinate descent: each knob over its candidates, the others at their best
     for (int knob = 0; knob < 6; knob++) {
         const int32_t* c = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : knob == 3 ? c_md : knob == 4 ? c_ov : c_rt;
-        const int n = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : knob == 3 ? 4 : knob == 4 ? 2 : 3;
+        const int n = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 6 : knob == 3 ? 5 : knob == 4 ? 2 : 3;
         for (int k = 0; k < n; k++) {
             prh_tuning t = best;
             int32_t* f = knob == 0 ? &t.item_prims
@@ -2448,8 +2448,8 @@ int prh_autotune_batch(prh_renderer* const* rs, int32_t n, int32_t frames, prh_t
     };
     static const int32_t c_ip[] = {256, 512, 1024, 2048, 4096};
     static const int32_t c_wg[] = {2, 4, 8, 16};
-    static const int32_t c_sd[] = {0, 16, 32, 64, 128};
-    static const int32_t c_md[] = {0, 24, 48, 96};
+    static const int32_t c_sd[] = {0, 16, 32, 64, 128, 256};
+    static const int32_t c_md[] = {0, 24, 48, 96, 192};
     static const int32_t c_rt[] = {8, 16, 32};
     const bool has[2] = {std::count(cls.begin(), cls.end(), 0) > 0, std::count(cls.begin(), cls.end(), 1) > 0};
     const prh_tuning first[2] = {cur[0], cur[1]};
@@ -2460,7 +2460,7 @@ int prh_autotune_batch(prh_renderer* const* rs, int32_t n, int32_t frames, prh_t
         for (int c = 0; c < 2; c++) {
             if (!has[c]) continue;
             const int32_t* cand = knob == 0 ? c_ip : knob == 1 ? c_wg : knob == 2 ? c_sd : knob == 3 ? c_md : c_rt;
-            const int nc = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 5 : knob == 3 ? 4 : 3;
+            const int nc = knob == 0 ? 5 : knob == 1 ? 4 : knob == 2 ? 6 : knob == 3 ? 5 : 3;
             for (int k = 0; k < nc; k++) {
                 prh_tuning t[2] = {cur[0], cur[1]};
                 int32_t* f = knob == 0 ? &t[c].item_prims

@@ -57,7 +57,7 @@ def test_autotune_keeps_the_frame(device):
     want = frame(r)
     t = r.Autotune(frames=2)
     assert t.item_prims in (256, 512, 1024, 2048, 4096) and t.raster_wg_per_cu in (1, 2, 4, 8)
-    assert t.shadow_direct_max in (0, 16, 32, 64, 128) and t.main_direct_max in (0, 24, 48, 96)
+    assert t.shadow_direct_max in (0, 16, 32, 64, 128, 256) and t.main_direct_max in (0, 24, 48, 96, 192)
     assert t.raster_tile in (8, 16, 32)
     assert t.ms > 0
     got = r.tuning()
@@ -82,7 +82,7 @@ def test_autotune_batch_keeps_every_view(device):
     small, large = render.autotune_batch(rs, frames=1)
     for t in (small, large):
         assert t.item_prims in (256, 512, 1024, 2048, 4096) and t.raster_wg_per_cu in (2, 4, 8, 16)
-        assert t.shadow_direct_max in (0, 16, 32, 64, 128) and t.main_direct_max in (0, 24, 48, 96)
+        assert t.shadow_direct_max in (0, 16, 32, 64, 128, 256) and t.main_direct_max in (0, 24, 48, 96, 192)
         assert t.raster_tile in (8, 16, 32)
         assert t.ms > 0
     for r, w in zip(rs, want):  # the batch the tuner left
