@@ -945,8 +945,20 @@ __device__ __forceinline__ void k_bin_body(const k_bin_args& a_, const VB vb) {
     }
     const int tx0 = x0 >> shift, ty0 = y0 >> shift, ntx = active ? (x1 >> shift) - tx0 + 1 : 0;
     const int n = active ? ntx * ((y1 >> shift) - ty0 + 1) : 0;
-    // the wave's (primitive, tile) pairs spread over its lanes, each lane's
-    // cursor atomic issued at once (not one returning atomic after another)
+    if (shadow) {
+        // a shadow map gathers a scene in a few of its tiles, so many of the
+        // wave's pairs share a tile: one returning atomic per distinct tile
+        for (int j = 0; __ballot(j < n) != 0; j++) {
+            const bool act = j < n;
+            const int key = act ? base + (ty0 + j / ntx) * tiles_x + tx0 + j % ntx : 0;
+            const int pos = wave_alloc(cursor, key, act);
+            if (act) list[pos] = i;
+        }
+        return;
+    }
+    // the main view's pairs are few and spread: the wave's (primitive, tile)
+    // pairs go over its lanes, each lane's cursor atomic issued at once (not
+    // one returning atomic per distinct tile after another)
     __shared__ int32_t mark[256];
     const int lane = threadIdx.x & 63;
     wave_spread(n, mark + (threadIdx.x & ~63), [&](int o, int j, bool valid) {
@@ -1723,7 +1735,8 @@ __device__ __forceinline__ V4 apply_diag_xf(V4 v, const double* d, const float* 
 }
 
 
-__device__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int fx, int fy, float depth) {
+__device__ __forceinline__ int occluded(const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b, int fx, int fy,
+                                        float depth) {
     int occ = 0;
     const int64_t len = (int64_t)c.SW * c.SH;
     // SW and SH exact in float32, y * SW within 32 bits
