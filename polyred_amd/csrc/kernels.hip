@@ -315,7 +315,7 @@ struct k_vertex_shadow_args {
     DevScene s;
     const M4* strans;
     ShadowConsts sc;
-    float4* ts;
+    float3* ts;  // [slot][vertex] light-space screen x y z
     FrameClear clr;  // the shadow passes' counters, cleared by this pass
     int32_t gx, gy;  // the grid of this launch
 };
@@ -324,7 +324,7 @@ __device__ __forceinline__ void k_vertex_shadow_body(const k_vertex_shadow_args&
     const DevScene& s = a_.s;
     const M4* __restrict__ strans = a_.strans;
     const ShadowConsts& sc = a_.sc;
-    float4* __restrict__ ts = a_.ts;
+    float3* __restrict__ ts = a_.ts;
 
     const int i = vb.bx * blockDim.x + threadIdx.x;
     if (i >= s.n_tverts) return;
@@ -333,7 +333,7 @@ __device__ __forceinline__ void k_vertex_shadow_body(const k_vertex_shadow_args&
     const V4 v = ld(s.vpos[mv]);
     for (int slot = 0; slot < sc.n_slots; slot++) {
         const V4 q = pos(apply(mulv(strans[(size_t)slot * s.n_objects + o], v), sc.vp));
-        ts[(size_t)slot * s.n_tverts + i] = make_float4(q.x, q.y, q.z, 0.0f);
+        ts[(size_t)slot * s.n_tverts + i] = make_float3(q.x, q.y, q.z);
     }
 }
 __global__ __launch_bounds__(256) void k_vertex_shadow(k_vertex_shadow_args a) {
@@ -586,7 +586,7 @@ struct k_setup_shadow_args {
     DevScene s;
     ShadowConsts sc;
     const int4* tri_rec;
-    const float4* ts;
+    const float3* ts;
     Prim* prims;
     uint2* pbox;
     int32_t* tile_count;
@@ -598,7 +598,7 @@ struct k_setup_shadow_args {
 __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a_, const VB vb) {
     const DevScene& s = a_.s;
     const ShadowConsts& sc = a_.sc;
-    const float4* __restrict__ ts = a_.ts;
+    const float3* __restrict__ ts = a_.ts;
     Prim* __restrict__ prims = a_.prims;
     uint2* __restrict__ pbox = a_.pbox;
     int32_t* __restrict__ tile_count = a_.tile_count;
@@ -613,8 +613,8 @@ __device__ __forceinline__ void k_setup_shadow_body(const k_setup_shadow_args& a
     if (g < s.n_tris) {
         const int4 rec = a_.tri_rec[g];  // (x < 0: not Triangle.IsValid)
         if (rec.x >= 0) {
-            const float4* T = ts + (size_t)slot * s.n_tverts;
-            float4 A = T[rec.x], B = T[rec.y], C = T[rec.z];
+            const float3* T = ts + (size_t)slot * s.n_tverts;
+            const float3 A = T[rec.x], B = T[rec.y], C = T[rec.z];
             p1 = v4(A.x, A.y, A.z, 1), p2 = v4(B.x, B.y, B.z, 1), p3 = v4(C.x, C.y, C.z, 1);
             if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {
                 Box3 tb = box_of(p1, p2, p3);
@@ -2578,7 +2578,7 @@ void launch_vertex_main(hipStream_t st, const DevScene& s, const M4* trans, cons
     go(k_vertex_main, k_vertex_main_wide, dim3(blocks_for(s.n_tverts, 256)), 256, st,
        k_vertex_main_args{s, trans, normal, vc, tv_scr, tv_nor, tv_wp, tv_col, tv_uv, clr, 0, 0});
 }
-void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float4* ts,
+void launch_vertex_shadow(hipStream_t st, const DevScene& s, const M4* strans, const ShadowConsts& sc, float3* ts,
                           const FrameClear& clr) {
     if (!s.n_tverts || !sc.n_slots) return clear_alone(st, clr);
     go(k_vertex_shadow, k_vertex_shadow_wide, dim3(blocks_for(s.n_tverts, 256)), 256, st,
@@ -2599,7 +2599,7 @@ void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, 
                    blk_prims + setup_blocks(s.n_tris), cnt, cap, 0, 0});
 }
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const int4* tri_rec,
-                         const float4* ts, Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
+                         const float3* ts, Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
                          uint32_t* smaps) {
     if (!s.n_tris || !sc.n_slots) return;
     go(k_setup_shadow, k_setup_shadow_wide, dim3(blocks_for(s.n_tris, 256), sc.n_slots), 256, st,

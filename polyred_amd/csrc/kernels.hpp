@@ -151,7 +151,7 @@ void launch_vertex_main(hipStream_t st, const DevScene& s, const gm::M4* trans, 
                         const ViewConsts& vc, float4* tv_scr, float4* tv_nor, float4* tv_wp, uint32_t* tv_col,
                         float2* tv_uv, const FrameClear& clr);
 void launch_vertex_shadow(hipStream_t st, const DevScene& s, const gm::M4* strans, const ShadowConsts& sc,
-                          float4* ts, const FrameClear& clr);
+                          float3* ts, const FrameClear& clr);
 // The setup's workgroups (each writes its unclipped primitives to
 // blk_prims[its index]) and then the clipper's (blk_prims[setup_blocks +
 // its index]: its pieces); blk_prims holds setup_blocks(n_tris) + CLIP_GRID.
@@ -164,7 +164,7 @@ void launch_setup_main(hipStream_t st, const DevScene& s, const ViewConsts& vc, 
                        ClipJob* clip_list, int32_t* tile_count, int32_t* blk_prims, Counters* cnt, Capacities cap,
                        unsigned long long* vis, int32_t draw_small, const int4* tri_rec);
 void launch_setup_shadow(hipStream_t st, const DevScene& s, const ShadowConsts& sc, const int4* tri_rec,
-                         const float4* ts, Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
+                         const float3* ts, Prim* prims, uint2* pbox, int32_t* tile_count, Counters* cnt, Capacities cap,
                          uint32_t* smaps);
 // Each global triangle's record, fixed by the scene: its transformed
 // vertices and its material in the flat table (-1: vertex colour, matAt's

@@ -409,7 +409,8 @@ struct prh_renderer {
     std::vector<void*> allocs;
     M4 *d_trans, *d_normal, *d_strans, *d_lview, *d_lproj;
     ShadowXf* d_lxf;
-    float4 *tv_scr, *tv_nor, *tv_wp, *ts;
+    float4 *tv_scr, *tv_nor, *tv_wp;
+    float3* ts;  // [slot][vertex] light-space screen positions
     int4* tri_rec;     // per triangle: transformed vertices and flat material (once, k_tri_rec)
     uint32_t* tv_col;  // per transformed vertex: colour
     float2* tv_uv;     // and UV (scenes with textures)
@@ -1278,6 +1279,9 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
         if (timed) (void)hipEventRecord(r->ev_sh[0], ss);
         launch_ao(ss, d, r->sh, b);
         if (timed) (void)hipEventRecord(r->ev_sh[1], ss);
+        // the uncovered colour (pixel (0,0)'s shading, its AO factor
+        // included) there too, beside the end of the shading's first half
+        launch_shade_bg(ss, d, r->sh, b);
         (void)hipEventRecord(r->ev_join, ss);
         (void)hipStreamWaitEvent(st, r->ev_sdone, 0);  // the shading reads the shadow maps
         launch_shade(st, d, r->sh, bp);
@@ -1287,7 +1291,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     if (timed) (void)hipEventRecord(r->ev[5], st);
     if (split) (void)hipStreamWaitEvent(st, r->ev_join, 0);  // the shadow maps (and the AO factors)
     wide_stage(14);
-    launch_shade_bg(st, d, r->sh, b);
+    if (!split_shade) launch_shade_bg(st, d, r->sh, b);
     if (timed) (void)hipEventRecord(r->ev[7], st);
     wide_stage(15);
     if (!split_shade) launch_shade(st, d, r->sh, b);
