@@ -667,13 +667,7 @@ __global__ __launch_bounds__(256) void k_setup_shadow_wide(const k_setup_shadow_
 constexpr int SCAN_BLOCK = 4096;
 constexpr int SCAN_WHOLE_MAX = 8192;
 
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-    for (int off = 1; off < 64; off <<= 1) {
-        int u = __shfl_up(v, off);
-        if (lane >= off) v += u;
-    }
-    return v;
-}
+__device__ __forceinline__ int wave_incl_scan(int v, int) { return wave_scan_incl<false>(v); }  // (v >= 0)
 __device__ __forceinline__ int wave_sum(int v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
@@ -761,6 +755,17 @@ __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, c
     __shared__ int32_t s_t[SCAN_WHOLE_MAX + SCAN_WHOLE_MAX / 32];  // whole scan: tile i at i + i / 32
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const bool whole = vb.gx == 1;
+    // the frame statistics' per-workgroup primitive counts, loaded at once
+    // by the workgroup that sums them (the last)
+    int np = 0;
+    if (whole || vb.bx == vb.gx - 1) {
+        int pv[8];  // (every load issued before the first is waited for)
+#pragma unroll
+        for (int j = 0; j < 8; j++) pv[j] = tid + j * NT < n_blk ? blk_prims[tid + j * NT] : 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) np += pv[j];
+        for (int k = tid + 8 * NT; k < n_blk; k += NT) np += blk_prims[k];
+    }
     int run = 0, runc = 0;
     if (whole) {
         // One workgroup for every tile: thread t takes tiles PER*t ..
@@ -871,8 +876,6 @@ __device__ __forceinline__ void k_scan_apply_body(const k_scan_apply_args& a_, c
         cnt->n_pairs = run;
         cnt->n_chunks = runc;
     }
-    int np = 0;
-    for (int k = tid; k < n_blk; k += NT) np += blk_prims[k];
     np = wave_sum(np);
     __syncthreads();  // every wave has read ws[] above
     if (lane == 0) ws[wv] = np;
@@ -1022,10 +1025,28 @@ __device__ __forceinline__ int clip_wave(int qi, const ClipJob& jb, const DevSce
     const Plane planes[6] = {{v4(w, 0, 0, 1), v4(-1, 0, 0, 1)}, {v4(0, 0, 0, 1), v4(1, 0, 0, 1)},
                              {v4(0, h, 0, 1), v4(0, -1, 0, 1)}, {v4(0, 0, 0, 1), v4(0, 1, 0, 1)},
                              {v4(0, 0, 1, 1), v4(0, 0, -1, 1)}, {v4(0, 0, -1, 1), v4(0, 0, 1, 1)}};
+    // A plane with all three vertices in front of it by a margin far above
+    // the rounding of the intersection points (2^-12 of the largest
+    // coordinate, plus 2^-10) keeps every vertex of every polygon the
+    // earlier planes make in front of it: those lie on the triangle's edges
+    // and chords up to a rounding of a few ulps of the coordinates (the
+    // planes are axis-aligned, so each intersection's numerator and
+    // denominator carry one or two roundings), so its pass would emit the
+    // polygon unchanged, in the same order: skipped.
+    float big = 0.0f;
+    big = fmaxf(big, fmaxf(fmaxf(__builtin_fabsf(p1.x), __builtin_fabsf(p1.y)), __builtin_fabsf(p1.z)));
+    big = fmaxf(big, fmaxf(fmaxf(__builtin_fabsf(p2.x), __builtin_fabsf(p2.y)), __builtin_fabsf(p2.z)));
+    big = fmaxf(big, fmaxf(fmaxf(__builtin_fabsf(p3.x), __builtin_fabsf(p3.y)), __builtin_fabsf(p3.z)));
+    const float margin = big * 0x1p-12f + 0x1p-10f;
+    auto clear_of = [&](const Plane& p) {
+        return (dot(sub(p1, p.pos), p.nor) > margin) & (dot(sub(p2, p.pos), p.nor) > margin) &
+               (dot(sub(p3, p.pos), p.nor) > margin);
+    };
     int na = 3, cur = 0;
 #pragma unroll
     for (int pi = 0; pi < 6; pi++) {
         if (na == 0) break;
+        if (__builtin_amdgcn_readfirstlane(clear_of(planes[pi]))) continue;
         const V4* a = poly[cur];
         V4* b = poly[cur ^ 1];
         int emit = 0;
@@ -2310,19 +2331,26 @@ __device__ __forceinline__ void shade_tile(const DevScene& s, const ShadeConsts&
     uint32_t col;
     float lin[3] = {__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf("")};
     bool hl = false;
+    uint8_t flag = 0;
     if (key == VIS_EMPTY) {
-        if (b.col_pre) return;  // (the second half shades uncovered pixels)
+        if (b.col_pre) {  // (the second half shades uncovered pixels)
+            b.post_flag[o] = 2;
+            return;
+        }
         col = b.bg[0];
         lin[0] = b.bg_lin[0], lin[1] = b.bg_lin[1], lin[2] = b.bg_lin[2];
         hl = true;
     } else {
         GFrag f = fetch_fragment(s, c, b, x, y, key);
         col = shade_fragment(s, c, b, f, x, y, f.mat, f.col, lin, hl);
+        if (b.col_pre) flag = f.mat >= 0 && s.mats[f.mat].ambient_occlusion;
     }
     if (b.col_pre) {
-        // first half of a split shading: the colour before AO, and the
-        // linear sums (which AO does not touch)
+        // first half of a split shading: the colour before AO, whether the
+        // AO factor is still to come, and the linear sums (which AO does
+        // not touch)
         b.col_pre[o] = col;
+        b.post_flag[o] = flag;
         if (b.linear) {
             b.linear[3 * o] = hl ? lin[0] : __builtin_nanf("");
             b.linear[3 * o + 1] = hl ? lin[1] : __builtin_nanf("");
@@ -2392,15 +2420,14 @@ __global__ __launch_bounds__(256) void k_shade_post(DevScene s, ShadeConsts c, S
     const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (o >= (size_t)c.v.W * c.v.H || b.cnt->overflow) return;
     const int x = (int)(o % (size_t)c.v.W), y = (int)(o / (size_t)c.v.W);
-    const uint64_t key = b.vis[o];
+    const uint8_t flag = b.post_flag[o];  // (the first half's: no dependent loads here)
     uint32_t col;
-    if (key == VIS_EMPTY) {
+    if (flag == 2) {
         col = b.bg[0];
         if (b.linear) b.linear[3 * o] = b.bg_lin[0], b.linear[3 * o + 1] = b.bg_lin[1], b.linear[3 * o + 2] = b.bg_lin[2];
     } else {
         col = b.col_pre[o];
-        const int mat = b.tri_rec[(uint32_t)key >> 3].w;
-        if (mat >= 0 && s.mats[mat].ambient_occlusion) col = ao_apply(col, b.ao[o]);
+        if (flag) col = ao_apply(col, b.ao[o]);
     }
     if (b.pregamma) b.pregamma[o] = col;
     b.image[(size_t)(c.v.H - 1 - y) * c.v.W + x] = gamma(c, col);
@@ -2686,7 +2713,7 @@ void launch_shade(hipStream_t st, const DevScene& s, const ShadeConsts& c, const
 
 bool launch_shade_post(hipStream_t st, const DevScene& s, const ShadeConsts& c, const ShadeBuffers& b) {
     const size_t P = (size_t)c.v.W * c.v.H;
-    if (!b.col_pre || !b.ao || !b.tri_rec) return false;  // it reads the first half's buffers: nothing launched
+    if (!b.col_pre || !b.post_flag || !b.ao) return false;  // it reads the first half's buffers: nothing launched
     if (P) hipLaunchKernelGGL(k_shade_post, dim3(blocks_for(P, 256)), dim3(256), 0, st, s, c, b);
     return true;
 }

@@ -62,6 +62,8 @@ struct ShadeBuffers {
     const float* ao_tab;       // AO march tables (tap offsets, diagonal distances)
     int32_t ao_fast;           // the tables hold for this frame size
     uint32_t* col_pre;         // set: k_shade is the first half of a split shading (colour before AO)
+    uint8_t* post_flag;        // and per pixel what the second half does: 0 colour as is, 1 with the AO
+                               // factor, 2 uncovered (the uncovered colour)
 };
 
 // The differentiable shading pass (diff.hip): parameters, fixed inputs and

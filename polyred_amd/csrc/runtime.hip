@@ -433,6 +433,7 @@ struct prh_renderer {
     // split shading: resolve done, the shadow passes done, the AO's span
     hipEvent_t ev_res, ev_sdone, ev_sh[2];
     uint32_t* col_pre;            // split shading: the colour before AO
+    uint8_t* post_flag;           // and what its second half does per pixel
     int32_t tile_slots;
     int32_t n_cu;
     uint32_t *first_all, *first_nan;
@@ -1058,6 +1059,7 @@ int prh_renderer_new(prh_ctx* ctx, prh_scene* s, const prh_camera* cam, const pr
     if (!rc && s->any_ao) chk(ralloc(r, &r->ao_xs, P));
     if (!rc && s->any_ao) chk(ralloc(r, &r->ao_cnt, (size_t)r->H));
     if (!rc && s->any_ao && r->n_slots > 0) chk(ralloc(r, &r->col_pre, P));
+    if (!rc && s->any_ao && r->n_slots > 0) chk(ralloc(r, &r->post_flag, P));
     if (!rc && msaa == 1) chk(ralloc(r, &r->image, P));
     if (!rc && msaa > 1) {
         // Resize(width, height, img): scale = old / new = MSAA (calcFactors,
@@ -1179,7 +1181,7 @@ ShadeBuffers shade_buffers(const prh_renderer* r) {
     b.ao = r->ao;
     b.ao_xs = r->ao_xs, b.ao_cnt = r->ao_cnt;
     b.ao_tab = r->ao_tab, b.ao_fast = r->ao_fast;
-    b.col_pre = nullptr;
+    b.col_pre = nullptr, b.post_flag = nullptr;
     return b;
 }
 
@@ -1211,7 +1213,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     // with AO as well, the AO runs on the second stream (after the shadow
     // passes) beside the shading's first half (everything but the AO factor)
     // on this one
-    const bool split_shade = split && r->sc->any_ao && r->col_pre;
+    const bool split_shade = split && r->sc->any_ao && r->col_pre && r->post_flag;
     if (shadows) {
         const int nt = r->n_slots * r->stiles_per_map;
         if (split) {
@@ -1269,7 +1271,7 @@ int enqueue_frame(prh_renderer* r, bool nan_mode, hipStream_t st, bool timed, bo
     const ShadeBuffers b = shade_buffers(r);
     // the split shading's buffers: both halves read and write col_pre
     ShadeBuffers bp = b;
-    bp.col_pre = r->col_pre;
+    bp.col_pre = r->col_pre, bp.post_flag = r->post_flag;
     r->split_shade_last = split_shade;
     if (split_shade) {
         // (k_shade there finishes last when it runs on the second stream)
