@@ -1270,9 +1270,9 @@ __device__ __forceinline__ bool eval_candidate(const LPrim& p, float px, float p
 // spread over many workgroups.  k_scan_apply lists each item's tile.
 // Persistent workgroups take items i = blockIdx.x, blockIdx.x + gridDim.x, ...
 // Each item evaluates its candidates (pixel box x tile) exactly as
-// drawClipped / drawDepth do, reduces them in LDS and writes the tile: with
-// plain stores when the tile is a single item, with global atomics (an
-// order-independent merge) when several items share it:
+// drawClipped / drawDepth do, reduces them in LDS and merges the tile's
+// covered pixels into the target with global atomics (order-independent,
+// and with what the setup drew and the tile's other items):
 //   MODE_VIS    64-bit min of (~ordered(z) << 32 | triangle*8+sub): largest
 //               depth wins, ties go to the first primitive in sequential
 //               order -- FragmentBuffer.DepthTest/Set under Workers(1)
@@ -1327,7 +1327,6 @@ __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB 
         const int t = SHADOW ? tile % rt.tiles_per_map : tile;
         const int px0 = (t % rt.tiles_x) * T, py0 = (t / rt.tiles_x) * T;
         const int first_item = chunk_off[tile];
-        const bool shared_tile = chunk_off[tile + 1] - first_item > 1;
         const int beg = tile_off[tile] + (item - first_item) * item_prims;
         const int n = min(item_prims, tile_off[tile + 1] - beg);
         for (int pp = tid; pp < TP; pp += 256) {
@@ -1360,23 +1359,15 @@ __device__ __forceinline__ void k_raster_body(const k_raster_args& a_, const VB 
         }
         for (int pp = tid; pp < TP; pp += 256) {
             const int x = px0 + (pp & (T - 1)), y = py0 + pp / T;
+            // (the setup may have drawn small triangles into the tile, and
+            // other items of it merge here too: an atomic, which needs no
+            // read of the old value first)
             if (SHADOW) {
-                if (x < rt.W && y < rt.H && smax[pp] > depth_ord(0.0f)) {
-                    uint32_t* m = &rt.smap_ord[(size_t)slot * rt.W * rt.H + (size_t)y * rt.W + x];
-                    if (shared_tile) atomicMax(m, smax[pp]);
-                    else if (smax[pp] > *m) *m = smax[pp];
-                }
+                if (x < rt.W && y < rt.H && smax[pp] > depth_ord(0.0f))
+                    atomicMax(&rt.smap_ord[(size_t)slot * rt.W * rt.H + (size_t)y * rt.W + x], smax[pp]);
             } else if (x < rt.W && y < rt.H) {
                 const size_t o = (size_t)y * rt.W + x;
-                // a single-item tile is this item's alone here, but the setup
-                // may have drawn small triangles into it already: keep the
-                // smaller key
-                if (!shared_tile) {
-                    const unsigned long long cur = rt.vis[o];
-                    rt.vis[o] = svis[pp] < cur ? svis[pp] : cur;
-                } else if (svis[pp] != VIS_EMPTY) {
-                    atomicMin(&rt.vis[o], svis[pp]);
-                }
+                if (svis[pp] != VIS_EMPTY) atomicMin(&rt.vis[o], svis[pp]);
                 if (EXACT) {
                     if (sfirst[pp] != 0xffffffffu) atomicMin(&rt.first_all[o], sfirst[pp]);
                     if (snan[pp] != 0xffffffffu) atomicMin(&rt.first_nan[o], snan[pp]);
