@@ -108,12 +108,14 @@ __device__ bool vp_intersect(const Box3& t, float w, float h) {
     float maxX = min2(w, t.mxx), maxY = min2(h, t.mxy), maxZ = min2(h, t.mxz);
     return minX <= maxX && minY <= maxY && minZ <= maxZ;
 }
+// (bitwise & and |: every term is a pure comparison, so evaluating them all
+// gives the same answer without a branch per term)
 __device__ __forceinline__ bool less_eq(float v1, float v2) {  // box.go:62-64
     const float eps = 1e-7f;
-    return approx_eq(v1, v2, eps) || (v1 < v2 && abs32(v1 - v2) > eps);
+    return approx_eq(v1, v2, eps) | ((v1 < v2) & (abs32(v1 - v2) > eps));
 }
 __device__ bool vp_contains(V4 p, float w, float h) {  // box.go:61-75 for one vertex
-    return less_eq(0.0f, p.x) && less_eq(0.0f, p.y) && less_eq(-1.0f, p.z) && less_eq(p.x, w) && less_eq(p.y, h) &&
+    return less_eq(0.0f, p.x) & less_eq(0.0f, p.y) & less_eq(-1.0f, p.z) & less_eq(p.x, w) & less_eq(p.y, h) &
            less_eq(p.z, 1.0f);
 }
 
@@ -519,7 +521,7 @@ __device__ __forceinline__ void k_setup_main_body(const k_setup_main_args& a_, c
             if (!(cross(sub(p2, p1), sub(p3, p1)).z < 0)) {  // cull.go:26-28
                 Box3 tb = box_of(p1, p2, p3);
                 if (vp_intersect(tb, w, h)) {
-                    if (vp_contains(p1, w, h) && vp_contains(p2, w, h) && vp_contains(p3, w, h))
+                    if (vp_contains(p1, w, h) & vp_contains(p2, w, h) & vp_contains(p3, w, h))
                         direct = pixel_box(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, vc.W, vc.H, x0, y0,
                                            x1, y1);
                     else {
